@@ -1,0 +1,79 @@
+"""ctypes mirror of include/gft.h (structures and constants)."""
+from __future__ import annotations
+
+import ctypes as C
+
+GFT_MODEL_SM = 0
+GFT_MODEL_2HDM = 1
+
+GFT_FORM_TWO_SLOT_F64 = 0
+GFT_FORM_TWO_SLOT_F32 = 1
+GFT_FORM_MOMENTUM_F32 = 2
+GFT_FORM_MOMENTUM_F64 = 3
+
+GFT_OK = 0
+GFT_EINVAL = -1
+GFT_EHIP = -2
+GFT_ENOMEM = -3
+GFT_EDOMAIN = -4
+GFT_ENODEV = -5
+
+NV = 15  # link reals per site: 3 directions x [y, c0, c1, c2, c3]
+
+
+class GftDesc(C.Structure):
+    _fields_ = [
+        ("model", C.c_int), ("form", C.c_int),
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+        ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double), ("dt", C.c_double),
+        ("g", C.c_double), ("gp", C.c_double), ("div_tol", C.c_double),
+        ("lambda_", C.c_double), ("eta_sqr", C.c_double),
+        ("m1", C.c_double), ("m2", C.c_double), ("l1", C.c_double), ("l2", C.c_double), ("l3", C.c_double),
+        ("break_to_son", C.c_int), ("break_to_uhalfn", C.c_int),
+        ("m12r", C.c_double), ("l4p5", C.c_double), ("l6r", C.c_double), ("l7r", C.c_double),
+        ("m12i", C.c_double), ("l4m5", C.c_double), ("l5i", C.c_double), ("l6i", C.c_double), ("l7i", C.c_double),
+        ("pot_norm", C.c_int), ("pot_offset", C.c_double),
+        ("damping", C.c_float), ("nt_damped", C.c_uint),
+    ]
+
+
+class GftObs(C.Structure):
+    _fields_ = [
+        ("time_iter", C.c_uint), ("has_energy", C.c_uint), ("has_gauss", C.c_uint), ("num_eq", C.c_uint),
+        ("energy", C.c_double * 6),
+        ("gauss_int", C.c_double * 4), ("gauss_max", C.c_double * 4), ("gauss_scale", C.c_double * 4),
+    ]
+
+
+class GftLoops(C.Structure):
+    _fields_ = [("w11_su2", C.c_double), ("w11_u1", C.c_double), ("w22_su2", C.c_double),
+                ("w22_u1", C.c_double), ("phi_sq", C.c_double)]
+
+
+def make_desc(model=GFT_MODEL_SM, form=GFT_FORM_TWO_SLOT_F64, n=(8, 8, 8), d=(0.5, 0.5, 0.5), dt=0.05,
+              g=0.65, gp=0.35, lam=0.13, eta_sqr=1.0, pot2=None, damping=0.0, nt_damped=0, div_tol=1e-9):
+    """A gft_desc with the SM couplings of the 32^3 / 256^3 workloads by default.
+
+    pot2: Double_SO_N parameters for the 2HDM (keys as in gft_desc)."""
+    ds = GftDesc()
+    ds.model, ds.form = model, form
+    ds.nx, ds.ny, ds.nz = n
+    ds.dx, ds.dy, ds.dz = d
+    ds.dt = dt
+    ds.g, ds.gp, ds.div_tol = g, gp, div_tol
+    ds.lambda_, ds.eta_sqr = lam, eta_sqr
+    ds.break_to_son = ds.break_to_uhalfn = 0
+    if pot2:
+        for k, v in pot2.items():
+            setattr(ds, k, v)
+    ds.damping, ds.nt_damped = damping, nt_damped
+    return ds
+
+
+# 2HDM potential of the 192^3 workload in Double_SO_N parameters:
+# Phi_k = (f_k0 + i f_k1, f_k2 + i f_k3)/sqrt(2), so Phi1^dag Phi2 = (D + iA)/2 with
+# D = f1.f2, A = f1^T J f2; then m1 = -m11^2/2, l1 = lambda1/8, l3 = lambda3/4,
+# l4p5 = (lambda4+lambda5)/4, l4m5 = (lambda4-lambda5)/4, m12r = m12^2.
+TWO_HDM_POTENTIAL = dict(m1=0.25, m2=0.25, l1=0.0325, l2=0.0325, l3=0.05, break_to_son=1, break_to_uhalfn=1,
+                         m12r=0.1, l4p5=-0.0125, l6r=0.0, l7r=0.0, m12i=0.0, l4m5=-0.0375, l5i=0.0,
+                         l6i=0.0, l7i=0.0, pot_norm=0, pot_offset=0.0)

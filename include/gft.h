@@ -1,0 +1,144 @@
+/*
+ * gft.h -- C ABI of libgft.so, the MI355X (gfx950) evolver for real-time
+ * lattice SU(2)xU(1)-Higgs dynamics in temporal gauge.
+ *
+ * The reference evolves one lattice through per-site virtual C++ calls
+ * (Potential / Gradient / WilsonLoop / Model, driven by Lattice::evolve(),
+ * Core/Source/Lattice.cpp:755-876).  Per-site calls cannot cross to a GPU,
+ * so this ABI cuts the path at whole-step granularity: one call advances the
+ * whole lattice N leapfrog steps, with the reference's Energy and
+ * GaugeCondition analysers folded into the same call.
+ *
+ * Conventions
+ *   - Every function returns 0 on success and a negative GFT_E* code on
+ *     failure; gft_last_error() gives the message.  No C++ exception crosses
+ *     this boundary (the reference throws std::runtime_error from its
+ *     constructors, Core/Source/Model.cpp:38,85,114; the C++ host wrappers in
+ *     gftevolver_amd/host rethrow with the same text).
+ *   - The caller owns every host array; the library copies in and out.
+ *   - A context owns its device buffers and one HIP stream.  A context is not
+ *     thread-safe (neither is the reference: Model keeps per-site scratch,
+ *     Core/Header/Model.hpp:255-262).
+ *   - Two-slot arrays use the reference layout
+ *       scalar[2][nx][ny][nz][Ns], vector[2][nx][ny][nz][Nv]
+ *     (Core/Source/Lattice.cpp:358-367, 658-671) with Nv = 15 (three
+ *     directions of [y, c0, c1, c2, c3]: U(1) phase then SU(2) quaternion).
+ *     At step index k the current slot is (k+1)%2 and new values go to slot
+ *     k%2 (Lattice.cpp:773-774).
+ */
+#ifndef GFT_H
+#define GFT_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- models (Core/Config/Model.cfg potential/gradient/wilson-loop ids) ---- */
+#define GFT_MODEL_SM    0  /* SO_N potential, Gradients::StandardModel, WilsonLoops::StandardModel; Ns=4 */
+#define GFT_MODEL_2HDM  1  /* Double_SO_N, Gradients::TwoHDM, WilsonLoops::TwoHDM with gaugeNum 4; Ns=8 */
+
+/* ---- state forms ---- */
+#define GFT_FORM_TWO_SLOT_F64 0  /* reference two-slot layout, FP64 storage (FP64 GPU oracle)            */
+#define GFT_FORM_TWO_SLOT_F32 1  /* reference two-slot layout, FP32 storage (the reference's own storage) */
+#define GFT_FORM_MOMENTUM_F32 2  /* structure-of-arrays kick/drift form, FP32 bulk (fast path)            */
+#define GFT_FORM_MOMENTUM_F64 3  /* structure-of-arrays kick/drift form, FP64 (plain FP64 GPU evolver)    */
+
+/* ---- error codes ---- */
+#define GFT_OK          0
+#define GFT_EINVAL     -1  /* invalid argument or configuration                */
+#define GFT_EHIP       -2  /* HIP runtime error                                */
+#define GFT_ENOMEM     -3  /* device allocation failed                         */
+#define GFT_EDOMAIN    -4  /* |R| >= 1 in a link update (asin/sqrt domain)      */
+#define GFT_ENODEV     -5  /* no usable gfx950 device                          */
+
+typedef struct gft_desc {
+    int      model;            /* GFT_MODEL_*                                                  */
+    int      form;             /* GFT_FORM_*                                                   */
+    int      nx, ny, nz;       /* periodic lattice size (Lattice.cfg line 11)                  */
+    double   dx, dy, dz, dt;   /* spacings and time step (Lattice.cfg lines 12, 14)            */
+    double   g, gp;            /* isospin / hypercharge couplings (WilsonLoops StandardModel.cfg) */
+    double   div_tol;          /* division-by-zero tolerance for 1/g^2 (same cfg), 1e-9        */
+    /* SO_N (Potentials/Source/SO_N.cpp:57-87): V = lambda/4 (|f|^2 - eta_sqr)^2              */
+    double   lambda, eta_sqr;
+    /* Double_SO_N (Potentials/Source/Double_SO_N.cpp:157-368)                                 */
+    double   m1, m2, l1, l2, l3;
+    int      break_to_son, break_to_uhalfn;
+    double   m12r, l4p5, l6r, l7r;
+    double   m12i, l4m5, l5i, l6i, l7i;
+    int      pot_norm;         /* add pot_offset to the potential-energy density (energy only)  */
+    double   pot_offset;
+    /* damping (Core/Source/Model.cpp:241-254, 345): gamma for time_iter < nt_damped          */
+    float    damping;
+    unsigned nt_damped;
+} gft_desc;
+
+/* Observables of one measured step.  energy[] columns follow the reference's
+ * global_quantities.dat order (Analysers/Source/Energy.cpp:169-189) and are
+ * already multiplied by dx*dy*dz; gauss_* follow global_quantities_gauge.dat
+ * (Analysers/Source/GaugeCondition.cpp:167-194).  All sums are FP64
+ * compensated (Neumaier) over FP64 per-site values. */
+typedef struct gft_obs {
+    unsigned time_iter;      /* step index the row belongs to                                  */
+    unsigned has_energy;     /* energy[] valid (measured before the update of this step)       */
+    unsigned has_gauss;      /* gauss_*[] valid (measured after the update of this step)       */
+    unsigned num_eq;         /* number of Gauss generators (4)                                 */
+    double   energy[6];      /* total, potential, gradient, kinetic, magnetic, electric        */
+    double   gauss_int[4];   /* sum_x |G^a(x)| dx dy dz, a = Y, 1, 2, 3                        */
+    double   gauss_max[4];   /* max_x |G^a(x)|                                                 */
+    double   gauss_scale[4]; /* sum_x (|div E^a| + g_a^2 |rho^a|) dx dy dz (relative measure)  */
+} gft_obs;
+
+typedef struct gft_ctx gft_ctx;
+
+/* Lifetime --------------------------------------------------------------- */
+int         gft_create(const gft_desc* desc, gft_ctx** out);
+void        gft_destroy(gft_ctx* ctx);
+const char* gft_last_error(const gft_ctx* ctx);   /* ctx may be NULL: last create error */
+int         gft_device_info(char* name, size_t name_len, int* cu_count);
+
+/* State transfer (reference two-slot layout, see above).  time_iter is the
+ * reference step counter of the next step to run; it fixes which slot is
+ * current. */
+int gft_import_two_slot_f32(gft_ctx* ctx, const float* scalar, const float* vec, unsigned time_iter);
+int gft_import_two_slot_f64(gft_ctx* ctx, const double* scalar, const double* vec, unsigned time_iter);
+int gft_export_two_slot_f32(gft_ctx* ctx, float* scalar, float* vec, unsigned* time_iter);
+int gft_export_two_slot_f64(gft_ctx* ctx, double* scalar, double* vec, unsigned* time_iter);
+
+/* Evolution.  Replaces Lattice::evolve() (Core/Source/Lattice.cpp:755-876)
+ * for nsteps steps.  energy_freq / gauss_freq reproduce the Energy and
+ * GaugeCondition cadence (measured on step 0 of the context and then every
+ * freq steps; 0 disables).  For the two-slot forms a measured energy step also
+ * normalises the last SU(2) product of the magnetic and electric terms, as the
+ * reference does (WilsonLoops/Source/StandardModel.cpp:374,433).  rows may be
+ * NULL when both frequencies are 0; otherwise it receives one gft_obs per
+ * measured step, up to max_rows, and *nrows the count. */
+int gft_evolve(gft_ctx* ctx, unsigned nsteps, unsigned energy_freq, unsigned gauss_freq,
+               gft_obs* rows, unsigned max_rows, unsigned* nrows);
+
+/* Observables of the current state without stepping (momentum forms: energy at
+ * the current time; two-slot forms: the reference's mixed-time energy). */
+int gft_observe(gft_ctx* ctx, gft_obs* out);
+
+/* Extra lattice observables (FP64): mean 1/2 tr of SU(2) Wilson loops and mean
+ * cos of U(1) Wilson loops of size 1x1 and 2x2 over the three spatial planes,
+ * and <|phi|^2>. */
+typedef struct gft_loops {
+    double w11_su2, w11_u1, w22_su2, w22_u1;
+    double phi_sq;
+} gft_loops;
+int gft_wilson_loops(gft_ctx* ctx, gft_loops* out);
+
+/* Wait for all queued work on the context's stream. */
+int gft_sync(gft_ctx* ctx);
+
+/* Timing: average duration (ms) of the step kernel over the last gft_evolve
+ * call, from HIP events recorded on the context's own stream around every
+ * step launch.  Returns GFT_EINVAL when no step ran. */
+int gft_last_step_ms(gft_ctx* ctx, double* avg_ms, double* total_ms, unsigned* nsteps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GFT_H */

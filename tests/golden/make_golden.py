@@ -1,0 +1,93 @@
+"""Writes the golden fixtures under tests/golden/ from the compiled reference.
+
+Run in the build container only (needs /root/reference):
+    make -C oracle ref && python tests/golden/make_golden.py
+
+Two kinds of fixture, both produced by the reference's own compiled code:
+  harness_<model>.npz  oracle/_ref/gft_ref_harness (reference Model / Potentials /
+                       Gradients / WilsonLoops objects) on a 4x5x6 periodic lattice
+                       from a seeded near-vacuum state with non-trivial links:
+                       fields after steps 1, 3 and 6, per-site energy densities of
+                       steps 0 and 3, per-site Gauss violations of steps 0 and 3
+  program_<model>.npz  the unmodified reference program (main.cpp + all libraries,
+                       oracle/_ref/gft_reference) on a 6x5x7 lattice with its
+                       RandomUniform initial state: Data/global_quantities.dat and
+                       Data/global_quantities_gauge.dat as printed (6 digits).
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+from gftevolver_amd._abi import TWO_HDM_POTENTIAL  # noqa: E402
+from tests import oracle_lib as O  # noqa: E402
+import refcfg  # noqa: E402
+
+HARNESS_N = (4, 5, 6)
+HARNESS_STEPS = 6
+HARNESS_KEEP = (0, 2, 5)          # step indices whose output fields are kept
+HARNESS_MEAS = 3                  # energy and Gauss measured on steps 0 and 3
+PROGRAM_N = (6, 5, 7)
+PROGRAM_STEPS = 12
+PROGRAM_EFREQ, PROGRAM_GFREQ = 3, 2
+PROGRAM_SEED, PROGRAM_RANGE = 7, (-0.8, 0.8)
+
+
+def harness_initial_state(model: int, seed: int = 2):
+    """Near-vacuum doublet(s) with a small t-dt offset; links near identity."""
+    N = int(np.prod(HARNESS_N))
+    Ns = 4 if model == 0 else 8
+    rng = np.random.default_rng(seed)
+    s = np.zeros((2, N, Ns), np.float32)
+    s[0] = (rng.normal(size=(N, Ns)) * 0.3).astype(np.float32)
+    s[0][:, 2] += 1.0
+    s[1] = s[0] + (rng.normal(size=(N, Ns)) * 0.01).astype(np.float32)
+    q = rng.normal(size=(2, N, 3, 4)) * 0.05
+    q[..., 0] += 1
+    q /= np.linalg.norm(q, axis=-1, keepdims=True)
+    y = rng.normal(size=(2, N, 3)) * 0.05
+    vv = np.zeros((2, N, 3, 5))
+    vv[..., 0] = y
+    vv[..., 1:] = q
+    return s, vv.reshape(2, N, 15).astype(np.float32)
+
+
+def main():
+    for model, tag in ((0, "sm"), (1, "2hdm")):
+        pot2 = TWO_HDM_POTENTIAL if model == 1 else None
+        Ns = 4 if model == 0 else 8
+        s, v = harness_initial_state(model)
+        with tempfile.TemporaryDirectory() as td:
+            h = O.run_harness(model, HARNESS_N, (0.5, 0.5, 0.5), 0.05, HARNESS_STEPS, HARNESS_MEAS, HARNESS_MEAS,
+                              s, v, td, pot2=pot2)
+        out = {"scalar0": s, "vec0": v}
+        for k in HARNESS_KEEP:
+            out[f"scalar_{k}"] = h[k]["scalar"]
+            out[f"vec_{k}"] = h[k]["vec"]
+        for k in range(0, HARNESS_STEPS, HARNESS_MEAS):
+            out[f"dens_{k}"] = h[k]["dens"]
+            out[f"viol_{k}"] = h[k]["viol"]
+        np.savez_compressed(os.path.join(HERE, f"harness_{tag}.npz"), **out)
+
+        refcfg.write_model(model, 0.65, 0.35, pot2=pot2)
+        refcfg.write_lattice(PROGRAM_N, (0.5, 0.5, 0.5), PROGRAM_STEPS, 0.05, Ns, ic_seed=PROGRAM_SEED,
+                             ic_range=PROGRAM_RANGE, efreq=PROGRAM_EFREQ, gfreq=PROGRAM_GFREQ)
+        subprocess.run([os.path.join(O.REF_DIR, "gft_reference")], check=True, stdout=subprocess.DEVNULL)
+        data = os.path.join(O.REF_DIR, "data")
+        E = np.loadtxt(os.path.join(data, "global_quantities.dat"), ndmin=2)
+        G = np.loadtxt(os.path.join(data, "global_quantities_gauge.dat"), ndmin=2)
+        np.savez_compressed(os.path.join(HERE, f"program_{tag}.npz"), energy=E, gauss=G)
+        print(tag, "written")
+
+
+if __name__ == "__main__":
+    main()

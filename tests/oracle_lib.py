@@ -1,0 +1,141 @@
+"""Loader for the CPU oracle (oracle/liboracle.so) and the reference harness.
+
+Test infrastructure: the oracle is the checker of the HIP evolver, never the
+thing measured.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from gftevolver_amd._abi import NV, GftDesc, GftObs
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_DIR = os.path.join(ROOT, "oracle", "_ref")
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "oracle"], check=True,
+                           stdout=subprocess.DEVNULL)
+        L = C.CDLL(ORACLE_SO)
+        for name, T in (("oracle_evolve_f32", C.c_float), ("oracle_evolve_f64", C.c_double)):
+            f = getattr(L, name)
+            f.argtypes = [C.POINTER(GftDesc), C.POINTER(T), C.POINTER(T), C.c_uint, C.c_uint, C.c_uint, C.c_uint,
+                          C.POINTER(C.c_double), C.POINTER(GftObs), C.c_uint, C.POINTER(C.c_uint), C.c_int,
+                          C.POINTER(C.c_ulong)]
+            f.restype = C.c_int
+        L.oracle_ic_random_uniform.argtypes = [C.POINTER(C.c_float), C.c_size_t, C.c_int, C.c_float, C.c_float,
+                                               C.c_int]
+        L.oracle_site_order.argtypes = [C.c_long, C.c_long, C.c_long, C.POINTER(C.c_long)]
+        L.oracle_site_order.restype = C.c_long
+        _lib = L
+    return _lib
+
+
+def evolve(desc: GftDesc, scalar: np.ndarray, vec: np.ndarray, nsteps: int, time_iter: int = 0,
+           energy_freq: int = 0, gauss_freq: int = 0, nthreads: int = 0):
+    """Advance two-slot arrays in place.  Returns (ref_rows [n,14], obs rows list, panics)."""
+    dtype = scalar.dtype
+    assert vec.dtype == dtype and scalar.flags.c_contiguous and vec.flags.c_contiguous
+    T = C.c_float if dtype == np.float32 else C.c_double
+    fn = lib().oracle_evolve_f32 if dtype == np.float32 else lib().oracle_evolve_f64
+    nmeas = 0
+    if energy_freq or gauss_freq:
+        nmeas = sum(1 for k in range(time_iter, time_iter + nsteps)
+                    if (energy_freq and k % energy_freq == 0) or (gauss_freq and k % gauss_freq == 0))
+    ref = np.zeros((max(nmeas, 1), 14), np.float64)
+    obs = (GftObs * max(nmeas, 1))()
+    nrows = C.c_uint(0)
+    panics = C.c_ulong(0)
+    rc = fn(C.byref(desc), scalar.ctypes.data_as(C.POINTER(T)), vec.ctypes.data_as(C.POINTER(T)), time_iter,
+            nsteps, energy_freq, gauss_freq, ref.ctypes.data_as(C.POINTER(C.c_double)), obs, max(nmeas, 1),
+            C.byref(nrows), nthreads, C.byref(panics))
+    if rc != 0:
+        raise RuntimeError(f"oracle evolve failed: {rc}")
+    return ref[:nrows.value], list(obs)[:nrows.value], panics.value
+
+
+def random_uniform(n: int, seed: int, lo: float, hi: float, set_equal: bool = True) -> np.ndarray:
+    out = np.zeros(n, np.float32)
+    lib().oracle_ic_random_uniform(out.ctypes.data_as(C.POINTER(C.c_float)), n, seed, lo, hi, int(set_equal))
+    return out
+
+
+def site_order(nx, ny, nz) -> np.ndarray:
+    out = np.zeros(nx * ny * nz, np.int64)
+    lib().oracle_site_order(nx, ny, nz, out.ctypes.data_as(C.POINTER(C.c_long)))
+    return out
+
+
+def identity_links(N: int, dtype=np.float32) -> np.ndarray:
+    v = np.zeros((2, N, NV), dtype)
+    v[:, :, 1::5] = 1.0
+    return v
+
+
+def ref_available() -> bool:
+    return os.path.exists(os.path.join(REF_DIR, "gft_ref_harness"))
+
+
+def run_harness(model: int, n, d, dt, nsteps, efreq, gfreq, scalar, vec, workdir, *, g=0.65, gp=0.35, lam=0.13,
+                eta_sqr=1.0, pot2=None, damping=0.0, nt_damped=0):
+    """Run the reference harness; returns a list of per-step dicts."""
+    import sys
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import refcfg
+    model_cfg = refcfg.write_model(model, g, gp, lam=lam, eta_sqr=eta_sqr, pot2=pot2, damping=damping,
+                                   nt_damped=nt_damped)
+    Ns = 4 if model == 0 else 8
+    N = n[0] * n[1] * n[2]
+    fin = os.path.join(workdir, "in.bin")
+    fout = os.path.join(workdir, "out.bin")
+    with open(fin, "wb") as f:
+        f.write(np.ascontiguousarray(scalar, np.float32).tobytes())
+        f.write(np.ascontiguousarray(vec, np.float32).tobytes())
+    subprocess.run([os.path.join(REF_DIR, "gft_ref_harness"), model_cfg, str(Ns), *map(str, n), *map(repr, d),
+                    repr(dt), str(nsteps), str(efreq), str(gfreq), fin, fout], check=True,
+                   stdout=subprocess.DEVNULL)
+    raw = np.fromfile(fout, np.float32)
+    out, pos = [], 0
+    for k in range(nsteps):
+        rec = {}
+        rec["scalar"] = raw[pos:pos + 2 * N * Ns].reshape(2, N, Ns); pos += 2 * N * Ns
+        rec["vec"] = raw[pos:pos + 2 * N * NV].reshape(2, N, NV); pos += 2 * N * NV
+        if efreq and k % efreq == 0:
+            rec["dens"] = raw[pos:pos + N * 5].reshape(N, 5); pos += N * 5
+        if gfreq and k % gfreq == 0:
+            rec["viol"] = raw[pos:pos + N * 4].reshape(N, 4); pos += N * 4
+        out.append(rec)
+    assert pos == raw.size
+    return out
+
+
+def energy_density_f32(desc, scalar, vec, time_iter):
+    L = lib()
+    N = desc.nx * desc.ny * desc.nz
+    out = np.zeros((N, 5), np.float32)
+    P = C.POINTER(C.c_float)
+    L.oracle_energy_density_f32.argtypes = [C.POINTER(GftDesc), P, P, C.c_uint, P]
+    L.oracle_energy_density_f32(C.byref(desc), scalar.ctypes.data_as(P), vec.ctypes.data_as(P), time_iter,
+                                out.ctypes.data_as(P))
+    return out
+
+
+def gauss_site_f32(desc, scalar, vec, time_iter):
+    L = lib()
+    N = desc.nx * desc.ny * desc.nz
+    out = np.zeros((N, 4), np.float32)
+    P = C.POINTER(C.c_float)
+    L.oracle_gauss_site_f32.argtypes = [C.POINTER(GftDesc), P, P, C.c_uint, P]
+    L.oracle_gauss_site_f32(C.byref(desc), scalar.ctypes.data_as(P), vec.ctypes.data_as(P), time_iter,
+                            out.ctypes.data_as(P))
+    return out
