@@ -1,0 +1,52 @@
+"""Builds the HIP library (libgft.so, gfx950) and, for the tests, the CPU checkers.
+
+    python -m gftevolver_amd.build            # library + oracle (+ reference build when present)
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libgft.so")
+REFERENCE = "/root/reference"
+
+HIPCC_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    # Packed FP32 instructions stay off: the compiler must not emit them.
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-result", "-Wno-unused-value",
+]
+
+
+def _run(cmd, **kw):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True, **kw)
+
+
+def build_library(verbose: bool = False):
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    tmp = LIB + ".tmp"
+    cmd = [hipcc, *HIPCC_FLAGS, "-o", tmp, os.path.join(CSRC, "gft_capi.hip")]
+    _run(cmd, stderr=None if verbose else subprocess.DEVNULL)
+    os.replace(tmp, LIB)
+
+
+def build_checkers():
+    oracle = os.path.join(ROOT, "oracle")
+    _run(["make", "-s", "-C", oracle, "oracle"])
+    if os.path.isdir(REFERENCE):
+        _run(["make", "-s", "-C", oracle, "ref", "-j8"])
+
+
+def build_all():
+    build_library()
+    build_checkers()
+
+
+if __name__ == "__main__":
+    build_library(verbose="-v" in sys.argv)
+    build_checkers()

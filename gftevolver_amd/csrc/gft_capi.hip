@@ -1,0 +1,445 @@
+// gft_capi.hip -- the C ABI of libgft.so (declared in include/gft.h).
+//
+// One translation unit: the kernel files are included here so every launch
+// site sees its kernel.  Each context owns its device buffers and one HIP
+// stream created at the default priority.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/gft.h"
+#include "gft_math.hpp"
+#include "gft_reduce.hpp"
+#include "gft_twoslot.hip"
+#include "gft_momentum.hip"
+
+using namespace gft;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+constexpr int kBlock = 256;
+constexpr int kEventPool = 256;
+
+bool is_two_slot(int form) { return form == GFT_FORM_TWO_SLOT_F64 || form == GFT_FORM_TWO_SLOT_F32; }
+bool is_f64(int form) { return form == GFT_FORM_TWO_SLOT_F64 || form == GFT_FORM_MOMENTUM_F64; }
+
+}  // namespace
+
+struct gft_ctx {
+    gft_desc desc{};
+    Coef coef{};
+    int form = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    long long N = 0;
+    // two-slot state (form TWO_SLOT_*): scalar[2][N][Ns], vector[2][N][15]
+    void* S = nullptr;
+    void* V = nullptr;
+    // momentum state (form MOMENTUM_*)
+    MomentumState mom{};
+    double* partial = nullptr;     // per-block reduction partials
+    size_t partial_cap = 0;        // doubles
+    double* obs_dev = nullptr;     // finalised observables (32 doubles)
+    double* obs_host = nullptr;    // pinned mirror
+    unsigned* err_dev = nullptr;
+    unsigned time_iter = 0;
+    std::string err;
+    // step-kernel timing
+    std::vector<hipEvent_t> ev_start, ev_stop;
+    int ev_used = 0;
+    double step_ms_total = 0.0;
+    unsigned step_count = 0;
+};
+
+#define CK(expr)                                                                                  \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess) {                                                                   \
+            ctx->err = std::string(#expr) + ": " + hipGetErrorString(e_);                        \
+            return GFT_EHIP;                                                                      \
+        }                                                                                         \
+    } while (0)
+
+static int fail(gft_ctx* ctx, int code, const std::string& msg) {
+    if (ctx) ctx->err = msg; else g_create_error = msg;
+    return code;
+}
+
+static Coef make_coef(const gft_desc& d) {
+    Coef c{};
+    c.ns = d.model == GFT_MODEL_2HDM ? 8 : 4;
+    c.nd = c.ns / 4;
+    c.nx = d.nx; c.ny = d.ny; c.nz = d.nz;
+    c.nsites = (long long)d.nx * d.ny * d.nz;
+    c.dt = d.dt;
+    const double sp[3] = {d.dx, d.dy, d.dz};
+    for (int a = 0; a < 3; a++) {
+        c.lap[a][0] = 1.0 / (sp[a] * sp[a]);
+        c.lap[a][1] = -2.0 / (sp[a] * sp[a]);
+        c.lap[a][2] = 1.0 / (sp[a] * sp[a]);
+        c.est[a] = 1.0 / sp[a];
+        c.inv_sp[a] = 1.0 / (sp[a] * sp[a]);
+    }
+    c.inv_sqr_dt = 1.0 / (d.dt * d.dt);
+    const double gs = d.g * d.g, gps = d.gp * d.gp;
+    c.inv_g_sqr = gs > d.div_tol ? 1.0 / gs : 0.0;
+    c.inv_gp_sqr = gps > d.div_tol ? 1.0 / gps : 0.0;
+    c.gsq[0] = (double)(float)gps;
+    c.gsq[1] = c.gsq[2] = c.gsq[3] = (double)(float)gs;
+    c.lambda = d.lambda; c.eta_sqr = d.eta_sqr;
+    c.m1 = d.m1; c.m2 = d.m2; c.l1 = d.l1; c.l2 = d.l2; c.l3 = d.l3;
+    c.m12r = d.m12r; c.l4p5 = d.l4p5; c.l6r = d.l6r; c.l7r = d.l7r;
+    c.m12i = d.m12i; c.l4m5 = d.l4m5; c.l5i = d.l5i; c.l6i = d.l6i; c.l7i = d.l7i;
+    c.break_son = d.break_to_son; c.break_uhalfn = d.break_to_uhalfn;
+    c.pot_norm = d.pot_norm; c.pot_offset = d.pot_offset;
+    c.dvol = d.dx * d.dy * d.dz;
+    return c;
+}
+
+static int ensure_partial(gft_ctx* ctx, size_t doubles) {
+    if (ctx->partial_cap >= doubles) return GFT_OK;
+    if (ctx->partial) hipFree(ctx->partial);
+    ctx->partial = nullptr;
+    CK(hipMalloc(&ctx->partial, doubles * sizeof(double)));
+    ctx->partial_cap = doubles;
+    return GFT_OK;
+}
+
+extern "C" {
+
+int gft_device_info(char* name, size_t name_len, int* cu_count) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1) return GFT_ENODEV;
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, 0) != hipSuccess) return GFT_ENODEV;
+    if (name && name_len) std::snprintf(name, name_len, "%s", p.gcnArchName);
+    if (cu_count) *cu_count = p.multiProcessorCount;
+    return GFT_OK;
+}
+
+const char* gft_last_error(const gft_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
+
+int gft_create(const gft_desc* desc, gft_ctx** out) {
+    if (!desc || !out) return fail(nullptr, GFT_EINVAL, "gft_create: null argument");
+    *out = nullptr;
+    const gft_desc& d = *desc;
+    if (d.model != GFT_MODEL_SM && d.model != GFT_MODEL_2HDM)
+        return fail(nullptr, GFT_EINVAL, "MODEL: The chosen model (" + std::to_string(d.model) + ") is invalid.");
+    if (d.form < GFT_FORM_TWO_SLOT_F64 || d.form > GFT_FORM_MOMENTUM_F64)
+        return fail(nullptr, GFT_EINVAL, "gft_create: invalid form " + std::to_string(d.form));
+    if (d.nx < 2 || d.ny < 2 || d.nz < 2)
+        return fail(nullptr, GFT_EINVAL, "LATTICE:: periodic lattices need at least 2 sites per direction");
+    if (!(d.dx > 0 && d.dy > 0 && d.dz > 0 && d.dt > 0))
+        return fail(nullptr, GFT_EINVAL, "LATTICE:: spacings and time step must be positive");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(nullptr, GFT_ENODEV, "gft_create: no HIP device");
+    gft_ctx* ctx = new gft_ctx();
+    ctx->desc = d;
+    ctx->form = d.form;
+    ctx->coef = make_coef(d);
+    ctx->N = ctx->coef.nsites;
+    hipGetDevice(&ctx->device);
+    auto bail = [&](int rc) { gft_destroy(ctx); return rc; };
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        g_create_error = "hipStreamCreate failed";
+        return bail(GFT_EHIP);
+    }
+    const size_t esz = is_f64(d.form) ? sizeof(double) : sizeof(float);
+    const long long N = ctx->N;
+    if (is_two_slot(d.form)) {
+        if (hipMalloc(&ctx->S, 2 * N * ctx->coef.ns * esz) != hipSuccess
+            || hipMalloc(&ctx->V, 2 * N * 15 * esz) != hipSuccess) {
+            g_create_error = "gft_create: device allocation failed";
+            return bail(GFT_ENOMEM);
+        }
+    } else {
+        int rc = momentum_alloc(ctx->mom, ctx->coef, is_f64(d.form));
+        if (rc) { g_create_error = "gft_create: device allocation failed"; return bail(rc); }
+    }
+    if (hipMalloc(&ctx->obs_dev, 64 * sizeof(double)) != hipSuccess
+        || hipHostMalloc(&ctx->obs_host, 64 * sizeof(double)) != hipSuccess
+        || hipMalloc(&ctx->err_dev, sizeof(unsigned)) != hipSuccess) {
+        g_create_error = "gft_create: device allocation failed";
+        return bail(GFT_ENOMEM);
+    }
+    hipMemsetAsync(ctx->err_dev, 0, sizeof(unsigned), ctx->stream);
+    ctx->ev_start.resize(kEventPool);
+    ctx->ev_stop.resize(kEventPool);
+    for (int i = 0; i < kEventPool; i++) {
+        if (hipEventCreate(&ctx->ev_start[i]) != hipSuccess || hipEventCreate(&ctx->ev_stop[i]) != hipSuccess) {
+            g_create_error = "hipEventCreate failed";
+            return bail(GFT_EHIP);
+        }
+    }
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) { g_create_error = "stream sync failed"; return bail(GFT_EHIP); }
+    *out = ctx;
+    return GFT_OK;
+}
+
+void gft_destroy(gft_ctx* ctx) {
+    if (!ctx) return;
+    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    if (ctx->S) hipFree(ctx->S);
+    if (ctx->V) hipFree(ctx->V);
+    momentum_free(ctx->mom);
+    if (ctx->partial) hipFree(ctx->partial);
+    if (ctx->obs_dev) hipFree(ctx->obs_dev);
+    if (ctx->obs_host) hipHostFree(ctx->obs_host);
+    if (ctx->err_dev) hipFree(ctx->err_dev);
+    for (auto e : ctx->ev_start) if (e) hipEventDestroy(e);
+    for (auto e : ctx->ev_stop) if (e) hipEventDestroy(e);
+    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+int gft_sync(gft_ctx* ctx) {
+    if (!ctx) return GFT_EINVAL;
+    CK(hipStreamSynchronize(ctx->stream));
+    return GFT_OK;
+}
+
+}  // extern "C"
+
+// ---- two-slot import/export --------------------------------------------------
+
+template <class Src>
+static int import_two_slot(gft_ctx* ctx, const Src* scalar, const Src* vec, unsigned time_iter) {
+    const long long N = ctx->N;
+    const size_t ns = 2 * N * ctx->coef.ns, nv = 2 * N * 15;
+    if (is_two_slot(ctx->form)) {
+        if (is_f64(ctx->form) == (sizeof(Src) == 8)) {
+            CK(hipMemcpyAsync(ctx->S, scalar, ns * sizeof(Src), hipMemcpyHostToDevice, ctx->stream));
+            CK(hipMemcpyAsync(ctx->V, vec, nv * sizeof(Src), hipMemcpyHostToDevice, ctx->stream));
+        } else if (is_f64(ctx->form)) {
+            std::vector<double> s(scalar, scalar + ns), v(vec, vec + nv);
+            CK(hipMemcpyAsync(ctx->S, s.data(), ns * 8, hipMemcpyHostToDevice, ctx->stream));
+            CK(hipMemcpyAsync(ctx->V, v.data(), nv * 8, hipMemcpyHostToDevice, ctx->stream));
+            CK(hipStreamSynchronize(ctx->stream));
+        } else {
+            std::vector<float> s(ns), v(nv);
+            for (size_t i = 0; i < ns; i++) s[i] = (float)scalar[i];
+            for (size_t i = 0; i < nv; i++) v[i] = (float)vec[i];
+            CK(hipMemcpyAsync(ctx->S, s.data(), ns * 4, hipMemcpyHostToDevice, ctx->stream));
+            CK(hipMemcpyAsync(ctx->V, v.data(), nv * 4, hipMemcpyHostToDevice, ctx->stream));
+            CK(hipStreamSynchronize(ctx->stream));
+        }
+    } else {
+        int rc = momentum_import(ctx->mom, ctx->coef, scalar, vec, time_iter, ctx->stream, ctx->err);
+        if (rc) return rc;
+    }
+    CK(hipStreamSynchronize(ctx->stream));
+    ctx->time_iter = time_iter;
+    return GFT_OK;
+}
+
+template <class Dst>
+static int export_two_slot(gft_ctx* ctx, Dst* scalar, Dst* vec, unsigned* time_iter) {
+    const long long N = ctx->N;
+    const size_t ns = 2 * N * ctx->coef.ns, nv = 2 * N * 15;
+    if (is_two_slot(ctx->form)) {
+        if (is_f64(ctx->form) == (sizeof(Dst) == 8)) {
+            CK(hipMemcpyAsync(scalar, ctx->S, ns * sizeof(Dst), hipMemcpyDeviceToHost, ctx->stream));
+            CK(hipMemcpyAsync(vec, ctx->V, nv * sizeof(Dst), hipMemcpyDeviceToHost, ctx->stream));
+            CK(hipStreamSynchronize(ctx->stream));
+        } else if (is_f64(ctx->form)) {
+            std::vector<double> s(ns), v(nv);
+            CK(hipMemcpyAsync(s.data(), ctx->S, ns * 8, hipMemcpyDeviceToHost, ctx->stream));
+            CK(hipMemcpyAsync(v.data(), ctx->V, nv * 8, hipMemcpyDeviceToHost, ctx->stream));
+            CK(hipStreamSynchronize(ctx->stream));
+            for (size_t i = 0; i < ns; i++) scalar[i] = (Dst)s[i];
+            for (size_t i = 0; i < nv; i++) vec[i] = (Dst)v[i];
+        } else {
+            std::vector<float> s(ns), v(nv);
+            CK(hipMemcpyAsync(s.data(), ctx->S, ns * 4, hipMemcpyDeviceToHost, ctx->stream));
+            CK(hipMemcpyAsync(v.data(), ctx->V, nv * 4, hipMemcpyDeviceToHost, ctx->stream));
+            CK(hipStreamSynchronize(ctx->stream));
+            for (size_t i = 0; i < ns; i++) scalar[i] = (Dst)s[i];
+            for (size_t i = 0; i < nv; i++) vec[i] = (Dst)v[i];
+        }
+    } else {
+        int rc = momentum_export(ctx->mom, ctx->coef, scalar, vec, ctx->time_iter, ctx->stream, ctx->err);
+        if (rc) return rc;
+    }
+    if (time_iter) *time_iter = ctx->time_iter;
+    return GFT_OK;
+}
+
+extern "C" {
+
+int gft_import_two_slot_f32(gft_ctx* ctx, const float* s, const float* v, unsigned it) {
+    if (!ctx || !s || !v) return GFT_EINVAL;
+    return import_two_slot(ctx, s, v, it);
+}
+int gft_import_two_slot_f64(gft_ctx* ctx, const double* s, const double* v, unsigned it) {
+    if (!ctx || !s || !v) return GFT_EINVAL;
+    return import_two_slot(ctx, s, v, it);
+}
+int gft_export_two_slot_f32(gft_ctx* ctx, float* s, float* v, unsigned* it) {
+    if (!ctx || !s || !v) return GFT_EINVAL;
+    return export_two_slot(ctx, s, v, it);
+}
+int gft_export_two_slot_f64(gft_ctx* ctx, double* s, double* v, unsigned* it) {
+    if (!ctx || !s || !v) return GFT_EINVAL;
+    return export_two_slot(ctx, s, v, it);
+}
+
+}  // extern "C"
+
+// ---- evolution ----------------------------------------------------------------
+
+static int timed_begin(gft_ctx* ctx) {
+    if (ctx->ev_used == kEventPool) {
+        CK(hipStreamSynchronize(ctx->stream));
+        for (int i = 0; i < kEventPool; i++) {
+            float ms = 0.f;
+            CK(hipEventElapsedTime(&ms, ctx->ev_start[i], ctx->ev_stop[i]));
+            ctx->step_ms_total += ms;
+        }
+        ctx->ev_used = 0;
+    }
+    CK(hipEventRecord(ctx->ev_start[ctx->ev_used], ctx->stream));
+    return GFT_OK;
+}
+
+static int timed_end(gft_ctx* ctx) {
+    CK(hipEventRecord(ctx->ev_stop[ctx->ev_used], ctx->stream));
+    ctx->ev_used++;
+    ctx->step_count++;
+    return GFT_OK;
+}
+
+static int flush_timing(gft_ctx* ctx) {
+    CK(hipStreamSynchronize(ctx->stream));
+    for (int i = 0; i < ctx->ev_used; i++) {
+        float ms = 0.f;
+        CK(hipEventElapsedTime(&ms, ctx->ev_start[i], ctx->ev_stop[i]));
+        ctx->step_ms_total += ms;
+    }
+    ctx->ev_used = 0;
+    return GFT_OK;
+}
+
+// Reads the finalised observables into a row.
+static int collect_row(gft_ctx* ctx, bool e, bool g, gft_obs* row) {
+    CK(hipMemcpyAsync(ctx->obs_host, ctx->obs_dev, 32 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    std::memset(row, 0, sizeof(*row));
+    row->time_iter = ctx->time_iter;
+    row->num_eq = 4;
+    const double dv = ctx->coef.dvol;
+    if (e) {
+        row->has_energy = 1;
+        double tot = 0.0;
+        for (int m = 0; m < 5; m++) {
+            row->energy[m + 1] = ctx->obs_host[m] * dv;
+            tot += ctx->obs_host[m];
+        }
+        row->energy[0] = tot * dv;
+    }
+    if (g) {
+        row->has_gauss = 1;
+        for (int m = 0; m < 4; m++) {
+            row->gauss_int[m] = ctx->obs_host[8 + m] * dv;
+            row->gauss_scale[m] = ctx->obs_host[12 + m] * dv;
+            row->gauss_max[m] = ctx->obs_host[16 + m];
+        }
+    }
+    return GFT_OK;
+}
+
+extern "C" int gft_evolve(gft_ctx* ctx, unsigned nsteps, unsigned energy_freq, unsigned gauss_freq, gft_obs* rows,
+                          unsigned max_rows, unsigned* nrows) {
+    if (!ctx) return GFT_EINVAL;
+    if ((energy_freq || gauss_freq) && !rows && max_rows) return fail(ctx, GFT_EINVAL, "gft_evolve: rows is NULL");
+    const long long N = ctx->N;
+    const long long nblocks = (N + kBlock - 1) / kBlock;
+    int rc = ensure_partial(ctx, (size_t)nblocks * 12);
+    if (rc) return rc;
+    unsigned row = 0;
+    ctx->step_ms_total = 0.0;
+    ctx->step_count = 0;
+    ctx->ev_used = 0;
+    for (unsigned k = 0; k < nsteps; k++) {
+        const unsigned it = ctx->time_iter;
+        const unsigned now = (it + 1) % 2;
+        const bool me = energy_freq && it % energy_freq == 0;
+        const bool mg = gauss_freq && it % gauss_freq == 0;
+        const float damp = it < ctx->desc.nt_damped ? ctx->desc.damping : 0.f;
+        const double damping_step = 0.5 * (double)damp * ctx->desc.dt;
+        if ((rc = timed_begin(ctx))) return rc;
+        if (is_two_slot(ctx->form)) {
+            if (ctx->form == GFT_FORM_TWO_SLOT_F64)
+                twoslot_step<double><<<nblocks, kBlock, 0, ctx->stream>>>(ctx->coef, (double*)ctx->S, (double*)ctx->V,
+                                                                          now, me, damping_step, ctx->partial,
+                                                                          ctx->err_dev);
+            else
+                twoslot_step<float><<<nblocks, kBlock, 0, ctx->stream>>>(ctx->coef, (float*)ctx->S, (float*)ctx->V,
+                                                                         now, me, damping_step, ctx->partial,
+                                                                         ctx->err_dev);
+            CK(hipGetLastError());
+        } else {
+            if ((rc = momentum_step(ctx->mom, ctx->coef, damping_step, ctx->stream, ctx->err_dev, ctx->err))) return rc;
+        }
+        if ((rc = timed_end(ctx))) return rc;
+        if (me) {
+            if (is_two_slot(ctx->form)) {
+                finalize_partials<<<1, 256, 0, ctx->stream>>>(ctx->partial, nblocks, 5, 0, ctx->obs_dev);
+            } else {
+                // Momentum forms measure the pre-step state in a separate pass.
+                ctx->err = "internal: momentum energy handled before the step";
+                return GFT_EINVAL;
+            }
+        }
+        if (mg) {
+            if (ctx->form == GFT_FORM_TWO_SLOT_F64)
+                twoslot_gauss<double><<<nblocks, kBlock, 0, ctx->stream>>>(ctx->coef, (const double*)ctx->S,
+                                                                           (const double*)ctx->V, now, ctx->partial);
+            else if (ctx->form == GFT_FORM_TWO_SLOT_F32)
+                twoslot_gauss<float><<<nblocks, kBlock, 0, ctx->stream>>>(ctx->coef, (const float*)ctx->S,
+                                                                          (const float*)ctx->V, now, ctx->partial);
+            else if ((rc = momentum_gauss(ctx->mom, ctx->coef, ctx->partial, nblocks, ctx->stream, ctx->err)))
+                return rc;
+            finalize_partials<<<1, 256, 0, ctx->stream>>>(ctx->partial, nblocks, 8, 4, ctx->obs_dev + 8);
+            CK(hipGetLastError());
+        }
+        if ((me || mg) && row < max_rows && rows) {
+            if ((rc = collect_row(ctx, me, mg, &rows[row]))) return rc;
+            row++;
+        }
+        ctx->time_iter++;
+    }
+    if ((rc = flush_timing(ctx))) return rc;
+    if (nrows) *nrows = row;
+    unsigned bad = 0;
+    CK(hipMemcpy(&bad, ctx->err_dev, sizeof(unsigned), hipMemcpyDeviceToHost));
+    if (bad) {
+        CK(hipMemset(ctx->err_dev, 0, sizeof(unsigned)));
+        return fail(ctx, GFT_EDOMAIN, "Panic in evolve: |R| > 1 in a link update (asin/sqrt out of domain)");
+    }
+    return GFT_OK;
+}
+
+extern "C" int gft_last_step_ms(gft_ctx* ctx, double* avg_ms, double* total_ms, unsigned* nsteps) {
+    if (!ctx || ctx->step_count == 0) return GFT_EINVAL;
+    if (avg_ms) *avg_ms = ctx->step_ms_total / ctx->step_count;
+    if (total_ms) *total_ms = ctx->step_ms_total;
+    if (nsteps) *nsteps = ctx->step_count;
+    return GFT_OK;
+}
+
+extern "C" int gft_observe(gft_ctx* ctx, gft_obs* out) {
+    if (!ctx || !out) return GFT_EINVAL;
+    return fail(ctx, GFT_EINVAL, "gft_observe: not available for this form yet");
+}
+
+extern "C" int gft_wilson_loops(gft_ctx* ctx, gft_loops* out) {
+    if (!ctx || !out) return GFT_EINVAL;
+    return fail(ctx, GFT_EINVAL, "gft_wilson_loops: not available for this form yet");
+}

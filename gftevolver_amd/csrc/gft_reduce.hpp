@@ -1,0 +1,85 @@
+// gft_reduce.hpp -- deterministic FP64 block reductions.
+//
+// Each block reduces per-thread partial sums (pairwise through wavefront
+// shuffles, then LDS across waves) and writes one partial per quantity; a
+// single-block finisher sums the per-block partials in block order with
+// Neumaier compensation.  The result does not depend on timing or atomics.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "gft_math.hpp"
+
+namespace gft {
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, kWave));
+    return v;
+}
+
+// Reduces K sums and M maxima over the block; thread 0 writes
+// out[blockIdx.x * (K + M) + k].  lds must hold (blockDim.x / 64) * (K + M) doubles.
+template <int K, int M>
+__device__ __forceinline__ void block_reduce_store(const double* sums, const double* maxs, double* lds, double* out,
+                                                   long long block_id) {
+    const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave, nw = blockDim.x / kWave;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        double v = wave_sum(sums[k]);
+        if (lane == 0) lds[wave * (K + M) + k] = v;
+    }
+#pragma unroll
+    for (int m = 0; m < M; m++) {
+        double v = wave_max(maxs[m]);
+        if (lane == 0) lds[wave * (K + M) + K + m] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < K + M) {
+        const int q = threadIdx.x;
+        double acc = lds[q];
+        for (int w = 1; w < nw; w++) acc = (q < K) ? acc + lds[w * (K + M) + q] : fmax(acc, lds[w * (K + M) + q]);
+        out[block_id * (K + M) + q] = acc;
+    }
+}
+
+// One block of 256 threads: out[q] = Neumaier sum (q < nsum) or max (q >= nsum)
+// over partial[b * (nsum + nmax) + q], b = 0 .. nblocks-1, in block order per
+// thread stripe, then combined in stripe order.
+__global__ void __launch_bounds__(256) finalize_partials(const double* __restrict__ partial, long long nblocks,
+                                                         int nsum, int nmax, double* __restrict__ out) {
+    __shared__ double s_hi[256], s_lo[256];
+    const int width = nsum + nmax;
+    for (int q = 0; q < width; q++) {
+        Kahan k;
+        k.init();
+        double mx = 0.0;
+        for (long long b = threadIdx.x; b < nblocks; b += blockDim.x) {
+            double v = partial[b * width + q];
+            if (q < nsum) k.add(v); else mx = fmax(mx, v);
+        }
+        s_hi[threadIdx.x] = (q < nsum) ? k.s : mx;
+        s_lo[threadIdx.x] = (q < nsum) ? k.c : 0.0;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            Kahan t;
+            t.init();
+            double m = 0.0;
+            for (int i = 0; i < (int)blockDim.x; i++) {
+                if (q < nsum) { t.add(s_hi[i]); t.add(s_lo[i]); } else m = fmax(m, s_hi[i]);
+            }
+            out[q] = (q < nsum) ? t.value() : m;
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace gft

@@ -1,0 +1,143 @@
+"""ctypes binding of libgft.so (the C ABI in include/gft.h).
+
+This is the Python caller of the drop-in boundary used by the tests and the
+benchmark; the C++ host API (gftevolver_amd/host) is the other caller.  There
+is no fallback: if the HIP library is missing, importing this module raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from ._abi import (GFT_FORM_MOMENTUM_F32, GFT_FORM_MOMENTUM_F64, GFT_FORM_TWO_SLOT_F32, GFT_FORM_TWO_SLOT_F64, NV,
+                   GftDesc, GftLoops, GftObs)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgft.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+
+_L = C.CDLL(LIB_PATH)
+
+_P = C.POINTER
+_L.gft_create.argtypes = [_P(GftDesc), _P(C.c_void_p)]
+_L.gft_destroy.argtypes = [C.c_void_p]
+_L.gft_destroy.restype = None
+_L.gft_last_error.argtypes = [C.c_void_p]
+_L.gft_last_error.restype = C.c_char_p
+_L.gft_device_info.argtypes = [C.c_char_p, C.c_size_t, _P(C.c_int)]
+for _n, _t in (("f32", C.c_float), ("f64", C.c_double)):
+    getattr(_L, f"gft_import_two_slot_{_n}").argtypes = [C.c_void_p, _P(_t), _P(_t), C.c_uint]
+    getattr(_L, f"gft_export_two_slot_{_n}").argtypes = [C.c_void_p, _P(_t), _P(_t), _P(C.c_uint)]
+_L.gft_evolve.argtypes = [C.c_void_p, C.c_uint, C.c_uint, C.c_uint, _P(GftObs), C.c_uint, _P(C.c_uint)]
+_L.gft_observe.argtypes = [C.c_void_p, _P(GftObs)]
+_L.gft_wilson_loops.argtypes = [C.c_void_p, _P(GftLoops)]
+_L.gft_sync.argtypes = [C.c_void_p]
+_L.gft_last_step_ms.argtypes = [C.c_void_p, _P(C.c_double), _P(C.c_double), _P(C.c_uint)]
+
+
+class GftError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"gft error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    return _L
+
+
+def device_info():
+    buf = C.create_string_buffer(128)
+    cu = C.c_int(0)
+    rc = _L.gft_device_info(buf, 128, C.byref(cu))
+    if rc != 0:
+        raise GftError(rc, "no HIP device")
+    return buf.value.decode(), cu.value
+
+
+class Evolver:
+    """One lattice on the GPU behind the C ABI."""
+
+    def __init__(self, desc: GftDesc):
+        self.desc = desc
+        self.ns = 8 if desc.model == 1 else 4
+        self.n = (desc.nx, desc.ny, desc.nz)
+        self.nsites = desc.nx * desc.ny * desc.nz
+        h = C.c_void_p()
+        rc = _L.gft_create(C.byref(desc), C.byref(h))
+        if rc != 0:
+            raise GftError(rc, _L.gft_last_error(None).decode())
+        self._h = h
+
+    def _check(self, rc):
+        if rc != 0:
+            raise GftError(rc, _L.gft_last_error(self._h).decode())
+
+    def close(self):
+        if self._h:
+            _L.gft_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def import_two_slot(self, scalar: np.ndarray, vec: np.ndarray, time_iter: int = 0):
+        dt = scalar.dtype
+        assert dt in (np.float32, np.float64) and vec.dtype == dt
+        assert scalar.size == 2 * self.nsites * self.ns and vec.size == 2 * self.nsites * NV
+        s = np.ascontiguousarray(scalar)
+        v = np.ascontiguousarray(vec)
+        T = C.c_float if dt == np.float32 else C.c_double
+        fn = _L.gft_import_two_slot_f32 if dt == np.float32 else _L.gft_import_two_slot_f64
+        self._check(fn(self._h, s.ctypes.data_as(C.POINTER(T)), v.ctypes.data_as(C.POINTER(T)), time_iter))
+
+    def export_two_slot(self, dtype=np.float64):
+        s = np.zeros((2, self.nsites, self.ns), dtype)
+        v = np.zeros((2, self.nsites, NV), dtype)
+        T = C.c_float if dtype == np.float32 else C.c_double
+        fn = _L.gft_export_two_slot_f32 if dtype == np.float32 else _L.gft_export_two_slot_f64
+        it = C.c_uint(0)
+        self._check(fn(self._h, s.ctypes.data_as(C.POINTER(T)), v.ctypes.data_as(C.POINTER(T)), C.byref(it)))
+        return s, v, it.value
+
+    def evolve(self, nsteps: int, energy_freq: int = 0, gauss_freq: int = 0, max_rows: int | None = None):
+        if max_rows is None:
+            max_rows = nsteps if (energy_freq or gauss_freq) else 0
+        rows = (GftObs * max(max_rows, 1))()
+        nrows = C.c_uint(0)
+        self._check(_L.gft_evolve(self._h, nsteps, energy_freq, gauss_freq, rows, max_rows, C.byref(nrows)))
+        return list(rows)[:nrows.value]
+
+    def observe(self) -> GftObs:
+        o = GftObs()
+        self._check(_L.gft_observe(self._h, C.byref(o)))
+        return o
+
+    def wilson_loops(self) -> GftLoops:
+        o = GftLoops()
+        self._check(_L.gft_wilson_loops(self._h, C.byref(o)))
+        return o
+
+    def sync(self):
+        self._check(_L.gft_sync(self._h))
+
+    def last_step_ms(self):
+        avg, tot, n = C.c_double(), C.c_double(), C.c_uint()
+        self._check(_L.gft_last_step_ms(self._h, C.byref(avg), C.byref(tot), C.byref(n)))
+        return avg.value, tot.value, n.value
+
+
+__all__ = ["Evolver", "GftError", "device_info", "lib", "GFT_FORM_TWO_SLOT_F64", "GFT_FORM_TWO_SLOT_F32",
+           "GFT_FORM_MOMENTUM_F32", "GFT_FORM_MOMENTUM_F64"]

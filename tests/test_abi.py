@@ -1,0 +1,48 @@
+"""CPU checks of the drop-in boundary: the HIP library loads (no GPU needed) and
+exports every symbol include/gft.h declares; the ctypes structures match the
+header's layout."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from gftevolver_amd import _abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "gft.h")
+LIB = os.path.join(ROOT, "gftevolver_amd", "libgft.so")
+
+
+def declared_functions():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(gft_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_header_declares_the_boundary():
+    names = declared_functions()
+    for required in ("gft_create", "gft_destroy", "gft_evolve", "gft_import_two_slot_f32",
+                     "gft_export_two_slot_f64", "gft_last_error", "gft_last_step_ms"):
+        assert required in names
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libgft.so not built")
+def test_library_exports_every_declared_symbol():
+    lib = C.CDLL(LIB)
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_struct_layouts_match_header():
+    # gft_obs: 4 unsigned then 6 + 3*4 doubles.
+    assert C.sizeof(_abi.GftObs) == 16 + 18 * 8
+    assert C.sizeof(_abi.GftLoops) == 5 * 8
+    fields = [f[0] for f in _abi.GftDesc._fields_]
+    txt = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    body = txt[txt.index("typedef struct gft_desc"):txt.index("} gft_desc;")]
+    hdr = re.findall(r"\b(?:int|double|float|unsigned)\s+([^;]+);", body)
+    names = [x.strip() for grp in hdr for x in grp.split(",")]
+    assert [n if n != "lambda" else "lambda_" for n in names] == fields
