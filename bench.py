@@ -1,0 +1,194 @@
+#!/usr/bin/env python3
+"""Throughput of the evolver on the 256^3 Standard-Model workload.
+
+One step = one leapfrog update of every site of the lattice (all fields).
+The timed region runs exactly --steps steps of the FP32-bulk momentum-form
+kernel on data already resident in HBM, bracketed by stream synchronisations;
+`value` is site-updates per second over that wall-clock interval.  Beside it:
+the FP64 momentum form (the plain FP64 GPU evolver) and the FP64 two-slot
+reference-layout kernel on the same state, the Gauss-law violation and energy
+drift after the run, the step kernel's roofline position (HIP events on the
+library's own stream) and the reference CPU program timed on a bounded sample.
+
+    python bench.py [--steps K] [--warmup W] [--n 256]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "site-updates/s + HBM GB/s, FP32-mixed vs FP64 at Gauss err<=1e-6, 256^3 SM"
+BYTES_F32 = 280   # 35 reals read + 35 written per site-update, 4 B each
+BYTES_F64 = 560
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def gauss_rel(row):
+    return max(row.gauss_int[m] / row.gauss_scale[m] for m in range(4) if row.gauss_scale[m] > 0)
+
+
+def run_form(rt, desc, s, v, steps, warmup, launch=None):
+    with rt.Evolver(desc) as ev:
+        if launch:
+            ev.set_launch(*launch)
+        ev.import_two_slot(s, v, 0)
+        e0 = ev.observe().energy[0] if desc.form >= 2 else None
+        ev.evolve(warmup)
+        ev.sync()
+        t0 = time.perf_counter()
+        ev.evolve(steps)
+        ev.sync()
+        wall = time.perf_counter() - t0
+        avg_ms, _, nk = ev.last_step_ms()
+        rows = ev.evolve(1, gauss_freq=1)
+        e1 = ev.observe().energy[0] if desc.form >= 2 else None
+    return {"wall_s": wall, "kernel_ms": avg_ms, "gauss_rel": gauss_rel(rows[0]), "e0": e0, "e1": e1}
+
+
+def cpu_baseline(nsteps=60):
+    """The reference program (oracle/_ref/gft_reference, built from the
+    reference's own sources) on a bounded sample: SM 32^3, `nsteps` steps,
+    analysers off, one core; its own 'Evolution finished in' timer."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "gft_reference")
+    n = 32
+    if os.path.exists(exe):
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import refcfg
+        refcfg.write_model(0, 0.65, 0.35, lam=0.13, eta_sqr=1.0)
+        refcfg.write_lattice((n, n, n), (0.5, 0.5, 0.5), nsteps, 0.05, 4, energy=False, gauss=False, progress=True,
+                             ic_range=(-0.1, 0.1))
+        r = subprocess.run([exe], capture_output=True, text=True, timeout=600)
+        m = re.findall(r"Evolution finished in ([0-9.eE+-]+) seconds", r.stdout)
+        if r.returncode == 0 and m:
+            sec = float(m[-1])
+            return {"value": n ** 3 * nsteps / sec, "unit": "site-updates/s", "cores": 1, "kind": "reference",
+                    "sample": f"reference program built from its sources, SM {n}^3, {nsteps} steps, analysers off, "
+                              f"single-threaded on one host core ({sec:.2f} s)"}
+    # Fallback: the FP64 CPU restatement on one thread.
+    import numpy as np
+    from gftevolver_amd._abi import make_desc
+    from gftevolver_amd.ic import gaussian_state
+    from tests import oracle_lib as O
+    d = make_desc(n=(n, n, n))
+    s, v = gaussian_state((n, n, n))
+    t0 = time.perf_counter()
+    O.evolve(d, s, v, nsteps, nthreads=1)
+    sec = time.perf_counter() - t0
+    return {"value": n ** 3 * nsteps / sec, "unit": "site-updates/s", "cores": 1, "kind": "port",
+            "sample": f"CPU restatement (FP64 two-slot), SM {n}^3, {nsteps} steps, one thread ({sec:.2f} s)"}
+
+
+def read_traffic():
+    """HBM bytes per step-kernel launch from the committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of known size)."""
+    p = os.path.join(ROOT, "profiles", "mom_step_f32_256_pmc.txt")
+    if not os.path.exists(p):
+        return None
+    m = re.search(r"traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)", open(p).read())
+    return float(m.group(1)) if m else None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--fp64-steps", type=int, default=100)
+    ap.add_argument("--twoslot-steps", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only-f32", action="store_true")
+    ap.add_argument("--launch", type=str, default="")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build runs every workload on a single MI355X "
+              f"(--gpus {args.gpus} is not supported).", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from gftevolver_amd import runtime as rt
+    from gftevolver_amd._abi import (GFT_FORM_MOMENTUM_F32, GFT_FORM_MOMENTUM_F64, GFT_FORM_TWO_SLOT_F64,
+                                     make_desc)
+    from gftevolver_amd.ic import grf_state
+
+    name, cus = rt.device_info()
+    n = (args.n,) * 3
+    N = args.n ** 3
+    log(f"device {name}, {cus} CUs; building GRF initial state {n}")
+    s, v = grf_state(n, d=(0.5, 0.5, 0.5), rms=0.1, vev=1.0, seed=20261203, dtype=np.float32)
+    launch = tuple(int(x) for x in args.launch.split(",")) if args.launch else None
+
+    d32 = make_desc(form=GFT_FORM_MOMENTUM_F32, n=n)
+    r32 = run_form(rt, d32, s, v, args.steps, args.warmup, launch)
+    value = N * args.steps / r32["wall_s"]
+    achieved = BYTES_F32 * N / (r32["kernel_ms"] * 1e-3) / 1e9
+    log(f"FP32 momentum: {r32['wall_s'] / args.steps * 1e3:.3f} ms/step wall, kernel {r32['kernel_ms']:.3f} ms, "
+        f"{value:.3e} site-updates/s, {achieved:.0f} GB/s algorithmic, Gauss rel {r32['gauss_rel']:.2e}")
+
+    extra = {}
+    if not args.only_f32:
+        d64 = make_desc(form=GFT_FORM_MOMENTUM_F64, n=n)
+        r64 = run_form(rt, d64, s, v, args.fp64_steps, 2)
+        v64 = N * args.fp64_steps / r64["wall_s"]
+        log(f"FP64 momentum: kernel {r64['kernel_ms']:.3f} ms, {v64:.3e} site-updates/s")
+        dts = make_desc(form=GFT_FORM_TWO_SLOT_F64, n=n)
+        rts = run_form(rt, dts, s, v, args.twoslot_steps, 2)
+        vts = N * args.twoslot_steps / rts["wall_s"]
+        log(f"FP64 two-slot: kernel {rts['kernel_ms']:.3f} ms, {vts:.3e} site-updates/s")
+        extra = {
+            "fp64_momentum": {"value": v64, "kernel_ms": r64["kernel_ms"],
+                              "achieved_GBps": BYTES_F64 * N / (r64["kernel_ms"] * 1e-3) / 1e9,
+                              "gauss_rel": r64["gauss_rel"]},
+            "fp64_two_slot": {"value": vts, "kernel_ms": rts["kernel_ms"], "gauss_rel": rts["gauss_rel"]},
+            "speedup_fp32_vs_fp64_momentum": value / v64,
+            "speedup_fp32_vs_fp64_two_slot": value / vts,
+        }
+
+    cpu = None if args.no_cpu else cpu_baseline()
+    traffic = read_traffic()
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "site-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r32["wall_s"] / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"SM {args.n}^3 periodic, FP32-bulk momentum-form leapfrog; GRF Higgs IC "
+                        f"(P(k)~1/k, k<=pi/(2dx), rms 0.1v), links identity, E=0; dx=0.5 dt=0.05 g=0.65 "
+                        f"g'=0.35 lambda=0.13 v=1",
+            "lattice": list(n),
+            "launch": list(launch) if launch else [64, 4, 1],
+        },
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "algorithmic_bytes_per_site_update": BYTES_F32,
+                     "kernel_ms": r32["kernel_ms"]},
+        "cpu_baseline": cpu,
+        "gauss_rel_violation": r32["gauss_rel"],
+        "energy_drift_rel": (r32["e1"] - r32["e0"]) / abs(r32["e0"]),
+        **extra,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -354,14 +354,16 @@ static int collect_row(gft_ctx* ctx, bool e, bool g, gft_obs* row) {
     return GFT_OK;
 }
 
+static long long obs_blocks(const gft_ctx* ctx) { return (ctx->N + kBlock - 1) / kBlock; }
+
 extern "C" int gft_evolve(gft_ctx* ctx, unsigned nsteps, unsigned energy_freq, unsigned gauss_freq, gft_obs* rows,
                           unsigned max_rows, unsigned* nrows) {
     if (!ctx) return GFT_EINVAL;
     if ((energy_freq || gauss_freq) && !rows && max_rows) return fail(ctx, GFT_EINVAL, "gft_evolve: rows is NULL");
-    const long long N = ctx->N;
-    const long long nblocks = (N + kBlock - 1) / kBlock;
+    const long long nblocks = obs_blocks(ctx);
     int rc = ensure_partial(ctx, (size_t)nblocks * 12);
     if (rc) return rc;
+    const bool two = is_two_slot(ctx->form);
     unsigned row = 0;
     ctx->step_ms_total = 0.0;
     ctx->step_count = 0;
@@ -373,8 +375,14 @@ extern "C" int gft_evolve(gft_ctx* ctx, unsigned nsteps, unsigned energy_freq, u
         const bool mg = gauss_freq && it % gauss_freq == 0;
         const float damp = it < ctx->desc.nt_damped ? ctx->desc.damping : 0.f;
         const double damping_step = 0.5 * (double)damp * ctx->desc.dt;
+        if (me && !two) {
+            // The momentum form measures the state before the update in its own pass.
+            if ((rc = momentum_observe(ctx->mom, ctx->coef, MomObs::Energy, ctx->partial, nblocks, ctx->stream, ctx->err)))
+                return rc;
+            finalize_partials<<<1, 256, 0, ctx->stream>>>(ctx->partial, nblocks, 5, 0, ctx->obs_dev);
+        }
         if ((rc = timed_begin(ctx))) return rc;
-        if (is_two_slot(ctx->form)) {
+        if (two) {
             if (ctx->form == GFT_FORM_TWO_SLOT_F64)
                 twoslot_step<double><<<nblocks, kBlock, 0, ctx->stream>>>(ctx->coef, (double*)ctx->S, (double*)ctx->V,
                                                                           now, me, damping_step, ctx->partial,
@@ -388,15 +396,7 @@ extern "C" int gft_evolve(gft_ctx* ctx, unsigned nsteps, unsigned energy_freq, u
             if ((rc = momentum_step(ctx->mom, ctx->coef, damping_step, ctx->stream, ctx->err_dev, ctx->err))) return rc;
         }
         if ((rc = timed_end(ctx))) return rc;
-        if (me) {
-            if (is_two_slot(ctx->form)) {
-                finalize_partials<<<1, 256, 0, ctx->stream>>>(ctx->partial, nblocks, 5, 0, ctx->obs_dev);
-            } else {
-                // Momentum forms measure the pre-step state in a separate pass.
-                ctx->err = "internal: momentum energy handled before the step";
-                return GFT_EINVAL;
-            }
-        }
+        if (me && two) finalize_partials<<<1, 256, 0, ctx->stream>>>(ctx->partial, nblocks, 5, 0, ctx->obs_dev);
         if (mg) {
             if (ctx->form == GFT_FORM_TWO_SLOT_F64)
                 twoslot_gauss<double><<<nblocks, kBlock, 0, ctx->stream>>>(ctx->coef, (const double*)ctx->S,
@@ -404,7 +404,8 @@ extern "C" int gft_evolve(gft_ctx* ctx, unsigned nsteps, unsigned energy_freq, u
             else if (ctx->form == GFT_FORM_TWO_SLOT_F32)
                 twoslot_gauss<float><<<nblocks, kBlock, 0, ctx->stream>>>(ctx->coef, (const float*)ctx->S,
                                                                           (const float*)ctx->V, now, ctx->partial);
-            else if ((rc = momentum_gauss(ctx->mom, ctx->coef, ctx->partial, nblocks, ctx->stream, ctx->err)))
+            else if ((rc = momentum_observe(ctx->mom, ctx->coef, MomObs::Gauss, ctx->partial, nblocks, ctx->stream,
+                                            ctx->err)))
                 return rc;
             finalize_partials<<<1, 256, 0, ctx->stream>>>(ctx->partial, nblocks, 8, 4, ctx->obs_dev + 8);
             CK(hipGetLastError());
@@ -426,6 +427,16 @@ extern "C" int gft_evolve(gft_ctx* ctx, unsigned nsteps, unsigned energy_freq, u
     return GFT_OK;
 }
 
+extern "C" int gft_set_launch(gft_ctx* ctx, int bz, int by, int xcd_remap) {
+    if (!ctx) return GFT_EINVAL;
+    if (bz < 1 || by < 1 || bz * by != 256 || (bz % 64 && bz != 32 && bz != 16))
+        return fail(ctx, GFT_EINVAL, "gft_set_launch: need bz*by = 256 with bz a multiple of 64, or 32 or 16");
+    ctx->mom.bz = bz;
+    ctx->mom.by = by;
+    ctx->mom.xcd_remap = xcd_remap ? 1 : 0;
+    return GFT_OK;
+}
+
 extern "C" int gft_last_step_ms(gft_ctx* ctx, double* avg_ms, double* total_ms, unsigned* nsteps) {
     if (!ctx || ctx->step_count == 0) return GFT_EINVAL;
     if (avg_ms) *avg_ms = ctx->step_ms_total / ctx->step_count;
@@ -436,10 +447,41 @@ extern "C" int gft_last_step_ms(gft_ctx* ctx, double* avg_ms, double* total_ms, 
 
 extern "C" int gft_observe(gft_ctx* ctx, gft_obs* out) {
     if (!ctx || !out) return GFT_EINVAL;
-    return fail(ctx, GFT_EINVAL, "gft_observe: not available for this form yet");
+    if (is_two_slot(ctx->form))
+        return fail(ctx, GFT_EINVAL, "gft_observe: the two-slot forms measure inside gft_evolve (reference cadence)");
+    const long long nblocks = obs_blocks(ctx);
+    int rc = ensure_partial(ctx, (size_t)nblocks * 12);
+    if (rc) return rc;
+    if ((rc = momentum_observe(ctx->mom, ctx->coef, MomObs::Energy, ctx->partial, nblocks, ctx->stream, ctx->err)))
+        return rc;
+    finalize_partials<<<1, 256, 0, ctx->stream>>>(ctx->partial, nblocks, 5, 0, ctx->obs_dev);
+    const bool g = ctx->mom.prev_valid;
+    if (g) {
+        if ((rc = momentum_observe(ctx->mom, ctx->coef, MomObs::Gauss, ctx->partial, nblocks, ctx->stream, ctx->err)))
+            return rc;
+        finalize_partials<<<1, 256, 0, ctx->stream>>>(ctx->partial, nblocks, 8, 4, ctx->obs_dev + 8);
+    }
+    CK(hipGetLastError());
+    return collect_row(ctx, true, g, out);
 }
 
 extern "C" int gft_wilson_loops(gft_ctx* ctx, gft_loops* out) {
     if (!ctx || !out) return GFT_EINVAL;
-    return fail(ctx, GFT_EINVAL, "gft_wilson_loops: not available for this form yet");
+    if (is_two_slot(ctx->form)) return fail(ctx, GFT_EINVAL, "gft_wilson_loops: available for the momentum forms");
+    const long long nblocks = obs_blocks(ctx);
+    int rc = ensure_partial(ctx, (size_t)nblocks * 12);
+    if (rc) return rc;
+    if ((rc = momentum_observe(ctx->mom, ctx->coef, MomObs::Loops, ctx->partial, nblocks, ctx->stream, ctx->err)))
+        return rc;
+    finalize_partials<<<1, 256, 0, ctx->stream>>>(ctx->partial, nblocks, 5, 0, ctx->obs_dev + 24);
+    CK(hipGetLastError());
+    CK(hipMemcpyAsync(ctx->obs_host, ctx->obs_dev, 32 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    const double n = (double)ctx->N;
+    out->w11_su2 = ctx->obs_host[24] / (3 * n);
+    out->w11_u1 = ctx->obs_host[25] / (3 * n);
+    out->w22_su2 = ctx->obs_host[26] / (3 * n);
+    out->w22_u1 = ctx->obs_host[27] / (3 * n);
+    out->phi_sq = ctx->obs_host[28] / n;
+    return GFT_OK;
 }

@@ -1,40 +1,863 @@
-// gft_momentum.hip -- structure-of-arrays kick/drift form (placeholder until
-// the fused kernel lands; every entry point reports "not available").
+// gft_momentum.hip -- the fast path: structure-of-arrays kick/drift form of the
+// reference leapfrog, one fused kernel per step.
+//
+// State per site (SM; the 2HDM doubles phi and P):
+//   phi  float4            scalar doublet at t                 (ping-pong)
+//   P    float4            phi(t) - phi(t-dt)                  (in place)
+//   y[3] float             U(1) link phases at t, wrapped      (ping-pong)
+//   q[3] float4            SU(2) link quaternions at t         (ping-pong)
+//   E[3] float4            (R^Y, R^1, R^2, R^3) of the last update (in place)
+// = 35 reals read and 35 written per site-update.  With P and E the reference's
+// update (Core/Source/Model.cpp:345-372, WilsonLoops/Source/StandardModel.cpp:
+// 456-514) becomes
+//   kick:  P <- (1 - g dt/2) P + dt^2 (Lap phi - V'(phi))
+//          E <- (1 - g dt/2) E + dt^2 (M + g_a^2 J)
+//   drift: phi <- phi + P;  y <- y + asin(E^Y);  q <- N[(sqrt(1-|E|^2), E) q]
+// which is the reference's two-slot recursion with El = vec(U(t) U(t-dt)^dag)
+// and sin(y(t) - y(t-dt)) replaced by the stored E of the previous step
+// (identical in exact arithmetic).  phi and the links ping-pong because
+// neighbours read them; P and E are read and written only at their own site.
+//
+// Layout: each field component group is its own array of N elements, sites in
+// the reference order s = (x*ny + y)*nz + z, so a wave of 64 threads along z
+// issues fully coalesced 16-byte loads per group.  Blocks are (BZ x BY) tiles
+// of one x-plane; consecutive logical blocks advance along x inside a (y,z)
+// column so the x-neighbour planes just read by the previous blocks are still
+// in L2, and logical blocks are dealt to the 8 XCDs in contiguous ranges
+// (hardware sends block b to XCD b % 8) so each XCD's L2 sees a compact region.
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
+#include <cstring>
 #include <string>
+#include <vector>
 
 #include "gft_math.hpp"
+#include "gft_reduce.hpp"
 
 namespace gft {
 
-struct MomentumState {
-    void* base = nullptr;
+template <class R> struct V4T;
+template <> struct V4T<float> { using type = float4; };
+template <> struct V4T<double> { using type = double4; };
+
+// Device pointers of one state.
+template <class R>
+struct MomFields {
+    using V4 = typename V4T<R>::type;
+    V4* phi[2][2];   // [buffer][doublet]
+    V4* P[2];        // [doublet]
+    R* y[2][3];      // [buffer][direction]
+    V4* q[2][3];     // [buffer][direction]
+    V4* E[3];        // [direction]
 };
 
-inline int momentum_alloc(MomentumState&, const Coef&, bool) { return GFT_EINVAL; }
+struct MomentumState {
+    void* base = nullptr;
+    size_t bytes = 0;
+    bool f64 = false;
+    int cur = 0;             // buffer holding (phi, U) at the current time
+    int prev_valid = 0;      // buffer !cur holds the state one step earlier
+    MomFields<float> ff{};
+    MomFields<double> fd{};
+    int bz = 64, by = 4;     // tile shape of the step kernel
+    int xcd_remap = 1;
+};
+
+// ---- float special functions on the small ranges the evolution produces ----
+// Taylor polynomials (coefficients 1/n!) after a quadrant reduction with a
+// three-part pi/2; |error| < 1e-9 absolute for |x| <= 16.
+__device__ __forceinline__ void sincos_fast(float x, float* s, float* c) {
+    const float q = rintf(x * 0.636619772367581343f);
+    float r = fmaf(-q, 1.57079637050628662109375f, x);
+    r = fmaf(-q, -4.371138828673793e-8f, r);
+    r = fmaf(-q, -1.7151245100058819e-15f, r);
+    const float r2 = r * r;
+    float sp = fmaf(r2, 2.75573192239858907e-6f, -1.98412698412698413e-4f);
+    sp = fmaf(r2, sp, 8.33333333333333333e-3f);
+    sp = fmaf(r2, sp, -1.66666666666666667e-1f);
+    const float sr = fmaf(r * r2, sp, r);
+    float cp = fmaf(r2, -2.75573192239858907e-7f, 2.48015873015873016e-5f);
+    cp = fmaf(r2, cp, -1.38888888888888889e-3f);
+    cp = fmaf(r2, cp, 4.16666666666666667e-2f);
+    cp = fmaf(r2, cp, -0.5f);
+    const float cr = fmaf(r2, cp, 1.0f);
+    const int k = static_cast<int>(q) & 3;
+    const float ss = (k & 1) ? cr : sr;
+    const float cc = (k & 1) ? sr : cr;
+    *s = (k & 2) ? -ss : ss;
+    *c = ((k + 1) & 2) ? -cc : cc;
+}
+
+__device__ __forceinline__ float sin_fast(float x) {
+    float s, c;
+    sincos_fast(x, &s, &c);
+    return s;
+}
+
+// asin by its Maclaurin series for |x| <= 1/8 (terms to x^11, error < 1e-13
+// relative), the library asinf beyond.
+__device__ __forceinline__ float asin_small(float x) {
+    if (fabsf(x) > 0.125f) return asinf(x);
+    const float x2 = x * x;
+    float p = fmaf(x2, 2.23721590909090909e-2f, 3.03819444444444444e-2f);
+    p = fmaf(x2, p, 4.46428571428571429e-2f);
+    p = fmaf(x2, p, 7.5e-2f);
+    p = fmaf(x2, p, 1.66666666666666667e-1f);
+    return fmaf(x * x2, p, x);
+}
+
+__device__ __forceinline__ void sincos_fast(double x, double* s, double* c) { sincos(x, s, c); }
+__device__ __forceinline__ double sin_fast(double x) { return sin(x); }
+__device__ __forceinline__ double asin_small(double x) { return asin(x); }
+__device__ __forceinline__ float rsqrt_r(float x) { return __frsqrt_rn(x); }
+__device__ __forceinline__ double rsqrt_r(double x) { return 1.0 / sqrt(x); }
+
+// Streaming (non-temporal) access to own-site data that no neighbour reads.
+typedef float f4n __attribute__((ext_vector_type(4)));
+typedef double d4n __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 ld_nt(const float4* p) {
+    const f4n v = __builtin_nontemporal_load(reinterpret_cast<const f4n*>(p));
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ double4 ld_nt(const double4* p) {
+    const d4n v = __builtin_nontemporal_load(reinterpret_cast<const d4n*>(p));
+    return make_double4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void st_nt(const float4& v, float4* p) {
+    const f4n n = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(n, reinterpret_cast<f4n*>(p));
+}
+__device__ __forceinline__ void st_nt(const double4& v, double4* p) {
+    const d4n n = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(n, reinterpret_cast<d4n*>(p));
+}
+__device__ __forceinline__ void st_nt(float v, float* p) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ void st_nt(double v, double* p) { __builtin_nontemporal_store(v, p); }
+
+template <class R>
+__device__ __forceinline__ R wrap_pi(R y) {
+    const R twopi = R(6.283185307179586476925286766559);
+    const R pi = R(3.1415926535897932384626433832795);
+    if (y > pi) y -= twopi;
+    if (y < -pi) y += twopi;
+    return y;
+}
+
+template <class R>
+struct Qt { R w, x, y, z; };   // quaternion (c0, c1, c2, c3)
+
+template <class V4, class R>
+__device__ __forceinline__ Qt<R> toq(const V4& v) { return Qt<R>{v.x, v.y, v.z, v.w}; }
+
+// a * b
+template <class R>
+__device__ __forceinline__ Qt<R> qmul(const Qt<R>& a, const Qt<R>& b) {
+    Qt<R> r;
+    r.w = a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z;
+    r.x = a.w * b.x + a.x * b.w - a.y * b.z + a.z * b.y;
+    r.y = a.w * b.y + a.y * b.w - a.z * b.x + a.x * b.z;
+    r.z = a.w * b.z + a.z * b.w - a.x * b.y + a.y * b.x;
+    return r;
+}
+template <class R>
+__device__ __forceinline__ Qt<R> qconj(const Qt<R>& a) { return Qt<R>{a.w, -a.x, -a.y, -a.z}; }
+// vector part of a * b
+template <class R>
+__device__ __forceinline__ void qvec(const Qt<R>& a, const Qt<R>& b, R* v) {
+    v[0] = a.w * b.x + a.x * b.w - a.y * b.z + a.z * b.y;
+    v[1] = a.w * b.y + a.y * b.w - a.z * b.x + a.x * b.z;
+    v[2] = a.w * b.z + a.z * b.w - a.x * b.y + a.y * b.x;
+}
+
+// SU(2) action (StandardModel.cpp:94-97) of quaternion u on doublet h.
+template <class R>
+__device__ __forceinline__ void act(const Qt<R>& u, const R* h, R* o) {
+    o[0] = u.w * h[0] - u.z * h[1] + u.y * h[2] - u.x * h[3];
+    o[1] = u.z * h[0] + u.w * h[1] + u.x * h[2] + u.y * h[3];
+    o[2] = -u.y * h[0] - u.x * h[1] + u.w * h[2] + u.z * h[3];
+    o[3] = u.x * h[0] - u.y * h[1] - u.z * h[2] + u.w * h[3];
+}
+
+// T phi = U^W (e^{iy} phi) for one doublet, with (c, s) = (cos y, sin y);
+// conj gives U^W^dag (e^{-iy} phi).
+template <class R>
+__device__ __forceinline__ void transport(const Qt<R>& u, R c, R s, bool conj, const R* h, R* o) {
+    const R ss = conj ? -s : s;
+    R t[4];
+    t[0] = c * h[0] - ss * h[1];
+    t[1] = c * h[1] + ss * h[0];
+    t[2] = c * h[2] - ss * h[3];
+    t[3] = c * h[3] + ss * h[2];
+    act(conj ? qconj(u) : u, t, o);
+}
+
+template <class V4, class R>
+__device__ __forceinline__ void unpack(const V4& v, R* o) { o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w; }
+
+template <class R>
+struct StepArgs {
+    using V4 = typename V4T<R>::type;
+    const V4* phi[2];
+    V4* phi_out[2];
+    V4* P[2];
+    const R* y[3];
+    R* y_out[3];
+    const V4* q[3];
+    V4* q_out[3];
+    V4* E[3];
+    int nx, ny, nz, nbz, nby;
+    long long nblocks;
+    int bz, by, xcd_remap;
+    R inv_sp[3];
+    R dt2, keep;             // dt^2 and 1 - gamma dt / 2
+    R gsq[4];
+    unsigned* err;
+};
+
+// Maps hardware block b to a logical (bz, by, x) tile.
+__device__ __forceinline__ long long logical_block(long long b, long long nb, int remap) {
+    if (!remap) return b;
+    const long long per = nb / 8, rem = nb % 8, xcd = b % 8, idx = b / 8;
+    return xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
+}
+
+template <class R, int ND>
+__global__ void __launch_bounds__(256) mom_step(StepArgs<R> a, Coef c) {
+    using V4 = typename V4T<R>::type;
+    const long long L = logical_block(blockIdx.x, a.nblocks, a.xcd_remap);
+    const int x = static_cast<int>(L % a.nx);
+    const long long rest = L / a.nx;
+    const int tby = static_cast<int>(rest % a.nby);
+    const int tbz = static_cast<int>(rest / a.nby);
+    const int z = tbz * a.bz + static_cast<int>(threadIdx.x) % a.bz;
+    const int y = tby * a.by + static_cast<int>(threadIdx.x) / a.bz;
+    if (z >= a.nz || y >= a.ny) return;
+
+    const long long sy = a.nz, sx = (long long)a.ny * a.nz;
+    const long long s = (long long)x * sx + (long long)y * sy + z;
+    // Offsets to the +e_a and -e_a neighbours with periodic wrap.
+    long long dp[3], dm[3];
+    dp[0] = (x == a.nx - 1) ? -(a.nx - 1) * sx : sx;
+    dm[0] = (x == 0) ? (a.nx - 1) * sx : -sx;
+    dp[1] = (y == a.ny - 1) ? -(long long)(a.ny - 1) * sy : sy;
+    dm[1] = (y == 0) ? (long long)(a.ny - 1) * sy : -sy;
+    dp[2] = (z == a.nz - 1) ? -(long long)(a.nz - 1) : 1;
+    dm[2] = (z == 0) ? (long long)(a.nz - 1) : -1;
+
+    // Own links and their phases.
+    Qt<R> U[3];
+    R cy[3], sy_[3], yv[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        U[i] = toq<V4, R>(a.q[i][s]);
+        yv[i] = a.y[i][s];
+        sincos_fast(yv[i], &sy_[i], &cy[i]);
+    }
+
+    R phi[4 * ND];
+#pragma unroll
+    for (int k = 0; k < ND; k++) unpack(a.phi[k][s], phi + 4 * k);
+
+    // Scalar force and currents.
+    R F[4 * ND];
+    potential_derivative<R>(c, phi, F);
+#pragma unroll
+    for (int k = 0; k < 4 * ND; k++) F[k] = -F[k];
+    R J[3][4];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const long long sp = s + dp[i], sm = s + dm[i];
+        const Qt<R> Um = toq<V4, R>(a.q[i][sm]);
+        R cm, smn;
+        sincos_fast(a.y[i][sm], &smn, &cm);
+        J[i][0] = J[i][1] = J[i][2] = J[i][3] = R(0);
+#pragma unroll
+        for (int k = 0; k < ND; k++) {
+            R fp[4], fm[4], tp[4], tm[4], tx[4];
+            unpack(a.phi[k][sp], fp);
+            unpack(a.phi[k][sm], fm);
+            transport(U[i], cy[i], sy_[i], true, fp, tp);   // U_i(x)^dag phi(x+i)
+            transport(Um, cm, smn, false, fm, tm);          // U_i(x-i) phi(x-i)
+            transport(U[i], cy[i], sy_[i], false, phi + 4 * k, tx);  // U_i(x) phi(x)
+#pragma unroll
+            for (int m = 0; m < 4; m++) F[4 * k + m] += a.inv_sp[i] * (tp[m] + tm[m] - R(2) * phi[4 * k + m]);
+            bilinears_add(tx, fp, J[i]);
+        }
+    }
+
+    // Magnetic force (staples, WilsonLoops/Source/StandardModel.cpp:324-409).
+    R M[3][4];
+#pragma unroll
+    for (int i = 0; i < 3; i++) M[i][0] = M[i][1] = M[i][2] = M[i][3] = R(0);
+    // Plaquettes based at x, shared between the two orientations of a plane:
+    //   P1(i,j) = U_j(x+i)^dag U_i(x+j) U_j(x) U_i(x)^dag = A C,
+    //   P1(j,i) = (C A)^dag.
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+#pragma unroll
+        for (int j = i + 1; j < 3; j++) {
+            const Qt<R> Ujxpi = toq<V4, R>(a.q[j][s + dp[i]]);
+            const Qt<R> Uixpj = toq<V4, R>(a.q[i][s + dp[j]]);
+            const R th = -a.y[j][s + dp[i]] + a.y[i][s + dp[j]] + yv[j] - yv[i];
+            const Qt<R> A = qmul(qconj(Ujxpi), Uixpj);
+            const Qt<R> C = qmul(U[j], qconj(U[i]));
+            R v1[3], v2[3];
+            qvec(A, C, v1);
+            qvec(C, A, v2);
+            const R st = sin_fast(th);
+            M[i][0] += a.inv_sp[j] * st;
+            M[j][0] -= a.inv_sp[i] * st;
+#pragma unroll
+            for (int m = 0; m < 3; m++) {
+                M[i][1 + m] += a.inv_sp[j] * v1[m];
+                M[j][1 + m] -= a.inv_sp[i] * v2[m];
+            }
+        }
+    }
+    // Plaquettes based at x-j:  P2(i,j) = U_i(x) U_j(x-j) U_i(x-j)^dag U_j(x+i-j)^dag.
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        const long long smj = s + dm[j];
+        const Qt<R> Ujm = toq<V4, R>(a.q[j][smj]);
+        const R yjm = a.y[j][smj];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            if (i == j) continue;
+            const long long sd = s + dp[i] + dm[j];
+            const Qt<R> Uim = toq<V4, R>(a.q[i][smj]);
+            const Qt<R> Ujd = toq<V4, R>(a.q[j][sd]);
+            const R th = -a.y[j][sd] - a.y[i][smj] + yjm + yv[i];
+            const Qt<R> B = qmul(qmul(Ujm, qconj(Uim)), qconj(Ujd));
+            R v[3];
+            qvec(U[i], B, v);
+            M[i][0] -= a.inv_sp[j] * sin_fast(th);
+#pragma unroll
+            for (int m = 0; m < 3; m++) M[i][1 + m] -= a.inv_sp[j] * v[m];
+        }
+    }
+
+    // Kick and drift.
+#pragma unroll
+    for (int k = 0; k < ND; k++) {
+        R p[4];
+        unpack(ld_nt(&a.P[k][s]), p);
+        V4 pn, fn;
+        pn.x = a.keep * p[0] + a.dt2 * F[4 * k + 0];
+        pn.y = a.keep * p[1] + a.dt2 * F[4 * k + 1];
+        pn.z = a.keep * p[2] + a.dt2 * F[4 * k + 2];
+        pn.w = a.keep * p[3] + a.dt2 * F[4 * k + 3];
+        fn.x = phi[4 * k + 0] + pn.x;
+        fn.y = phi[4 * k + 1] + pn.y;
+        fn.z = phi[4 * k + 2] + pn.z;
+        fn.w = phi[4 * k + 3] + pn.w;
+        st_nt(pn, &a.P[k][s]);
+        st_nt(fn, &a.phi_out[k][s]);
+    }
+    unsigned bad = 0;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        R e[4];
+        unpack(ld_nt(&a.E[i][s]), e);
+        V4 en;
+        en.x = a.keep * e[0] + a.dt2 * (M[i][0] + a.gsq[0] * J[i][0]);
+        en.y = a.keep * e[1] + a.dt2 * (M[i][1] + a.gsq[1] * J[i][1]);
+        en.z = a.keep * e[2] + a.dt2 * (M[i][2] + a.gsq[2] * J[i][2]);
+        en.w = a.keep * e[3] + a.dt2 * (M[i][3] + a.gsq[3] * J[i][3]);
+        const R e2 = en.y * en.y + en.z * en.z + en.w * en.w;
+        if (!(e2 < R(1)) || !(fabs(en.x) < R(1))) bad = 1;
+        const R yn = wrap_pi(yv[i] + asin_small(en.x));
+        const Qt<R> u{sqrt(R(1) - e2), en.y, en.z, en.w};
+        Qt<R> qn = qmul(u, U[i]);
+        const R inv = rsqrt_r(qn.w * qn.w + qn.x * qn.x + qn.y * qn.y + qn.z * qn.z);
+        V4 qo;
+        qo.x = qn.w * inv;
+        qo.y = qn.x * inv;
+        qo.z = qn.y * inv;
+        qo.w = qn.z * inv;
+        st_nt(en, &a.E[i][s]);
+        st_nt(yn, &a.y_out[i][s]);
+        st_nt(qo, &a.q_out[i][s]);
+    }
+    if (bad) atomicOr(a.err, 1u);
+}
+
+// ---- FP64 observables of the momentum form ------------------------------------
+
+template <class R>
+struct ObsArgs {
+    using V4 = typename V4T<R>::type;
+    const V4* phi[2];       // current
+    const V4* phi_old[2];   // one step earlier (Gauss only)
+    const V4* P[2];
+    const R* y[3];
+    const V4* q[3];
+    const V4* q_old[3];     // links one step earlier (Gauss only)
+    const V4* E[3];
+    int nx, ny, nz;
+};
+
+__device__ __forceinline__ long long nb_index(int nx, int ny, int nz, long long s, int a, int delta) {
+    long long z = s % nz, y = (s / nz) % ny, x = s / ((long long)nz * ny);
+    if (a == 0) x = (x + delta + nx) % nx;
+    else if (a == 1) y = (y + delta + ny) % ny;
+    else z = (z + delta + nz) % nz;
+    return (x * ny + y) * nz + z;
+}
+
+template <class V4>
+__device__ __forceinline__ void load_d(const V4& v, double* o) {
+    o[0] = (double)v.x; o[1] = (double)v.y; o[2] = (double)v.z; o[3] = (double)v.w;
+}
+
+// 1 - sqrt(1 - x) without cancellation.
+__device__ __forceinline__ double one_minus_sqrt1m(double x) { return x / (1.0 + sqrt(1.0 - x)); }
+
+// Energy densities of the current state (potential, gradient, kinetic, magnetic,
+// electric) with the reference's definitions (Analysers/Source/Energy.cpp:126-132;
+// kinetic and electric from the last update, as the reference's two slots give).
+template <class R, int ND>
+__global__ void __launch_bounds__(256) mom_energy(ObsArgs<R> a, Coef c, double* __restrict__ partial) {
+    __shared__ double lds[4 * 5];
+    const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double d[5] = {0, 0, 0, 0, 0};
+    if (s < c.nsites) {
+        double phi[8];
+        for (int k = 0; k < ND; k++) load_d(a.phi[k][s], phi + 4 * k);
+        d[0] = potential_energy(c, phi);
+        double ge = 0.0;
+        Qt<double> U[3];
+        double yv[3];
+        for (int i = 0; i < 3; i++) {
+            double t[4];
+            load_d(a.q[i][s], t);
+            U[i] = Qt<double>{t[0], t[1], t[2], t[3]};
+            yv[i] = (double)a.y[i][s];
+        }
+        for (int i = 0; i < 3; i++) {
+            const long long sp = nb_index(a.nx, a.ny, a.nz, s, i, 1);
+            for (int k = 0; k < ND; k++) {
+                double fp[4], tx[4];
+                load_d(a.phi[k][sp], fp);
+                transport(U[i], cos(yv[i]), sin(yv[i]), false, phi + 4 * k, tx);
+                for (int m = 0; m < 4; m++) {
+                    const double dd = (fp[m] - tx[m]) * c.est[i];
+                    ge += dd * dd;
+                }
+            }
+        }
+        d[1] = 0.5 * ge;
+        double ke = 0.0;
+        for (int k = 0; k < ND; k++) {
+            double p[4];
+            load_d(a.P[k][s], p);
+            for (int m = 0; m < 4; m++) ke += p[m] * p[m];
+        }
+        d[2] = 0.5 * ke * c.inv_sqr_dt;
+        double me = 0.0;
+        for (int i = 0; i < 3; i++) {
+            for (int j = i + 1; j < 3; j++) {
+                const long long spi = nb_index(a.nx, a.ny, a.nz, s, i, 1);
+                const long long spj = nb_index(a.nx, a.ny, a.nz, s, j, 1);
+                double t1[4], t2[4];
+                load_d(a.q[j][spi], t1);
+                load_d(a.q[i][spj], t2);
+                const Qt<double> A = qmul(qconj(Qt<double>{t1[0], t1[1], t1[2], t1[3]}),
+                                          Qt<double>{t2[0], t2[1], t2[2], t2[3]});
+                const Qt<double> P1 = qmul(A, qmul(U[j], qconj(U[i])));
+                const double n = sqrt(P1.w * P1.w + P1.x * P1.x + P1.y * P1.y + P1.z * P1.z);
+                const double v2 = P1.x * P1.x + P1.y * P1.y + P1.z * P1.z;
+                // 1 - P0/|P| = (|P| - P0)/|P| = |v|^2 / (|P| (|P| + P0)) for P0 > 0.
+                const double one_m_p0 = P1.w > 0 ? v2 / (n * (n + P1.w)) : 1.0 - P1.w / n;
+                const double th = -(double)a.y[j][spi] + (double)a.y[i][spj] + yv[j] - yv[i];
+                const double sh = sin(0.5 * th);
+                // Both orientations (i,j) and (j,i) of the plane contribute equally.
+                me += 2.0 * 2.0 * c.inv_sp[i] * c.inv_sp[j] * (c.inv_g_sqr * one_m_p0 + c.inv_gp_sqr * 2.0 * sh * sh);
+            }
+        }
+        d[3] = me;
+        double ee = 0.0;
+        for (int i = 0; i < 3; i++) {
+            double e[4];
+            load_d(a.E[i][s], e);
+            const double ea2 = e[1] * e[1] + e[2] * e[2] + e[3] * e[3];
+            ee += 4.0 * c.inv_sp[i] * (c.inv_g_sqr * one_minus_sqrt1m(ea2) + c.inv_gp_sqr * one_minus_sqrt1m(e[0] * e[0]));
+        }
+        d[4] = c.inv_sqr_dt * ee;
+    }
+    block_reduce_store<5, 0>(d, nullptr, lds, partial, blockIdx.x);
+}
+
+// Gauss law after a step, from the stored E (the momentum form's electric
+// field): G^Y = sum_i [E^Y_i(x) - E^Y_i(x-i)]/dx_i^2 - g'^2 rho^Y,
+// G^a = sum_i [vec(U_i(x,t)^dag E_i(x) U_i(x,t)) - E_i(x-i)]/dx_i^2 - g^2 rho^a,
+// rho = 1/4 bilinears(phi(t), phi(t+dt)).  This equals the reference's
+// link-difference form (WilsonLoops/Source/StandardModel.cpp:274-322) in exact
+// arithmetic, since U(t+dt) U(t)^dag = (sqrt(1-|E|^2), E).
+template <class R, int ND>
+__global__ void __launch_bounds__(256) mom_gauss(ObsArgs<R> a, Coef c, double* __restrict__ partial) {
+    __shared__ double lds[4 * 12];
+    const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double sums[8] = {0, 0, 0, 0, 0, 0, 0, 0}, maxs[4] = {0, 0, 0, 0};
+    if (s < c.nsites) {
+        double rho[4] = {0, 0, 0, 0};
+        for (int k = 0; k < ND; k++) {
+            double t[4], n[4];
+            load_d(a.phi_old[k][s], t);
+            load_d(a.phi[k][s], n);
+            bilinears_add(t, n, rho);
+        }
+        double div[4] = {0, 0, 0, 0};
+        for (int i = 0; i < 3; i++) {
+            const long long sm = nb_index(a.nx, a.ny, a.nz, s, i, -1);
+            double e[4], em[4], u[4];
+            load_d(a.E[i][s], e);
+            load_d(a.E[i][sm], em);
+            load_d(a.q_old[i][s], u);
+            const Qt<double> Ut{u[0], u[1], u[2], u[3]};
+            const Qt<double> Ev{0.0, e[1], e[2], e[3]};
+            const Qt<double> rot = qmul(qconj(Ut), qmul(Ev, Ut));
+            div[0] += (e[0] - em[0]) * c.inv_sp[i];
+            div[1] += (rot.x - em[1]) * c.inv_sp[i];
+            div[2] += (rot.y - em[2]) * c.inv_sp[i];
+            div[3] += (rot.z - em[3]) * c.inv_sp[i];
+        }
+        for (int k = 0; k < 4; k++) {
+            const double g = div[k] - c.gsq[k] * rho[k];
+            sums[k] = fabs(g);
+            sums[4 + k] = fabs(div[k]) + fabs(c.gsq[k] * rho[k]);
+            maxs[k] = fabs(g);
+        }
+    }
+    block_reduce_store<8, 4>(sums, maxs, lds, partial, blockIdx.x);
+}
+
+// Wilson loops: mean 1/2 tr and mean cos of the 1x1 and 2x2 loops over the three
+// planes at every site, and <|phi|^2>.  The SU(2) trace is the c0 of the
+// product of the loop's links; the U(1) loop is the cosine of the summed phases.
+template <class R, int ND>
+__global__ void __launch_bounds__(256) mom_loops(ObsArgs<R> a, Coef c, double* __restrict__ partial) {
+    __shared__ double lds[4 * 5];
+    const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double d[5] = {0, 0, 0, 0, 0};
+    if (s < c.nsites) {
+        for (int k = 0; k < ND; k++) {
+            double f[4];
+            load_d(a.phi[k][s], f);
+            d[4] += f[0] * f[0] + f[1] * f[1] + f[2] * f[2] + f[3] * f[3];
+        }
+        for (int i = 0; i < 3; i++) {
+            for (int j = i + 1; j < 3; j++) {
+                for (int L = 1; L <= 2; L++) {
+                    // Loop x -> x+L i -> x+L i+L j -> x+L j -> x as a product of
+                    // transporters (right to left): U_i's along i, then U_j's, then back.
+                    Qt<double> W{1.0, 0.0, 0.0, 0.0};
+                    double ph = 0.0;
+                    long long p = s;
+                    for (int st = 0; st < L; st++) {
+                        double t[4];
+                        load_d(a.q[i][p], t);
+                        W = qmul(Qt<double>{t[0], t[1], t[2], t[3]}, W);
+                        ph += (double)a.y[i][p];
+                        p = nb_index(a.nx, a.ny, a.nz, p, i, 1);
+                    }
+                    for (int st = 0; st < L; st++) {
+                        double t[4];
+                        load_d(a.q[j][p], t);
+                        W = qmul(Qt<double>{t[0], t[1], t[2], t[3]}, W);
+                        ph += (double)a.y[j][p];
+                        p = nb_index(a.nx, a.ny, a.nz, p, j, 1);
+                    }
+                    for (int st = 0; st < L; st++) {
+                        p = nb_index(a.nx, a.ny, a.nz, p, i, -1);
+                        double t[4];
+                        load_d(a.q[i][p], t);
+                        W = qmul(qconj(Qt<double>{t[0], t[1], t[2], t[3]}), W);
+                        ph -= (double)a.y[i][p];
+                    }
+                    for (int st = 0; st < L; st++) {
+                        p = nb_index(a.nx, a.ny, a.nz, p, j, -1);
+                        double t[4];
+                        load_d(a.q[j][p], t);
+                        W = qmul(qconj(Qt<double>{t[0], t[1], t[2], t[3]}), W);
+                        ph -= (double)a.y[j][p];
+                    }
+                    const double n = sqrt(W.w * W.w + W.x * W.x + W.y * W.y + W.z * W.z);
+                    d[(L - 1) * 2] += W.w / n;
+                    d[(L - 1) * 2 + 1] += cos(ph);
+                }
+            }
+        }
+    }
+    block_reduce_store<5, 0>(d, nullptr, lds, partial, blockIdx.x);
+}
+
+// ---- host side ----------------------------------------------------------------
+
+template <class R>
+inline size_t carve(MomFields<R>* f, char* base, long long N) {
+    using V4 = typename V4T<R>::type;
+    size_t off = 0;
+    auto take = [&](size_t elems, size_t esz) {
+        const size_t at = off;
+        off += (elems * esz + 255) & ~size_t(255);
+        return base ? base + at : nullptr;
+    };
+    for (int b = 0; b < 2; b++)
+        for (int k = 0; k < 2; k++) { char* p = take(N, sizeof(V4)); if (f) f->phi[b][k] = reinterpret_cast<V4*>(p); }
+    for (int k = 0; k < 2; k++) { char* p = take(N, sizeof(V4)); if (f) f->P[k] = reinterpret_cast<V4*>(p); }
+    for (int b = 0; b < 2; b++)
+        for (int i = 0; i < 3; i++) { char* p = take(N, sizeof(R)); if (f) f->y[b][i] = reinterpret_cast<R*>(p); }
+    for (int b = 0; b < 2; b++)
+        for (int i = 0; i < 3; i++) { char* p = take(N, sizeof(V4)); if (f) f->q[b][i] = reinterpret_cast<V4*>(p); }
+    for (int i = 0; i < 3; i++) { char* p = take(N, sizeof(V4)); if (f) f->E[i] = reinterpret_cast<V4*>(p); }
+    return off;
+}
+
+inline int momentum_alloc(MomentumState& m, const Coef& c, bool f64) {
+    m.f64 = f64;
+    const long long N = c.nsites;
+    // The second-doublet arrays are carved for the SM too (unused there).
+    m.bytes = f64 ? carve<double>(nullptr, nullptr, N) : carve<float>(nullptr, nullptr, N);
+    if (hipMalloc(&m.base, m.bytes) != hipSuccess) { m.base = nullptr; return GFT_ENOMEM; }
+    if (f64) carve<double>(&m.fd, static_cast<char*>(m.base), N);
+    else carve<float>(&m.ff, static_cast<char*>(m.base), N);
+    m.cur = 0;
+    m.prev_valid = 0;
+    return GFT_OK;
+}
+
 inline void momentum_free(MomentumState& m) {
-    if (m.base) hipFree(m.base);
+    if (m.base) (void)hipFree(m.base);
     m.base = nullptr;
 }
+
+inline long long mom_nblocks(const MomentumState& m, const Coef& c) {
+    const long long nbz = (c.nz + m.bz - 1) / m.bz, nby = (c.ny + m.by - 1) / m.by;
+    return nbz * nby * c.nx;
+}
+
+inline double wrap_host(double y) {
+    const double twopi = 6.283185307179586476925286766559;
+    y = std::fmod(y, twopi);
+    if (y >= 3.1415926535897932384626433832795) y -= twopi;
+    if (y < -3.1415926535897932384626433832795) y += twopi;
+    return y;
+}
+
+// Two-slot (reference layout) -> momentum form, on the host in FP64.
+template <class R, class Src>
+int momentum_import_t(MomentumState& m, MomFields<R>& f, const Coef& c, const Src* S, const Src* V,
+                      unsigned time_iter, hipStream_t st, std::string& err) {
+    const long long N = c.nsites;
+    const int Ns = c.ns, nd = c.nd;
+    const unsigned now = (time_iter + 1) % 2, past = !now;
+    std::vector<R> phi(4 * N * 2, R(0)), P(4 * N * 2, R(0)), y(3 * N), q(12 * N), E(12 * N);
+    for (long long s = 0; s < N; s++) {
+        for (int k = 0; k < nd; k++)
+            for (int m4 = 0; m4 < 4; m4++) {
+                const double a = (double)S[(now * N + s) * Ns + 4 * k + m4];
+                const double b = (double)S[(past * N + s) * Ns + 4 * k + m4];
+                phi[(k * N + s) * 4 + m4] = (R)a;
+                P[(k * N + s) * 4 + m4] = (R)(a - b);
+            }
+        for (int i = 0; i < 3; i++) {
+            const Src* ln = V + (now * N + s) * 15 + 5 * i;
+            const Src* lp = V + (past * N + s) * 15 + 5 * i;
+            y[i * N + s] = (R)wrap_host((double)ln[0]);
+            const double un[4] = {(double)ln[1], (double)ln[2], (double)ln[3], (double)ln[4]};
+            const double up[4] = {(double)lp[1], -(double)lp[2], -(double)lp[3], -(double)lp[4]};
+            for (int k = 0; k < 4; k++) q[(i * N + s) * 4 + k] = (R)un[k];
+            double r[4];
+            qmul_full(un, up, r);
+            E[(i * N + s) * 4 + 0] = (R)std::sin((double)ln[0] - (double)lp[0]);
+            E[(i * N + s) * 4 + 1] = (R)r[1];
+            E[(i * N + s) * 4 + 2] = (R)r[2];
+            E[(i * N + s) * 4 + 3] = (R)r[3];
+        }
+    }
+    m.cur = 0;
+    m.prev_valid = 0;
+    auto cp = [&](void* dst, const void* src, size_t bytes) {
+        return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st) == hipSuccess;
+    };
+    bool ok = true;
+    for (int k = 0; k < nd; k++) {
+        ok &= cp(f.phi[0][k], &phi[k * N * 4], N * 4 * sizeof(R));
+        ok &= cp(f.P[k], &P[k * N * 4], N * 4 * sizeof(R));
+    }
+    for (int i = 0; i < 3; i++) {
+        ok &= cp(f.y[0][i], &y[i * N], N * sizeof(R));
+        ok &= cp(f.q[0][i], &q[i * N * 4], N * 4 * sizeof(R));
+        ok &= cp(f.E[i], &E[i * N * 4], N * 4 * sizeof(R));
+    }
+    ok &= hipStreamSynchronize(st) == hipSuccess;
+    if (!ok) { err = "momentum import: copy failed"; return GFT_EHIP; }
+    return GFT_OK;
+}
+
+// Momentum form -> two-slot: the current slot gets (phi, y, q); the other slot
+// gets phi - P, y - asin(E^Y), (sqrt(1-|E|^2), E)^dag q.
+template <class R, class Dst>
+int momentum_export_t(MomentumState& m, MomFields<R>& f, const Coef& c, Dst* S, Dst* V, unsigned time_iter,
+                      hipStream_t st, std::string& err) {
+    const long long N = c.nsites;
+    const int Ns = c.ns, nd = c.nd;
+    const unsigned now = (time_iter + 1) % 2, past = !now;
+    std::vector<R> phi(4 * N * 2), P(4 * N * 2), y(3 * N), q(12 * N), E(12 * N);
+    auto cp = [&](void* dst, const void* src, size_t bytes) {
+        return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st) == hipSuccess;
+    };
+    bool ok = true;
+    for (int k = 0; k < nd; k++) {
+        ok &= cp(&phi[k * N * 4], f.phi[m.cur][k], N * 4 * sizeof(R));
+        ok &= cp(&P[k * N * 4], f.P[k], N * 4 * sizeof(R));
+    }
+    for (int i = 0; i < 3; i++) {
+        ok &= cp(&y[i * N], f.y[m.cur][i], N * sizeof(R));
+        ok &= cp(&q[i * N * 4], f.q[m.cur][i], N * 4 * sizeof(R));
+        ok &= cp(&E[i * N * 4], f.E[i], N * 4 * sizeof(R));
+    }
+    ok &= hipStreamSynchronize(st) == hipSuccess;
+    if (!ok) { err = "momentum export: copy failed"; return GFT_EHIP; }
+    for (long long s = 0; s < N; s++) {
+        for (int k = 0; k < nd; k++)
+            for (int m4 = 0; m4 < 4; m4++) {
+                const double a = (double)phi[(k * N + s) * 4 + m4];
+                S[(now * N + s) * Ns + 4 * k + m4] = (Dst)a;
+                S[(past * N + s) * Ns + 4 * k + m4] = (Dst)(a - (double)P[(k * N + s) * 4 + m4]);
+            }
+        for (int i = 0; i < 3; i++) {
+            Dst* ln = V + (now * N + s) * 15 + 5 * i;
+            Dst* lp = V + (past * N + s) * 15 + 5 * i;
+            const double yy = (double)y[i * N + s];
+            const double e[4] = {(double)E[(i * N + s) * 4], (double)E[(i * N + s) * 4 + 1],
+                                 (double)E[(i * N + s) * 4 + 2], (double)E[(i * N + s) * 4 + 3]};
+            const double un[4] = {(double)q[(i * N + s) * 4], (double)q[(i * N + s) * 4 + 1],
+                                  (double)q[(i * N + s) * 4 + 2], (double)q[(i * N + s) * 4 + 3]};
+            const double ud[4] = {std::sqrt(std::max(0.0, 1.0 - e[1] * e[1] - e[2] * e[2] - e[3] * e[3])), -e[1], -e[2],
+                                  -e[3]};
+            double up[4];
+            qmul_full(ud, un, up);
+            ln[0] = (Dst)yy;
+            lp[0] = (Dst)(yy - std::asin(e[0]));
+            for (int k = 0; k < 4; k++) {
+                ln[1 + k] = (Dst)un[k];
+                lp[1 + k] = (Dst)up[k];
+            }
+        }
+    }
+    return GFT_OK;
+}
+
 template <class Src>
-inline int momentum_import(MomentumState&, const Coef&, const Src*, const Src*, unsigned, hipStream_t, std::string& e) {
-    e = "momentum form not available";
-    return GFT_EINVAL;
+inline int momentum_import(MomentumState& m, const Coef& c, const Src* S, const Src* V, unsigned it, hipStream_t st,
+                           std::string& err) {
+    return m.f64 ? momentum_import_t<double>(m, m.fd, c, S, V, it, st, err)
+                 : momentum_import_t<float>(m, m.ff, c, S, V, it, st, err);
 }
+
 template <class Dst>
-inline int momentum_export(MomentumState&, const Coef&, Dst*, Dst*, unsigned, hipStream_t, std::string& e) {
-    e = "momentum form not available";
-    return GFT_EINVAL;
+inline int momentum_export(MomentumState& m, const Coef& c, Dst* S, Dst* V, unsigned it, hipStream_t st,
+                           std::string& err) {
+    return m.f64 ? momentum_export_t<double>(m, m.fd, c, S, V, it, st, err)
+                 : momentum_export_t<float>(m, m.ff, c, S, V, it, st, err);
 }
-inline int momentum_step(MomentumState&, const Coef&, double, hipStream_t, unsigned*, std::string& e) {
-    e = "momentum form not available";
-    return GFT_EINVAL;
+
+template <class R>
+StepArgs<R> make_step_args(MomentumState& m, MomFields<R>& f, const Coef& c, double damping_step, unsigned* err) {
+    StepArgs<R> a{};
+    const int cur = m.cur, nxt = !m.cur;
+    for (int k = 0; k < 2; k++) {
+        a.phi[k] = f.phi[cur][k];
+        a.phi_out[k] = f.phi[nxt][k];
+        a.P[k] = f.P[k];
+    }
+    for (int i = 0; i < 3; i++) {
+        a.y[i] = f.y[cur][i];
+        a.y_out[i] = f.y[nxt][i];
+        a.q[i] = f.q[cur][i];
+        a.q_out[i] = f.q[nxt][i];
+        a.E[i] = f.E[i];
+        a.inv_sp[i] = (R)c.inv_sp[i];
+    }
+    a.nx = c.nx; a.ny = c.ny; a.nz = c.nz;
+    a.bz = m.bz; a.by = m.by;
+    a.nbz = (c.nz + m.bz - 1) / m.bz;
+    a.nby = (c.ny + m.by - 1) / m.by;
+    a.nblocks = (long long)a.nbz * a.nby * c.nx;
+    a.xcd_remap = m.xcd_remap;
+    a.dt2 = (R)(c.dt * c.dt);
+    a.keep = (R)(1.0 - damping_step);
+    for (int k = 0; k < 4; k++) a.gsq[k] = (R)c.gsq[k];
+    a.err = err;
+    return a;
 }
-inline int momentum_gauss(MomentumState&, const Coef&, double*, long long, hipStream_t, std::string& e) {
-    e = "momentum form not available";
-    return GFT_EINVAL;
+
+inline int momentum_step(MomentumState& m, const Coef& c, double damping_step, hipStream_t st, unsigned* err,
+                         std::string& msg) {
+    if (m.bz * m.by != 256 || m.bz < 1 || m.by < 1) { msg = "momentum step: tile must hold 256 sites"; return GFT_EINVAL; }
+    const long long nb = mom_nblocks(m, c);
+    if (m.f64) {
+        auto a = make_step_args<double>(m, m.fd, c, damping_step, err);
+        if (c.nd == 1) mom_step<double, 1><<<nb, 256, 0, st>>>(a, c);
+        else mom_step<double, 2><<<nb, 256, 0, st>>>(a, c);
+    } else {
+        auto a = make_step_args<float>(m, m.ff, c, damping_step, err);
+        if (c.nd == 1) mom_step<float, 1><<<nb, 256, 0, st>>>(a, c);
+        else mom_step<float, 2><<<nb, 256, 0, st>>>(a, c);
+    }
+    if (hipGetLastError() != hipSuccess) { msg = "momentum step launch failed"; return GFT_EHIP; }
+    m.cur = !m.cur;
+    m.prev_valid = 1;
+    return GFT_OK;
+}
+
+template <class R>
+ObsArgs<R> make_obs_args(const MomentumState& m, const MomFields<R>& f, const Coef& c) {
+    ObsArgs<R> a{};
+    const int cur = m.cur, old = !m.cur;
+    for (int k = 0; k < 2; k++) {
+        a.phi[k] = f.phi[cur][k];
+        a.phi_old[k] = f.phi[old][k];
+        a.P[k] = f.P[k];
+    }
+    for (int i = 0; i < 3; i++) {
+        a.y[i] = f.y[cur][i];
+        a.q[i] = f.q[cur][i];
+        a.q_old[i] = f.q[old][i];
+        a.E[i] = f.E[i];
+    }
+    a.nx = c.nx; a.ny = c.ny; a.nz = c.nz;
+    return a;
+}
+
+enum class MomObs { Energy, Gauss, Loops };
+
+inline int momentum_observe(const MomentumState& m, const Coef& c, MomObs what, double* partial, long long nblocks,
+                            hipStream_t st, std::string& msg) {
+    if (what == MomObs::Gauss && !m.prev_valid) {
+        msg = "Gauss law needs the state of the previous step: run at least one step after import";
+        return GFT_EINVAL;
+    }
+#define GFT_MOM_LAUNCH(KERNEL)                                                                           \
+    if (m.f64) {                                                                                         \
+        auto a = make_obs_args<double>(m, m.fd, c);                                                      \
+        if (c.nd == 1) KERNEL<double, 1><<<nblocks, 256, 0, st>>>(a, c, partial);                        \
+        else KERNEL<double, 2><<<nblocks, 256, 0, st>>>(a, c, partial);                                  \
+    } else {                                                                                             \
+        auto a = make_obs_args<float>(m, m.ff, c);                                                       \
+        if (c.nd == 1) KERNEL<float, 1><<<nblocks, 256, 0, st>>>(a, c, partial);                         \
+        else KERNEL<float, 2><<<nblocks, 256, 0, st>>>(a, c, partial);                                   \
+    }
+    if (what == MomObs::Energy) { GFT_MOM_LAUNCH(mom_energy) }
+    else if (what == MomObs::Gauss) { GFT_MOM_LAUNCH(mom_gauss) }
+    else { GFT_MOM_LAUNCH(mom_loops) }
+#undef GFT_MOM_LAUNCH
+    if (hipGetLastError() != hipSuccess) { msg = "momentum observable launch failed"; return GFT_EHIP; }
+    return GFT_OK;
 }
 
 }  // namespace gft

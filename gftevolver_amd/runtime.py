@@ -36,6 +36,7 @@ _L.gft_evolve.argtypes = [C.c_void_p, C.c_uint, C.c_uint, C.c_uint, _P(GftObs), 
 _L.gft_observe.argtypes = [C.c_void_p, _P(GftObs)]
 _L.gft_wilson_loops.argtypes = [C.c_void_p, _P(GftLoops)]
 _L.gft_sync.argtypes = [C.c_void_p]
+_L.gft_set_launch.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
 _L.gft_last_step_ms.argtypes = [C.c_void_p, _P(C.c_double), _P(C.c_double), _P(C.c_uint)]
 
 
@@ -129,6 +130,9 @@ class Evolver:
         o = GftLoops()
         self._check(_L.gft_wilson_loops(self._h, C.byref(o)))
         return o
+
+    def set_launch(self, bz: int, by: int, xcd_remap: bool = True):
+        self._check(_L.gft_set_launch(self._h, bz, by, int(xcd_remap)))
 
     def sync(self):
         self._check(_L.gft_sync(self._h))

@@ -130,6 +130,13 @@ typedef struct gft_loops {
 } gft_loops;
 int gft_wilson_loops(gft_ctx* ctx, gft_loops* out);
 
+/* Launch shape of the momentum-form step kernel: tiles of bz x by sites
+ * (bz along z, by along y, bz*by = 256) and whether blocks are dealt to the
+ * XCDs in contiguous ranges.  Every shape gives bitwise the same fields; the
+ * autotuner (gftevolver_amd/tune.py) picks the fastest.  No reference
+ * counterpart (the reference has no launch configuration). */
+int gft_set_launch(gft_ctx* ctx, int bz, int by, int xcd_remap);
+
 /* Wait for all queued work on the context's stream. */
 int gft_sync(gft_ctx* ctx);
 

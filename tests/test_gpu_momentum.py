@@ -1,0 +1,178 @@
+"""GPU parity of the momentum-form (structure-of-arrays kick/drift) evolver.
+
+The momentum form reorders the reference's arithmetic (stored E instead of
+vec(U(t) U(t-dt)^dag), staples shared between orientations, no intermediate
+renormalisations), so parity is to a tolerance, not bitwise:
+  * FP64 storage vs the FP64 CPU oracle: <= 1e-11 relative at field level after
+    100 steps of the 32^3 SM workload (reordering changes the last bits only);
+  * FP32 storage vs the FP64 CPU oracle: <= 2e-5 relative at field level after
+    100 steps (float rounding of ~1e-7 per step, accumulated);
+  * energies: <= 1e-9 (FP64) / 1e-5 (FP32) relative to the total energy;
+  * Gauss law: conserved to <= 1e-12 (FP64) relative from a constraint-
+    satisfying state; the FP32 value is recorded and bounded at 1e-5.
+Every launch shape must give bitwise identical fields (the autotuner relies on it).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from gftevolver_amd._abi import (GFT_FORM_MOMENTUM_F32, GFT_FORM_MOMENTUM_F64, GFT_FORM_TWO_SLOT_F64,
+                                 GFT_MODEL_2HDM, GFT_MODEL_SM, TWO_HDM_POTENTIAL, make_desc)
+from tests import lattice_np as L
+from tests import oracle_lib as O
+from tests.test_gpu_twoslot import gaussian_ic
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def rt():
+    from gftevolver_amd import runtime
+    return runtime
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def _phase_diff(va, vb):
+    """Link arrays compared with phases modulo 2 pi (the momentum form wraps them)."""
+    d = (va - vb).reshape(va.shape[:-1] + (3, 5)).copy()
+    d[..., 0] = (d[..., 0] + np.pi) % (2 * np.pi) - np.pi
+    return d
+
+
+@pytest.mark.parametrize("form,tol", [(GFT_FORM_MOMENTUM_F64, 1e-11), (GFT_FORM_MOMENTUM_F32, 2e-5)],
+                         ids=["f64", "f32"])
+def test_momentum_matches_oracle_32cube_100_steps(rt, form, tol):
+    n = (32, 32, 32)
+    d = make_desc(form=form, n=n)
+    s, v = gaussian_ic(n)
+    with rt.Evolver(d) as ev:
+        ev.import_two_slot(s, v, 0)
+        rows = ev.evolve(100, energy_freq=20)
+        gs, gv, it = ev.export_two_slot(np.float64)
+    assert it == 100
+    _, orows, _ = O.evolve(d, s, v, 100, energy_freq=20, nthreads=8)
+    cur = (100 + 1) % 2
+    rs = _rel(gs[cur], s[cur])
+    rv = np.abs(_phase_diff(gv[cur], v[cur])).max()
+    print(f"form {form}: field error vs FP64 oracle after 100 steps: scalar {rs:.2e}, links {rv:.2e}")
+    assert rs <= tol and rv <= tol
+    etol = 1e-9 if form == GFT_FORM_MOMENTUM_F64 else 1e-5
+    for a, b in zip(rows, orows):
+        assert a.time_iter == b.time_iter
+        err = np.abs(np.array(a.energy[:]) - np.array(b.energy[:])).max() / abs(b.energy[0])
+        print(f"  step {a.time_iter}: energy {a.energy[0]:.9e} vs {b.energy[0]:.9e} (rel col err {err:.1e})")
+        assert err <= etol
+
+
+@pytest.mark.parametrize("form", [GFT_FORM_MOMENTUM_F64, GFT_FORM_MOMENTUM_F32], ids=["f64", "f32"])
+def test_momentum_2hdm_matches_oracle(rt, form):
+    n = (8, 6, 10)
+    d = make_desc(model=GFT_MODEL_2HDM, form=form, n=n, pot2=TWO_HDM_POTENTIAL)
+    s, v = L.near_vacuum_state(n, 8, seed=9, amp=0.1, mom=0.01, link_eps=0.05, vev=0.7)
+    with rt.Evolver(d) as ev:
+        ev.import_two_slot(s, v, 0)
+        ev.evolve(40)
+        gs, gv, _ = ev.export_two_slot(np.float64)
+    O.evolve(d, s, v, 40, nthreads=8)
+    cur = (40 + 1) % 2
+    tol = 1e-11 if form == GFT_FORM_MOMENTUM_F64 else 2e-5
+    assert _rel(gs[cur], s[cur]) <= tol
+    assert np.abs(_phase_diff(gv[cur], v[cur])).max() <= tol
+
+
+@pytest.mark.parametrize("form,tol", [(GFT_FORM_MOMENTUM_F64, 1e-12), (GFT_FORM_MOMENTUM_F32, 1e-5)],
+                         ids=["f64", "f32"])
+def test_momentum_gauss_law_conserved(rt, form, tol):
+    """From E = 0, P = 0 the discrete Gauss law holds exactly; it must stay at
+    rounding level.  Ragged lattice, unequal spacings, non-trivial links."""
+    n = (12, 10, 14)
+    d = make_desc(form=form, n=n, d=(0.5, 0.45, 0.55))
+    s, v = L.near_vacuum_state(n, 4, seed=4, amp=0.2, mom=0.0, link_eps=0.2)
+    s[0], v[0] = s[1], v[1]
+    with rt.Evolver(d) as ev:
+        ev.import_two_slot(s, v, 0)
+        rows = ev.evolve(200, gauss_freq=50)
+    rel = [max(r.gauss_int[m] / r.gauss_scale[m] for m in range(4)) for r in rows]
+    print(f"form {form}: relative Gauss violation every 50 steps: {['%.2e' % x for x in rel]}")
+    assert max(rel) <= tol
+
+
+def test_momentum_launch_shapes_bitwise_identical(rt):
+    n = (16, 12, 20)
+    d = make_desc(form=GFT_FORM_MOMENTUM_F32, n=n)
+    s, v = L.near_vacuum_state(n, 4, seed=6, amp=0.1, mom=0.01, link_eps=0.05)
+    out = []
+    for bz, by, remap in ((64, 4, 1), (64, 4, 0), (256, 1, 1), (128, 2, 1), (32, 8, 1), (16, 16, 0)):
+        with rt.Evolver(d) as ev:
+            ev.set_launch(bz, by, remap)
+            ev.import_two_slot(s, v, 0)
+            ev.evolve(7)
+            out.append(ev.export_two_slot(np.float32))
+    for o in out[1:]:
+        assert np.array_equal(o[0], out[0][0]) and np.array_equal(o[1], out[0][1])
+
+
+def test_momentum_import_export_roundtrip(rt):
+    n = (6, 5, 4)
+    d = make_desc(form=GFT_FORM_MOMENTUM_F64, n=n)
+    s, v = L.near_vacuum_state(n, 4, seed=7, amp=0.2, mom=0.02, link_eps=0.2)
+    with rt.Evolver(d) as ev:
+        ev.import_two_slot(s, v, 5)
+        gs, gv, it = ev.export_two_slot(np.float64)
+    assert it == 5
+    assert np.abs(gs - s).max() < 1e-14
+    assert np.abs(_phase_diff(gv, v)).max() < 1e-13
+
+
+def test_momentum_energy_matches_two_slot_reference_definition(rt):
+    """gft_observe's energy equals the reference's mixed-time energy of the same
+    state (as measured by the FP64 oracle before a step)."""
+    n = (8, 8, 8)
+    d = make_desc(form=GFT_FORM_MOMENTUM_F64, n=n)
+    s, v = L.near_vacuum_state(n, 4, seed=8, amp=0.1, mom=0.01, link_eps=0.05)
+    with rt.Evolver(d) as ev:
+        ev.import_two_slot(s, v, 0)
+        o = ev.observe()
+    _, orows, _ = O.evolve(d, s.copy(), v.copy(), 1, energy_freq=1)
+    np.testing.assert_allclose(np.array(o.energy[:]), np.array(orows[0].energy[:]), rtol=1e-9,
+                               atol=1e-11 * abs(orows[0].energy[0]))
+
+
+def test_wilson_loops_identity_and_plaquette_energy_relation(rt):
+    """Identity links give W = 1; the mean 1x1 loops reproduce the magnetic
+    energy: E_mag = N dV sum_{i<j} 4/(dx_i^2 dx_j^2) [(1 - W_su2)/g^2 + (1 - W_u1)/g'^2]
+    for an isotropic lattice."""
+    n = (6, 6, 6)
+    d = make_desc(form=GFT_FORM_MOMENTUM_F64, n=n)
+    s, v = L.near_vacuum_state(n, 4, seed=8, amp=0.1, mom=0.0, link_eps=0.0)
+    v = O.identity_links(216, np.float64)
+    with rt.Evolver(d) as ev:
+        ev.import_two_slot(s, v, 0)
+        w = ev.wilson_loops()
+    assert abs(w.w11_su2 - 1) < 1e-14 and abs(w.w22_su2 - 1) < 1e-14 and abs(w.w11_u1 - 1) < 1e-14
+    assert abs(w.phi_sq - (s[1] ** 2).sum(axis=1).mean()) < 1e-12
+    s, v = L.near_vacuum_state(n, 4, seed=9, amp=0.1, mom=0.0, link_eps=0.3)
+    with rt.Evolver(d) as ev:
+        ev.import_two_slot(s, v, 0)
+        w = ev.wilson_loops()
+        o = ev.observe()
+    dx = d.dx
+    N = 216
+    emag = N * dx ** 3 * 3 * 4 / dx ** 4 * ((1 - w.w11_su2) / d.g ** 2 + (1 - w.w11_u1) / d.gp ** 2)
+    assert abs(emag - o.energy[4]) <= 1e-10 * abs(o.energy[4])
+    assert w.w22_su2 < w.w11_su2 < 1
+
+
+def test_momentum_domain_error_is_reported(rt):
+    n = (4, 4, 4)
+    d = make_desc(form=GFT_FORM_MOMENTUM_F32, n=n, dt=3.0)
+    s, v = L.near_vacuum_state(n, 4, seed=2, amp=0.5, mom=0.2, link_eps=0.5)
+    with rt.Evolver(d) as ev:
+        ev.import_two_slot(s, v, 0)
+        with pytest.raises(rt.GftError) as e:
+            ev.evolve(3)
+        assert e.value.code == -4
