@@ -39,12 +39,21 @@ def gauss_rel(row):
     return max(row.gauss_int[m] / row.gauss_scale[m] for m in range(4) if row.gauss_scale[m] > 0)
 
 
-def run_form(rt, desc, s, v, steps, warmup, launch=None):
+def run_form(rt, desc, s, v, steps, warmup, launch=None, tune=False, occupancy=False):
+    """Imports the state, optionally autotunes the launch shape (untimed), runs
+    `warmup` untimed steps, then times exactly `steps` steps."""
+    info = {}
     with rt.Evolver(desc) as ev:
-        if launch:
-            ev.set_launch(*launch)
         ev.import_two_slot(s, v, 0)
         e0 = ev.observe().energy[0] if desc.form >= 2 else None
+        if launch:
+            ev.set_launch(*launch)
+            info["launch"] = list(launch)
+        elif tune and desc.form >= 2:
+            from gftevolver_amd.tune import tune as autotune
+            best, timing = autotune(ev, fp32_sm=(desc.form == 2 and desc.model == 0), occupancy=occupancy)
+            info["launch"] = list(best)
+            info["tune_ms"] = {",".join(map(str, k)): round(v, 4) for k, v in sorted(timing.items(), key=lambda kv: kv[1])[:6]}
         ev.evolve(warmup)
         ev.sync()
         t0 = time.perf_counter()
@@ -54,7 +63,7 @@ def run_form(rt, desc, s, v, steps, warmup, launch=None):
         avg_ms, _, nk = ev.last_step_ms()
         rows = ev.evolve(1, gauss_freq=1)
         e1 = ev.observe().energy[0] if desc.form >= 2 else None
-    return {"wall_s": wall, "kernel_ms": avg_ms, "gauss_rel": gauss_rel(rows[0]), "e0": e0, "e1": e1}
+    return {"wall_s": wall, "kernel_ms": avg_ms, "gauss_rel": gauss_rel(rows[0]), "e0": e0, "e1": e1, **info}
 
 
 def cpu_baseline(nsteps=60):
@@ -64,12 +73,15 @@ def cpu_baseline(nsteps=60):
     exe = os.path.join(ROOT, "oracle", "_ref", "gft_reference")
     n = 32
     if os.path.exists(exe):
+        import tempfile
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import refcfg
-        refcfg.write_model(0, 0.65, 0.35, lam=0.13, eta_sqr=1.0)
-        refcfg.write_lattice((n, n, n), (0.5, 0.5, 0.5), nsteps, 0.05, 4, energy=False, gauss=False, progress=True,
-                             ic_range=(-0.1, 0.1))
-        r = subprocess.run([exe], capture_output=True, text=True, timeout=600)
+        with tempfile.TemporaryDirectory() as td:
+            refcfg.prepare(td)
+            refcfg.write_model(td, 0, 0.65, 0.35, lam=0.13, eta_sqr=1.0)
+            refcfg.write_lattice(td, (n, n, n), (0.5, 0.5, 0.5), nsteps, 0.05, 4, energy=False, gauss=False,
+                                 progress=True, ic_range=(-0.1, 0.1))
+            r = subprocess.run([exe], capture_output=True, text=True, timeout=600, cwd=td)
         m = re.findall(r"Evolution finished in ([0-9.eE+-]+) seconds", r.stdout)
         if r.returncode == 0 and m:
             sec = float(m[-1])
@@ -110,7 +122,12 @@ def main():
     ap.add_argument("--twoslot-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only-f32", action="store_true")
-    ap.add_argument("--launch", type=str, default="")
+    ap.add_argument("--launch", type=str, default="", help="bz,by,order,min_waves (skips the autotuner)")
+    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--occupancy-variants", action="store_true",
+                    help="let the tuner try the 3/4-wave kernels (last-bit different results)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first copy 1 GiB device-to-device (a kernel of known traffic for PMC scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs every workload on a single MI355X "
@@ -124,6 +141,13 @@ def main():
     from gftevolver_amd.ic import grf_state
 
     name, cus = rt.device_info()
+    copy_gbps = None
+    if args.calibrate_copy or not args.only_f32:
+        # A plain copy of 1 GiB: its achieved bandwidth is the practical HBM
+        # ceiling, and its PMC traffic calibrates FETCH_SIZE / WRITE_SIZE.
+        cms = rt.bench_copy(1 << 30, 5)
+        copy_gbps = 2 * (1 << 30) / (cms * 1e-3) / 1e9
+        log(f"copy kernel: 1 GiB in {cms:.3f} ms = {copy_gbps:.0f} GB/s (read + write)")
     n = (args.n,) * 3
     N = args.n ** 3
     log(f"device {name}, {cus} CUs; building GRF initial state {n}")
@@ -131,7 +155,9 @@ def main():
     launch = tuple(int(x) for x in args.launch.split(",")) if args.launch else None
 
     d32 = make_desc(form=GFT_FORM_MOMENTUM_F32, n=n)
-    r32 = run_form(rt, d32, s, v, args.steps, args.warmup, launch)
+    r32 = run_form(rt, d32, s, v, args.steps, args.warmup, launch, tune=not args.no_tune,
+                   occupancy=args.occupancy_variants)
+    log(f"FP32 launch {r32.get('launch')}; tuner {r32.get('tune_ms')}")
     value = N * args.steps / r32["wall_s"]
     achieved = BYTES_F32 * N / (r32["kernel_ms"] * 1e-3) / 1e9
     log(f"FP32 momentum: {r32['wall_s'] / args.steps * 1e3:.3f} ms/step wall, kernel {r32['kernel_ms']:.3f} ms, "
@@ -140,9 +166,10 @@ def main():
     extra = {}
     if not args.only_f32:
         d64 = make_desc(form=GFT_FORM_MOMENTUM_F64, n=n)
-        r64 = run_form(rt, d64, s, v, args.fp64_steps, 2)
+        r64 = run_form(rt, d64, s, v, args.fp64_steps, 2, tune=not args.no_tune)
         v64 = N * args.fp64_steps / r64["wall_s"]
-        log(f"FP64 momentum: kernel {r64['kernel_ms']:.3f} ms, {v64:.3e} site-updates/s")
+        log(f"FP64 momentum: kernel {r64['kernel_ms']:.3f} ms, {v64:.3e} site-updates/s, "
+            f"energy drift {(r64['e1'] - r64['e0']) / abs(r64['e0']):.2e}")
         dts = make_desc(form=GFT_FORM_TWO_SLOT_F64, n=n)
         rts = run_form(rt, dts, s, v, args.twoslot_steps, 2)
         vts = N * args.twoslot_steps / rts["wall_s"]
@@ -150,7 +177,8 @@ def main():
         extra = {
             "fp64_momentum": {"value": v64, "kernel_ms": r64["kernel_ms"],
                               "achieved_GBps": BYTES_F64 * N / (r64["kernel_ms"] * 1e-3) / 1e9,
-                              "gauss_rel": r64["gauss_rel"]},
+                              "gauss_rel": r64["gauss_rel"],
+                              "energy_drift_rel": (r64["e1"] - r64["e0"]) / abs(r64["e0"])},
             "fp64_two_slot": {"value": vts, "kernel_ms": rts["kernel_ms"], "gauss_rel": rts["gauss_rel"]},
             "speedup_fp32_vs_fp64_momentum": value / v64,
             "speedup_fp32_vs_fp64_two_slot": value / vts,
@@ -176,12 +204,12 @@ def main():
                         f"(P(k)~1/k, k<=pi/(2dx), rms 0.1v), links identity, E=0; dx=0.5 dt=0.05 g=0.65 "
                         f"g'=0.35 lambda=0.13 v=1",
             "lattice": list(n),
-            "launch": list(launch) if launch else [64, 4, 1],
+            "launch": r32.get("launch", [64, 4, 2, 3]),
         },
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_site_update": BYTES_F32,
-                     "kernel_ms": r32["kernel_ms"]},
+                     "kernel_ms": r32["kernel_ms"], "copy_kernel_GBps": copy_gbps},
         "cpu_baseline": cpu,
         "gauss_rel_violation": r32["gauss_rel"],
         "energy_drift_rel": (r32["e1"] - r32["e0"]) / abs(r32["e0"]),

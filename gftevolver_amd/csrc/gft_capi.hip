@@ -135,6 +135,8 @@ int gft_create(const gft_desc* desc, gft_ctx** out) {
         return fail(nullptr, GFT_EINVAL, "gft_create: invalid form " + std::to_string(d.form));
     if (d.nx < 2 || d.ny < 2 || d.nz < 2)
         return fail(nullptr, GFT_EINVAL, "LATTICE:: periodic lattices need at least 2 sites per direction");
+    if ((long long)d.nx * d.ny * d.nz > (1LL << 28))
+        return fail(nullptr, GFT_EINVAL, "gft_create: at most 2^28 sites per device (32-bit site offsets)");
     if (!(d.dx > 0 && d.dy > 0 && d.dz > 0 && d.dt > 0))
         return fail(nullptr, GFT_EINVAL, "LATTICE:: spacings and time step must be positive");
     int ndev = 0;
@@ -427,13 +429,17 @@ extern "C" int gft_evolve(gft_ctx* ctx, unsigned nsteps, unsigned energy_freq, u
     return GFT_OK;
 }
 
-extern "C" int gft_set_launch(gft_ctx* ctx, int bz, int by, int xcd_remap) {
+extern "C" int gft_set_launch(gft_ctx* ctx, int bz, int by, int order, int min_waves) {
     if (!ctx) return GFT_EINVAL;
+    if (min_waves != 0 && min_waves != 3 && min_waves != 4)
+        return fail(ctx, GFT_EINVAL, "gft_set_launch: min_waves must be 0, 3 or 4");
+    if (order < 0 || order > 2) return fail(ctx, GFT_EINVAL, "gft_set_launch: order must be 0, 1 or 2");
     if (bz < 1 || by < 1 || bz * by != 256 || (bz % 64 && bz != 32 && bz != 16))
         return fail(ctx, GFT_EINVAL, "gft_set_launch: need bz*by = 256 with bz a multiple of 64, or 32 or 16");
     ctx->mom.bz = bz;
     ctx->mom.by = by;
-    ctx->mom.xcd_remap = xcd_remap ? 1 : 0;
+    ctx->mom.order = order;
+    ctx->mom.min_waves = min_waves;
     return GFT_OK;
 }
 
@@ -484,4 +490,41 @@ extern "C" int gft_wilson_loops(gft_ctx* ctx, gft_loops* out) {
     out->w22_u1 = ctx->obs_host[27] / (3 * n);
     out->phi_sq = ctx->obs_host[28] / n;
     return GFT_OK;
+}
+
+// Streaming copy of n float4 elements (grid-stride); traffic = 32 n bytes.
+__global__ void __launch_bounds__(256) gft_copy_kernel(const float4* __restrict__ src, float4* __restrict__ dst,
+                                                       long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+extern "C" int gft_bench_copy(size_t bytes, int reps, double* avg_ms) {
+    if (bytes < 16 || reps < 1 || !avg_ms) return GFT_EINVAL;
+    const long long n = (long long)(bytes / 16);
+    float4 *a = nullptr, *b = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = GFT_OK;
+    if (hipMalloc(&a, n * 16) != hipSuccess || hipMalloc(&b, n * 16) != hipSuccess) rc = GFT_ENOMEM;
+    if (!rc && (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&e0) != hipSuccess
+                || hipEventCreate(&e1) != hipSuccess))
+        rc = GFT_EHIP;
+    if (!rc && hipMemsetAsync(a, 0, n * 16, st) != hipSuccess) rc = GFT_EHIP;
+    if (!rc) {
+        const int grid = 256 * 32;
+        gft_copy_kernel<<<grid, 256, 0, st>>>(a, b, n);
+        hipEventRecord(e0, st);
+        for (int r = 0; r < reps; r++) gft_copy_kernel<<<grid, 256, 0, st>>>(a, b, n);
+        hipEventRecord(e1, st);
+        float ms = 0.f;
+        if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = GFT_EHIP;
+        *avg_ms = ms / reps;
+    }
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    if (st) hipStreamDestroy(st);
+    if (a) hipFree(a);
+    if (b) hipFree(b);
+    return rc;
 }

@@ -62,6 +62,7 @@ GFT_HD void qmul_full(const R* a, const R* b, R* r) {
 // SU2Product(a, b, trace): full product normalised to unit length when trace,
 // else the vector part with r[0] = 0 (WilsonLoops/Source/StandardModel.cpp:160-191).
 GFT_HD void su2_product(const double* a, const double* b, bool trace, double* out) {
+#pragma clang fp contract(off)
     double r[4];
     r[1] = a[0] * b[1] + a[1] * b[0] - a[2] * b[3] + a[3] * b[2];
     r[2] = a[0] * b[2] + a[2] * b[0] - a[3] * b[1] + a[1] * b[3];

@@ -63,7 +63,8 @@ struct MomentumState {
     MomFields<float> ff{};
     MomFields<double> fd{};
     int bz = 64, by = 4;     // tile shape of the step kernel
-    int xcd_remap = 1;
+    int order = 2;           // block order (see decode_block)
+    int min_waves = 3;       // occupancy target of the FP32 SM step kernel (0 = unconstrained, 3, 4)
 };
 
 // ---- float special functions on the small ranges the evolution produces ----
@@ -146,6 +147,17 @@ __device__ __forceinline__ R wrap_pi(R y) {
     return y;
 }
 
+// Element idx of an array addressed by a 32-bit byte offset from its base
+// (scalar base + vector offset addressing).
+template <class T>
+__device__ __forceinline__ const T& at(const T* base, unsigned idx) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + idx * static_cast<unsigned>(sizeof(T)));
+}
+template <class T>
+__device__ __forceinline__ T* ptr_at(T* base, unsigned idx) {
+    return reinterpret_cast<T*>(reinterpret_cast<char*>(base) + idx * static_cast<unsigned>(sizeof(T)));
+}
+
 template <class R>
 struct Qt { R w, x, y, z; };   // quaternion (c0, c1, c2, c3)
 
@@ -210,56 +222,86 @@ struct StepArgs {
     V4* E[3];
     int nx, ny, nz, nbz, nby;
     long long nblocks;
-    int bz, by, xcd_remap;
+    int bz, by, order;
     R inv_sp[3];
     R dt2, keep;             // dt^2 and 1 - gamma dt / 2
     R gsq[4];
     unsigned* err;
 };
 
-// Maps hardware block b to a logical (bz, by, x) tile.
-__device__ __forceinline__ long long logical_block(long long b, long long nb, int remap) {
-    if (!remap) return b;
+// Block orders (the hardware sends block b to XCD b % 8):
+//   0  hardware order, x fastest inside (y, z) column tiles;
+//   1  as 0, with logical blocks dealt to the XCDs in contiguous ranges;
+//   2  contiguous ranges per XCD over y-bands: the y-tiles are cut into 8
+//      bands, and inside a band the order is x (slowest), then y-tile, then
+//      z-tile.  Each XCD then sweeps x-planes of its own band, so the
+//      neighbour planes x-1 and x+1 of a band (tens of rows) stay in its L2.
+__device__ __forceinline__ long long xcd_contiguous(long long b, long long nb) {
     const long long per = nb / 8, rem = nb % 8, xcd = b % 8, idx = b / 8;
     return xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
 }
 
-template <class R, int ND>
-__global__ void __launch_bounds__(256) mom_step(StepArgs<R> a, Coef c) {
+// Decodes the logical tile (x, y-tile, z-tile) of hardware block b.
+__device__ __forceinline__ void decode_block(long long b, long long nb, int order, int nx, int nby, int nbz, int* x,
+                                             int* ty, int* tz) {
+    const long long L = order == 0 ? b : xcd_contiguous(b, nb);
+    if (order <= 1) {
+        *x = static_cast<int>(L % nx);
+        const long long rest = L / nx;
+        *ty = static_cast<int>(rest % nby);
+        *tz = static_cast<int>(rest / nby);
+        return;
+    }
+    const int br = (nby + 7) / 8;                      // y-tiles per band
+    const int nbands = (nby + br - 1) / br;
+    const long long per_band = (long long)br * nx * nbz;
+    int band = static_cast<int>(L / per_band);
+    if (band > nbands - 1) band = nbands - 1;
+    const long long loc = L - band * per_band;
+    const int rows = min(br, nby - band * br);         // the last band may be shorter
+    *tz = static_cast<int>(loc % nbz);
+    const long long t = loc / nbz;
+    *ty = band * br + static_cast<int>(t % rows);
+    *x = static_cast<int>(t / rows);
+}
+
+// MINW: minimum waves per SIMD the register allocation must allow (0: the
+// compiler's choice).  Higher occupancy hides more memory latency but may spill.
+template <class R, int ND, int MINW>
+__global__ void __launch_bounds__(256, MINW > 0 ? MINW : 1) mom_step(StepArgs<R> a, Coef c) {
     using V4 = typename V4T<R>::type;
-    const long long L = logical_block(blockIdx.x, a.nblocks, a.xcd_remap);
-    const int x = static_cast<int>(L % a.nx);
-    const long long rest = L / a.nx;
-    const int tby = static_cast<int>(rest % a.nby);
-    const int tbz = static_cast<int>(rest / a.nby);
+    int x, tby, tbz;
+    decode_block(blockIdx.x, a.nblocks, a.order, a.nx, a.nby, a.nbz, &x, &tby, &tbz);
     const int z = tbz * a.bz + static_cast<int>(threadIdx.x) % a.bz;
     const int y = tby * a.by + static_cast<int>(threadIdx.x) / a.bz;
     if (z >= a.nz || y >= a.ny) return;
 
-    const long long sy = a.nz, sx = (long long)a.ny * a.nz;
-    const long long s = (long long)x * sx + (long long)y * sy + z;
+    // 32-bit site offsets (the host limits a lattice to 2^28 sites): addresses
+    // become a scalar base plus one 32-bit vector offset, with no 64-bit math.
+    const int sy = a.nz, sx = a.ny * a.nz;
+    const unsigned s = static_cast<unsigned>(x * sx + y * sy + z);
     // Offsets to the +e_a and -e_a neighbours with periodic wrap.
-    long long dp[3], dm[3];
+    int dp[3], dm[3];
     dp[0] = (x == a.nx - 1) ? -(a.nx - 1) * sx : sx;
     dm[0] = (x == 0) ? (a.nx - 1) * sx : -sx;
-    dp[1] = (y == a.ny - 1) ? -(long long)(a.ny - 1) * sy : sy;
-    dm[1] = (y == 0) ? (long long)(a.ny - 1) * sy : -sy;
-    dp[2] = (z == a.nz - 1) ? -(long long)(a.nz - 1) : 1;
-    dm[2] = (z == 0) ? (long long)(a.nz - 1) : -1;
+    dp[1] = (y == a.ny - 1) ? -(a.ny - 1) * sy : sy;
+    dm[1] = (y == 0) ? (a.ny - 1) * sy : -sy;
+    dp[2] = (z == a.nz - 1) ? -(a.nz - 1) : 1;
+    dm[2] = (z == 0) ? (a.nz - 1) : -1;
 
     // Own links and their phases.
     Qt<R> U[3];
     R cy[3], sy_[3], yv[3];
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-        U[i] = toq<V4, R>(a.q[i][s]);
-        yv[i] = a.y[i][s];
+        U[i] = toq<V4, R>(at(a.q[i], s));
+        yv[i] = at(a.y[i], s);
         sincos_fast(yv[i], &sy_[i], &cy[i]);
     }
 
     R phi[4 * ND];
 #pragma unroll
-    for (int k = 0; k < ND; k++) unpack(a.phi[k][s], phi + 4 * k);
+    for (int k = 0; k < ND; k++) unpack(at(a.phi[k], s), phi + 4 * k);
 
     // Scalar force and currents.
     R F[4 * ND];
@@ -269,16 +311,16 @@ __global__ void __launch_bounds__(256) mom_step(StepArgs<R> a, Coef c) {
     R J[3][4];
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-        const long long sp = s + dp[i], sm = s + dm[i];
-        const Qt<R> Um = toq<V4, R>(a.q[i][sm]);
+        const unsigned sp = s + dp[i], sm = s + dm[i];
+        const Qt<R> Um = toq<V4, R>(at(a.q[i], sm));
         R cm, smn;
-        sincos_fast(a.y[i][sm], &smn, &cm);
+        sincos_fast(at(a.y[i], sm), &smn, &cm);
         J[i][0] = J[i][1] = J[i][2] = J[i][3] = R(0);
 #pragma unroll
         for (int k = 0; k < ND; k++) {
             R fp[4], fm[4], tp[4], tm[4], tx[4];
-            unpack(a.phi[k][sp], fp);
-            unpack(a.phi[k][sm], fm);
+            unpack(at(a.phi[k], sp), fp);
+            unpack(at(a.phi[k], sm), fm);
             transport(U[i], cy[i], sy_[i], true, fp, tp);   // U_i(x)^dag phi(x+i)
             transport(Um, cm, smn, false, fm, tm);          // U_i(x-i) phi(x-i)
             transport(U[i], cy[i], sy_[i], false, phi + 4 * k, tx);  // U_i(x) phi(x)
@@ -299,9 +341,9 @@ __global__ void __launch_bounds__(256) mom_step(StepArgs<R> a, Coef c) {
     for (int i = 0; i < 3; i++) {
 #pragma unroll
         for (int j = i + 1; j < 3; j++) {
-            const Qt<R> Ujxpi = toq<V4, R>(a.q[j][s + dp[i]]);
-            const Qt<R> Uixpj = toq<V4, R>(a.q[i][s + dp[j]]);
-            const R th = -a.y[j][s + dp[i]] + a.y[i][s + dp[j]] + yv[j] - yv[i];
+            const Qt<R> Ujxpi = toq<V4, R>(at(a.q[j], s + dp[i]));
+            const Qt<R> Uixpj = toq<V4, R>(at(a.q[i], s + dp[j]));
+            const R th = -at(a.y[j], s + dp[i]) + at(a.y[i], s + dp[j]) + yv[j] - yv[i];
             const Qt<R> A = qmul(qconj(Ujxpi), Uixpj);
             const Qt<R> C = qmul(U[j], qconj(U[i]));
             R v1[3], v2[3];
@@ -320,16 +362,16 @@ __global__ void __launch_bounds__(256) mom_step(StepArgs<R> a, Coef c) {
     // Plaquettes based at x-j:  P2(i,j) = U_i(x) U_j(x-j) U_i(x-j)^dag U_j(x+i-j)^dag.
 #pragma unroll
     for (int j = 0; j < 3; j++) {
-        const long long smj = s + dm[j];
-        const Qt<R> Ujm = toq<V4, R>(a.q[j][smj]);
-        const R yjm = a.y[j][smj];
+        const unsigned smj = s + dm[j];
+        const Qt<R> Ujm = toq<V4, R>(at(a.q[j], smj));
+        const R yjm = at(a.y[j], smj);
 #pragma unroll
         for (int i = 0; i < 3; i++) {
             if (i == j) continue;
-            const long long sd = s + dp[i] + dm[j];
-            const Qt<R> Uim = toq<V4, R>(a.q[i][smj]);
-            const Qt<R> Ujd = toq<V4, R>(a.q[j][sd]);
-            const R th = -a.y[j][sd] - a.y[i][smj] + yjm + yv[i];
+            const unsigned sd = s + dp[i] + dm[j];
+            const Qt<R> Uim = toq<V4, R>(at(a.q[i], smj));
+            const Qt<R> Ujd = toq<V4, R>(at(a.q[j], sd));
+            const R th = -at(a.y[j], sd) - at(a.y[i], smj) + yjm + yv[i];
             const Qt<R> B = qmul(qmul(Ujm, qconj(Uim)), qconj(Ujd));
             R v[3];
             qvec(U[i], B, v);
@@ -343,7 +385,7 @@ __global__ void __launch_bounds__(256) mom_step(StepArgs<R> a, Coef c) {
 #pragma unroll
     for (int k = 0; k < ND; k++) {
         R p[4];
-        unpack(ld_nt(&a.P[k][s]), p);
+        unpack(ld_nt(ptr_at(a.P[k], s)), p);
         V4 pn, fn;
         pn.x = a.keep * p[0] + a.dt2 * F[4 * k + 0];
         pn.y = a.keep * p[1] + a.dt2 * F[4 * k + 1];
@@ -353,14 +395,14 @@ __global__ void __launch_bounds__(256) mom_step(StepArgs<R> a, Coef c) {
         fn.y = phi[4 * k + 1] + pn.y;
         fn.z = phi[4 * k + 2] + pn.z;
         fn.w = phi[4 * k + 3] + pn.w;
-        st_nt(pn, &a.P[k][s]);
-        st_nt(fn, &a.phi_out[k][s]);
+        st_nt(pn, ptr_at(a.P[k], s));
+        st_nt(fn, ptr_at(a.phi_out[k], s));
     }
     unsigned bad = 0;
 #pragma unroll
     for (int i = 0; i < 3; i++) {
         R e[4];
-        unpack(ld_nt(&a.E[i][s]), e);
+        unpack(ld_nt(ptr_at(a.E[i], s)), e);
         V4 en;
         en.x = a.keep * e[0] + a.dt2 * (M[i][0] + a.gsq[0] * J[i][0]);
         en.y = a.keep * e[1] + a.dt2 * (M[i][1] + a.gsq[1] * J[i][1]);
@@ -377,9 +419,9 @@ __global__ void __launch_bounds__(256) mom_step(StepArgs<R> a, Coef c) {
         qo.y = qn.x * inv;
         qo.z = qn.y * inv;
         qo.w = qn.z * inv;
-        st_nt(en, &a.E[i][s]);
-        st_nt(yn, &a.y_out[i][s]);
-        st_nt(qo, &a.q_out[i][s]);
+        st_nt(en, ptr_at(a.E[i], s));
+        st_nt(yn, ptr_at(a.y_out[i], s));
+        st_nt(qo, ptr_at(a.q_out[i], s));
     }
     if (bad) atomicOr(a.err, 1u);
 }
@@ -493,7 +535,7 @@ __global__ void __launch_bounds__(256) mom_energy(ObsArgs<R> a, Coef c, double* 
 // Gauss law after a step, from the stored E (the momentum form's electric
 // field): G^Y = sum_i [E^Y_i(x) - E^Y_i(x-i)]/dx_i^2 - g'^2 rho^Y,
 // G^a = sum_i [vec(U_i(x,t)^dag E_i(x) U_i(x,t)) - E_i(x-i)]/dx_i^2 - g^2 rho^a,
-// rho = 1/4 bilinears(phi(t), phi(t+dt)).  This equals the reference's
+// rho = 1/4 bilinears(phi(t), phi(t+dt)) evaluated as 1/4 bilinears(phi(t), P).  This equals the reference's
 // link-difference form (WilsonLoops/Source/StandardModel.cpp:274-322) in exact
 // arithmetic, since U(t+dt) U(t)^dag = (sqrt(1-|E|^2), E).
 template <class R, int ND>
@@ -502,14 +544,17 @@ __global__ void __launch_bounds__(256) mom_gauss(ObsArgs<R> a, Coef c, double* _
     const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     double sums[8] = {0, 0, 0, 0, 0, 0, 0, 0}, maxs[4] = {0, 0, 0, 0};
     if (s < c.nsites) {
+        // rho = 1/4 B(phi(t), phi(t+dt)) = 1/4 B(phi(t), P): the bilinears are
+        // antisymmetric, so B(phi, phi) = 0.  Using the stored P avoids the
+        // rounding of phi(t+dt) = phi(t) + P, which is large next to the charge.
         double rho[4] = {0, 0, 0, 0};
         for (int k = 0; k < ND; k++) {
             double t[4], n[4];
             load_d(a.phi_old[k][s], t);
-            load_d(a.phi[k][s], n);
+            load_d(a.P[k][s], n);
             bilinears_add(t, n, rho);
         }
-        double div[4] = {0, 0, 0, 0};
+        double div[4] = {0, 0, 0, 0}, mag[4] = {0, 0, 0, 0};
         for (int i = 0; i < 3; i++) {
             const long long sm = nb_index(a.nx, a.ny, a.nz, s, i, -1);
             double e[4], em[4], u[4];
@@ -523,11 +568,15 @@ __global__ void __launch_bounds__(256) mom_gauss(ObsArgs<R> a, Coef c, double* _
             div[1] += (rot.x - em[1]) * c.inv_sp[i];
             div[2] += (rot.y - em[2]) * c.inv_sp[i];
             div[3] += (rot.z - em[3]) * c.inv_sp[i];
+            mag[0] += (fabs(e[0]) + fabs(em[0])) * c.inv_sp[i];
+            mag[1] += (fabs(rot.x) + fabs(em[1])) * c.inv_sp[i];
+            mag[2] += (fabs(rot.y) + fabs(em[2])) * c.inv_sp[i];
+            mag[3] += (fabs(rot.z) + fabs(em[3])) * c.inv_sp[i];
         }
         for (int k = 0; k < 4; k++) {
             const double g = div[k] - c.gsq[k] * rho[k];
             sums[k] = fabs(g);
-            sums[4 + k] = fabs(div[k]) + fabs(c.gsq[k] * rho[k]);
+            sums[4 + k] = mag[k] + fabs(c.gsq[k] * rho[k]);
             maxs[k] = fabs(g);
         }
     }
@@ -788,7 +837,7 @@ StepArgs<R> make_step_args(MomentumState& m, MomFields<R>& f, const Coef& c, dou
     a.nbz = (c.nz + m.bz - 1) / m.bz;
     a.nby = (c.ny + m.by - 1) / m.by;
     a.nblocks = (long long)a.nbz * a.nby * c.nx;
-    a.xcd_remap = m.xcd_remap;
+    a.order = m.order;
     a.dt2 = (R)(c.dt * c.dt);
     a.keep = (R)(1.0 - damping_step);
     for (int k = 0; k < 4; k++) a.gsq[k] = (R)c.gsq[k];
@@ -802,12 +851,14 @@ inline int momentum_step(MomentumState& m, const Coef& c, double damping_step, h
     const long long nb = mom_nblocks(m, c);
     if (m.f64) {
         auto a = make_step_args<double>(m, m.fd, c, damping_step, err);
-        if (c.nd == 1) mom_step<double, 1><<<nb, 256, 0, st>>>(a, c);
-        else mom_step<double, 2><<<nb, 256, 0, st>>>(a, c);
+        if (c.nd == 1) mom_step<double, 1, 0><<<nb, 256, 0, st>>>(a, c);
+        else mom_step<double, 2, 0><<<nb, 256, 0, st>>>(a, c);
     } else {
         auto a = make_step_args<float>(m, m.ff, c, damping_step, err);
-        if (c.nd == 1) mom_step<float, 1><<<nb, 256, 0, st>>>(a, c);
-        else mom_step<float, 2><<<nb, 256, 0, st>>>(a, c);
+        if (c.nd == 2) mom_step<float, 2, 0><<<nb, 256, 0, st>>>(a, c);
+        else if (m.min_waves == 3) mom_step<float, 1, 3><<<nb, 256, 0, st>>>(a, c);
+        else if (m.min_waves == 4) mom_step<float, 1, 4><<<nb, 256, 0, st>>>(a, c);
+        else mom_step<float, 1, 0><<<nb, 256, 0, st>>>(a, c);
     }
     if (hipGetLastError() != hipSuccess) { msg = "momentum step launch failed"; return GFT_EHIP; }
     m.cur = !m.cur;

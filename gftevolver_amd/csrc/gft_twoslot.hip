@@ -20,8 +20,9 @@
 #include "gft_math.hpp"
 #include "gft_reduce.hpp"
 
-#pragma clang fp contract(off)
-
+// FP contraction is switched off inside each function of this file (the pragma
+// at the top of a compound statement is scoped to it), so the momentum-form
+// kernels compiled in the same translation unit keep their FMAs.
 namespace gft {
 
 constexpr int NV = 15;
@@ -34,10 +35,20 @@ __device__ __forceinline__ long long site_shift(const Coef& c, long long s, int 
     return (x * c.ny + y) * c.nz + z;
 }
 
+// SU(2) action on one doublet, contraction off (StandardModel.cpp:94-97).
+__device__ __forceinline__ void su2_act_ref(const double* U, const double* h, double* o) {
+#pragma clang fp contract(off)
+    o[0] = U[0] * h[0] - U[3] * h[1] + U[2] * h[2] - U[1] * h[3];
+    o[1] = U[3] * h[0] + U[0] * h[1] + U[1] * h[2] + U[2] * h[3];
+    o[2] = -U[2] * h[0] - U[1] * h[1] + U[0] * h[2] + U[3] * h[3];
+    o[3] = U[1] * h[0] - U[2] * h[1] - U[3] * h[2] + U[0] * h[3];
+}
+
 // transform(): U(1) phase then SU(2), optionally conjugated
 // (Gradients/Source/StandardModel.cpp:72-100, TwoHDM.cpp:72-104).
 template <class T>
 __device__ __forceinline__ void transform_ref(const double* sf, int nd, const T* link, bool conj, double* out) {
+#pragma clang fp contract(off)
     const double y = static_cast<double>(link[0]);
     const double cy = cos(y);
     const double sy = sin(conj ? -y : y);
@@ -48,12 +59,13 @@ __device__ __forceinline__ void transform_ref(const double* sf, int nd, const T*
     }
     double U[4];
     link_rep(link, conj, U);
-    for (int k = 0; k < nd; k++) su2_act(U, h + 4 * k, out + 4 * k);
+    for (int k = 0; k < nd; k++) su2_act_ref(U, h + 4 * k, out + 4 * k);
 }
 
 // Potential derivative in the reference's operation order (SO_N.cpp:57-75,
 // Double_SO_N.cpp:157-254).
 __device__ void potential_derivative_ref(const Coef& d, const double* f, double* pc) {
+#pragma clang fp contract(off)
     if (d.ns == 4) {
         double fsq = 0.0;
         for (int c = 0; c < 4; c++) fsq += f[c] * f[c];
@@ -108,6 +120,7 @@ template <class T>
 __global__ void __launch_bounds__(256) twoslot_step(Coef c, T* __restrict__ S, T* __restrict__ V, unsigned now,
                                                     int measure, double damping_step, double* __restrict__ partial,
                                                     unsigned* __restrict__ err) {
+#pragma clang fp contract(off)
     __shared__ double lds[4 * 5];
     const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     double dens[5] = {0, 0, 0, 0, 0};
@@ -269,10 +282,11 @@ __global__ void __launch_bounds__(256) twoslot_step(Coef c, T* __restrict__ S, T
 // Gauss law after a step (slot `now` = t, slot !now = t+dt), FP64
 // (Core/Source/Model.cpp:323-337; Gradients/Source/StandardModel.cpp:250-270;
 // WilsonLoops/Source/StandardModel.cpp:274-322).  Reduces per block:
-// sum |G^a| (4), sum (|div E^a| + g_a^2 |rho^a|) (4), max |G^a| (4).
+// sum |G^a| (4), sum (sum_i |electric terms| + g_a^2 |rho^a|) (4), max |G^a| (4).
 template <class T>
 __global__ void __launch_bounds__(256) twoslot_gauss(Coef c, const T* __restrict__ S, const T* __restrict__ V,
                                                      unsigned now, double* __restrict__ partial) {
+#pragma clang fp contract(off)
     __shared__ double lds[4 * 12];
     const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     double sums[8] = {0, 0, 0, 0, 0, 0, 0, 0}, maxs[4] = {0, 0, 0, 0};
@@ -290,7 +304,7 @@ __global__ void __launch_bounds__(256) twoslot_gauss(Coef c, const T* __restrict
             for (int m = 0; m < 4; m++) { t[m] = static_cast<double>(sn[4 * k + m]); n[m] = static_cast<double>(sf[4 * k + m]); }
             bilinears_add(t, n, rho);
         }
-        double div[4] = {0, 0, 0, 0};
+        double div[4] = {0, 0, 0, 0}, mag[4] = {0, 0, 0, 0};
         for (int a = 0; a < 3; a++) {
             const long long xm = site_shift(c, s, a, -1);
             const T* Ln = Vn + s * NV + a * 5;
@@ -300,6 +314,7 @@ __global__ void __launch_bounds__(256) twoslot_gauss(Coef c, const T* __restrict
             const double la = static_cast<double>(Lf[0]) - static_cast<double>(Ln[0]);
             const double na = static_cast<double>(Mf[0]) - static_cast<double>(Mn[0]);
             div[0] += (sin(la) - sin(na)) * c.inv_sp[a];
+            mag[0] += (fabs(sin(la)) + fabs(sin(na))) * c.inv_sp[a];
             double P1[4], P2[4], Um[4];
             link_rep(Lf, false, P1);
             link_rep(Ln, true, Um);
@@ -307,12 +322,15 @@ __global__ void __launch_bounds__(256) twoslot_gauss(Coef c, const T* __restrict
             link_rep(Mn, true, P2);
             link_rep(Mf, false, Um);
             su2_product(Um, P2, false, P2);
-            for (int k = 1; k < 4; k++) div[k] += (P1[k] - P2[k]) * c.inv_sp[a];
+            for (int k = 1; k < 4; k++) {
+                div[k] += (P1[k] - P2[k]) * c.inv_sp[a];
+                mag[k] += (fabs(P1[k]) + fabs(P2[k])) * c.inv_sp[a];
+            }
         }
         for (int k = 0; k < 4; k++) {
             const double g = div[k] - c.gsq[k] * rho[k];
             sums[k] = fabs(g);
-            sums[4 + k] = fabs(div[k]) + fabs(c.gsq[k] * rho[k]);
+            sums[4 + k] = mag[k] + fabs(c.gsq[k] * rho[k]);
             maxs[k] = fabs(g);
         }
     }

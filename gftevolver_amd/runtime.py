@@ -36,7 +36,8 @@ _L.gft_evolve.argtypes = [C.c_void_p, C.c_uint, C.c_uint, C.c_uint, _P(GftObs), 
 _L.gft_observe.argtypes = [C.c_void_p, _P(GftObs)]
 _L.gft_wilson_loops.argtypes = [C.c_void_p, _P(GftLoops)]
 _L.gft_sync.argtypes = [C.c_void_p]
-_L.gft_set_launch.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
+_L.gft_set_launch.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]
+_L.gft_bench_copy.argtypes = [C.c_size_t, C.c_int, _P(C.c_double)]
 _L.gft_last_step_ms.argtypes = [C.c_void_p, _P(C.c_double), _P(C.c_double), _P(C.c_uint)]
 
 
@@ -57,6 +58,15 @@ def device_info():
     if rc != 0:
         raise GftError(rc, "no HIP device")
     return buf.value.decode(), cu.value
+
+
+def bench_copy(nbytes: int, reps: int = 5) -> float:
+    """Average ms of a plain device copy of nbytes (read nbytes, write nbytes)."""
+    ms = C.c_double()
+    rc = _L.gft_bench_copy(nbytes, reps, C.byref(ms))
+    if rc != 0:
+        raise GftError(rc, "gft_bench_copy failed")
+    return ms.value
 
 
 class Evolver:
@@ -131,8 +141,8 @@ class Evolver:
         self._check(_L.gft_wilson_loops(self._h, C.byref(o)))
         return o
 
-    def set_launch(self, bz: int, by: int, xcd_remap: bool = True):
-        self._check(_L.gft_set_launch(self._h, bz, by, int(xcd_remap)))
+    def set_launch(self, bz: int, by: int, order: int = 2, min_waves: int = 3):
+        self._check(_L.gft_set_launch(self._h, bz, by, int(order), min_waves))
 
     def sync(self):
         self._check(_L.gft_sync(self._h))

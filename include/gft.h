@@ -87,7 +87,9 @@ typedef struct gft_obs {
     double   energy[6];      /* total, potential, gradient, kinetic, magnetic, electric        */
     double   gauss_int[4];   /* sum_x |G^a(x)| dx dy dz, a = Y, 1, 2, 3                        */
     double   gauss_max[4];   /* max_x |G^a(x)|                                                 */
-    double   gauss_scale[4]; /* sum_x (|div E^a| + g_a^2 |rho^a|) dx dy dz (relative measure)  */
+    double   gauss_scale[4]; /* scale of the relative violation: sum_x of the magnitudes of the
+                                terms that cancel in G^a, sum_i (|E term at x| + |E term at x-i|)
+                                / dx_i^2 + g_a^2 |rho^a|, times dx dy dz                      */
 } gft_obs;
 
 typedef struct gft_ctx gft_ctx;
@@ -131,11 +133,22 @@ typedef struct gft_loops {
 int gft_wilson_loops(gft_ctx* ctx, gft_loops* out);
 
 /* Launch shape of the momentum-form step kernel: tiles of bz x by sites
- * (bz along z, by along y, bz*by = 256) and whether blocks are dealt to the
- * XCDs in contiguous ranges.  Every shape gives bitwise the same fields; the
- * autotuner (gftevolver_amd/tune.py) picks the fastest.  No reference
+ * (bz along z, by along y, bz*by = 256); the block order (0: hardware order,
+ * x fastest in (y,z) column tiles; 1: the same dealt to the XCDs in
+ * contiguous ranges; 2, the default: each XCD sweeps the x-planes of its own
+ * y-band); and the occupancy target of the FP32 SM kernel (min_waves 3 or 4
+ * waves per SIMD, the default being 3; 0 leaves the register allocation
+ * unconstrained).  Every tile shape and order gives bitwise the same fields;
+ * the autotuner (gftevolver_amd/tune.py) picks the fastest.  No reference
  * counterpart (the reference has no launch configuration). */
-int gft_set_launch(gft_ctx* ctx, int bz, int by, int xcd_remap);
+int gft_set_launch(gft_ctx* ctx, int bz, int by, int order, int min_waves);
+
+/* Measurement helper (no reference counterpart): copies `bytes` between two
+ * device buffers `reps` times with a plain streaming copy kernel
+ * (gft_copy_kernel) and returns the average duration.  Its traffic is known
+ * exactly (bytes read + bytes written per launch), so rocprofv3's FETCH_SIZE
+ * and WRITE_SIZE of this kernel calibrate the same counters of the step kernel. */
+int gft_bench_copy(size_t bytes, int reps, double* avg_ms);
 
 /* Wait for all queued work on the context's stream. */
 int gft_sync(gft_ctx* ctx);

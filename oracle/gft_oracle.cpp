@@ -522,7 +522,7 @@ void site_update(const Params& p, T* S, T* V, long s, unsigned now, bool store_e
 
 // Gauss-law violation at s after the update (Model::calcConstraintViolation):
 // slot `now` holds t, slot `!now` holds t+dt.  Outputs per generator the
-// violation and, for the FP64 flavour, the scale |div E| + g_a^2 |rho|.
+// violation and, for the FP64 flavour, the scale sum_i |electric terms| + g_a^2 |rho|.
 template <class T, class A>
 void site_gauss(const Params& p, const T* S, const T* V, long s, unsigned now, A* viol, double* scale) {
     const int Ns = p.Ns, nd = p.nd;
@@ -575,7 +575,7 @@ void site_gauss(const Params& p, const T* S, const T* V, long s, unsigned now, A
         const double na = static_cast<double>(Mf[0]) - static_cast<double>(Mn[0]);
         const double d0 = (std::sin(la) - std::sin(na)) * p.inv_sp[a];
         elc[0] = static_cast<A>(elc[0] + d0);
-        divabs[0] += d0;
+        divabs[0] += (std::fabs(std::sin(la)) + std::fabs(std::sin(na))) * p.inv_sp[a];
         double P1[4], P2[4], Um[4];
         rep(Lf, false, P1);
         rep(Ln, true, Um);
@@ -586,13 +586,14 @@ void site_gauss(const Params& p, const T* S, const T* V, long s, unsigned now, A
         for (int k = 1; k < 4; k++) {
             const double dk = (P1[k] - P2[k]) * p.inv_sp[a];
             elc[k] += static_cast<A>(dk);
-            divabs[k] += dk;
+            divabs[k] += (std::fabs(P1[k]) + std::fabs(P2[k])) * p.inv_sp[a];
         }
     }
     for (int k = 0; k < 4; k++) {
         if constexpr (sizeof(A) == sizeof(float)) viol[k] = elc[k] - p.gsq_f[k] * grad[k];
         else viol[k] = elc[k] - static_cast<double>(p.gsq_f[k]) * grad[k];
-        if (scale) scale[k] = std::fabs(divabs[k]) + std::fabs(static_cast<double>(p.gsq_f[k]) * static_cast<double>(grad[k]));
+        // Scale of the relative violation: the magnitudes of the terms that cancel.
+        if (scale) scale[k] = divabs[k] + std::fabs(static_cast<double>(p.gsq_f[k]) * static_cast<double>(grad[k]));
     }
 }
 

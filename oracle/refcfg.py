@@ -1,20 +1,37 @@
-"""Writes the reference's positional .cfg files for one run.
+"""Prepares run directories for the reference binaries and writes their
+positional .cfg files.
 
-TEST INFRASTRUCTURE ONLY (used to drive oracle/_ref/gft_reference and
-oracle/_ref/gft_ref_harness in the build container).
+TEST INFRASTRUCTURE ONLY (drives oracle/_ref/gft_reference and
+oracle/_ref/gft_ref_harness).
 
 The reference reads each .cfg positionally: it skips a header, then reads the
 value(s) after each ':' in order (e.g. Core/Source/Lattice.cpp:47-143,
-Core/Source/Model.cpp:173-221).  `oracle/Makefile` copies the shipped files into
-oracle/_ref/cfg/<Lib>/Config; this module rewrites the values of the k-th value
-line after the '=====' separator, leaving the description text untouched.
+Core/Source/Model.cpp:173-221).  oracle/Makefile compiles the reference with
+config and data paths relative to the working directory and keeps the shipped
+files in oracle/_ref/cfg as a template; `prepare(root)` copies the template to
+root/cfg, and set_values() rewrites the values of the k-th value line after
+the '=====' separator, leaving the description text untouched.
 """
 from __future__ import annotations
 
 import os
+import shutil
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-CFG = os.path.join(HERE, "_ref", "cfg")
+TEMPLATE = os.path.join(HERE, "_ref", "cfg")
+
+
+def prepare(root: str) -> str:
+    """Fresh run directory: root/cfg (copy of the template) and root/data/Continual."""
+    dst = os.path.join(root, "cfg")
+    if os.path.exists(dst):
+        shutil.rmtree(dst)
+    shutil.copytree(TEMPLATE, dst)
+    for dp, _, files in os.walk(dst):
+        for f in files:
+            os.chmod(os.path.join(dp, f), 0o644)
+    os.makedirs(os.path.join(root, "data", "Continual"), exist_ok=True)
+    return root
 
 
 def _fmt(v):
@@ -25,9 +42,9 @@ def _fmt(v):
     return str(v)
 
 
-def set_values(lib: str, name: str, values: list) -> str:
+def set_values(root: str, lib: str, name: str, values: list) -> str:
     """values[k] (a scalar or a sequence) replaces the k-th value line."""
-    path = os.path.join(CFG, lib, "Config", name)
+    path = os.path.join(root, "cfg", lib, "Config", name)
     with open(path) as f:
         lines = f.read().split("\n")
     sep = next(i for i, l in enumerate(lines) if l.startswith("====="))
@@ -38,10 +55,7 @@ def set_values(lib: str, name: str, values: list) -> str:
         if k >= len(values):
             break
         v = values[k]
-        if isinstance(v, (list, tuple)):
-            txt = " ".join(_fmt(x) for x in v)
-        else:
-            txt = _fmt(v)
+        txt = " ".join(_fmt(x) for x in v) if isinstance(v, (list, tuple)) else _fmt(v)
         head = lines[i].split(":", 1)[0]
         lines[i] = f"{head}:    {txt}"
         k += 1
@@ -52,9 +66,10 @@ def set_values(lib: str, name: str, values: list) -> str:
     return path
 
 
-def write_model(model: int, g: float, gp: float, *, lam=0.13, eta_sqr=1.0, pot2=None,
+def write_model(root: str, model: int, g: float, gp: float, *, lam=0.13, eta_sqr=1.0, pot2=None,
                 damping=0.0, nt_damped=0, tol=1e-9) -> str:
-    """Model.cfg plus the potential / gradient / Wilson-loop configs.
+    """Model.cfg plus the potential / gradient / Wilson-loop configs; returns the
+    Model.cfg path (relative to root, as the harness opens it from there).
 
     model 0 = SM (SO_N, SM gradient, SM loops); 1 = 2HDM with gaugeNum 4
     (Double_SO_N, TwoHDM gradient, TwoHDM loops, quaternion links).
@@ -63,38 +78,39 @@ def write_model(model: int, g: float, gp: float, *, lam=0.13, eta_sqr=1.0, pot2=
     """
     if model == 0:
         ids = (1, 2, 1)
-        set_values("Potentials", "SO_N.cfg", [lam, eta_sqr])
-        set_values("WilsonLoops", "StandardModel.cfg", [g, gp, tol])
-        set_values("Gradients", "StandardModel.cfg", [tol])
+        set_values(root, "Potentials", "SO_N.cfg", [lam, eta_sqr])
+        set_values(root, "WilsonLoops", "StandardModel.cfg", [g, gp, tol])
+        set_values(root, "Gradients", "StandardModel.cfg", [tol])
     else:
         ids = (2, 3, 2)
         q = pot2
-        set_values("Potentials", "Double_SO_N.cfg", [
+        set_values(root, "Potentials", "Double_SO_N.cfg", [
             4, int(q.get("pot_norm", 0)), q.get("pot_offset", 0.0),
             (q["m1"], q["m2"]), (q["l1"], q["l2"]), q["l3"],
             int(q.get("break_to_son", 1)), q["m12r"], q["l4p5"], (q["l6r"], q["l7r"]),
             int(q.get("break_to_uhalfn", 1)), q["m12i"], (q["l4m5"], q["l5i"]), (q["l6i"], q["l7i"]),
         ])
-        set_values("WilsonLoops", "TwoHDM.cfg", [g, gp, 0.5, 0, tol])
-        set_values("Gradients", "TwoHDM.cfg", [tol])
-    return set_values("Core", "Model.cfg", [ids[0], ids[1], ids[2], float(damping), nt_damped, 0, 0.0])
+        set_values(root, "WilsonLoops", "TwoHDM.cfg", [g, gp, 0.5, 0, tol])
+        set_values(root, "Gradients", "TwoHDM.cfg", [tol])
+    set_values(root, "Core", "Model.cfg", [ids[0], ids[1], ids[2], float(damping), nt_damped, 0, 0.0])
+    return os.path.join("cfg", "Core", "Config", "Model.cfg")
 
 
-def write_lattice(n, d, nt, dt, ns, *, ic_seed=42, ic_range=(-1.0, 1.0), efreq=1, gfreq=1,
+def write_lattice(root: str, n, d, nt, dt, ns, *, ic_seed=42, ic_range=(-1.0, 1.0), efreq=1, gfreq=1,
                   energy=True, gauss=True, progress=False):
     """Lattice.cfg, RandomUniform.cfg and analyser configs for a full program run
     with a RandomUniform scalar initial state and identity links."""
-    set_values("Core", "Lattice.cfg", [
+    set_values(root, "Core", "Lattice.cfg", [
         tuple(n), tuple(d), nt, dt, ns, 5, 0, 3.0, (1, 0), 0, int(energy), int(gauss), (3, 3, 3, 3, 3, 3),
         int(progress), max(nt, 1),
     ])
-    set_values("InitialConditions", "RandomUniform.cfg", [ic_seed, tuple(ic_range), 1])
-    set_values("Analysers", "Energy.cfg", [
+    set_values(root, "InitialConditions", "RandomUniform.cfg", [ic_seed, tuple(ic_range), 1])
+    set_values(root, "Analysers", "Energy.cfg", [
         "global_quantities.dat", 1, 1, 1, 1, 1, 1, efreq,
         "Continual/local_quantities", 0, 0, 0, 0, 0, 0, efreq,
     ])
-    set_values("Analysers", "GaugeCondition.cfg", [
+    set_values(root, "Analysers", "GaugeCondition.cfg", [
         "global_quantities_gauge.dat", 1, 1, gfreq, "Continual/local_quantities_gauge", 0, gfreq,
     ])
-    set_values("Analysers", "OutputFields.cfg", [0, 0, 0, 0, 10, "initial_fields.dat",
-                                                 "Continual/fields", "final_fields.dat"])
+    set_values(root, "Analysers", "OutputFields.cfg", [0, 0, 0, 0, 10, "initial_fields.dat",
+                                                       "Continual/fields", "final_fields.dat"])

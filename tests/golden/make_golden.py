@@ -78,13 +78,16 @@ def main():
             out[f"viol_{k}"] = h[k]["viol"]
         np.savez_compressed(os.path.join(HERE, f"harness_{tag}.npz"), **out)
 
-        refcfg.write_model(model, 0.65, 0.35, pot2=pot2)
-        refcfg.write_lattice(PROGRAM_N, (0.5, 0.5, 0.5), PROGRAM_STEPS, 0.05, Ns, ic_seed=PROGRAM_SEED,
-                             ic_range=PROGRAM_RANGE, efreq=PROGRAM_EFREQ, gfreq=PROGRAM_GFREQ)
-        subprocess.run([os.path.join(O.REF_DIR, "gft_reference")], check=True, stdout=subprocess.DEVNULL)
-        data = os.path.join(O.REF_DIR, "data")
-        E = np.loadtxt(os.path.join(data, "global_quantities.dat"), ndmin=2)
-        G = np.loadtxt(os.path.join(data, "global_quantities_gauge.dat"), ndmin=2)
+        with tempfile.TemporaryDirectory() as td:
+            refcfg.prepare(td)
+            refcfg.write_model(td, model, 0.65, 0.35, pot2=pot2)
+            refcfg.write_lattice(td, PROGRAM_N, (0.5, 0.5, 0.5), PROGRAM_STEPS, 0.05, Ns, ic_seed=PROGRAM_SEED,
+                                 ic_range=PROGRAM_RANGE, efreq=PROGRAM_EFREQ, gfreq=PROGRAM_GFREQ)
+            subprocess.run([os.path.join(O.REF_DIR, "gft_reference")], check=True, stdout=subprocess.DEVNULL,
+                           stderr=subprocess.DEVNULL, cwd=td)
+            data = os.path.join(td, "data")
+            E = np.loadtxt(os.path.join(data, "global_quantities.dat"), ndmin=2)
+            G = np.loadtxt(os.path.join(data, "global_quantities_gauge.dat"), ndmin=2)
         np.savez_compressed(os.path.join(HERE, f"program_{tag}.npz"), energy=E, gauss=G)
         print(tag, "written")
 

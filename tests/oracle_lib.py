@@ -92,7 +92,8 @@ def run_harness(model: int, n, d, dt, nsteps, efreq, gfreq, scalar, vec, workdir
     import sys
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import refcfg
-    model_cfg = refcfg.write_model(model, g, gp, lam=lam, eta_sqr=eta_sqr, pot2=pot2, damping=damping,
+    refcfg.prepare(workdir)
+    model_cfg = refcfg.write_model(workdir, model, g, gp, lam=lam, eta_sqr=eta_sqr, pot2=pot2, damping=damping,
                                    nt_damped=nt_damped)
     Ns = 4 if model == 0 else 8
     N = n[0] * n[1] * n[2]
@@ -103,7 +104,7 @@ def run_harness(model: int, n, d, dt, nsteps, efreq, gfreq, scalar, vec, workdir
         f.write(np.ascontiguousarray(vec, np.float32).tobytes())
     subprocess.run([os.path.join(REF_DIR, "gft_ref_harness"), model_cfg, str(Ns), *map(str, n), *map(repr, d),
                     repr(dt), str(nsteps), str(efreq), str(gfreq), fin, fout], check=True,
-                   stdout=subprocess.DEVNULL)
+                   stdout=subprocess.DEVNULL, cwd=workdir)
     raw = np.fromfile(fout, np.float32)
     out, pos = [], 0
     for k in range(nsteps):

@@ -8,9 +8,11 @@ renormalisations), so parity is to a tolerance, not bitwise:
   * FP32 storage vs the FP64 CPU oracle: <= 2e-5 relative at field level after
     100 steps (float rounding of ~1e-7 per step, accumulated);
   * energies: <= 1e-9 (FP64) / 1e-5 (FP32) relative to the total energy;
-  * Gauss law: conserved to <= 1e-12 (FP64) relative from a constraint-
-    satisfying state; the FP32 value is recorded and bounded at 1e-5.
-Every launch shape must give bitwise identical fields (the autotuner relies on it).
+  * Gauss law, relative to the magnitude of the cancelling terms
+    (gft_obs.gauss_scale): <= 1e-12 (FP64) and <= 1e-6 (FP32) over 200 steps
+    of a hot state from an exactly constraint-satisfying start.
+Every tile shape and block order must give bitwise identical fields (the autotuner
+relies on it); the occupancy variants are separate kernels, equal to 2 ulps.
 """
 from __future__ import annotations
 
@@ -84,7 +86,7 @@ def test_momentum_2hdm_matches_oracle(rt, form):
     assert np.abs(_phase_diff(gv[cur], v[cur])).max() <= tol
 
 
-@pytest.mark.parametrize("form,tol", [(GFT_FORM_MOMENTUM_F64, 1e-12), (GFT_FORM_MOMENTUM_F32, 1e-5)],
+@pytest.mark.parametrize("form,tol", [(GFT_FORM_MOMENTUM_F64, 1e-12), (GFT_FORM_MOMENTUM_F32, 1e-6)],
                          ids=["f64", "f32"])
 def test_momentum_gauss_law_conserved(rt, form, tol):
     """From E = 0, P = 0 the discrete Gauss law holds exactly; it must stay at
@@ -105,15 +107,34 @@ def test_momentum_launch_shapes_bitwise_identical(rt):
     n = (16, 12, 20)
     d = make_desc(form=GFT_FORM_MOMENTUM_F32, n=n)
     s, v = L.near_vacuum_state(n, 4, seed=6, amp=0.1, mom=0.01, link_eps=0.05)
-    out = []
-    for bz, by, remap in ((64, 4, 1), (64, 4, 0), (256, 1, 1), (128, 2, 1), (32, 8, 1), (16, 16, 0)):
+    cfgs = ((64, 4, 2, 3), (64, 4, 0, 3), (64, 4, 1, 3), (256, 1, 2, 3), (128, 2, 2, 3), (32, 8, 1, 3),
+            (16, 16, 0, 3), (16, 16, 2, 3), (64, 4, 2, 0), (64, 4, 2, 4))
+    out = {}
+    for c in cfgs:
         with rt.Evolver(d) as ev:
-            ev.set_launch(bz, by, remap)
+            ev.set_launch(*c)
             ev.import_two_slot(s, v, 0)
             ev.evolve(7)
-            out.append(ev.export_two_slot(np.float32))
-    for o in out[1:]:
-        assert np.array_equal(o[0], out[0][0]) and np.array_equal(o[1], out[0][1])
+            out[c] = ev.export_two_slot(np.float32)
+    ref = out[cfgs[0]]
+    bad = []
+    for c in cfgs[1:]:
+        ds = np.abs(out[c][0] - ref[0]).max()
+        dv = np.abs(out[c][1] - ref[1]).max()
+        nd = int((out[c][0] != ref[0]).sum() + (out[c][1] != ref[1]).sum())
+        print(f"launch {c}: max |diff| scalar {ds:.2e} links {dv:.2e}, {nd} differing values")
+        if c[3] == 3:
+            # Tile shape and block order: same kernel, bitwise the same fields.
+            if nd:
+                bad.append(c)
+        else:
+            # Occupancy variants are separately compiled kernels; their FMA
+            # formation may differ in the last bit.
+            # Two float ulps at the scale of the field (|phi| ~ 1).
+            ulp = np.spacing(np.float32(np.abs(ref[0]).max()))
+            if ds > 2 * ulp or dv > 2 * np.spacing(np.float32(1)):
+                bad.append(c)
+    assert not bad, bad
 
 
 def test_momentum_import_export_roundtrip(rt):
