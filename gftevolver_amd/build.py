@@ -35,6 +35,15 @@ def build_library(verbose: bool = False):
     os.replace(tmp, LIB)
 
 
+def build_host():
+    """C++ host API (gftevolver_amd/host) and the gft_run driver, linked to libgft.so."""
+    host = os.path.join(HERE, "host")
+    out = os.path.join(HERE, "gft_run")
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-o", out + ".tmp", os.path.join(host, "gft_host.cpp"),
+          os.path.join(host, "gft_run.cpp"), "-L" + HERE, "-lgft", "-Wl,-rpath,$ORIGIN"])
+    os.replace(out + ".tmp", out)
+
+
 def build_checkers():
     oracle = os.path.join(ROOT, "oracle")
     _run(["make", "-s", "-C", oracle, "oracle"])
@@ -44,9 +53,11 @@ def build_checkers():
 
 def build_all():
     build_library()
+    build_host()
     build_checkers()
 
 
 if __name__ == "__main__":
     build_library(verbose="-v" in sys.argv)
+    build_host()
     build_checkers()

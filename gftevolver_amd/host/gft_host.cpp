@@ -1,0 +1,530 @@
+// gft_host.cpp -- the C++ host API (see gft_host.hpp).
+#include "gft_host.hpp"
+
+#include <chrono>
+#include <cmath>
+#include <fstream>
+#include <iostream>
+#include <random>
+#include <sstream>
+
+namespace gft {
+
+// ---- ConfigReader ---------------------------------------------------------------
+
+ConfigReader::ConfigReader(const std::string& path) : path_(path) {
+    std::ifstream ifs(path);
+    if (ifs.is_open()) {
+        std::stringstream ss;
+        ss << ifs.rdbuf();
+        text_ = ss.str();
+        open_ = true;
+    }
+}
+
+void ConfigReader::skip_lines(int n) {
+    for (int i = 0; i < n && pos_ < text_.size(); i++) {
+        const size_t e = text_.find('\n', pos_);
+        pos_ = (e == std::string::npos) ? text_.size() : e + 1;
+    }
+}
+
+template <class T>
+T ConfigReader::value() {
+    T v{};
+    values(&v, 1);
+    return v;
+}
+
+template <class T>
+void ConfigReader::values(T* out, int n) {
+    if (!open_) return;
+    const size_t c = text_.find(':', pos_);
+    if (c == std::string::npos) { pos_ = text_.size(); return; }
+    std::istringstream is(text_.substr(c + 1));
+    for (int i = 0; i < n; i++) is >> out[i];
+    const std::streamoff used = is.fail() ? 0 : static_cast<std::streamoff>(is.tellg());
+    pos_ = c + 1 + (used > 0 ? static_cast<size_t>(used) : 0);
+}
+
+template unsigned ConfigReader::value<unsigned>();
+template int ConfigReader::value<int>();
+template bool ConfigReader::value<bool>();
+template double ConfigReader::value<double>();
+template float ConfigReader::value<float>();
+template std::string ConfigReader::value<std::string>();
+
+static std::string cfg(const std::string& root, const std::string& lib, const std::string& name) {
+    return root + "/" + lib + "/Config/" + name;
+}
+
+// ---- Potentials -----------------------------------------------------------------
+
+SO_N::SO_N(const std::string& root, unsigned num_components) : numComponents(num_components) {
+    ConfigReader r(cfg(root, "Potentials", "SO_N.cfg"));   // SO_N.cpp:12-29
+    r.skip_lines(15);
+    lambda = r.value<double>();
+    etaSqr = r.value<double>();
+}
+
+void SO_N::fill(gft_desc& d) const {
+    d.lambda = lambda;
+    d.eta_sqr = etaSqr;
+}
+
+Double_SO_N::Double_SO_N(const std::string& root, unsigned total) {
+    ConfigReader r(cfg(root, "Potentials", "Double_SO_N.cfg"));   // Double_SO_N.cpp:35-128
+    r.skip_lines(40);
+    numComponents[0] = r.value<unsigned>();
+    applyPotentialNormalisation = r.value<bool>();
+    potentialEnergyOffset = r.value<double>();
+    if (numComponents[0] > total)
+        throw std::runtime_error("POTENTIAL::DOUBLE_SO_N:: The number of field components in f1 has been set to "
+                                 + std::to_string(numComponents[0]) + " and the total number has been set to "
+                                 + std::to_string(total) + ".\n The total number must be greater or equal to the number in f1.");
+    numComponents[1] = total - numComponents[0];
+    r.skip_lines(1);
+    double mm[2], ll[2];
+    r.values(mm, 2);
+    m1 = mm[0]; m2 = mm[1];
+    r.values(ll, 2);
+    l1 = ll[0]; l2 = ll[1];
+    l3 = r.value<double>();
+    r.skip_lines(4);
+    breakToSON = r.value<bool>();
+    if (breakToSON) {
+        if (numComponents[0] != numComponents[1])
+            throw std::runtime_error("POTENTIAL::DOUBLE_SO_N:: Attempting to use the potential parameters that explicitly "
+                                     "break the symmetry down to a single SO(N) requires that the number of components in f1 "
+                                     "and f2 are the same.");
+        r.skip_lines(1);
+        m12r = r.value<double>();
+        l4p5 = r.value<double>();
+        double l67[2];
+        r.values(l67, 2);
+        l6r = l67[0]; l7r = l67[1];
+        r.skip_lines(5);
+        breakToUHalfN = r.value<bool>();
+        if (breakToUHalfN) {
+            if (numComponents[0] % 2 != 0)
+                throw std::runtime_error("POTENTIAL::DOUBLE_SO_N:: Attempting to use the potential parameters that explicitly "
+                                         "break the symmetry down to a single U(N/2) requires that the number of components "
+                                         "in f1 (equal to f2) is even.");
+            r.skip_lines(1);
+            m12i = r.value<double>();
+            double a[2], b[2];
+            r.values(a, 2);
+            l4m5 = a[0]; l5i = a[1];
+            r.values(b, 2);
+            l6i = b[0]; l7i = b[1];
+        }
+    }
+    if (numComponents[0] != 4 || numComponents[1] != 4)
+        throw std::runtime_error("POTENTIAL::DOUBLE_SO_N:: the GPU evolver supports two four-component doublets");
+}
+
+void Double_SO_N::fill(gft_desc& d) const {
+    d.m1 = m1; d.m2 = m2; d.l1 = l1; d.l2 = l2; d.l3 = l3;
+    d.break_to_son = breakToSON; d.break_to_uhalfn = breakToUHalfN;
+    d.m12r = m12r; d.l4p5 = l4p5; d.l6r = l6r; d.l7r = l7r;
+    d.m12i = m12i; d.l4m5 = l4m5; d.l5i = l5i; d.l6i = l6i; d.l7i = l7i;
+    d.pot_norm = applyPotentialNormalisation; d.pot_offset = potentialEnergyOffset;
+}
+
+// ---- Gradients / Wilson loops -------------------------------------------------------
+
+namespace Gradients {
+StandardModel::StandardModel(const std::string& root, unsigned ns, unsigned nv) {
+    ConfigReader r(cfg(root, "Gradients", "StandardModel.cfg"));   // StandardModel.cpp:46-59
+    r.skip_lines(12);
+    divisionByZeroTolerance = r.value<double>();
+    if (ns != 4)
+        throw std::runtime_error("GRADIENTS::STANDARDMODEL:: 4 scalar components are required but " + std::to_string(ns)
+                                 + " have been assigned.");
+    if (nv != 15)
+        throw std::runtime_error("GRADIENTS::STANDARDMODEL:: the GPU evolver uses the quaternion representation "
+                                 "(15 vector components) but " + std::to_string(nv) + " have been assigned.");
+}
+
+TwoHDM::TwoHDM(const std::string& root, unsigned ns, unsigned nv) {
+    ConfigReader r(cfg(root, "Gradients", "TwoHDM.cfg"));
+    r.skip_lines(12);
+    divisionByZeroTolerance = r.value<double>();
+    if (ns != 8)
+        throw std::runtime_error("GRADIENTS::TWOHDM:: 8 scalar components are required but " + std::to_string(ns)
+                                 + " have been assigned.");
+    if (nv != 15)
+        throw std::runtime_error("GRADIENTS::TWOHDM:: the GPU evolver supports the SM gauge group in the quaternion "
+                                 "representation (15 vector components) but " + std::to_string(nv) + " have been assigned.");
+}
+}  // namespace Gradients
+
+namespace WilsonLoops {
+StandardModel::StandardModel(const std::string& root, unsigned nv) {
+    ConfigReader r(cfg(root, "WilsonLoops", "StandardModel.cfg"));   // StandardModel.cpp:5-44
+    r.skip_lines(12);
+    g = r.value<double>();
+    gp = r.value<double>();
+    r.skip_lines(1);
+    divisionByZeroTolerance = r.value<double>();
+    if (nv != 12 && nv != 15)
+        throw std::runtime_error("WILSONLOOPS::STANDARDMODEL:: Either 12 or 15 vector components are required "
+                                 "(depending on whether the generator or quaternion representation is used) but "
+                                 + std::to_string(nv) + " have been assigned.");
+    usingGeneratorRepresentation = (nv == 12);
+    if (usingGeneratorRepresentation)
+        throw std::runtime_error("WILSONLOOPS::STANDARDMODEL:: the GPU evolver uses the quaternion representation");
+}
+
+float StandardModel::getSqrCouplings(unsigned c) const {
+    return static_cast<float>(c % 4 == 0 ? gp * gp : g * g);
+}
+
+TwoHDM::TwoHDM(const std::string& root, unsigned nv) {
+    ConfigReader r(cfg(root, "WilsonLoops", "TwoHDM.cfg"));   // TwoHDM.cpp:5-44
+    r.skip_lines(12);
+    g = r.value<double>();
+    gp = r.value<double>();
+    gpp = r.value<double>();
+    r.skip_lines(1);
+    usingGeneratorRepresentation = r.value<bool>();
+    r.skip_lines(1);
+    divisionByZeroTolerance = r.value<double>();
+    if (usingGeneratorRepresentation || nv != 15)
+        throw std::runtime_error("WILSONLOOPS::TWOHDM:: the GPU evolver supports the SM gauge group (gaugeNum 4) in "
+                                 "the quaternion representation (15 vector components)");
+}
+
+float TwoHDM::getSqrCouplings(unsigned c) const {
+    return static_cast<float>(c % 4 == 0 ? gp * gp : g * g);
+}
+}  // namespace WilsonLoops
+
+// ---- Model --------------------------------------------------------------------------
+
+void Model::configure(const std::string& model_cfg, const std::string& root, unsigned ns, unsigned nv, unsigned,
+                      unsigned, unsigned, double, double, double, double dt) {
+    ConfigReader r(model_cfg);   // Core/Source/Model.cpp:145-234
+    r.skip_lines(11);
+    potentialType = r.value<int>();
+    switch (potentialType) {
+    case SO_N_POTENTIAL: potential = std::make_unique<SO_N>(root, ns); break;
+    case DOUBLE_SO_N_POTENTIAL: potential = std::make_unique<Double_SO_N>(root, ns); break;
+    default:
+        throw std::runtime_error("MODEL:: The chosen potential type (" + std::to_string(potentialType) + ") is invalid.");
+    }
+    r.skip_lines(1);
+    gradientType = r.value<int>();
+    r.skip_lines(1);
+    wilsonLoopType = r.value<int>();
+    switch (wilsonLoopType) {
+    case SM_WILSON_LOOP: wilsonLoop = std::make_unique<WilsonLoops::StandardModel>(root, nv); break;
+    case TWOHDM_WILSON_LOOP: wilsonLoop = std::make_unique<WilsonLoops::TwoHDM>(root, nv); break;
+    default:
+        throw std::runtime_error("MODEL: The chosen wilson loop type (" + std::to_string(wilsonLoopType) + ") is invalid.");
+    }
+    switch (gradientType) {
+    case SM_GRADIENT: gradient = std::make_unique<Gradients::StandardModel>(root, ns, nv); break;
+    case TWOHDM_GRADIENT: gradient = std::make_unique<Gradients::TwoHDM>(root, ns, nv); break;
+    default:
+        throw std::runtime_error("MODEL::ERROR: The chosen gradient type (" + std::to_string(gradientType) + ") is invalid.");
+    }
+    r.skip_lines(1);
+    dampingFactor = r.value<float>();
+    ntDamped = r.value<unsigned>();
+    const bool override_nt = r.value<bool>();
+    const float t_end = r.value<float>();
+    if (override_nt) ntDamped = static_cast<int>(t_end / dt);
+    const bool sm = potentialType == SO_N_POTENTIAL && gradientType == SM_GRADIENT && wilsonLoopType == SM_WILSON_LOOP;
+    const bool thdm = potentialType == DOUBLE_SO_N_POTENTIAL && gradientType == TWOHDM_GRADIENT
+                      && wilsonLoopType == TWOHDM_WILSON_LOOP;
+    if (!sm && !thdm)
+        throw std::runtime_error("MODEL:: the GPU evolver runs the SM (potential 1, gradient 2, Wilson loop 1) or the "
+                                 "2HDM (potential 2, gradient 3, Wilson loop 2)");
+    getNumberOfConstraintEquations();
+}
+
+unsigned Model::getNumberOfConstraintEquations() const {
+    const unsigned a = gradient->getNumberOfConstraintEquations(), b = wilsonLoop->getNumberOfConstraintEquations();
+    if (a != b)   // Core/Source/Model.cpp:269-279
+        throw std::runtime_error("MODEL: The number of constraint equations expected by the gradients (" + std::to_string(a)
+                                 + ") does not match the number expected by the wilson loops (" + std::to_string(b) + ").");
+    return b;
+}
+
+bool Model::isUsingQuaternionRepresentation() const { return !wilsonLoop->isUsingGeneratorRepresentation(); }
+
+void Model::fill(gft_desc& d) const {
+    d.model = potentialType == DOUBLE_SO_N_POTENTIAL ? GFT_MODEL_2HDM : GFT_MODEL_SM;
+    d.g = wilsonLoop->g;
+    d.gp = wilsonLoop->gp;
+    d.div_tol = wilsonLoop->divisionByZeroTolerance;
+    potential->fill(d);
+    d.damping = dampingFactor;
+    d.nt_damped = ntDamped;
+}
+
+// ---- Initial conditions ---------------------------------------------------------------
+
+RandomUniform::RandomUniform(const std::string& root) {
+    ConfigReader r(cfg(root, "InitialConditions", "RandomUniform.cfg"));   // RandomUniform.cpp:12-36
+    r.skip_lines(12);
+    seed = r.value<int>();
+    float mm[2] = {0, 0};
+    r.values(mm, 2);
+    minRange = mm[0]; maxRange = mm[1];
+    setTimestepsEqual = r.value<bool>();
+}
+
+void RandomUniform::setInitialFields(std::vector<float>& field) const {
+    std::mt19937 generator(seed);
+    std::uniform_real_distribution<float> distribution(minRange, maxRange);
+    const size_t m = setTimestepsEqual ? field.size() / 2 : field.size();
+    for (size_t i = 0; i < m; i++) {
+        field[i] = distribution(generator);
+        if (setTimestepsEqual) field[i + m] = field[i];
+    }
+}
+
+Gaussian::Gaussian(const std::string& root, unsigned ns) : numScalar(ns) {
+    ConfigReader r(cfg(root, "InitialConditions", "Gaussian.cfg"));
+    r.skip_lines(1);
+    seed = r.value<unsigned>();
+    sigma = r.value<double>();
+    vev = r.value<double>();
+}
+
+void Gaussian::setInitialFields(std::vector<float>& field) const {
+    std::mt19937_64 gen(seed);
+    std::normal_distribution<double> nd(0.0, sigma);
+    const size_t m = field.size() / 2;
+    for (size_t i = 0; i < m; i++) {
+        const unsigned c = i % numScalar;
+        field[m + i] = static_cast<float>(nd(gen) + ((c % 4) == 2 ? vev : 0.0));
+        field[i] = field[m + i];
+    }
+}
+
+// ---- Analysers --------------------------------------------------------------------------
+
+Energy::Energy(const std::string& root, const std::string& data_dir, double, double, double) : dataDir(data_dir) {
+    ConfigReader r(cfg(root, "Analysers", "Energy.cfg"));   // Energy.cpp:5-57
+    r.skip_lines(13);
+    globalQuantitiesPath = r.value<std::string>();
+    for (int i = 0; i < 6; i++) {
+        globalOptions[i] = r.value<bool>();
+        anyGlobalOptions = anyGlobalOptions || globalOptions[i];
+    }
+    globalFrequency = r.value<unsigned>();
+    if (globalFrequency == 0) globalFrequency = 1;
+    if (anyGlobalOptions) std::ofstream(dataDir + "/" + globalQuantitiesPath).close();
+}
+
+void Energy::timestepAnalysis(const gft_obs& row) {
+    if (!row.has_energy) return;
+    std::ofstream ofs(dataDir + "/" + globalQuantitiesPath, std::ios::app);   // Energy.cpp:169-189
+    for (int m = 0; m < 6; m++)
+        if (globalOptions[m]) ofs << row.energy[m] << " ";
+    ofs << "\n";
+}
+
+GaugeCondition::GaugeCondition(const std::string& root, const std::string& data_dir) : dataDir(data_dir) {
+    ConfigReader r(cfg(root, "Analysers", "GaugeCondition.cfg"));   // GaugeCondition.cpp:5-57
+    r.skip_lines(13);
+    globalQuantitiesPath = r.value<std::string>();
+    for (int i = 0; i < 2; i++) {
+        globalOptions[i] = r.value<bool>();
+        anyGlobalOptions = anyGlobalOptions || globalOptions[i];
+    }
+    globalFrequency = r.value<unsigned>();
+    if (globalFrequency == 0) globalFrequency = 1;
+    if (anyGlobalOptions) std::ofstream(dataDir + "/" + globalQuantitiesPath).close();
+}
+
+void GaugeCondition::timestepAnalysis(const gft_obs& row) {
+    if (!row.has_gauss) return;
+    std::ofstream ofs(dataDir + "/" + globalQuantitiesPath, std::ios::app);   // GaugeCondition.cpp:167-194
+    if (globalOptions[0])
+        for (unsigned m = 0; m < row.num_eq; m++) ofs << static_cast<float>(row.gauss_int[m]) << " ";
+    if (globalOptions[1])
+        for (unsigned m = 0; m < row.num_eq; m++) ofs << static_cast<float>(row.gauss_max[m]) << " ";
+    ofs << "\n";
+}
+
+OutputFields::OutputFields(const std::string& root, const std::string& data_dir, const std::vector<float>& scalar,
+                           unsigned ns_, const std::vector<float>& vec, unsigned nv_)
+    : dataDir(data_dir), scalarFields(scalar), vectorFields(vec), ns(ns_), nv(nv_) {
+    ConfigReader r(cfg(root, "Analysers", "OutputFields.cfg"));   // OutputFields.cpp:5-53
+    r.skip_lines(13);
+    outputBothTimesteps = r.value<bool>();
+    outputInitial = r.value<bool>();
+    outputFinal = r.value<bool>();
+    r.skip_lines(1);
+    outputContinual = r.value<bool>();
+    outputFrequency = r.value<unsigned>();
+    if (outputFrequency == 0) outputFrequency = 1;
+    r.skip_lines(1);
+    initialAnalysisPath = r.value<std::string>();
+    continualAnalysisPath = r.value<std::string>();
+    finalAnalysisPath = r.value<std::string>();
+}
+
+void OutputFields::outputFields(const std::string& path, unsigned time_step) const {
+    std::ofstream ofs(dataDir + "/" + path);   // OutputFields.cpp:63-133
+    if (!ofs.is_open()) return;
+    const size_t nsites = scalarFields.size() / (2 * ns);
+    const size_t lo = outputBothTimesteps ? 0 : ((time_step + 1) % 2) * nsites;
+    const size_t hi = outputBothTimesteps ? 2 * nsites : lo + nsites;
+    for (size_t i = lo; i < hi; i++) {
+        for (unsigned c = 0; c < ns; c++) ofs << scalarFields[i * ns + c] << " ";
+        for (unsigned c = 0; c < nv; c++) ofs << vectorFields[i * nv + c] << " ";
+        ofs << std::endl;
+    }
+}
+
+// ---- Lattice -------------------------------------------------------------------------------
+
+Lattice::Lattice(const std::string& config_root, const std::string& data_dir, int form_)
+    : root(config_root), dataDir(data_dir), form(form_) {
+    configure(cfg(root, "Core", "Lattice.cfg"));
+    model.configure(cfg(root, "Core", "Model.cfg"), root, numScalarFieldComponents, numVectorFieldComponents, nx, ny,
+                    nz, dx, dy, dz, dt);
+    initFields();
+}
+
+Lattice::~Lattice() {
+    if (ctx) gft_destroy(ctx);
+}
+
+void Lattice::configure(const std::string& path) {
+    ConfigReader r(path);   // Core/Source/Lattice.cpp:33-170
+    if (!r.is_open()) throw std::runtime_error("LATTICE:: cannot open " + path);
+    r.skip_lines(10);
+    unsigned n3[3];
+    r.values(n3, 3);
+    nx = n3[0]; ny = n3[1]; nz = n3[2];
+    double d3[3];
+    r.values(d3, 3);
+    dx = d3[0]; dy = d3[1]; dz = d3[2];
+    nt = r.value<unsigned>();
+    dt = r.value<double>();
+    r.skip_lines(1);
+    numScalarFieldComponents = r.value<unsigned>();
+    numVectorFieldComponents = 3 * r.value<unsigned>();
+    r.skip_lines(2);
+    if (r.value<bool>()) {
+        const float crossings = r.value<float>();
+        nt = static_cast<int>(crossings * static_cast<float>(nx) * dx / (2 * dt));
+    }
+    r.skip_lines(3);
+    int ic[2] = {0, 0};
+    r.values(ic, 2);
+    for (int k = 0; k < 2; k++) {
+        std::unique_ptr<InitialCondition>& slot = k == 0 ? scalarInitialConditions : vectorInitialConditions;
+        if (ic[k] == RANDOM_UNIFORM_IC) slot = std::make_unique<RandomUniform>(root);
+        else if (ic[k] == GAUSSIAN_IC && k == 0) slot = std::make_unique<Gaussian>(root, numScalarFieldComponents);
+        else if (ic[k] != ZERO_IC)
+            std::cout << "LATTICE::WARNING: Initial condition type (" << ic[k]
+                      << ") is invalid.\nReverting to default behaviour (initialisation to zero)\n" << std::endl;
+    }
+    r.skip_lines(1);
+    const bool want_fields = r.value<bool>();
+    const bool want_energy = r.value<bool>();
+    const bool want_gauss = r.value<bool>();
+    r.skip_lines(7);
+    int faces[6] = {0, 0, 0, 0, 0, 0};
+    r.values(faces, 6);
+    for (int f = 0; f < 6; f++) {
+        if (faces[f] != FIXED && faces[f] != PERIODIC)
+            throw std::runtime_error("LATTICE:: Face boundary condition (" + std::to_string(f) + ") has unsupported type "
+                                     + std::to_string(faces[f]) + ".");
+        if (faces[f] == FIXED)
+            throw std::runtime_error("LATTICE:: the GPU evolver supports periodic boundaries only");
+    }
+    r.skip_lines(1);
+    progressReport = r.value<bool>();
+    reportFrequency = r.value<unsigned>();
+    if (reportFrequency == 0) reportFrequency = 1;
+    if (want_fields) {
+        auto of = std::make_unique<OutputFields>(root, dataDir, scalarFields, numScalarFieldComponents, vectorFields,
+                                                 numVectorFieldComponents);
+        outputFields = of.get();
+        analysers.push_back(std::move(of));
+    }
+    if (want_energy) analysers.push_back(std::make_unique<Energy>(root, dataDir, dx, dy, dz));
+    if (want_gauss) analysers.push_back(std::make_unique<GaugeCondition>(root, dataDir));
+}
+
+void Lattice::initFields() {
+    const size_t N = static_cast<size_t>(nx) * ny * nz;
+    scalarFields.assign(2 * N * numScalarFieldComponents, 0.f);
+    vectorFields.assign(2 * N * numVectorFieldComponents, 0.f);
+    if (model.isUsingQuaternionRepresentation())   // identity links, Lattice.cpp:369-395
+        for (size_t i = 0; i < 2 * N; i++)
+            for (unsigned d = 0; d < 3; d++) vectorFields[i * numVectorFieldComponents + d * 5 + 1] = 1.f;
+    if (scalarInitialConditions) scalarInitialConditions->setInitialFields(scalarFields);
+    if (vectorInitialConditions) vectorInitialConditions->setInitialFields(vectorFields);
+}
+
+void Lattice::initialAnalysis() {
+    if (outputFields && outputFields->outputInitial) outputFields->outputFields(outputFields->initialAnalysisPath, 0);
+}
+
+void Lattice::evolve() {
+    gft_desc d{};
+    d.form = form;
+    d.nx = nx; d.ny = ny; d.nz = nz;
+    d.dx = dx; d.dy = dy; d.dz = dz; d.dt = dt;
+    model.fill(d);
+    if (!ctx) {
+        if (gft_create(&d, &ctx) != GFT_OK) throw std::runtime_error(gft_last_error(nullptr));
+        if (gft_import_two_slot_f32(ctx, scalarFields.data(), vectorFields.data(), timeIter) != GFT_OK)
+            throw std::runtime_error(gft_last_error(ctx));
+    }
+    unsigned efreq = 0, gfreq = 0;
+    for (auto& a : analysers) {
+        if (a->energyFrequency()) efreq = a->energyFrequency();
+        if (a->gaussFrequency()) gfreq = a->gaussFrequency();
+    }
+    // Chunks end where continual field dumps or progress reports fall.
+    unsigned chunk = 1000;
+    if (outputFields && outputFields->outputContinual) chunk = std::min(chunk, outputFields->outputFrequency);
+    if (progressReport) chunk = std::min(chunk, reportFrequency);
+    if (progressReport) std::cout << "Beginning evolution." << std::endl;
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<gft_obs> rows(chunk);
+    for (unsigned done = 0; done < nt;) {
+        const unsigned n = std::min(chunk - (timeIter % chunk), nt - done);
+        unsigned nrows = 0;
+        if (gft_evolve(ctx, n, efreq, gfreq, rows.data(), n, &nrows) != GFT_OK)
+            throw std::runtime_error(gft_last_error(ctx));
+        for (unsigned k = 0; k < nrows; k++)
+            for (auto& a : analysers) a->timestepAnalysis(rows[k]);
+        done += n;
+        timeIter += n;
+        if (outputFields && outputFields->outputContinual && timeIter % outputFields->outputFrequency == 0) {
+            unsigned it = 0;
+            if (gft_export_two_slot_f32(ctx, scalarFields.data(), vectorFields.data(), &it) != GFT_OK)
+                throw std::runtime_error(gft_last_error(ctx));
+            outputFields->outputFields(outputFields->continualAnalysisPath + "_" + std::to_string(timeIter) + ".dat",
+                                       timeIter);
+        }
+        if (progressReport && timeIter % reportFrequency == 0)
+            std::cout << "Timestep " << timeIter << " completed.\r" << std::flush;
+    }
+    unsigned it = 0;
+    if (gft_export_two_slot_f32(ctx, scalarFields.data(), vectorFields.data(), &it) != GFT_OK)
+        throw std::runtime_error(gft_last_error(ctx));
+    const std::chrono::duration<double> el = std::chrono::steady_clock::now() - t0;
+    lastEvolveSeconds = el.count();
+    if (progressReport) {
+        std::cout << "Timestep " << nt << " completed.\n";
+        std::cout << "Evolution finished in " << lastEvolveSeconds << " seconds." << std::endl;
+    }
+}
+
+void Lattice::finalAnalysis() {
+    if (outputFields && outputFields->outputFinal) outputFields->outputFields(outputFields->finalAnalysisPath, timeIter);
+}
+
+}  // namespace gft

@@ -1,0 +1,85 @@
+"""The C++ host API (gftevolver_amd/host) as a drop-in for the reference program:
+it reads the reference's own positional .cfg files and writes the reference's
+Data/global_quantities*.dat.
+
+CPU: the config reader against values written into the shipped config files.
+GPU: gft_run on the configuration of the golden program run reproduces the
+reference program's energy file (the reference accumulates in float, the GPU
+in FP64, so 2e-5 relative) and a Gauss violation no larger than the
+reference's own float-rounding level."""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from gftevolver_amd._abi import TWO_HDM_POTENTIAL
+from tests import oracle_lib as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GFT_RUN = os.path.join(ROOT, "gftevolver_amd", "gft_run")
+TEMPLATE = os.path.join(ROOT, "oracle", "_ref", "cfg")
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+needs_cfg = pytest.mark.skipif(not (os.path.isdir(TEMPLATE) and os.path.exists(GFT_RUN)),
+                               reason="reference config template or gft_run not built")
+
+
+def _prepare(td, model, n, nt, ns, efreq=3, gfreq=2):
+    import refcfg
+    refcfg.prepare(td)
+    refcfg.write_model(td, model, 0.65, 0.35, pot2=TWO_HDM_POTENTIAL if model == 1 else None)
+    refcfg.write_lattice(td, n, (0.5, 0.5, 0.5), nt, 0.05, ns, ic_seed=7, ic_range=(-0.8, 0.8), efreq=efreq,
+                         gfreq=gfreq)
+    return os.path.join(td, "cfg")
+
+
+@needs_cfg
+@pytest.mark.parametrize("model", [0, 1], ids=["sm", "2hdm"])
+def test_config_reader_matches_values_written(tmp_path, model):
+    ns = 4 if model == 0 else 8
+    root = _prepare(str(tmp_path), model, (6, 5, 7), 12, ns)
+    out = subprocess.run([GFT_RUN, root, str(tmp_path / "data"), "dump_config"], capture_output=True, text=True,
+                         check=True).stdout
+    kv = {l.split()[0]: l.split()[1:] for l in out.strip().splitlines()}
+    assert kv["n"] == ["6", "5", "7"] and float(kv["dt"][0]) == 0.05 and kv["nt"] == ["12"]
+    assert kv["ns"] == [str(ns)] and kv["nv"] == ["15"] and kv["model"] == [str(model)]
+    assert float(kv["g"][0]) == 0.65 and float(kv["gp"][0]) == 0.35
+    if model == 0:
+        assert float(kv["lambda"][0]) == 0.13 and float(kv["eta_sqr"][0]) == 1.0
+    else:
+        p = TWO_HDM_POTENTIAL
+        assert float(kv["m1"][0]) == p["m1"] and float(kv["l3"][0]) == p["l3"]
+        assert float(kv["m12r"][0]) == p["m12r"] and float(kv["l4m5"][0]) == p["l4m5"]
+    # RandomUniform initial state: same generator and sequence as the oracle's restatement.
+    N = 6 * 5 * 7
+    ref = O.random_uniform(2 * N * ns, 7, -0.8, 0.8)
+    assert abs(float(kv["scalar_sum"][0]) - float(ref.astype(np.float64).sum())) < 1e-9
+
+
+@needs_cfg
+def test_config_errors_keep_reference_messages(tmp_path):
+    import refcfg
+    root = _prepare(str(tmp_path), 0, (6, 5, 7), 12, 4)
+    refcfg.set_values(str(tmp_path), "Core", "Model.cfg", [7, 2, 1, 0.0, 0, 0, 0.0])
+    r = subprocess.run([GFT_RUN, root, str(tmp_path / "data"), "dump_config"], capture_output=True, text=True)
+    assert r.returncode != 0 and "The chosen potential type (7) is invalid." in r.stderr
+
+
+@needs_cfg
+@pytest.mark.gpu
+@pytest.mark.parametrize("model,tag", [(0, "sm"), (1, "2hdm")], ids=["sm", "2hdm"])
+def test_gft_run_reproduces_reference_program_output(tmp_path, model, tag):
+    g = np.load(os.path.join(ROOT, "tests", "golden", f"program_{tag}.npz"))
+    ns = 4 if model == 0 else 8
+    root = _prepare(str(tmp_path), model, (6, 5, 7), 12, ns)
+    data = tmp_path / "data"
+    subprocess.run([GFT_RUN, root, str(data)], check=True, capture_output=True, timeout=120)
+    E = np.loadtxt(data / "global_quantities.dat", ndmin=2)
+    G = np.loadtxt(data / "global_quantities_gauge.dat", ndmin=2)
+    assert E.shape == g["energy"].shape and G.shape == g["gauss"].shape
+    np.testing.assert_allclose(E, g["energy"], rtol=2e-5, atol=1e-5 * np.abs(g["energy"]).max())
+    assert (G <= 10 * g["gauss"] + 1e-7).all()
