@@ -49,6 +49,7 @@ struct gft_ctx {
     double* obs_host = nullptr;    // pinned mirror
     unsigned* err_dev = nullptr;
     unsigned time_iter = 0;
+    bool imported = false;
     std::string err;
     // step-kernel timing
     std::vector<hipEvent_t> ev_start, ev_stop;
@@ -238,6 +239,7 @@ static int import_two_slot(gft_ctx* ctx, const Src* scalar, const Src* vec, unsi
     }
     CK(hipStreamSynchronize(ctx->stream));
     ctx->time_iter = time_iter;
+    ctx->imported = true;
     return GFT_OK;
 }
 
@@ -434,8 +436,8 @@ extern "C" int gft_set_launch(gft_ctx* ctx, int bz, int by, int order, int min_w
     if (min_waves != 0 && min_waves != 3 && min_waves != 4)
         return fail(ctx, GFT_EINVAL, "gft_set_launch: min_waves must be 0, 3 or 4");
     if (order < 0 || order > 2) return fail(ctx, GFT_EINVAL, "gft_set_launch: order must be 0, 1 or 2");
-    if (bz < 1 || by < 1 || bz * by != 256 || (bz % 64 && bz != 32 && bz != 16))
-        return fail(ctx, GFT_EINVAL, "gft_set_launch: need bz*by = 256 with bz a multiple of 64, or 32 or 16");
+    if (bz < 1 || by < 1 || bz * by != 256 || (bz & (bz - 1)) || bz < 16)
+        return fail(ctx, GFT_EINVAL, "gft_set_launch: need bz*by = 256 with bz a power of two >= 16");
     ctx->mom.bz = bz;
     ctx->mom.by = by;
     ctx->mom.order = order;
@@ -528,3 +530,208 @@ extern "C" int gft_bench_copy(size_t bytes, int reps, double* avg_ms) {
     if (b) hipFree(b);
     return rc;
 }
+
+// ---- batched ensembles -----------------------------------------------------------
+
+struct gft_ensemble {
+    std::vector<gft_ctx*> members;
+    hipStream_t stream = nullptr;
+    EnsMember* args[2] = {nullptr, nullptr};   // step arguments with buffer 0 / 1 current
+    int4* work = nullptr;
+    long long nwork = 0;
+    int grid = 0, blocks_per_xcd = 0;
+    unsigned* err_dev = nullptr;
+    bool dirty = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int graph_parity = -1;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    double last_ms = 0.0;
+    unsigned last_steps = 0;
+    std::string err;
+};
+
+static int efail(gft_ensemble* e, int code, const std::string& msg) {
+    if (e) e->err = msg; else g_create_error = msg;
+    return code;
+}
+
+#define EK(expr)                                                                                  \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess) return efail(ens, GFT_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+static void ens_drop_graph(gft_ensemble* ens) {
+    if (ens->exec) (void)hipGraphExecDestroy(ens->exec);
+    if (ens->graph) (void)hipGraphDestroy(ens->graph);
+    ens->exec = nullptr;
+    ens->graph = nullptr;
+    ens->graph_parity = -1;
+}
+
+// Uploads the per-member step arguments (both parities) and the work list.
+static int ens_prepare(gft_ensemble* ens) {
+    if (!ens->dirty) return GFT_OK;
+    ens_drop_graph(ens);
+    const size_t nm = ens->members.size();
+    std::vector<EnsMember> host[2];
+    std::vector<int4> work;
+    for (size_t i = 0; i < nm; i++) {
+        gft_ctx* c = ens->members[i];
+        const int saved = c->mom.cur;
+        for (int p = 0; p < 2; p++) {
+            c->mom.cur = p;
+            EnsMember em{};
+            em.a = make_step_args<float>(c->mom, c->mom.ff, c->coef, 0.0, ens->err_dev);
+            em.a.tiles = nullptr;
+            em.c = c->coef;
+            host[p].push_back(em);
+        }
+        c->mom.cur = saved;
+        const std::vector<int4> t = build_tiles((int)i, c->coef.nx, c->coef.ny, c->coef.nz, c->mom.bz, c->mom.by, 0);
+        work.insert(work.end(), t.begin(), t.end());
+    }
+    for (int p = 0; p < 2; p++) {
+        if (!ens->args[p]) EK(hipMalloc(&ens->args[p], nm * sizeof(EnsMember)));
+        EK(hipMemcpyAsync(ens->args[p], host[p].data(), nm * sizeof(EnsMember), hipMemcpyHostToDevice, ens->stream));
+    }
+    if (ens->work) (void)hipFree(ens->work);
+    ens->work = nullptr;
+    EK(hipMalloc(&ens->work, work.size() * sizeof(int4)));
+    EK(hipMemcpyAsync(ens->work, work.data(), work.size() * sizeof(int4), hipMemcpyHostToDevice, ens->stream));
+    EK(hipStreamSynchronize(ens->stream));
+    ens->nwork = (long long)work.size();
+    int cus = 0;
+    EK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0));
+    // Three 256-thread blocks per CU (the kernel's occupancy), a multiple of 8.
+    long long grid = std::min<long long>(ens->nwork, 3LL * cus);
+    grid = std::max<long long>(8, (grid / 8) * 8);
+    ens->grid = (int)grid;
+    ens->blocks_per_xcd = (int)(grid / 8);
+    ens->dirty = false;
+    return GFT_OK;
+}
+
+extern "C" {
+
+int gft_ensemble_create(const gft_desc* descs, int nmembers, gft_ensemble** out) {
+    if (!descs || nmembers < 1 || !out) return efail(nullptr, GFT_EINVAL, "gft_ensemble_create: bad arguments");
+    *out = nullptr;
+    for (int i = 0; i < nmembers; i++)
+        if (descs[i].form != GFT_FORM_MOMENTUM_F32 || descs[i].model != GFT_MODEL_SM)
+            return efail(nullptr, GFT_EINVAL, "gft_ensemble_create: members must be FP32 momentum-form SM lattices");
+        else if (descs[i].damping != 0.f && descs[i].nt_damped > 0)
+            return efail(nullptr, GFT_EINVAL, "gft_ensemble_create: damped members are not supported");
+    gft_ensemble* ens = new gft_ensemble();
+    auto bail = [&](int rc) { gft_ensemble_destroy(ens); return rc; };
+    for (int i = 0; i < nmembers; i++) {
+        gft_ctx* c = nullptr;
+        int rc = gft_create(&descs[i], &c);
+        if (rc) return bail(rc);
+        ens->members.push_back(c);
+    }
+    if (hipStreamCreateWithFlags(&ens->stream, hipStreamNonBlocking) != hipSuccess
+        || hipMalloc(&ens->err_dev, sizeof(unsigned)) != hipSuccess || hipEventCreate(&ens->e0) != hipSuccess
+        || hipEventCreate(&ens->e1) != hipSuccess) {
+        g_create_error = "gft_ensemble_create: HIP resources";
+        return bail(GFT_EHIP);
+    }
+    if (hipMemset(ens->err_dev, 0, sizeof(unsigned)) != hipSuccess) return bail(GFT_EHIP);
+    *out = ens;
+    return GFT_OK;
+}
+
+void gft_ensemble_destroy(gft_ensemble* ens) {
+    if (!ens) return;
+    if (ens->stream) (void)hipStreamSynchronize(ens->stream);
+    ens_drop_graph(ens);
+    for (auto c : ens->members) gft_destroy(c);
+    for (int p = 0; p < 2; p++) if (ens->args[p]) (void)hipFree(ens->args[p]);
+    if (ens->work) (void)hipFree(ens->work);
+    if (ens->err_dev) (void)hipFree(ens->err_dev);
+    if (ens->e0) (void)hipEventDestroy(ens->e0);
+    if (ens->e1) (void)hipEventDestroy(ens->e1);
+    if (ens->stream) (void)hipStreamDestroy(ens->stream);
+    delete ens;
+}
+
+const char* gft_ensemble_last_error(const gft_ensemble* ens) { return ens ? ens->err.c_str() : g_create_error.c_str(); }
+
+int gft_ensemble_member(gft_ensemble* ens, int member, gft_ctx** ctx) {
+    if (!ens || !ctx || member < 0 || member >= (int)ens->members.size())
+        return efail(ens, GFT_EINVAL, "gft_ensemble_member: no such member");
+    *ctx = ens->members[member];
+    return GFT_OK;
+}
+
+int gft_ensemble_set_launch(gft_ensemble* ens, int member, int bz, int by) {
+    gft_ctx* c = nullptr;
+    int rc = gft_ensemble_member(ens, member, &c);
+    if (rc) return rc;
+    rc = gft_set_launch(c, bz, by, c->mom.order, c->mom.min_waves);
+    if (rc) return efail(ens, rc, c->err);
+    ens->dirty = true;
+    return GFT_OK;
+}
+
+int gft_ensemble_evolve(gft_ensemble* ens, unsigned nsteps, int use_graph) {
+    if (!ens) return GFT_EINVAL;
+    const int parity = ens->members[0]->mom.cur;
+    for (auto c : ens->members)
+        if (c->mom.cur != parity || c->time_iter != ens->members[0]->time_iter)
+            return efail(ens, GFT_EINVAL, "gft_ensemble_evolve: members are not at the same step (import them all first)");
+    for (auto c : ens->members) {
+        if (!c->imported) return efail(ens, GFT_EINVAL, "gft_ensemble_evolve: import every member first");
+        if (gft_sync(c) != GFT_OK) return efail(ens, GFT_EHIP, c->err);
+    }
+    int rc = ens_prepare(ens);
+    if (rc) return rc;
+    const dim3 grid(ens->grid), block(256);
+    auto launch = [&](int p) {
+        ens_step<3><<<grid, block, 0, ens->stream>>>(ens->args[p], ens->work, ens->nwork, ens->blocks_per_xcd);
+    };
+    if (use_graph && nsteps >= 2 && ens->graph_parity != parity) {
+        ens_drop_graph(ens);
+        EK(hipStreamBeginCapture(ens->stream, hipStreamCaptureModeThreadLocal));
+        launch(parity);
+        launch(!parity);
+        EK(hipStreamEndCapture(ens->stream, &ens->graph));
+        EK(hipGraphInstantiate(&ens->exec, ens->graph, nullptr, nullptr, 0));
+        ens->graph_parity = parity;
+    }
+    EK(hipEventRecord(ens->e0, ens->stream));
+    unsigned k = 0;
+    if (use_graph && nsteps >= 2)
+        for (; k + 2 <= nsteps; k += 2) EK(hipGraphLaunch(ens->exec, ens->stream));
+    for (; k < nsteps; k++) {
+        launch((parity + k) % 2);
+        EK(hipGetLastError());
+    }
+    EK(hipEventRecord(ens->e1, ens->stream));
+    EK(hipEventSynchronize(ens->e1));
+    float ms = 0.f;
+    EK(hipEventElapsedTime(&ms, ens->e0, ens->e1));
+    ens->last_ms = ms;
+    ens->last_steps = nsteps;
+    for (auto c : ens->members) {
+        c->mom.cur = (parity + nsteps) % 2;
+        if (nsteps) c->mom.prev_valid = 1;
+        c->time_iter += nsteps;
+    }
+    unsigned bad = 0;
+    EK(hipMemcpy(&bad, ens->err_dev, sizeof(unsigned), hipMemcpyDeviceToHost));
+    if (bad) {
+        EK(hipMemset(ens->err_dev, 0, sizeof(unsigned)));
+        return efail(ens, GFT_EDOMAIN, "Panic in evolve: |R| > 1 in a link update (asin/sqrt out of domain)");
+    }
+    return GFT_OK;
+}
+
+int gft_ensemble_last_step_ms(gft_ensemble* ens, double* avg_ms) {
+    if (!ens || !avg_ms || !ens->last_steps) return GFT_EINVAL;
+    *avg_ms = ens->last_ms / ens->last_steps;
+    return GFT_OK;
+}
+
+}  // extern "C"

@@ -65,6 +65,8 @@ struct MomentumState {
     int bz = 64, by = 4;     // tile shape of the step kernel
     int order = 2;           // block order (see decode_block)
     int min_waves = 3;       // occupancy target of the FP32 SM step kernel (0 = unconstrained, 3, 4)
+    int4* tiles = nullptr;   // device tile table of the current launch shape
+    int tiles_key[3] = {0, 0, -1};
 };
 
 // ---- float special functions on the small ranges the evolution produces ----
@@ -222,7 +224,8 @@ struct StepArgs {
     V4* E[3];
     int nx, ny, nz, nbz, nby;
     long long nblocks;
-    int bz, by, order;
+    int bz, by, order, bz_shift;   // bz = 1 << bz_shift
+    const int4* tiles;       // per hardware block: (member, x, y-tile, z-tile), built on the host
     R inv_sp[3];
     R dt2, keep;             // dt^2 and 1 - gamma dt / 2
     R gsq[4];
@@ -236,14 +239,13 @@ struct StepArgs {
 //      bands, and inside a band the order is x (slowest), then y-tile, then
 //      z-tile.  Each XCD then sweeps x-planes of its own band, so the
 //      neighbour planes x-1 and x+1 of a band (tens of rows) stay in its L2.
-__device__ __forceinline__ long long xcd_contiguous(long long b, long long nb) {
+GFT_HD long long xcd_contiguous(long long b, long long nb) {
     const long long per = nb / 8, rem = nb % 8, xcd = b % 8, idx = b / 8;
     return xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
 }
 
 // Decodes the logical tile (x, y-tile, z-tile) of hardware block b.
-__device__ __forceinline__ void decode_block(long long b, long long nb, int order, int nx, int nby, int nbz, int* x,
-                                             int* ty, int* tz) {
+GFT_HD void decode_block(long long b, long long nb, int order, int nx, int nby, int nbz, int* x, int* ty, int* tz) {
     const long long L = order == 0 ? b : xcd_contiguous(b, nb);
     if (order <= 1) {
         *x = static_cast<int>(L % nx);
@@ -258,23 +260,18 @@ __device__ __forceinline__ void decode_block(long long b, long long nb, int orde
     int band = static_cast<int>(L / per_band);
     if (band > nbands - 1) band = nbands - 1;
     const long long loc = L - band * per_band;
-    const int rows = min(br, nby - band * br);         // the last band may be shorter
+    const int rows = (nby - band * br) < br ? (nby - band * br) : br;   // the last band may be shorter
     *tz = static_cast<int>(loc % nbz);
     const long long t = loc / nbz;
     *ty = band * br + static_cast<int>(t % rows);
     *x = static_cast<int>(t / rows);
 }
 
-// MINW: minimum waves per SIMD the register allocation must allow (0: the
-// compiler's choice).  Higher occupancy hides more memory latency but may spill.
-template <class R, int ND, int MINW>
-__global__ void __launch_bounds__(256, MINW > 0 ? MINW : 1) mom_step(StepArgs<R> a, Coef c) {
+// One site-update at (x, y, z): forces, kick and drift (shared by the
+// single-lattice kernel and the ensemble kernel).
+template <class R, int ND>
+__device__ __forceinline__ void mom_site(const StepArgs<R>& a, const Coef& c, int x, int y, int z) {
     using V4 = typename V4T<R>::type;
-    int x, tby, tbz;
-    decode_block(blockIdx.x, a.nblocks, a.order, a.nx, a.nby, a.nbz, &x, &tby, &tbz);
-    const int z = tbz * a.bz + static_cast<int>(threadIdx.x) % a.bz;
-    const int y = tby * a.by + static_cast<int>(threadIdx.x) / a.bz;
-    if (z >= a.nz || y >= a.ny) return;
 
     // 32-bit site offsets (the host limits a lattice to 2^28 sites): addresses
     // become a scalar base plus one 32-bit vector offset, with no 64-bit math.
@@ -424,6 +421,46 @@ __global__ void __launch_bounds__(256, MINW > 0 ? MINW : 1) mom_step(StepArgs<R>
         st_nt(qo, ptr_at(a.q_out[i], s));
     }
     if (bad) atomicOr(a.err, 1u);
+}
+
+// MINW: minimum waves per SIMD the register allocation must allow (0: the
+// compiler's choice).  Higher occupancy hides more memory latency but may spill.
+template <class R, int ND, int MINW>
+__global__ void __launch_bounds__(256, MINW > 0 ? MINW : 1) mom_step(StepArgs<R> a, Coef c) {
+    const int4 t = a.tiles[blockIdx.x];
+    const int z = t.w * a.bz + (static_cast<int>(threadIdx.x) & (a.bz - 1));
+    const int y = t.z * a.by + (static_cast<int>(threadIdx.x) >> a.bz_shift);
+    if (z >= a.nz || y >= a.ny) return;
+    mom_site<R, ND>(a, c, t.y, y, z);
+}
+
+// One member of a batched ensemble: its step arguments and coefficients.
+struct EnsMember {
+    StepArgs<float> a;
+    Coef c;
+};
+
+// One step of every member of an ensemble in one persistent launch.  The work
+// list holds (member, x, y-tile, z-tile) per 256-site tile, members one after
+// another; XCD k (hardware blocks b with b % 8 == k) takes the k-th eighth of
+// the list and its blocks stride through it, so each XCD works on a compact
+// region of a few members.  Every block reaches the end of its range and exits.
+template <int MINW>
+__global__ void __launch_bounds__(256, MINW) ens_step(const EnsMember* __restrict__ members, const int4* __restrict__ work,
+                                                      long long nwork, int blocks_per_xcd) {
+    const long long xcd = blockIdx.x % 8, idx = blockIdx.x / 8;
+    const long long lo = xcd * nwork / 8, hi = (xcd + 1) * nwork / 8;
+    for (long long w = lo + idx; w < hi; w += blocks_per_xcd) {
+        const int4 t = work[w];
+        // Member arguments copied before any store, as a single-lattice launch
+        // holds them in its kernel arguments: the per-site arithmetic then
+        // compiles to the same instructions as mom_step's.
+        const StepArgs<float> a = members[t.x].a;
+        const Coef c = members[t.x].c;
+        const int z = t.w * a.bz + (static_cast<int>(threadIdx.x) & (a.bz - 1));
+        const int y = t.z * a.by + (static_cast<int>(threadIdx.x) >> a.bz_shift);
+        if (z < a.nz && y < a.ny) mom_site<float, 1>(a, c, t.y, y, z);
+    }
 }
 
 // ---- FP64 observables of the momentum form ------------------------------------
@@ -680,7 +717,37 @@ inline int momentum_alloc(MomentumState& m, const Coef& c, bool f64) {
 
 inline void momentum_free(MomentumState& m) {
     if (m.base) (void)hipFree(m.base);
+    if (m.tiles) (void)hipFree(m.tiles);
     m.base = nullptr;
+    m.tiles = nullptr;
+}
+
+// Host-built table mapping hardware block -> (member, x, y-tile, z-tile).
+inline std::vector<int4> build_tiles(int member, int nx, int ny, int nz, int bz, int by, int order) {
+    const int nbz = (nz + bz - 1) / bz, nby = (ny + by - 1) / by;
+    const long long nb = (long long)nbz * nby * nx;
+    std::vector<int4> t(nb);
+    for (long long b = 0; b < nb; b++) {
+        int x, ty, tz;
+        decode_block(b, nb, order, nx, nby, nbz, &x, &ty, &tz);
+        t[b] = make_int4(member, x, ty, tz);
+    }
+    return t;
+}
+
+inline int ensure_tiles(MomentumState& m, const Coef& c, hipStream_t st, std::string& msg) {
+    if (m.tiles && m.tiles_key[0] == m.bz && m.tiles_key[1] == m.by && m.tiles_key[2] == m.order) return GFT_OK;
+    const std::vector<int4> t = build_tiles(0, c.nx, c.ny, c.nz, m.bz, m.by, m.order);
+    if (m.tiles) (void)hipFree(m.tiles);
+    m.tiles = nullptr;
+    if (hipMalloc(&m.tiles, t.size() * sizeof(int4)) != hipSuccess) { msg = "tile table allocation failed"; return GFT_ENOMEM; }
+    if (hipMemcpyAsync(m.tiles, t.data(), t.size() * sizeof(int4), hipMemcpyHostToDevice, st) != hipSuccess
+        || hipStreamSynchronize(st) != hipSuccess) {
+        msg = "tile table upload failed";
+        return GFT_EHIP;
+    }
+    m.tiles_key[0] = m.bz; m.tiles_key[1] = m.by; m.tiles_key[2] = m.order;
+    return GFT_OK;
 }
 
 inline long long mom_nblocks(const MomentumState& m, const Coef& c) {
@@ -838,6 +905,9 @@ StepArgs<R> make_step_args(MomentumState& m, MomFields<R>& f, const Coef& c, dou
     a.nby = (c.ny + m.by - 1) / m.by;
     a.nblocks = (long long)a.nbz * a.nby * c.nx;
     a.order = m.order;
+    a.tiles = m.tiles;
+    a.bz_shift = 0;
+    while ((1 << a.bz_shift) < m.bz) a.bz_shift++;
     a.dt2 = (R)(c.dt * c.dt);
     a.keep = (R)(1.0 - damping_step);
     for (int k = 0; k < 4; k++) a.gsq[k] = (R)c.gsq[k];
@@ -848,6 +918,7 @@ StepArgs<R> make_step_args(MomentumState& m, MomFields<R>& f, const Coef& c, dou
 inline int momentum_step(MomentumState& m, const Coef& c, double damping_step, hipStream_t st, unsigned* err,
                          std::string& msg) {
     if (m.bz * m.by != 256 || m.bz < 1 || m.by < 1) { msg = "momentum step: tile must hold 256 sites"; return GFT_EINVAL; }
+    if (int rc = ensure_tiles(m, c, st, msg)) return rc;
     const long long nb = mom_nblocks(m, c);
     if (m.f64) {
         auto a = make_step_args<double>(m, m.fd, c, damping_step, err);

@@ -39,6 +39,15 @@ _L.gft_sync.argtypes = [C.c_void_p]
 _L.gft_set_launch.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]
 _L.gft_bench_copy.argtypes = [C.c_size_t, C.c_int, _P(C.c_double)]
 _L.gft_last_step_ms.argtypes = [C.c_void_p, _P(C.c_double), _P(C.c_double), _P(C.c_uint)]
+_L.gft_ensemble_create.argtypes = [_P(GftDesc), C.c_int, _P(C.c_void_p)]
+_L.gft_ensemble_destroy.argtypes = [C.c_void_p]
+_L.gft_ensemble_destroy.restype = None
+_L.gft_ensemble_last_error.argtypes = [C.c_void_p]
+_L.gft_ensemble_last_error.restype = C.c_char_p
+_L.gft_ensemble_member.argtypes = [C.c_void_p, C.c_int, _P(C.c_void_p)]
+_L.gft_ensemble_set_launch.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
+_L.gft_ensemble_evolve.argtypes = [C.c_void_p, C.c_uint, C.c_int]
+_L.gft_ensemble_last_step_ms.argtypes = [C.c_void_p, _P(C.c_double)]
 
 
 class GftError(RuntimeError):
@@ -72,11 +81,15 @@ def bench_copy(nbytes: int, reps: int = 5) -> float:
 class Evolver:
     """One lattice on the GPU behind the C ABI."""
 
-    def __init__(self, desc: GftDesc):
+    def __init__(self, desc: GftDesc, _handle=None):
         self.desc = desc
         self.ns = 8 if desc.model == 1 else 4
         self.n = (desc.nx, desc.ny, desc.nz)
         self.nsites = desc.nx * desc.ny * desc.nz
+        self._owned = _handle is None
+        if _handle is not None:
+            self._h = _handle
+            return
         h = C.c_void_p()
         rc = _L.gft_create(C.byref(desc), C.byref(h))
         if rc != 0:
@@ -88,9 +101,9 @@ class Evolver:
             raise GftError(rc, _L.gft_last_error(self._h).decode())
 
     def close(self):
-        if self._h:
+        if self._h and self._owned:
             _L.gft_destroy(self._h)
-            self._h = None
+        self._h = None
 
     def __del__(self):
         try:
@@ -153,5 +166,59 @@ class Evolver:
         return avg.value, tot.value, n.value
 
 
-__all__ = ["Evolver", "GftError", "device_info", "lib", "GFT_FORM_TWO_SLOT_F64", "GFT_FORM_TWO_SLOT_F32",
+class Ensemble:
+    """Independent FP32 momentum-form lattices advanced together by one
+    persistent kernel per step (gft_ensemble_* in include/gft.h).  members[i]
+    is an Evolver view of member i for import, export and observables; it
+    stays valid while the ensemble is open."""
+
+    def __init__(self, descs):
+        arr = (GftDesc * len(descs))(*descs)
+        h = C.c_void_p()
+        rc = _L.gft_ensemble_create(arr, len(descs), C.byref(h))
+        if rc != 0:
+            raise GftError(rc, _L.gft_ensemble_last_error(None).decode())
+        self._h = h
+        self.members = []
+        for i, d in enumerate(descs):
+            m = C.c_void_p()
+            self._check(_L.gft_ensemble_member(self._h, i, C.byref(m)))
+            self.members.append(Evolver(d, _handle=m))
+
+    def _check(self, rc):
+        if rc != 0:
+            raise GftError(rc, _L.gft_ensemble_last_error(self._h).decode())
+
+    def close(self):
+        if self._h:
+            for m in self.members:
+                m._h = None
+            _L.gft_ensemble_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_launch(self, member: int, bz: int, by: int):
+        self._check(_L.gft_ensemble_set_launch(self._h, member, bz, by))
+
+    def evolve(self, nsteps: int, use_graph: bool = True):
+        self._check(_L.gft_ensemble_evolve(self._h, nsteps, int(use_graph)))
+
+    def last_step_ms(self) -> float:
+        ms = C.c_double()
+        self._check(_L.gft_ensemble_last_step_ms(self._h, C.byref(ms)))
+        return ms.value
+
+
+__all__ = ["Evolver", "Ensemble", "GftError", "device_info", "lib", "GFT_FORM_TWO_SLOT_F64", "GFT_FORM_TWO_SLOT_F32",
            "GFT_FORM_MOMENTUM_F32", "GFT_FORM_MOMENTUM_F64"]

@@ -150,6 +150,22 @@ int gft_set_launch(gft_ctx* ctx, int bz, int by, int order, int min_waves);
  * and WRITE_SIZE of this kernel calibrate the same counters of the step kernel. */
 int gft_bench_copy(size_t bytes, int reps, double* avg_ms);
 
+/* Batched ensembles (no reference counterpart: the reference evolves one
+ * lattice per process).  Independent FP32 momentum-form SM lattices of any
+ * sizes advance together: one persistent kernel launch per step over a work
+ * list of 256-site tiles of all members, optionally replayed from a hipGraph
+ * that holds two steps.  Each member is an ordinary context (gft_ensemble_member)
+ * for import, export, gft_observe and gft_wilson_loops; its fields are bitwise
+ * those of the same member evolved alone. */
+typedef struct gft_ensemble gft_ensemble;
+int         gft_ensemble_create(const gft_desc* members, int nmembers, gft_ensemble** out);
+void        gft_ensemble_destroy(gft_ensemble* ens);
+const char* gft_ensemble_last_error(const gft_ensemble* ens);
+int         gft_ensemble_member(gft_ensemble* ens, int member, gft_ctx** ctx);
+int         gft_ensemble_set_launch(gft_ensemble* ens, int member, int bz, int by);   /* tile shape per member */
+int         gft_ensemble_evolve(gft_ensemble* ens, unsigned nsteps, int use_graph);
+int         gft_ensemble_last_step_ms(gft_ensemble* ens, double* avg_ms);
+
 /* Wait for all queued work on the context's stream. */
 int gft_sync(gft_ctx* ctx);
 

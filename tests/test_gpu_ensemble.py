@@ -1,0 +1,90 @@
+"""GPU checks of the batched ensemble (gft_ensemble_*): independent FP32
+momentum-form lattices of different sizes advanced by one persistent kernel
+per step, with and without a captured two-step hipGraph.
+
+Each member must be bitwise the same lattice evolved alone by the same kernel
+(a one-member ensemble with the same tile shape): any difference is a bug in
+the work list, the member table or the parity bookkeeping.  Against the
+single-lattice kernel (gft_evolve) the per-site arithmetic is the same but the
+compiler may contract a different product of a sum into an FMA, so that
+comparison allows a few ulps of the field scale."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from gftevolver_amd._abi import GFT_FORM_MOMENTUM_F32, make_desc
+from tests import lattice_np as L
+
+pytestmark = pytest.mark.gpu
+
+# Ragged sizes: partial z and y tiles, odd extents, one member with nx = 1 row of tiles.
+SIZES = [(8, 12, 16), (16, 16, 16), (12, 8, 20), (5, 7, 33), (24, 24, 24)]
+TILES = [(64, 4), (16, 16), (32, 8), (256, 1), (128, 2)]
+
+
+@pytest.fixture(scope="module")
+def rt():
+    from gftevolver_amd import runtime
+    return runtime
+
+
+def _states():
+    return [L.near_vacuum_state(n, 4, seed=100 + i, amp=0.2, mom=0.02, link_eps=0.1) for i, n in enumerate(SIZES)]
+
+
+def _run_alone(rt, d, s, v, steps, tile, use_graph):
+    with rt.Ensemble([d]) as ens:
+        ens.members[0].import_two_slot(s.astype(np.float32), v.astype(np.float32), 0)
+        ens.set_launch(0, *tile)
+        ens.evolve(steps, use_graph=use_graph)
+        return ens.members[0].export_two_slot(np.float32), ens.members[0].observe()
+
+
+def _run_single_kernel(rt, d, s, v, steps, tile):
+    with rt.Evolver(d) as ev:
+        ev.import_two_slot(s.astype(np.float32), v.astype(np.float32), 0)
+        ev.set_launch(tile[0], tile[1], order=0, min_waves=3)
+        ev.evolve(steps)
+        return ev.export_two_slot(np.float32)
+
+
+@pytest.mark.parametrize("use_graph", [True, False], ids=["graph", "stream"])
+@pytest.mark.parametrize("steps", [7, 10])
+def test_ensemble_members_bitwise_equal_single_runs(rt, use_graph, steps):
+    descs = [make_desc(form=GFT_FORM_MOMENTUM_F32, n=n) for n in SIZES]
+    states = _states()
+    with rt.Ensemble(descs) as ens:
+        for i, (m, (s, v)) in enumerate(zip(ens.members, states)):
+            m.import_two_slot(s.astype(np.float32), v.astype(np.float32), 0)
+            ens.set_launch(i, *TILES[i])
+        # Split into two calls: the second starts from the other parity.
+        ens.evolve(3, use_graph=use_graph)
+        ens.evolve(steps - 3, use_graph=use_graph)
+        got = [m.export_two_slot(np.float32) for m in ens.members]
+        gobs = [m.observe() for m in ens.members]
+    for i, (d, (s, v)) in enumerate(zip(descs, states)):
+        # The alone run uses the other launch mode, so graph and stream are compared too.
+        (rs, rv, rit), robs = _run_alone(rt, d, s, v, steps, TILES[i], not use_graph)
+        gs, gv, git = got[i]
+        print(f"member {i} {SIZES[i]} tile {TILES[i]}: time_iter {git}, "
+              f"max |diff| scalar {np.abs(gs - rs).max():.3e} links {np.abs(gv - rv).max():.3e}")
+        assert git == rit == steps
+        assert np.array_equal(gs, rs) and np.array_equal(gv, rv)
+        assert gobs[i].energy[0] == robs.energy[0]
+        ks, kv, _ = _run_single_kernel(rt, d, s, v, steps, TILES[i])
+        ds, dv = np.abs(gs - ks).max(), np.abs(gv - kv).max()
+        print(f"  vs gft_evolve: scalar {ds:.3e} links {dv:.3e}")
+        assert ds <= 8 * 2.0**-24 * np.abs(ks).max() and dv <= 8 * 2.0**-24 * max(np.abs(kv).max(), 1.0)
+
+
+def test_ensemble_rejects_mixed_steps(rt):
+    descs = [make_desc(form=GFT_FORM_MOMENTUM_F32, n=n) for n in SIZES[:2]]
+    states = _states()[:2]
+    with rt.Ensemble(descs) as ens:
+        ens.members[0].import_two_slot(states[0][0].astype(np.float32), states[0][1].astype(np.float32), 0)
+        with pytest.raises(rt.GftError, match="import every member"):
+            ens.evolve(2)
+        ens.members[1].import_two_slot(states[1][0].astype(np.float32), states[1][1].astype(np.float32), 5)
+        with pytest.raises(rt.GftError, match="same step"):
+            ens.evolve(2)
