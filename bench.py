@@ -102,6 +102,31 @@ def cpu_baseline(nsteps=60):
             "sample": f"CPU restatement (FP64 two-slot), SM {n}^3, {nsteps} steps, one thread ({sec:.2f} s)"}
 
 
+def ensemble_leg(rt, steps):
+    """The 64-member ensemble workload: aggregate FP32 throughput of the
+    persistent graph-replayed launch, and its ensemble-mean observables against
+    the same members run one by one in FP64."""
+    import numpy as np
+    from gftevolver_amd import ensemble as E
+    specs = E.members()
+    sites = sum(int(np.prod(n)) for n, _, _ in specs)
+    every = max(steps // 4, 1)
+    o32, sec, kms, tiles = E.run_fp32(rt, specs, steps, every, log=log)
+    value = sites * steps / sec
+    log(f"ensemble: {len(specs)} members, {sites} sites, {steps} steps in {sec:.3f} s = {value:.3e} site-updates/s "
+        f"({kms:.3f} ms per step by HIP events)")
+    o64 = E.run_fp64_sequential(rt, specs, steps, every)
+    m32, m64 = o32.mean(axis=1), o64.mean(axis=1)
+    diff = np.abs(m32 - m64).max(axis=0)
+    log(f"ensemble means at step {steps}: FP32 {m32[-1]}, FP64 {m64[-1]}, max |diff| over time {diff}")
+    return {"value": value, "unit": "site-updates/s", "members": len(specs), "sites": sites, "steps": steps,
+            "kernel_ms": kms, "achieved_GBps": BYTES_F32 * sites / (kms * 1e-3) / 1e9,
+            "tiles_by_size": {str(k): list(v) for k, v in zip((32, 48, 64, 96), (tiles or {}).values())},
+            "observables": list(E.OBSERVABLES),
+            "mean_fp32_final": m32[-1].tolist(), "mean_fp64_final": m64[-1].tolist(),
+            "max_abs_mean_diff_vs_fp64": diff.tolist(), "measured_every": every}
+
+
 def read_traffic():
     """HBM bytes per step-kernel launch from the committed rocprofv3 PMC summary
     (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of known size)."""
@@ -120,6 +145,8 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--fp64-steps", type=int, default=100)
     ap.add_argument("--twoslot-steps", type=int, default=20)
+    ap.add_argument("--ensemble-steps", type=int, default=2000,
+                    help="steps of the 64-member ensemble workload (0 skips it)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only-f32", action="store_true")
     ap.add_argument("--launch", type=str, default="", help="bz,by,order,min_waves (skips the autotuner)")
@@ -183,6 +210,9 @@ def main():
             "speedup_fp32_vs_fp64_momentum": value / v64,
             "speedup_fp32_vs_fp64_two_slot": value / vts,
         }
+
+    if not args.only_f32 and args.ensemble_steps:
+        extra["ensemble"] = ensemble_leg(rt, args.ensemble_steps)
 
     cpu = None if args.no_cpu else cpu_baseline()
     traffic = read_traffic()

@@ -10,6 +10,7 @@ GFT_FORM_TWO_SLOT_F64 = 0
 GFT_FORM_TWO_SLOT_F32 = 1
 GFT_FORM_MOMENTUM_F32 = 2
 GFT_FORM_MOMENTUM_F64 = 3
+GFT_FORM_MOMENTUM_F32_PLAIN = 4
 
 GFT_OK = 0
 GFT_EINVAL = -1

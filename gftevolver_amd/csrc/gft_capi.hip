@@ -132,7 +132,7 @@ int gft_create(const gft_desc* desc, gft_ctx** out) {
     const gft_desc& d = *desc;
     if (d.model != GFT_MODEL_SM && d.model != GFT_MODEL_2HDM)
         return fail(nullptr, GFT_EINVAL, "MODEL: The chosen model (" + std::to_string(d.model) + ") is invalid.");
-    if (d.form < GFT_FORM_TWO_SLOT_F64 || d.form > GFT_FORM_MOMENTUM_F64)
+    if (d.form < GFT_FORM_TWO_SLOT_F64 || d.form > GFT_FORM_MOMENTUM_F32_PLAIN)
         return fail(nullptr, GFT_EINVAL, "gft_create: invalid form " + std::to_string(d.form));
     if (d.nx < 2 || d.ny < 2 || d.nz < 2)
         return fail(nullptr, GFT_EINVAL, "LATTICE:: periodic lattices need at least 2 sites per direction");
@@ -163,7 +163,7 @@ int gft_create(const gft_desc* desc, gft_ctx** out) {
             return bail(GFT_ENOMEM);
         }
     } else {
-        int rc = momentum_alloc(ctx->mom, ctx->coef, is_f64(d.form));
+        int rc = momentum_alloc(ctx->mom, ctx->coef, is_f64(d.form), d.form == GFT_FORM_MOMENTUM_F32);
         if (rc) { g_create_error = "gft_create: device allocation failed"; return bail(rc); }
     }
     if (hipMalloc(&ctx->obs_dev, 64 * sizeof(double)) != hipSuccess

@@ -52,12 +52,17 @@ struct MomFields {
     R* y[2][3];      // [buffer][direction]
     V4* q[2][3];     // [buffer][direction]
     V4* E[3];        // [direction]
+    // Compensated FP32 storage only: BF16 residuals of (phi, P) per doublet
+    // (phi_lo[4], P_lo[4]) and of E per direction; null otherwise.
+    uint4* lo_s[2];  // [doublet]
+    uint2* lo_E[3];  // [direction]
 };
 
 struct MomentumState {
     void* base = nullptr;
     size_t bytes = 0;
     bool f64 = false;
+    bool cmp = false;        // FP32 storage with BF16 residuals of phi, P and E
     int cur = 0;             // buffer holding (phi, U) at the current time
     int prev_valid = 0;      // buffer !cur holds the state one step earlier
     MomFields<float> ff{};
@@ -137,6 +142,24 @@ __device__ __forceinline__ void st_nt(const double4& v, double4* p) {
     const d4n n = {v.x, v.y, v.z, v.w};
     __builtin_nontemporal_store(n, reinterpret_cast<d4n*>(p));
 }
+typedef unsigned u4n __attribute__((ext_vector_type(4)));
+typedef unsigned u2n __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint4 ld_nt(const uint4* p) {
+    const u4n v = __builtin_nontemporal_load(reinterpret_cast<const u4n*>(p));
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ uint2 ld_nt(const uint2* p) {
+    const u2n v = __builtin_nontemporal_load(reinterpret_cast<const u2n*>(p));
+    return make_uint2(v.x, v.y);
+}
+__device__ __forceinline__ void st_nt(const uint4& v, uint4* p) {
+    const u4n n = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(n, reinterpret_cast<u4n*>(p));
+}
+__device__ __forceinline__ void st_nt(const uint2& v, uint2* p) {
+    const u2n n = {v.x, v.y};
+    __builtin_nontemporal_store(n, reinterpret_cast<u2n*>(p));
+}
 __device__ __forceinline__ void st_nt(float v, float* p) { __builtin_nontemporal_store(v, p); }
 __device__ __forceinline__ void st_nt(double v, double* p) { __builtin_nontemporal_store(v, p); }
 
@@ -166,14 +189,21 @@ struct Qt { R w, x, y, z; };   // quaternion (c0, c1, c2, c3)
 template <class V4, class R>
 __device__ __forceinline__ Qt<R> toq(const V4& v) { return Qt<R>{v.x, v.y, v.z, v.w}; }
 
+// Quaternion products with a fixed operation order (explicit FMAs, contraction
+// off): a product of the same operands rounds the same way in every kernel and
+// at every call site, which the plaquette sharing below relies on.
+__device__ __forceinline__ float fma_r(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_r(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 // a * b
 template <class R>
 __device__ __forceinline__ Qt<R> qmul(const Qt<R>& a, const Qt<R>& b) {
+#pragma clang fp contract(off)
     Qt<R> r;
-    r.w = a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z;
-    r.x = a.w * b.x + a.x * b.w - a.y * b.z + a.z * b.y;
-    r.y = a.w * b.y + a.y * b.w - a.z * b.x + a.x * b.z;
-    r.z = a.w * b.z + a.z * b.w - a.x * b.y + a.y * b.x;
+    r.w = fma_r(a.w, b.w, -fma_r(a.x, b.x, fma_r(a.y, b.y, a.z * b.z)));
+    r.x = fma_r(a.w, b.x, fma_r(a.x, b.w, fma_r(-a.y, b.z, a.z * b.y)));
+    r.y = fma_r(a.w, b.y, fma_r(a.y, b.w, fma_r(-a.z, b.x, a.x * b.z)));
+    r.z = fma_r(a.w, b.z, fma_r(a.z, b.w, fma_r(-a.x, b.y, a.y * b.x)));
     return r;
 }
 template <class R>
@@ -181,30 +211,87 @@ __device__ __forceinline__ Qt<R> qconj(const Qt<R>& a) { return Qt<R>{a.w, -a.x,
 // vector part of a * b
 template <class R>
 __device__ __forceinline__ void qvec(const Qt<R>& a, const Qt<R>& b, R* v) {
-    v[0] = a.w * b.x + a.x * b.w - a.y * b.z + a.z * b.y;
-    v[1] = a.w * b.y + a.y * b.w - a.z * b.x + a.x * b.z;
-    v[2] = a.w * b.z + a.z * b.w - a.x * b.y + a.y * b.x;
+#pragma clang fp contract(off)
+    v[0] = fma_r(a.w, b.x, fma_r(a.x, b.w, fma_r(-a.y, b.z, a.z * b.y)));
+    v[1] = fma_r(a.w, b.y, fma_r(a.y, b.w, fma_r(-a.z, b.x, a.x * b.z)));
+    v[2] = fma_r(a.w, b.z, fma_r(a.z, b.w, fma_r(-a.x, b.y, a.y * b.x)));
+}
+// vec(q v q^dag) for a unit quaternion q and a pure vector v (the product
+// convention of qmul): with u = vec(q) and c = v x u, v + 2 (q.w c + c x u).
+template <class R>
+__device__ __forceinline__ void qrot(const Qt<R>& q, const R* v, R* o) {
+#pragma clang fp contract(off)
+    const R c0 = fma_r(v[1], q.z, -v[2] * q.y);
+    const R c1 = fma_r(v[2], q.x, -v[0] * q.z);
+    const R c2 = fma_r(v[0], q.y, -v[1] * q.x);
+    const R d0 = fma_r(q.w, c0, fma_r(c1, q.z, -c2 * q.y));
+    const R d1 = fma_r(q.w, c1, fma_r(c2, q.x, -c0 * q.z));
+    const R d2 = fma_r(q.w, c2, fma_r(c0, q.y, -c1 * q.x));
+    o[0] = fma_r(R(2), d0, v[0]);
+    o[1] = fma_r(R(2), d1, v[1]);
+    o[2] = fma_r(R(2), d2, v[2]);
 }
 
-// SU(2) action (StandardModel.cpp:94-97) of quaternion u on doublet h.
+// ---- compensated FP32 storage ------------------------------------------------
+// A stored value is hi + lo: hi the FP32 value the stencil reads, lo a BF16
+// residual kept at the owning site only.  Updates are error-free transformations
+// (TwoSum, exact products by FMA), so the only rounding per step is that of the
+// residual to BF16, 2^-9 of an FP32 half-ulp: the state keeps about 32
+// significant bits while the kernel's arithmetic and neighbour traffic stay FP32.
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+typedef float f2v_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ unsigned bf16_pack2(float a, float b) {   // round to nearest even
+    const f2v_t v = {a, b};
+    return __builtin_bit_cast(unsigned, __builtin_convertvector(v, bf16x2_t));
+}
+__device__ __forceinline__ float bf16_lo(unsigned w) { return __uint_as_float(w << 16); }
+__device__ __forceinline__ float bf16_hi(unsigned w) { return __uint_as_float(w & 0xffff0000u); }
+
+// (h, l) <- keep (h + l) + delta; returns the new hi, *r the exact residual.
+__device__ __forceinline__ float cmp_update(float h, float l, float keep, float delta, float* r) {
+#pragma clang fp contract(off)
+    const float p = keep * h;
+    const float pe = __builtin_fmaf(keep, h, -p);
+    const float t = __builtin_fmaf(keep, l, delta) + pe;
+    const float sum = p + t;
+    const float bb = sum - p;
+    *r = (p - (sum - bb)) + (t - bb);
+    return sum;
+}
+// (h1 + l1) + (h2 + l2) with |l1|, |l2| small; returns the hi, *r the residual.
+__device__ __forceinline__ float cmp_add(float h1, float l1, float h2, float l2, float* r) {
+#pragma clang fp contract(off)
+    const float sum = h1 + h2;
+    const float bb = sum - h1;
+    const float e = (h1 - (sum - bb)) + (h2 - bb);
+    const float t = (l1 + l2) + e;
+    const float hi = sum + t;
+    *r = t - (hi - sum);
+    return hi;
+}
+
+// SU(2) action (StandardModel.cpp:94-97) of quaternion u on doublet h, in a
+// fixed operation order like qmul.
 template <class R>
 __device__ __forceinline__ void act(const Qt<R>& u, const R* h, R* o) {
-    o[0] = u.w * h[0] - u.z * h[1] + u.y * h[2] - u.x * h[3];
-    o[1] = u.z * h[0] + u.w * h[1] + u.x * h[2] + u.y * h[3];
-    o[2] = -u.y * h[0] - u.x * h[1] + u.w * h[2] + u.z * h[3];
-    o[3] = u.x * h[0] - u.y * h[1] - u.z * h[2] + u.w * h[3];
+#pragma clang fp contract(off)
+    o[0] = fma_r(u.w, h[0], fma_r(-u.z, h[1], fma_r(u.y, h[2], -u.x * h[3])));
+    o[1] = fma_r(u.z, h[0], fma_r(u.w, h[1], fma_r(u.x, h[2], u.y * h[3])));
+    o[2] = fma_r(-u.y, h[0], fma_r(-u.x, h[1], fma_r(u.w, h[2], u.z * h[3])));
+    o[3] = fma_r(u.x, h[0], fma_r(-u.y, h[1], fma_r(-u.z, h[2], u.w * h[3])));
 }
 
 // T phi = U^W (e^{iy} phi) for one doublet, with (c, s) = (cos y, sin y);
 // conj gives U^W^dag (e^{-iy} phi).
 template <class R>
 __device__ __forceinline__ void transport(const Qt<R>& u, R c, R s, bool conj, const R* h, R* o) {
+#pragma clang fp contract(off)
     const R ss = conj ? -s : s;
     R t[4];
-    t[0] = c * h[0] - ss * h[1];
-    t[1] = c * h[1] + ss * h[0];
-    t[2] = c * h[2] - ss * h[3];
-    t[3] = c * h[3] + ss * h[2];
+    t[0] = fma_r(c, h[0], -ss * h[1]);
+    t[1] = fma_r(c, h[1], ss * h[0]);
+    t[2] = fma_r(c, h[2], -ss * h[3]);
+    t[3] = fma_r(c, h[3], ss * h[2]);
     act(conj ? qconj(u) : u, t, o);
 }
 
@@ -222,6 +309,8 @@ struct StepArgs {
     const V4* q[3];
     V4* q_out[3];
     V4* E[3];
+    uint4* lo_s[2];          // compensated storage: residuals of (phi, P) and E
+    uint2* lo_E[3];
     int nx, ny, nz, nbz, nby;
     long long nblocks;
     int bz, by, order, bz_shift;   // bz = 1 << bz_shift
@@ -269,7 +358,7 @@ GFT_HD void decode_block(long long b, long long nb, int order, int nx, int nby, 
 
 // One site-update at (x, y, z): forces, kick and drift (shared by the
 // single-lattice kernel and the ensemble kernel).
-template <class R, int ND>
+template <class R, int ND, bool CMP>
 __device__ __forceinline__ void mom_site(const StepArgs<R>& a, const Coef& c, int x, int y, int z) {
     using V4 = typename V4T<R>::type;
 
@@ -315,48 +404,72 @@ __device__ __forceinline__ void mom_site(const StepArgs<R>& a, const Coef& c, in
         J[i][0] = J[i][1] = J[i][2] = J[i][3] = R(0);
 #pragma unroll
         for (int k = 0; k < ND; k++) {
-            R fp[4], fm[4], tp[4], tm[4], tx[4];
+            // The covariant differences along link i at x and at x-i,
+            //   d = phi(x+i) - T_i(x) phi(x),  dm = phi(x) - T_i(x-i) phi(x-i),
+            // carry the Laplacian and the current: T^dag phi(x+i) + T phi(x-i) - 2 phi
+            // = T_i(x)^dag d - dm and B(T phi, phi(x+i)) = B(T phi, d) (the
+            // bilinears vanish on equal arguments).  dm is bitwise the d of site x-i,
+            // so the current on a link and the charges it moves at its two ends
+            // are computed from the same small differences, not from O(1) terms.
+            R fp[4], fm[4], tx[4], tm[4], d[4], dmv[4], dpv[4];
             unpack(at(a.phi[k], sp), fp);
             unpack(at(a.phi[k], sm), fm);
-            transport(U[i], cy[i], sy_[i], true, fp, tp);   // U_i(x)^dag phi(x+i)
-            transport(Um, cm, smn, false, fm, tm);          // U_i(x-i) phi(x-i)
-            transport(U[i], cy[i], sy_[i], false, phi + 4 * k, tx);  // U_i(x) phi(x)
+            transport(U[i], cy[i], sy_[i], false, phi + 4 * k, tx);  // T_i(x) phi(x)
+            transport(Um, cm, smn, false, fm, tm);                    // T_i(x-i) phi(x-i)
 #pragma unroll
-            for (int m = 0; m < 4; m++) F[4 * k + m] += a.inv_sp[i] * (tp[m] + tm[m] - R(2) * phi[4 * k + m]);
-            bilinears_add(tx, fp, J[i]);
+            for (int m = 0; m < 4; m++) {
+                d[m] = fp[m] - tx[m];
+                dmv[m] = phi[4 * k + m] - tm[m];
+            }
+            transport(U[i], cy[i], sy_[i], true, d, dpv);             // T_i(x)^dag d
+#pragma unroll
+            for (int m = 0; m < 4; m++) F[4 * k + m] += a.inv_sp[i] * (dpv[m] - dmv[m]);
+            bilinears_add(tx, d, J[i]);
         }
     }
 
     // Magnetic force (staples, WilsonLoops/Source/StandardModel.cpp:324-409).
+    // Each plaquette of the plane (lo, hi), lo < hi, based at corner b enters the
+    // forces on its four links through one rounded quaternion and one rounded
+    // angle, computed in the same operation order wherever it is needed:
+    //   K_b = A_b C_b,  A_b = U_hi(b+lo)^dag U_lo(b+hi),  C_b = U_hi(b) U_lo(b)^dag,
+    //   theta_b = (y_lo(b+hi) - y_lo(b)) + (y_hi(b) - y_hi(b+lo)).
+    // With v = vec(K_b) the four contributions are
+    //   link lo at b:      +vec K_b            link hi at b:      -rot(C_b) v
+    //   link lo at b+hi:   -rot(U_hi(b+lo)) v  link hi at b+lo:   +rot(U_hi(b+lo)) v
+    // (rot(q) v = vec(q v q^dag)), with sin(theta_b) for the U(1) part in the same
+    // pattern; in exact arithmetic these are the reference's staple sums.  Sharing
+    // the rounded K_b and theta_b makes the contributions of one plaquette cancel in
+    // the Gauss law up to the rounding of the rotations, which is relative to the
+    // plaquette's small deviation from 1 instead of to the size of the links.
     R M[3][4];
 #pragma unroll
     for (int i = 0; i < 3; i++) M[i][0] = M[i][1] = M[i][2] = M[i][3] = R(0);
-    // Plaquettes based at x, shared between the two orientations of a plane:
-    //   P1(i,j) = U_j(x+i)^dag U_i(x+j) U_j(x) U_i(x)^dag = A C,
-    //   P1(j,i) = (C A)^dag.
+    // Plaquettes based at x (b = x, lo = i, hi = j).
 #pragma unroll
     for (int i = 0; i < 3; i++) {
 #pragma unroll
         for (int j = i + 1; j < 3; j++) {
             const Qt<R> Ujxpi = toq<V4, R>(at(a.q[j], s + dp[i]));
             const Qt<R> Uixpj = toq<V4, R>(at(a.q[i], s + dp[j]));
-            const R th = -at(a.y[j], s + dp[i]) + at(a.y[i], s + dp[j]) + yv[j] - yv[i];
+            const R th = (at(a.y[i], s + dp[j]) - yv[i]) + (yv[j] - at(a.y[j], s + dp[i]));
             const Qt<R> A = qmul(qconj(Ujxpi), Uixpj);
             const Qt<R> C = qmul(U[j], qconj(U[i]));
-            R v1[3], v2[3];
-            qvec(A, C, v1);
-            qvec(C, A, v2);
+            R v[3], w[3];
+            qvec(A, C, v);
+            qrot(C, v, w);
             const R st = sin_fast(th);
             M[i][0] += a.inv_sp[j] * st;
             M[j][0] -= a.inv_sp[i] * st;
 #pragma unroll
             for (int m = 0; m < 3; m++) {
-                M[i][1 + m] += a.inv_sp[j] * v1[m];
-                M[j][1 + m] -= a.inv_sp[i] * v2[m];
+                M[i][1 + m] += a.inv_sp[j] * v[m];
+                M[j][1 + m] -= a.inv_sp[i] * w[m];
             }
         }
     }
-    // Plaquettes based at x-j:  P2(i,j) = U_i(x) U_j(x-j) U_i(x-j)^dag U_j(x+i-j)^dag.
+    // Plaquettes based at b = x - j, for link i at x: i < j makes x = b + hi
+    // (link lo at b+hi), i > j makes x = b + lo (link hi at b+lo).
 #pragma unroll
     for (int j = 0; j < 3; j++) {
         const unsigned smj = s + dm[j];
@@ -368,13 +481,25 @@ __device__ __forceinline__ void mom_site(const StepArgs<R>& a, const Coef& c, in
             const unsigned sd = s + dp[i] + dm[j];
             const Qt<R> Uim = toq<V4, R>(at(a.q[i], smj));
             const Qt<R> Ujd = toq<V4, R>(at(a.q[j], sd));
-            const R th = -at(a.y[j], sd) - at(a.y[i], smj) + yjm + yv[i];
-            const Qt<R> B = qmul(qmul(Ujm, qconj(Uim)), qconj(Ujd));
-            R v[3];
-            qvec(U[i], B, v);
-            M[i][0] -= a.inv_sp[j] * sin_fast(th);
+            const R yim = at(a.y[i], smj), yjd = at(a.y[j], sd);
+            R v[3], w[3];
+            if (i < j) {
+                // lo = i, hi = j: A_b = U_j(x+i-j)^dag U_i(x), C_b = U_j(x-j) U_i(x-j)^dag.
+                const R th = (yv[i] - yim) + (yjm - yjd);
+                qvec(qmul(qconj(Ujd), U[i]), qmul(Ujm, qconj(Uim)), v);
+                qrot(Ujd, v, w);
+                M[i][0] -= a.inv_sp[j] * sin_fast(th);
 #pragma unroll
-            for (int m = 0; m < 3; m++) M[i][1 + m] -= a.inv_sp[j] * v[m];
+                for (int m = 0; m < 3; m++) M[i][1 + m] -= a.inv_sp[j] * w[m];
+            } else {
+                // lo = j, hi = i: A_b = U_i(x)^dag U_j(x+i-j), C_b = U_i(x-j) U_j(x-j)^dag.
+                const R th = (yjd - yjm) + (yim - yv[i]);
+                qvec(qmul(qconj(U[i]), Ujd), qmul(Uim, qconj(Ujm)), v);
+                qrot(U[i], v, w);
+                M[i][0] += a.inv_sp[j] * sin_fast(th);
+#pragma unroll
+                for (int m = 0; m < 3; m++) M[i][1 + m] += a.inv_sp[j] * w[m];
+            }
         }
     }
 
@@ -384,14 +509,31 @@ __device__ __forceinline__ void mom_site(const StepArgs<R>& a, const Coef& c, in
         R p[4];
         unpack(ld_nt(ptr_at(a.P[k], s)), p);
         V4 pn, fn;
-        pn.x = a.keep * p[0] + a.dt2 * F[4 * k + 0];
-        pn.y = a.keep * p[1] + a.dt2 * F[4 * k + 1];
-        pn.z = a.keep * p[2] + a.dt2 * F[4 * k + 2];
-        pn.w = a.keep * p[3] + a.dt2 * F[4 * k + 3];
-        fn.x = phi[4 * k + 0] + pn.x;
-        fn.y = phi[4 * k + 1] + pn.y;
-        fn.z = phi[4 * k + 2] + pn.z;
-        fn.w = phi[4 * k + 3] + pn.w;
+        if constexpr (CMP) {
+            const uint4 L = ld_nt(ptr_at(a.lo_s[k], s));
+            const float fl[4] = {bf16_lo(L.x), bf16_hi(L.x), bf16_lo(L.y), bf16_hi(L.y)};
+            const float pl[4] = {bf16_lo(L.z), bf16_hi(L.z), bf16_lo(L.w), bf16_hi(L.w)};
+            float ph[4], pr[4], fh[4], fr[4];
+#pragma unroll
+            for (int m = 0; m < 4; m++) ph[m] = cmp_update(p[m], pl[m], a.keep, a.dt2 * F[4 * k + m], &pr[m]);
+            // phi(t+dt) = phi(t) + P(t+dt) with P as stored (its residual rounded).
+            const unsigned pw0 = bf16_pack2(pr[0], pr[1]), pw1 = bf16_pack2(pr[2], pr[3]);
+            const float prr[4] = {bf16_lo(pw0), bf16_hi(pw0), bf16_lo(pw1), bf16_hi(pw1)};
+#pragma unroll
+            for (int m = 0; m < 4; m++) fh[m] = cmp_add(phi[4 * k + m], fl[m], ph[m], prr[m], &fr[m]);
+            pn = make_float4(ph[0], ph[1], ph[2], ph[3]);
+            fn = make_float4(fh[0], fh[1], fh[2], fh[3]);
+            st_nt(make_uint4(bf16_pack2(fr[0], fr[1]), bf16_pack2(fr[2], fr[3]), pw0, pw1), ptr_at(a.lo_s[k], s));
+        } else {
+            pn.x = a.keep * p[0] + a.dt2 * F[4 * k + 0];
+            pn.y = a.keep * p[1] + a.dt2 * F[4 * k + 1];
+            pn.z = a.keep * p[2] + a.dt2 * F[4 * k + 2];
+            pn.w = a.keep * p[3] + a.dt2 * F[4 * k + 3];
+            fn.x = phi[4 * k + 0] + pn.x;
+            fn.y = phi[4 * k + 1] + pn.y;
+            fn.z = phi[4 * k + 2] + pn.z;
+            fn.w = phi[4 * k + 3] + pn.w;
+        }
         st_nt(pn, ptr_at(a.P[k], s));
         st_nt(fn, ptr_at(a.phi_out[k], s));
     }
@@ -401,10 +543,21 @@ __device__ __forceinline__ void mom_site(const StepArgs<R>& a, const Coef& c, in
         R e[4];
         unpack(ld_nt(ptr_at(a.E[i], s)), e);
         V4 en;
-        en.x = a.keep * e[0] + a.dt2 * (M[i][0] + a.gsq[0] * J[i][0]);
-        en.y = a.keep * e[1] + a.dt2 * (M[i][1] + a.gsq[1] * J[i][1]);
-        en.z = a.keep * e[2] + a.dt2 * (M[i][2] + a.gsq[2] * J[i][2]);
-        en.w = a.keep * e[3] + a.dt2 * (M[i][3] + a.gsq[3] * J[i][3]);
+        if constexpr (CMP) {
+            const uint2 L = ld_nt(ptr_at(a.lo_E[i], s));
+            const float el[4] = {bf16_lo(L.x), bf16_hi(L.x), bf16_lo(L.y), bf16_hi(L.y)};
+            float eh[4], er[4];
+#pragma unroll
+            for (int m = 0; m < 4; m++)
+                eh[m] = cmp_update(e[m], el[m], a.keep, a.dt2 * (M[i][m] + a.gsq[m] * J[i][m]), &er[m]);
+            en = make_float4(eh[0], eh[1], eh[2], eh[3]);
+            st_nt(make_uint2(bf16_pack2(er[0], er[1]), bf16_pack2(er[2], er[3])), ptr_at(a.lo_E[i], s));
+        } else {
+            en.x = a.keep * e[0] + a.dt2 * (M[i][0] + a.gsq[0] * J[i][0]);
+            en.y = a.keep * e[1] + a.dt2 * (M[i][1] + a.gsq[1] * J[i][1]);
+            en.z = a.keep * e[2] + a.dt2 * (M[i][2] + a.gsq[2] * J[i][2]);
+            en.w = a.keep * e[3] + a.dt2 * (M[i][3] + a.gsq[3] * J[i][3]);
+        }
         const R e2 = en.y * en.y + en.z * en.z + en.w * en.w;
         if (!(e2 < R(1)) || !(fabs(en.x) < R(1))) bad = 1;
         const R yn = wrap_pi(yv[i] + asin_small(en.x));
@@ -425,13 +578,13 @@ __device__ __forceinline__ void mom_site(const StepArgs<R>& a, const Coef& c, in
 
 // MINW: minimum waves per SIMD the register allocation must allow (0: the
 // compiler's choice).  Higher occupancy hides more memory latency but may spill.
-template <class R, int ND, int MINW>
+template <class R, int ND, int MINW, bool CMP>
 __global__ void __launch_bounds__(256, MINW > 0 ? MINW : 1) mom_step(StepArgs<R> a, Coef c) {
     const int4 t = a.tiles[blockIdx.x];
     const int z = t.w * a.bz + (static_cast<int>(threadIdx.x) & (a.bz - 1));
     const int y = t.z * a.by + (static_cast<int>(threadIdx.x) >> a.bz_shift);
     if (z >= a.nz || y >= a.ny) return;
-    mom_site<R, ND>(a, c, t.y, y, z);
+    mom_site<R, ND, CMP>(a, c, t.y, y, z);
 }
 
 // One member of a batched ensemble: its step arguments and coefficients.
@@ -459,7 +612,7 @@ __global__ void __launch_bounds__(256, MINW) ens_step(const EnsMember* __restric
         const Coef c = members[t.x].c;
         const int z = t.w * a.bz + (static_cast<int>(threadIdx.x) & (a.bz - 1));
         const int y = t.z * a.by + (static_cast<int>(threadIdx.x) >> a.bz_shift);
-        if (z < a.nz && y < a.ny) mom_site<float, 1>(a, c, t.y, y, z);
+        if (z < a.nz && y < a.ny) mom_site<float, 1, true>(a, c, t.y, y, z);
     }
 }
 
@@ -475,6 +628,8 @@ struct ObsArgs {
     const V4* q[3];
     const V4* q_old[3];     // links one step earlier (Gauss only)
     const V4* E[3];
+    const uint4* lo_s[2];   // residuals of compensated storage (null otherwise)
+    const uint2* lo_E[3];
     int nx, ny, nz;
 };
 
@@ -491,6 +646,33 @@ __device__ __forceinline__ void load_d(const V4& v, double* o) {
     o[0] = (double)v.x; o[1] = (double)v.y; o[2] = (double)v.z; o[3] = (double)v.w;
 }
 
+// Full-precision values of the compensated FP32 storage (hi + BF16 residual;
+// exact in FP64), or the stored values when there are no residuals.
+template <class R>
+__device__ __forceinline__ void obs_phi(const ObsArgs<R>& a, int k, long long s, double* o) {
+    load_d(a.phi[k][s], o);
+    if (a.lo_s[k]) {
+        const uint4 L = a.lo_s[k][s];
+        o[0] += bf16_lo(L.x); o[1] += bf16_hi(L.x); o[2] += bf16_lo(L.y); o[3] += bf16_hi(L.y);
+    }
+}
+template <class R>
+__device__ __forceinline__ void obs_P(const ObsArgs<R>& a, int k, long long s, double* o) {
+    load_d(a.P[k][s], o);
+    if (a.lo_s[k]) {
+        const uint4 L = a.lo_s[k][s];
+        o[0] += bf16_lo(L.z); o[1] += bf16_hi(L.z); o[2] += bf16_lo(L.w); o[3] += bf16_hi(L.w);
+    }
+}
+template <class R>
+__device__ __forceinline__ void obs_E(const ObsArgs<R>& a, int i, long long s, double* o) {
+    load_d(a.E[i][s], o);
+    if (a.lo_E[i]) {
+        const uint2 L = a.lo_E[i][s];
+        o[0] += bf16_lo(L.x); o[1] += bf16_hi(L.x); o[2] += bf16_lo(L.y); o[3] += bf16_hi(L.y);
+    }
+}
+
 // 1 - sqrt(1 - x) without cancellation.
 __device__ __forceinline__ double one_minus_sqrt1m(double x) { return x / (1.0 + sqrt(1.0 - x)); }
 
@@ -504,7 +686,7 @@ __global__ void __launch_bounds__(256) mom_energy(ObsArgs<R> a, Coef c, double* 
     double d[5] = {0, 0, 0, 0, 0};
     if (s < c.nsites) {
         double phi[8];
-        for (int k = 0; k < ND; k++) load_d(a.phi[k][s], phi + 4 * k);
+        for (int k = 0; k < ND; k++) obs_phi(a, k, s, phi + 4 * k);
         d[0] = potential_energy(c, phi);
         double ge = 0.0;
         Qt<double> U[3];
@@ -519,7 +701,7 @@ __global__ void __launch_bounds__(256) mom_energy(ObsArgs<R> a, Coef c, double* 
             const long long sp = nb_index(a.nx, a.ny, a.nz, s, i, 1);
             for (int k = 0; k < ND; k++) {
                 double fp[4], tx[4];
-                load_d(a.phi[k][sp], fp);
+                obs_phi(a, k, sp, fp);
                 transport(U[i], cos(yv[i]), sin(yv[i]), false, phi + 4 * k, tx);
                 for (int m = 0; m < 4; m++) {
                     const double dd = (fp[m] - tx[m]) * c.est[i];
@@ -531,7 +713,7 @@ __global__ void __launch_bounds__(256) mom_energy(ObsArgs<R> a, Coef c, double* 
         double ke = 0.0;
         for (int k = 0; k < ND; k++) {
             double p[4];
-            load_d(a.P[k][s], p);
+            obs_P(a, k, s, p);
             for (int m = 0; m < 4; m++) ke += p[m] * p[m];
         }
         d[2] = 0.5 * ke * c.inv_sqr_dt;
@@ -560,7 +742,7 @@ __global__ void __launch_bounds__(256) mom_energy(ObsArgs<R> a, Coef c, double* 
         double ee = 0.0;
         for (int i = 0; i < 3; i++) {
             double e[4];
-            load_d(a.E[i][s], e);
+            obs_E(a, i, s, e);
             const double ea2 = e[1] * e[1] + e[2] * e[2] + e[3] * e[3];
             ee += 4.0 * c.inv_sp[i] * (c.inv_g_sqr * one_minus_sqrt1m(ea2) + c.inv_gp_sqr * one_minus_sqrt1m(e[0] * e[0]));
         }
@@ -587,16 +769,22 @@ __global__ void __launch_bounds__(256) mom_gauss(ObsArgs<R> a, Coef c, double* _
         double rho[4] = {0, 0, 0, 0};
         for (int k = 0; k < ND; k++) {
             double t[4], n[4];
-            load_d(a.phi_old[k][s], t);
-            load_d(a.P[k][s], n);
+            obs_P(a, k, s, n);
+            if (a.lo_s[k]) {
+                // phi(t) = phi(t+dt) - P: exact to the residuals' rounding.
+                obs_phi(a, k, s, t);
+                for (int m = 0; m < 4; m++) t[m] -= n[m];
+            } else {
+                load_d(a.phi_old[k][s], t);
+            }
             bilinears_add(t, n, rho);
         }
         double div[4] = {0, 0, 0, 0}, mag[4] = {0, 0, 0, 0};
         for (int i = 0; i < 3; i++) {
             const long long sm = nb_index(a.nx, a.ny, a.nz, s, i, -1);
             double e[4], em[4], u[4];
-            load_d(a.E[i][s], e);
-            load_d(a.E[i][sm], em);
+            obs_E(a, i, s, e);
+            obs_E(a, i, sm, em);
             load_d(a.q_old[i][s], u);
             const Qt<double> Ut{u[0], u[1], u[2], u[3]};
             const Qt<double> Ev{0.0, e[1], e[2], e[3]};
@@ -631,7 +819,7 @@ __global__ void __launch_bounds__(256) mom_loops(ObsArgs<R> a, Coef c, double* _
     if (s < c.nsites) {
         for (int k = 0; k < ND; k++) {
             double f[4];
-            load_d(a.phi[k][s], f);
+            obs_phi(a, k, s, f);
             d[4] += f[0] * f[0] + f[1] * f[1] + f[2] * f[2] + f[3] * f[3];
         }
         for (int i = 0; i < 3; i++) {
@@ -683,7 +871,7 @@ __global__ void __launch_bounds__(256) mom_loops(ObsArgs<R> a, Coef c, double* _
 // ---- host side ----------------------------------------------------------------
 
 template <class R>
-inline size_t carve(MomFields<R>* f, char* base, long long N) {
+inline size_t carve(MomFields<R>* f, char* base, long long N, bool cmp) {
     using V4 = typename V4T<R>::type;
     size_t off = 0;
     auto take = [&](size_t elems, size_t esz) {
@@ -699,17 +887,20 @@ inline size_t carve(MomFields<R>* f, char* base, long long N) {
     for (int b = 0; b < 2; b++)
         for (int i = 0; i < 3; i++) { char* p = take(N, sizeof(V4)); if (f) f->q[b][i] = reinterpret_cast<V4*>(p); }
     for (int i = 0; i < 3; i++) { char* p = take(N, sizeof(V4)); if (f) f->E[i] = reinterpret_cast<V4*>(p); }
+    for (int k = 0; k < 2; k++) { char* p = cmp ? take(N, sizeof(uint4)) : nullptr; if (f) f->lo_s[k] = reinterpret_cast<uint4*>(p); }
+    for (int i = 0; i < 3; i++) { char* p = cmp ? take(N, sizeof(uint2)) : nullptr; if (f) f->lo_E[i] = reinterpret_cast<uint2*>(p); }
     return off;
 }
 
-inline int momentum_alloc(MomentumState& m, const Coef& c, bool f64) {
+inline int momentum_alloc(MomentumState& m, const Coef& c, bool f64, bool cmp) {
     m.f64 = f64;
+    m.cmp = cmp && !f64;
     const long long N = c.nsites;
     // The second-doublet arrays are carved for the SM too (unused there).
-    m.bytes = f64 ? carve<double>(nullptr, nullptr, N) : carve<float>(nullptr, nullptr, N);
+    m.bytes = f64 ? carve<double>(nullptr, nullptr, N, false) : carve<float>(nullptr, nullptr, N, m.cmp);
     if (hipMalloc(&m.base, m.bytes) != hipSuccess) { m.base = nullptr; return GFT_ENOMEM; }
-    if (f64) carve<double>(&m.fd, static_cast<char*>(m.base), N);
-    else carve<float>(&m.ff, static_cast<char*>(m.base), N);
+    if (f64) carve<double>(&m.fd, static_cast<char*>(m.base), N, false);
+    else carve<float>(&m.ff, static_cast<char*>(m.base), N, m.cmp);
     m.cur = 0;
     m.prev_valid = 0;
     return GFT_OK;
@@ -763,7 +954,23 @@ inline double wrap_host(double y) {
     return y;
 }
 
-// Two-slot (reference layout) -> momentum form, on the host in FP64.
+// BF16 residuals on the host (round to nearest even, as the kernel's conversion).
+inline unsigned bf16_bits_host(double v) {
+    const float f = (float)v;
+    unsigned u;
+    std::memcpy(&u, &f, 4);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return u >> 16;
+}
+inline double bf16_value_host(unsigned h) {
+    const unsigned u = (h & 0xffffu) << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return (double)f;
+}
+
+// Two-slot (reference layout) -> momentum form, on the host in FP64.  With
+// compensated storage each value is split into its FP32 hi and BF16 residual.
 template <class R, class Src>
 int momentum_import_t(MomentumState& m, MomFields<R>& f, const Coef& c, const Src* S, const Src* V,
                       unsigned time_iter, hipStream_t st, std::string& err) {
@@ -771,14 +978,23 @@ int momentum_import_t(MomentumState& m, MomFields<R>& f, const Coef& c, const Sr
     const int Ns = c.ns, nd = c.nd;
     const unsigned now = (time_iter + 1) % 2, past = !now;
     std::vector<R> phi(4 * N * 2, R(0)), P(4 * N * 2, R(0)), y(3 * N), q(12 * N), E(12 * N);
+    const bool cmp = f.lo_s[0] != nullptr;
+    std::vector<uint4> lo_s(cmp ? 2 * N : 0, make_uint4(0, 0, 0, 0));
+    std::vector<uint2> lo_E(cmp ? 3 * N : 0, make_uint2(0, 0));
+    auto pair = [](unsigned a, unsigned b) { return a | (b << 16); };
     for (long long s = 0; s < N; s++) {
-        for (int k = 0; k < nd; k++)
+        for (int k = 0; k < nd; k++) {
+            unsigned lf[4], lp[4];
             for (int m4 = 0; m4 < 4; m4++) {
                 const double a = (double)S[(now * N + s) * Ns + 4 * k + m4];
                 const double b = (double)S[(past * N + s) * Ns + 4 * k + m4];
                 phi[(k * N + s) * 4 + m4] = (R)a;
                 P[(k * N + s) * 4 + m4] = (R)(a - b);
+                lf[m4] = bf16_bits_host(a - (double)(R)a);
+                lp[m4] = bf16_bits_host((a - b) - (double)(R)(a - b));
             }
+            if (cmp) lo_s[k * N + s] = make_uint4(pair(lf[0], lf[1]), pair(lf[2], lf[3]), pair(lp[0], lp[1]), pair(lp[2], lp[3]));
+        }
         for (int i = 0; i < 3; i++) {
             const Src* ln = V + (now * N + s) * 15 + 5 * i;
             const Src* lp = V + (past * N + s) * 15 + 5 * i;
@@ -788,10 +1004,13 @@ int momentum_import_t(MomentumState& m, MomFields<R>& f, const Coef& c, const Sr
             for (int k = 0; k < 4; k++) q[(i * N + s) * 4 + k] = (R)un[k];
             double r[4];
             qmul_full(un, up, r);
-            E[(i * N + s) * 4 + 0] = (R)std::sin((double)ln[0] - (double)lp[0]);
-            E[(i * N + s) * 4 + 1] = (R)r[1];
-            E[(i * N + s) * 4 + 2] = (R)r[2];
-            E[(i * N + s) * 4 + 3] = (R)r[3];
+            const double ev[4] = {std::sin((double)ln[0] - (double)lp[0]), r[1], r[2], r[3]};
+            unsigned le[4];
+            for (int k = 0; k < 4; k++) {
+                E[(i * N + s) * 4 + k] = (R)ev[k];
+                le[k] = bf16_bits_host(ev[k] - (double)(R)ev[k]);
+            }
+            if (cmp) lo_E[i * N + s] = make_uint2(pair(le[0], le[1]), pair(le[2], le[3]));
         }
     }
     m.cur = 0;
@@ -808,7 +1027,10 @@ int momentum_import_t(MomentumState& m, MomFields<R>& f, const Coef& c, const Sr
         ok &= cp(f.y[0][i], &y[i * N], N * sizeof(R));
         ok &= cp(f.q[0][i], &q[i * N * 4], N * 4 * sizeof(R));
         ok &= cp(f.E[i], &E[i * N * 4], N * 4 * sizeof(R));
+        if (cmp) ok &= cp(f.lo_E[i], &lo_E[i * N], N * sizeof(uint2));
     }
+    if (cmp)
+        for (int k = 0; k < nd; k++) ok &= cp(f.lo_s[k], &lo_s[k * N], N * sizeof(uint4));
     ok &= hipStreamSynchronize(st) == hipSuccess;
     if (!ok) { err = "momentum import: copy failed"; return GFT_EHIP; }
     return GFT_OK;
@@ -823,6 +1045,9 @@ int momentum_export_t(MomentumState& m, MomFields<R>& f, const Coef& c, Dst* S, 
     const int Ns = c.ns, nd = c.nd;
     const unsigned now = (time_iter + 1) % 2, past = !now;
     std::vector<R> phi(4 * N * 2), P(4 * N * 2), y(3 * N), q(12 * N), E(12 * N);
+    const bool cmp = f.lo_s[0] != nullptr;
+    std::vector<uint4> lo_s(cmp ? 2 * N : 0);
+    std::vector<uint2> lo_E(cmp ? 3 * N : 0);
     auto cp = [&](void* dst, const void* src, size_t bytes) {
         return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st) == hipSuccess;
     };
@@ -835,22 +1060,38 @@ int momentum_export_t(MomentumState& m, MomFields<R>& f, const Coef& c, Dst* S, 
         ok &= cp(&y[i * N], f.y[m.cur][i], N * sizeof(R));
         ok &= cp(&q[i * N * 4], f.q[m.cur][i], N * 4 * sizeof(R));
         ok &= cp(&E[i * N * 4], f.E[i], N * 4 * sizeof(R));
+        if (cmp) ok &= cp(&lo_E[i * N], f.lo_E[i], N * sizeof(uint2));
     }
+    if (cmp)
+        for (int k = 0; k < nd; k++) ok &= cp(&lo_s[k * N], f.lo_s[k], N * sizeof(uint4));
     ok &= hipStreamSynchronize(st) == hipSuccess;
     if (!ok) { err = "momentum export: copy failed"; return GFT_EHIP; }
+    // Residual m (0..3) of a pair of packed BF16 words.
+    auto res = [](unsigned w0, unsigned w1, int m) {
+        const unsigned w = m < 2 ? w0 : w1;
+        return bf16_value_host((m & 1) ? (w >> 16) : w);
+    };
     for (long long s = 0; s < N; s++) {
         for (int k = 0; k < nd; k++)
             for (int m4 = 0; m4 < 4; m4++) {
-                const double a = (double)phi[(k * N + s) * 4 + m4];
+                double a = (double)phi[(k * N + s) * 4 + m4];
+                double pp = (double)P[(k * N + s) * 4 + m4];
+                if (cmp) {
+                    const uint4 L = lo_s[k * N + s];
+                    a += res(L.x, L.y, m4);
+                    pp += res(L.z, L.w, m4);
+                }
                 S[(now * N + s) * Ns + 4 * k + m4] = (Dst)a;
-                S[(past * N + s) * Ns + 4 * k + m4] = (Dst)(a - (double)P[(k * N + s) * 4 + m4]);
+                S[(past * N + s) * Ns + 4 * k + m4] = (Dst)(a - pp);
             }
         for (int i = 0; i < 3; i++) {
             Dst* ln = V + (now * N + s) * 15 + 5 * i;
             Dst* lp = V + (past * N + s) * 15 + 5 * i;
             const double yy = (double)y[i * N + s];
-            const double e[4] = {(double)E[(i * N + s) * 4], (double)E[(i * N + s) * 4 + 1],
-                                 (double)E[(i * N + s) * 4 + 2], (double)E[(i * N + s) * 4 + 3]};
+            double e[4] = {(double)E[(i * N + s) * 4], (double)E[(i * N + s) * 4 + 1],
+                           (double)E[(i * N + s) * 4 + 2], (double)E[(i * N + s) * 4 + 3]};
+            if (cmp)
+                for (int k = 0; k < 4; k++) e[k] += res(lo_E[i * N + s].x, lo_E[i * N + s].y, k);
             const double un[4] = {(double)q[(i * N + s) * 4], (double)q[(i * N + s) * 4 + 1],
                                   (double)q[(i * N + s) * 4 + 2], (double)q[(i * N + s) * 4 + 3]};
             const double ud[4] = {std::sqrt(std::max(0.0, 1.0 - e[1] * e[1] - e[2] * e[2] - e[3] * e[3])), -e[1], -e[2],
@@ -897,8 +1138,11 @@ StepArgs<R> make_step_args(MomentumState& m, MomFields<R>& f, const Coef& c, dou
         a.q[i] = f.q[cur][i];
         a.q_out[i] = f.q[nxt][i];
         a.E[i] = f.E[i];
+        a.lo_E[i] = f.lo_E[i];
         a.inv_sp[i] = (R)c.inv_sp[i];
     }
+    a.lo_s[0] = f.lo_s[0];
+    a.lo_s[1] = f.lo_s[1];
     a.nx = c.nx; a.ny = c.ny; a.nz = c.nz;
     a.bz = m.bz; a.by = m.by;
     a.nbz = (c.nz + m.bz - 1) / m.bz;
@@ -922,14 +1166,21 @@ inline int momentum_step(MomentumState& m, const Coef& c, double damping_step, h
     const long long nb = mom_nblocks(m, c);
     if (m.f64) {
         auto a = make_step_args<double>(m, m.fd, c, damping_step, err);
-        if (c.nd == 1) mom_step<double, 1, 0><<<nb, 256, 0, st>>>(a, c);
-        else mom_step<double, 2, 0><<<nb, 256, 0, st>>>(a, c);
+        if (c.nd == 1) mom_step<double, 1, 0, false><<<nb, 256, 0, st>>>(a, c);
+        else mom_step<double, 2, 0, false><<<nb, 256, 0, st>>>(a, c);
     } else {
         auto a = make_step_args<float>(m, m.ff, c, damping_step, err);
-        if (c.nd == 2) mom_step<float, 2, 0><<<nb, 256, 0, st>>>(a, c);
-        else if (m.min_waves == 3) mom_step<float, 1, 3><<<nb, 256, 0, st>>>(a, c);
-        else if (m.min_waves == 4) mom_step<float, 1, 4><<<nb, 256, 0, st>>>(a, c);
-        else mom_step<float, 1, 0><<<nb, 256, 0, st>>>(a, c);
+        if (m.cmp) {
+            if (c.nd == 2) mom_step<float, 2, 0, true><<<nb, 256, 0, st>>>(a, c);
+            else if (m.min_waves == 3) mom_step<float, 1, 3, true><<<nb, 256, 0, st>>>(a, c);
+            else if (m.min_waves == 4) mom_step<float, 1, 4, true><<<nb, 256, 0, st>>>(a, c);
+            else mom_step<float, 1, 0, true><<<nb, 256, 0, st>>>(a, c);
+        } else {
+            if (c.nd == 2) mom_step<float, 2, 0, false><<<nb, 256, 0, st>>>(a, c);
+            else if (m.min_waves == 3) mom_step<float, 1, 3, false><<<nb, 256, 0, st>>>(a, c);
+            else if (m.min_waves == 4) mom_step<float, 1, 4, false><<<nb, 256, 0, st>>>(a, c);
+            else mom_step<float, 1, 0, false><<<nb, 256, 0, st>>>(a, c);
+        }
     }
     if (hipGetLastError() != hipSuccess) { msg = "momentum step launch failed"; return GFT_EHIP; }
     m.cur = !m.cur;
@@ -951,7 +1202,10 @@ ObsArgs<R> make_obs_args(const MomentumState& m, const MomFields<R>& f, const Co
         a.q[i] = f.q[cur][i];
         a.q_old[i] = f.q[old][i];
         a.E[i] = f.E[i];
+        a.lo_E[i] = f.lo_E[i];
     }
+    a.lo_s[0] = f.lo_s[0];
+    a.lo_s[1] = f.lo_s[1];
     a.nx = c.nx; a.ny = c.ny; a.nz = c.nz;
     return a;
 }

@@ -1,0 +1,126 @@
+"""Classical-statistical ensembles of SM lattices on one GPU.
+
+The ensemble workload: 64 independent SM lattices of uneven sizes
+(16 x 32^3, 24 x 48^3, 16 x 64^3, 8 x 96^3; 14 450 688 sites in all), each a
+Gaussian random field around the vev with P(k) ~ 1/k up to k_max = pi/(2 dx),
+its own seed and an rms amplitude drawn uniformly from [0.05, 0.2]; couplings
+and spacings of the 256^3 workload.  All members advance together in the
+persistent ensemble kernel (gft_ensemble_*), replayed from a two-step
+hipGraph, with per-member tile shapes picked by the autotuner.
+
+Observables per member, all FP64 reductions on the GPU:
+  phi_sq    <|phi|^2> / v^2                       (gft_wilson_loops)
+  e_frac    electric energy / total energy        (gft_observe)
+  b_frac    magnetic energy / total energy
+  plaq_su2  mean 1/2 tr of the 1x1 SU(2) Wilson loop
+  plaq_u1   mean cos of the 1x1 U(1) Wilson loop
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from ._abi import GFT_FORM_MOMENTUM_F32, GFT_FORM_MOMENTUM_F64, make_desc
+from .ic import grf_state
+
+OBSERVABLES = ("phi_sq", "e_frac", "b_frac", "plaq_su2", "plaq_u1")
+GROUPS = ((16, 32), (24, 48), (16, 64), (8, 96))
+SEED = 20261203
+
+
+def members(groups=GROUPS, seed=SEED, rms_range=(0.05, 0.2)):
+    """[(n, rms, seed)] of the ensemble workload; member i has seed `seed + i`."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for count, L in groups:
+        for _ in range(count):
+            out.append(((L, L, L), float(rng.uniform(*rms_range)), seed + len(out)))
+    return out
+
+
+def member_state(n, rms, seed):
+    return grf_state(n, d=(0.5, 0.5, 0.5), rms=rms, vev=1.0, seed=seed, dtype=np.float32)
+
+
+def observe(ev):
+    """The five ensemble observables of one member (an Evolver or an ensemble member view)."""
+    o = ev.observe()
+    w = ev.wilson_loops()
+    tot = o.energy[0]
+    return np.array([w.phi_sq, o.energy[5] / tot, o.energy[4] / tot, w.w11_su2, w.w11_u1])
+
+
+def tune_tiles(ens, groups_idx, tiles=None, steps=4):
+    """Per-group tile shapes by coordinate descent on the whole ensemble's step
+    time: for each group of equal-size members, try every tile shape with the
+    other groups fixed and keep the fastest.  Every shape gives bitwise the
+    same fields, so tuning advances the state but never changes results.
+    Returns {group: (bz, by)}."""
+    from .tune import TILES
+    tiles = tiles or TILES
+    best = {}
+    for g, idx in enumerate(groups_idx):
+        timing = {}
+        for t in tiles:
+            for i in idx:
+                ens.set_launch(i, *t)
+            ens.evolve(2)
+            ens.evolve(steps)
+            timing[t] = ens.last_step_ms()
+        best[g] = min(timing, key=timing.get)
+        for i in idx:
+            ens.set_launch(i, *best[g])
+    return best
+
+
+def run_fp32(rt, specs, steps, measure_every, tune=True, use_graph=True, log=None):
+    """Advances the whole ensemble `steps` steps from t = 0.  Returns
+    (observables [n_meas][n_members][5], timed seconds, avg kernel ms per step,
+    tiles).  Timing excludes tuning and measurements: only the evolve calls."""
+    descs = [make_desc(form=GFT_FORM_MOMENTUM_F32, n=n) for n, _, _ in specs]
+    states = [member_state(*s) for s in specs]
+    sizes = sorted({n for n, _, _ in specs})
+    groups_idx = [[i for i, s in enumerate(specs) if s[0] == n] for n in sizes]
+    with rt.Ensemble(descs) as ens:
+        tiles = None
+        if tune:
+            # Tune on a throw-away copy of the state, then import afresh.
+            for m, (s, v) in zip(ens.members, states):
+                m.import_two_slot(s, v, 0)
+            tiles = tune_tiles(ens, groups_idx)
+            if log:
+                log(f"ensemble tiles per size {dict(zip(sizes, tiles.values()))}")
+        for m, (s, v) in zip(ens.members, states):
+            m.import_two_slot(s, v, 0)
+        obs = [[observe(m) for m in ens.members]]
+        timed = 0.0
+        kms = []
+        done = 0
+        while done < steps:
+            k = min(measure_every, steps - done)
+            t0 = time.perf_counter()
+            ens.evolve(k, use_graph=use_graph)
+            timed += time.perf_counter() - t0
+            kms.append(ens.last_step_ms())
+            done += k
+            obs.append([observe(m) for m in ens.members])
+    return np.array(obs), timed, float(np.mean(kms)), tiles
+
+
+def run_fp64_sequential(rt, specs, steps, measure_every):
+    """The same members one after another in the FP64 momentum form."""
+    out = []
+    for n, rms, seed in specs:
+        s, v = member_state(n, rms, seed)
+        with rt.Evolver(make_desc(form=GFT_FORM_MOMENTUM_F64, n=n)) as ev:
+            ev.import_two_slot(s.astype(np.float64), v.astype(np.float64), 0)
+            rows = [observe(ev)]
+            done = 0
+            while done < steps:
+                k = min(measure_every, steps - done)
+                ev.evolve(k)
+                done += k
+                rows.append(observe(ev))
+        out.append(rows)
+    return np.array(out).transpose(1, 0, 2)

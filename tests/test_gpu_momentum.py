@@ -19,7 +19,8 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from gftevolver_amd._abi import (GFT_FORM_MOMENTUM_F32, GFT_FORM_MOMENTUM_F64, GFT_FORM_TWO_SLOT_F64,
+from gftevolver_amd._abi import (GFT_FORM_MOMENTUM_F32, GFT_FORM_MOMENTUM_F32_PLAIN, GFT_FORM_MOMENTUM_F64,
+                                 GFT_FORM_TWO_SLOT_F64,
                                  GFT_MODEL_2HDM, GFT_MODEL_SM, TWO_HDM_POTENTIAL, make_desc)
 from tests import lattice_np as L
 from tests import oracle_lib as O
@@ -45,8 +46,8 @@ def _phase_diff(va, vb):
     return d
 
 
-@pytest.mark.parametrize("form,tol", [(GFT_FORM_MOMENTUM_F64, 1e-11), (GFT_FORM_MOMENTUM_F32, 2e-5)],
-                         ids=["f64", "f32"])
+@pytest.mark.parametrize("form,tol", [(GFT_FORM_MOMENTUM_F64, 1e-11), (GFT_FORM_MOMENTUM_F32, 2e-5),
+                                      (GFT_FORM_MOMENTUM_F32_PLAIN, 2e-5)], ids=["f64", "f32", "f32plain"])
 def test_momentum_matches_oracle_32cube_100_steps(rt, form, tol):
     n = (32, 32, 32)
     d = make_desc(form=form, n=n)
@@ -70,7 +71,8 @@ def test_momentum_matches_oracle_32cube_100_steps(rt, form, tol):
         assert err <= etol
 
 
-@pytest.mark.parametrize("form", [GFT_FORM_MOMENTUM_F64, GFT_FORM_MOMENTUM_F32], ids=["f64", "f32"])
+@pytest.mark.parametrize("form", [GFT_FORM_MOMENTUM_F64, GFT_FORM_MOMENTUM_F32, GFT_FORM_MOMENTUM_F32_PLAIN],
+                         ids=["f64", "f32", "f32plain"])
 def test_momentum_2hdm_matches_oracle(rt, form):
     n = (8, 6, 10)
     d = make_desc(model=GFT_MODEL_2HDM, form=form, n=n, pot2=TWO_HDM_POTENTIAL)
@@ -86,11 +88,13 @@ def test_momentum_2hdm_matches_oracle(rt, form):
     assert np.abs(_phase_diff(gv[cur], v[cur])).max() <= tol
 
 
-@pytest.mark.parametrize("form,tol", [(GFT_FORM_MOMENTUM_F64, 1e-12), (GFT_FORM_MOMENTUM_F32, 1e-6)],
-                         ids=["f64", "f32"])
+@pytest.mark.parametrize("form,tol", [(GFT_FORM_MOMENTUM_F64, 1e-12), (GFT_FORM_MOMENTUM_F32, 1e-6),
+                                      (GFT_FORM_MOMENTUM_F32_PLAIN, 1e-6)], ids=["f64", "f32", "f32plain"])
 def test_momentum_gauss_law_conserved(rt, form, tol):
     """From E = 0, P = 0 the discrete Gauss law holds exactly; it must stay at
-    rounding level.  Ragged lattice, unequal spacings, non-trivial links."""
+    rounding level.  Ragged lattice, unequal spacings, non-trivial links: a hot
+    state whose plaquettes are far from 1, so the FP32 rounding of the forces
+    themselves (not of the storage) sets the FP32 level here."""
     n = (12, 10, 14)
     d = make_desc(form=form, n=n, d=(0.5, 0.45, 0.55))
     s, v = L.near_vacuum_state(n, 4, seed=4, amp=0.2, mom=0.0, link_eps=0.2)
@@ -103,9 +107,10 @@ def test_momentum_gauss_law_conserved(rt, form, tol):
     assert max(rel) <= tol
 
 
-def test_momentum_launch_shapes_bitwise_identical(rt):
+@pytest.mark.parametrize("form", [GFT_FORM_MOMENTUM_F32, GFT_FORM_MOMENTUM_F32_PLAIN], ids=["f32", "f32plain"])
+def test_momentum_launch_shapes_bitwise_identical(rt, form):
     n = (16, 12, 20)
-    d = make_desc(form=GFT_FORM_MOMENTUM_F32, n=n)
+    d = make_desc(form=form, n=n)
     s, v = L.near_vacuum_state(n, 4, seed=6, amp=0.1, mom=0.01, link_eps=0.05)
     cfgs = ((64, 4, 2, 3), (64, 4, 0, 3), (64, 4, 1, 3), (256, 1, 2, 3), (128, 2, 2, 3), (32, 8, 1, 3),
             (16, 16, 0, 3), (16, 16, 2, 3), (64, 4, 2, 0), (64, 4, 2, 4))
@@ -135,6 +140,48 @@ def test_momentum_launch_shapes_bitwise_identical(rt):
             if ds > 2 * ulp or dv > 2 * np.spacing(np.float32(1)):
                 bad.append(c)
     assert not bad, bad
+
+
+@pytest.mark.parametrize("form,tol", [(GFT_FORM_MOMENTUM_F32, 2e-9), (GFT_FORM_MOMENTUM_F32_PLAIN, 1.2e-7)],
+                         ids=["f32", "f32plain"])
+def test_momentum_f32_storage_precision(rt, form, tol):
+    """An FP64 state survives import and export to the storage's precision:
+    FP32 + BF16 residual keeps about 32 significant bits (phi, P, E), plain FP32
+    24.  P is phi(t) - phi(t-dt), so the previous slot carries P's precision."""
+    n = (6, 5, 4)
+    d = make_desc(form=form, n=n)
+    s, v = L.near_vacuum_state(n, 4, seed=7, amp=0.2, mom=0.02, link_eps=0.2)
+    with rt.Evolver(d) as ev:
+        ev.import_two_slot(s, v, 0)
+        gs, gv, _ = ev.export_two_slot(np.float64)
+    cur, old = 1, 0
+    e_phi = np.abs(gs[cur] - s[cur]).max() / np.abs(s[cur]).max()
+    e_p = np.abs((gs[cur] - gs[old]) - (s[cur] - s[old])).max() / np.abs(s[cur] - s[old]).max()
+    print(f"form {form}: relative error phi {e_phi:.2e}, P {e_p:.2e}")
+    assert e_phi <= tol and e_p <= tol
+
+
+@pytest.mark.parametrize("form,tol", [(GFT_FORM_MOMENTUM_F32, 6e-7), (GFT_FORM_MOMENTUM_F32_PLAIN, 2.5e-6)],
+                         ids=["f32", "f32plain"])
+def test_momentum_gauss_law_after_1e4_steps(rt, form, tol):
+    """SM 64^3 from a GRF Higgs state (P(k) ~ 1/k, rms 0.1 v, links identity,
+    E = 0) for 10^4 steps: the relative Gauss violation (integrated |G| over the
+    magnitude of the cancelling terms, per generator) stays near 3e-7 with
+    compensated storage (the random walk of the FP32 force rounding) and
+    reaches ~1.2e-6 with plain FP32 storage (the random walk of its storage
+    rounding)."""
+    from gftevolver_amd.ic import grf_state
+    n = (64, 64, 64)
+    s, v = grf_state(n, rms=0.1, seed=20261203, dtype=np.float32)
+    with rt.Evolver(make_desc(form=form, n=n)) as ev:
+        ev.import_two_slot(s, v, 0)
+        rel = []
+        for _ in range(4):
+            ev.evolve(2500)
+            o = ev.observe()
+            rel.append(max(o.gauss_int[m] / o.gauss_scale[m] for m in range(4)))
+    print(f"form {form}: relative Gauss violation every 2500 steps: {['%.2e' % x for x in rel]}")
+    assert max(rel) <= tol
 
 
 def test_momentum_import_export_roundtrip(rt):
