@@ -2,13 +2,16 @@
 """Throughput of the evolver on the 256^3 Standard-Model workload.
 
 One step = one leapfrog update of every site of the lattice (all fields).
-The timed region runs exactly --steps steps of the FP32-bulk momentum-form
-kernel on data already resident in HBM, bracketed by stream synchronisations;
-`value` is site-updates per second over that wall-clock interval.  Beside it:
-the FP64 momentum form (the plain FP64 GPU evolver) and the FP64 two-slot
-reference-layout kernel on the same state, the Gauss-law violation and energy
-drift after the run, the step kernel's roofline position (HIP events on the
-library's own stream) and the reference CPU program timed on a bounded sample.
+The timed region runs exactly --steps steps (default 10^4) of the FP32-bulk
+momentum-form kernel (FP32 arithmetic, phi/P/E stored as FP32 + BF16
+residual) on data already resident in HBM, bracketed by stream
+synchronisations; `value` is site-updates per second over that wall-clock
+interval, and the Gauss-law violation is measured right after it.  Beside it:
+the same kernel with plain FP32 storage, the FP64 momentum form (the plain
+FP64 GPU evolver) and the FP64 two-slot reference-layout kernel on the same
+state, the 64-member ensemble workload, the step kernel's roofline position
+(HIP events on the library's own stream) and the reference CPU program timed
+on a bounded sample.
 
     python bench.py [--steps K] [--warmup W] [--n 256]
 """
@@ -26,7 +29,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "site-updates/s + HBM GB/s, FP32-mixed vs FP64 at Gauss err<=1e-6, 256^3 SM"
-BYTES_F32 = 280   # 35 reals read + 35 written per site-update, 4 B each
+BYTES_F32 = 280   # 35 reals read + 35 written per site-update, 4 B each (algorithmic)
+STORED_F32C = 360  # what the compensated form moves: + BF16 residuals of phi, P, E (20 x 2 B each way)
 BYTES_F64 = 560
 HBM_PEAK_GBS = 8000.0
 
@@ -140,7 +144,8 @@ def read_traffic():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=10000)
+    ap.add_argument("--plain-steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--fp64-steps", type=int, default=100)
@@ -163,8 +168,8 @@ def main():
 
     import numpy as np
     from gftevolver_amd import runtime as rt
-    from gftevolver_amd._abi import (GFT_FORM_MOMENTUM_F32, GFT_FORM_MOMENTUM_F64, GFT_FORM_TWO_SLOT_F64,
-                                     make_desc)
+    from gftevolver_amd._abi import (GFT_FORM_MOMENTUM_F32, GFT_FORM_MOMENTUM_F32_PLAIN, GFT_FORM_MOMENTUM_F64,
+                                     GFT_FORM_TWO_SLOT_F64, make_desc)
     from gftevolver_amd.ic import grf_state
 
     name, cus = rt.device_info()
@@ -188,10 +193,19 @@ def main():
     value = N * args.steps / r32["wall_s"]
     achieved = BYTES_F32 * N / (r32["kernel_ms"] * 1e-3) / 1e9
     log(f"FP32 momentum: {r32['wall_s'] / args.steps * 1e3:.3f} ms/step wall, kernel {r32['kernel_ms']:.3f} ms, "
-        f"{value:.3e} site-updates/s, {achieved:.0f} GB/s algorithmic, Gauss rel {r32['gauss_rel']:.2e}")
+        f"{value:.3e} site-updates/s, {achieved:.0f} GB/s algorithmic, Gauss rel {r32['gauss_rel']:.2e} "
+        f"after {args.steps + args.warmup + 1} steps")
 
     extra = {}
     if not args.only_f32:
+        dpl = make_desc(form=GFT_FORM_MOMENTUM_F32_PLAIN, n=n)
+        rpl = run_form(rt, dpl, s, v, args.plain_steps, args.warmup, launch, tune=not args.no_tune)
+        vpl = N * args.plain_steps / rpl["wall_s"]
+        log(f"FP32 plain storage: kernel {rpl['kernel_ms']:.3f} ms, {vpl:.3e} site-updates/s, "
+            f"Gauss rel {rpl['gauss_rel']:.2e} after {args.plain_steps + args.warmup + 1} steps")
+        extra["fp32_plain_storage"] = {"value": vpl, "kernel_ms": rpl["kernel_ms"], "steps": args.plain_steps,
+                                       "achieved_GBps": BYTES_F32 * N / (rpl["kernel_ms"] * 1e-3) / 1e9,
+                                       "gauss_rel": rpl["gauss_rel"], "launch": rpl.get("launch")}
         d64 = make_desc(form=GFT_FORM_MOMENTUM_F64, n=n)
         r64 = run_form(rt, d64, s, v, args.fp64_steps, 2, tune=not args.no_tune)
         v64 = N * args.fp64_steps / r64["wall_s"]
@@ -201,7 +215,7 @@ def main():
         rts = run_form(rt, dts, s, v, args.twoslot_steps, 2)
         vts = N * args.twoslot_steps / rts["wall_s"]
         log(f"FP64 two-slot: kernel {rts['kernel_ms']:.3f} ms, {vts:.3e} site-updates/s")
-        extra = {
+        extra.update({
             "fp64_momentum": {"value": v64, "kernel_ms": r64["kernel_ms"],
                               "achieved_GBps": BYTES_F64 * N / (r64["kernel_ms"] * 1e-3) / 1e9,
                               "gauss_rel": r64["gauss_rel"],
@@ -209,7 +223,7 @@ def main():
             "fp64_two_slot": {"value": vts, "kernel_ms": rts["kernel_ms"], "gauss_rel": rts["gauss_rel"]},
             "speedup_fp32_vs_fp64_momentum": value / v64,
             "speedup_fp32_vs_fp64_two_slot": value / vts,
-        }
+        })
 
     if not args.only_f32 and args.ensemble_steps:
         extra["ensemble"] = ensemble_leg(rt, args.ensemble_steps)
@@ -230,7 +244,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"SM {args.n}^3 periodic, FP32-bulk momentum-form leapfrog; GRF Higgs IC "
+            "workload": f"SM {args.n}^3 periodic, FP32-bulk momentum-form leapfrog (FP32 arithmetic, phi/P/E "
+                        f"stored FP32 + BF16 residual); GRF Higgs IC "
                         f"(P(k)~1/k, k<=pi/(2dx), rms 0.1v), links identity, E=0; dx=0.5 dt=0.05 g=0.65 "
                         f"g'=0.35 lambda=0.13 v=1",
             "lattice": list(n),
@@ -239,9 +254,11 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_site_update": BYTES_F32,
+                     "stored_bytes_per_site_update": STORED_F32C,
                      "kernel_ms": r32["kernel_ms"], "copy_kernel_GBps": copy_gbps},
         "cpu_baseline": cpu,
         "gauss_rel_violation": r32["gauss_rel"],
+        "gauss_measured_after_steps": args.steps + args.warmup + 1,
         "energy_drift_rel": (r32["e1"] - r32["e0"]) / abs(r32["e0"]),
         **extra,
     }

@@ -383,7 +383,7 @@ extern "C" int gft_evolve(gft_ctx* ctx, unsigned nsteps, unsigned energy_freq, u
             // The momentum form measures the state before the update in its own pass.
             if ((rc = momentum_observe(ctx->mom, ctx->coef, MomObs::Energy, ctx->partial, nblocks, ctx->stream, ctx->err)))
                 return rc;
-            finalize_partials<<<1, 256, 0, ctx->stream>>>(ctx->partial, nblocks, 5, 0, ctx->obs_dev);
+            finalize_partials<<<1, kFinThreads, 0, ctx->stream>>>(ctx->partial, nblocks, 5, 0, ctx->obs_dev);
         }
         if ((rc = timed_begin(ctx))) return rc;
         if (two) {
@@ -400,7 +400,7 @@ extern "C" int gft_evolve(gft_ctx* ctx, unsigned nsteps, unsigned energy_freq, u
             if ((rc = momentum_step(ctx->mom, ctx->coef, damping_step, ctx->stream, ctx->err_dev, ctx->err))) return rc;
         }
         if ((rc = timed_end(ctx))) return rc;
-        if (me && two) finalize_partials<<<1, 256, 0, ctx->stream>>>(ctx->partial, nblocks, 5, 0, ctx->obs_dev);
+        if (me && two) finalize_partials<<<1, kFinThreads, 0, ctx->stream>>>(ctx->partial, nblocks, 5, 0, ctx->obs_dev);
         if (mg) {
             if (ctx->form == GFT_FORM_TWO_SLOT_F64)
                 twoslot_gauss<double><<<nblocks, kBlock, 0, ctx->stream>>>(ctx->coef, (const double*)ctx->S,
@@ -411,7 +411,7 @@ extern "C" int gft_evolve(gft_ctx* ctx, unsigned nsteps, unsigned energy_freq, u
             else if ((rc = momentum_observe(ctx->mom, ctx->coef, MomObs::Gauss, ctx->partial, nblocks, ctx->stream,
                                             ctx->err)))
                 return rc;
-            finalize_partials<<<1, 256, 0, ctx->stream>>>(ctx->partial, nblocks, 8, 4, ctx->obs_dev + 8);
+            finalize_partials<<<1, kFinThreads, 0, ctx->stream>>>(ctx->partial, nblocks, 8, 4, ctx->obs_dev + 8);
             CK(hipGetLastError());
         }
         if ((me || mg) && row < max_rows && rows) {
@@ -462,12 +462,12 @@ extern "C" int gft_observe(gft_ctx* ctx, gft_obs* out) {
     if (rc) return rc;
     if ((rc = momentum_observe(ctx->mom, ctx->coef, MomObs::Energy, ctx->partial, nblocks, ctx->stream, ctx->err)))
         return rc;
-    finalize_partials<<<1, 256, 0, ctx->stream>>>(ctx->partial, nblocks, 5, 0, ctx->obs_dev);
+    finalize_partials<<<1, kFinThreads, 0, ctx->stream>>>(ctx->partial, nblocks, 5, 0, ctx->obs_dev);
     const bool g = ctx->mom.prev_valid;
     if (g) {
         if ((rc = momentum_observe(ctx->mom, ctx->coef, MomObs::Gauss, ctx->partial, nblocks, ctx->stream, ctx->err)))
             return rc;
-        finalize_partials<<<1, 256, 0, ctx->stream>>>(ctx->partial, nblocks, 8, 4, ctx->obs_dev + 8);
+        finalize_partials<<<1, kFinThreads, 0, ctx->stream>>>(ctx->partial, nblocks, 8, 4, ctx->obs_dev + 8);
     }
     CK(hipGetLastError());
     return collect_row(ctx, true, g, out);
@@ -481,7 +481,7 @@ extern "C" int gft_wilson_loops(gft_ctx* ctx, gft_loops* out) {
     if (rc) return rc;
     if ((rc = momentum_observe(ctx->mom, ctx->coef, MomObs::Loops, ctx->partial, nblocks, ctx->stream, ctx->err)))
         return rc;
-    finalize_partials<<<1, 256, 0, ctx->stream>>>(ctx->partial, nblocks, 5, 0, ctx->obs_dev + 24);
+    finalize_partials<<<1, kFinThreads, 0, ctx->stream>>>(ctx->partial, nblocks, 5, 0, ctx->obs_dev + 24);
     CK(hipGetLastError());
     CK(hipMemcpyAsync(ctx->obs_host, ctx->obs_dev, 32 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     CK(hipStreamSynchronize(ctx->stream));
@@ -495,10 +495,12 @@ extern "C" int gft_wilson_loops(gft_ctx* ctx, gft_loops* out) {
 }
 
 // Streaming copy of n float4 elements (grid-stride); traffic = 32 n bytes.
+// One 16-byte element per thread: the simplest form reaches the practical HBM
+// bandwidth (a grid-stride loop over fewer blocks stays ~25% below it).
 __global__ void __launch_bounds__(256) gft_copy_kernel(const float4* __restrict__ src, float4* __restrict__ dst,
                                                        long long n) {
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-        dst[i] = src[i];
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
 }
 
 extern "C" int gft_bench_copy(size_t bytes, int reps, double* avg_ms) {
@@ -513,8 +515,9 @@ extern "C" int gft_bench_copy(size_t bytes, int reps, double* avg_ms) {
                 || hipEventCreate(&e1) != hipSuccess))
         rc = GFT_EHIP;
     if (!rc && hipMemsetAsync(a, 0, n * 16, st) != hipSuccess) rc = GFT_EHIP;
+    if (!rc && n > (long long)INT32_MAX * 256) rc = GFT_EINVAL;
     if (!rc) {
-        const int grid = 256 * 32;
+        const int grid = (int)((n + 255) / 256);
         gft_copy_kernel<<<grid, 256, 0, st>>>(a, b, n);
         hipEventRecord(e0, st);
         for (int r = 0; r < reps; r++) gft_copy_kernel<<<grid, 256, 0, st>>>(a, b, n);

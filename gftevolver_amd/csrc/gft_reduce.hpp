@@ -2,8 +2,8 @@
 //
 // Each block reduces per-thread partial sums (pairwise through wavefront
 // shuffles, then LDS across waves) and writes one partial per quantity; a
-// single-block finisher sums the per-block partials in block order with
-// Neumaier compensation.  The result does not depend on timing or atomics.
+// single-block finisher sums the per-block partials with Neumaier compensation
+// in a fixed order.  The result does not depend on timing or atomics.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -51,34 +51,46 @@ __device__ __forceinline__ void block_reduce_store(const double* sums, const dou
     }
 }
 
-// One block of 256 threads: out[q] = Neumaier sum (q < nsum) or max (q >= nsum)
-// over partial[b * (nsum + nmax) + q], b = 0 .. nblocks-1, in block order per
-// thread stripe, then combined in stripe order.
-__global__ void __launch_bounds__(256) finalize_partials(const double* __restrict__ partial, long long nblocks,
-                                                         int nsum, int nmax, double* __restrict__ out) {
-    __shared__ double s_hi[256], s_lo[256];
+// One block of kFinThreads threads: out[q] = Neumaier sum (q < nsum) or max
+// (q >= nsum) of partial[b * (nsum + nmax) + q] over b = 0 .. nblocks-1.  Thread
+// t takes rows t, t + kFinThreads, ... for every q at once (independent
+// accumulator chains), then thread q combines the per-thread (sum,
+// compensation) pairs in thread order: a fixed order, so the result is
+// deterministic.  nsum + nmax <= kFinWidth.
+constexpr int kFinThreads = 256;
+constexpr int kFinWidth = 12;
+__global__ void __launch_bounds__(kFinThreads) finalize_partials(const double* __restrict__ partial, long long nblocks,
+                                                                 int nsum, int nmax, double* __restrict__ out) {
+    __shared__ double s_hi[kFinWidth * kFinThreads], s_lo[kFinWidth * kFinThreads];
     const int width = nsum + nmax;
-    for (int q = 0; q < width; q++) {
-        Kahan k;
-        k.init();
-        double mx = 0.0;
-        for (long long b = threadIdx.x; b < nblocks; b += blockDim.x) {
-            double v = partial[b * width + q];
-            if (q < nsum) k.add(v); else mx = fmax(mx, v);
+    Kahan k[kFinWidth];
+    double mx[kFinWidth];
+#pragma unroll
+    for (int q = 0; q < kFinWidth; q++) { k[q].init(); mx[q] = 0.0; }
+    for (long long b = threadIdx.x; b < nblocks; b += kFinThreads) {
+        const double* row = partial + b * width;
+#pragma unroll
+        for (int q = 0; q < kFinWidth; q++) {
+            if (q < nsum) k[q].add(row[q]);
+            else if (q < width) mx[q] = fmax(mx[q], row[q]);
         }
-        s_hi[threadIdx.x] = (q < nsum) ? k.s : mx;
-        s_lo[threadIdx.x] = (q < nsum) ? k.c : 0.0;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            Kahan t;
-            t.init();
-            double m = 0.0;
-            for (int i = 0; i < (int)blockDim.x; i++) {
-                if (q < nsum) { t.add(s_hi[i]); t.add(s_lo[i]); } else m = fmax(m, s_hi[i]);
-            }
-            out[q] = (q < nsum) ? t.value() : m;
+    }
+#pragma unroll
+    for (int q = 0; q < kFinWidth; q++) {
+        s_hi[q * kFinThreads + threadIdx.x] = (q < nsum) ? k[q].s : mx[q];
+        s_lo[q * kFinThreads + threadIdx.x] = (q < nsum) ? k[q].c : 0.0;
+    }
+    __syncthreads();
+    if (threadIdx.x < width) {
+        const int q = threadIdx.x;
+        Kahan t;
+        t.init();
+        double m = 0.0;
+        for (int i = 0; i < kFinThreads; i++) {
+            if (q < nsum) { t.add(s_hi[q * kFinThreads + i]); t.add(s_lo[q * kFinThreads + i]); }
+            else m = fmax(m, s_hi[q * kFinThreads + i]);
         }
-        __syncthreads();
+        out[q] = (q < nsum) ? t.value() : m;
     }
 }
 

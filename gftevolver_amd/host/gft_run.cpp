@@ -5,7 +5,8 @@
 // CONFIG_ROOT holds the reference's config tree (<Lib>/Config/*.cfg, default
 // "."); DATA_DIR receives global_quantities*.dat and field dumps (default
 // CONFIG_ROOT/Data); FORM is two_slot_f32 (default: the reference's float
-// storage and operation order), two_slot_f64, momentum_f32 or momentum_f64;
+// storage and operation order), two_slot_f64, momentum_f32, momentum_f32_plain
+// or momentum_f64;
 // FORM = dump_config parses the configuration, prints it and exits (no GPU).
 #include <filesystem>
 #include <iostream>
@@ -42,6 +43,7 @@ int main(int argc, char** argv) {
         if (form_name == "two_slot_f64") form = GFT_FORM_TWO_SLOT_F64;
         else if (form_name == "momentum_f32") form = GFT_FORM_MOMENTUM_F32;
         else if (form_name == "momentum_f64") form = GFT_FORM_MOMENTUM_F64;
+        else if (form_name == "momentum_f32_plain") form = GFT_FORM_MOMENTUM_F32_PLAIN;
         else if (form_name != "two_slot_f32") throw std::runtime_error("unknown form " + form_name);
         const std::filesystem::path dd(data);
         if (std::filesystem::exists(dd) && !std::filesystem::is_empty(dd))
