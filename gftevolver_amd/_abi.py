@@ -78,3 +78,8 @@ def make_desc(model=GFT_MODEL_SM, form=GFT_FORM_TWO_SLOT_F64, n=(8, 8, 8), d=(0.
 TWO_HDM_POTENTIAL = dict(m1=0.25, m2=0.25, l1=0.0325, l2=0.0325, l3=0.05, break_to_son=1, break_to_uhalfn=1,
                          m12r=0.1, l4p5=-0.0125, l6r=0.0, l7r=0.0, m12i=0.0, l4m5=-0.0375, l5i=0.0,
                          l6i=0.0, l7i=0.0, pot_norm=0, pot_offset=0.0)
+
+
+class GftMlpDesc(C.Structure):
+    _fields_ = [("n_in", C.c_int), ("width", C.c_int), ("n_hidden", C.c_int), ("n_heads", C.c_int),
+                ("n_out", C.c_int), ("max_batch", C.c_int)]

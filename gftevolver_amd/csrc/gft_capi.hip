@@ -16,6 +16,7 @@
 #include "gft_reduce.hpp"
 #include "gft_twoslot.hip"
 #include "gft_momentum.hip"
+#include "gft_mlp.hip"
 
 using namespace gft;
 

@@ -12,7 +12,7 @@ import os
 import numpy as np
 
 from ._abi import (GFT_FORM_MOMENTUM_F32, GFT_FORM_MOMENTUM_F64, GFT_FORM_TWO_SLOT_F32, GFT_FORM_TWO_SLOT_F64, NV,
-                   GftDesc, GftLoops, GftObs)
+                   GftDesc, GftLoops, GftMlpDesc, GftObs)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libgft.so")
@@ -48,6 +48,19 @@ _L.gft_ensemble_member.argtypes = [C.c_void_p, C.c_int, _P(C.c_void_p)]
 _L.gft_ensemble_set_launch.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
 _L.gft_ensemble_evolve.argtypes = [C.c_void_p, C.c_uint, C.c_int]
 _L.gft_ensemble_last_step_ms.argtypes = [C.c_void_p, _P(C.c_double)]
+_F = _P(C.c_float)
+_L.gft_mlp_create.argtypes = [_P(GftMlpDesc), _P(C.c_void_p)]
+_L.gft_mlp_destroy.argtypes = [C.c_void_p]
+_L.gft_mlp_destroy.restype = None
+_L.gft_mlp_last_error.argtypes = [C.c_void_p]
+_L.gft_mlp_last_error.restype = C.c_char_p
+_L.gft_mlp_num_params.argtypes = [C.c_void_p]
+_L.gft_mlp_num_params.restype = C.c_longlong
+_L.gft_mlp_set_params.argtypes = [C.c_void_p, _F]
+_L.gft_mlp_get_params.argtypes = [C.c_void_p, _F]
+_L.gft_mlp_forward.argtypes = [C.c_void_p, _F, C.c_int, _F]
+_L.gft_mlp_gradients.argtypes = [C.c_void_p, _F, _F, _F, C.c_int, _F, _F]
+_L.gft_mlp_train.argtypes = [C.c_void_p, _F, _F, _F, C.c_int, C.c_int, C.c_int, C.c_float, C.c_uint, _F]
 
 
 class GftError(RuntimeError):
@@ -220,5 +233,83 @@ class Ensemble:
         return ms.value
 
 
-__all__ = ["Evolver", "Ensemble", "GftError", "device_info", "lib", "GFT_FORM_TWO_SLOT_F64", "GFT_FORM_TWO_SLOT_F32",
+def _fp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class Mlp:
+    """The surrogate network on the GPU (gft_mlp_* in include/gft.h)."""
+
+    def __init__(self, n_in=8, width=256, n_hidden=4, n_heads=4, n_out=256, max_batch=512):
+        self.desc = GftMlpDesc(n_in, width, n_hidden, n_heads, n_out, max_batch)
+        h = C.c_void_p()
+        rc = _L.gft_mlp_create(C.byref(self.desc), C.byref(h))
+        if rc != 0:
+            raise GftError(rc, _L.gft_mlp_last_error(None).decode())
+        self._h = h
+        self.n_params = int(_L.gft_mlp_num_params(h))
+
+    def _check(self, rc):
+        if rc != 0:
+            raise GftError(rc, _L.gft_mlp_last_error(self._h).decode())
+
+    def close(self):
+        if self._h:
+            _L.gft_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def layout(self):
+        """[(name, shape)] of the flat parameter vector, in order."""
+        d = self.desc
+        out = []
+        for l in range(d.n_hidden):
+            out += [(f"W{l}", (d.width, d.n_in if l == 0 else d.width)), (f"b{l}", (d.width,))]
+        out += [("Wh", (d.n_heads * d.n_out, d.width)), ("bh", (d.n_heads * d.n_out,))]
+        return out
+
+    def set_params(self, p: np.ndarray):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.n_params
+        self._check(_L.gft_mlp_set_params(self._h, _fp(p)))
+
+    def get_params(self) -> np.ndarray:
+        p = np.zeros(self.n_params, np.float32)
+        self._check(_L.gft_mlp_get_params(self._h, _fp(p)))
+        return p
+
+    def forward(self, x: np.ndarray) -> np.ndarray:
+        """[n, n_heads, n_out] for x [n, n_in]."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        n = x.shape[0]
+        y = np.zeros((n, self.desc.n_heads, self.desc.n_out), np.float32)
+        self._check(_L.gft_mlp_forward(self._h, _fp(x), n, _fp(y)))
+        return y
+
+    def gradients(self, x, t, w):
+        x, t, w = (np.ascontiguousarray(a, dtype=np.float32) for a in (x, t, w))
+        g = np.zeros(self.n_params, np.float32)
+        loss = np.zeros(1, np.float32)
+        self._check(_L.gft_mlp_gradients(self._h, _fp(x), _fp(t), _fp(w), x.shape[0], _fp(loss), _fp(g)))
+        return float(loss[0]), g
+
+    def train(self, x, t, w, batch=256, epochs=100, lr=1e-3, seed=0):
+        x, t, w = (np.ascontiguousarray(a, dtype=np.float32) for a in (x, t, w))
+        hist = np.zeros(max(epochs, 1), np.float32)
+        self._check(_L.gft_mlp_train(self._h, _fp(x), _fp(t), _fp(w), x.shape[0], batch, epochs, lr, seed, _fp(hist)))
+        return hist[:epochs]
+
+
+__all__ = ["Evolver", "Ensemble", "Mlp", "GftError", "device_info", "lib", "GFT_FORM_TWO_SLOT_F64", "GFT_FORM_TWO_SLOT_F32",
            "GFT_FORM_MOMENTUM_F32", "GFT_FORM_MOMENTUM_F64"]

@@ -169,6 +169,41 @@ int         gft_ensemble_set_launch(gft_ensemble* ens, int member, int bz, int b
 int         gft_ensemble_evolve(gft_ensemble* ens, unsigned nsteps, int use_graph);
 int         gft_ensemble_last_step_ms(gft_ensemble* ens, double* avg_ms);
 
+/* Surrogate network (no reference counterpart): an MLP mapping initial-condition
+ * and coupling parameters to observable time series, BF16 weights and
+ * activations with FP32 accumulation on the MFMA matrix cores, trained on the
+ * GPU.  n_in inputs -> n_hidden layers of `width` with GELU -> n_heads linear
+ * heads of n_out outputs each, sharing the trunk.  Parameters are FP32 in one
+ * flat vector: for each hidden layer l, W_l[width][in_l] (row-major, in_0 =
+ * n_in, else width) then b_l[width]; then W_head[n_heads * n_out][width] and
+ * b_head[n_heads * n_out].  The loss of a batch is
+ *   sum_{s,h,o} w[s][h] (y[s][h][o] - t[s][o])^2 / (n_out * batch * n_heads),
+ * with per-head sample weights w (bootstrap resampling of the heads). */
+typedef struct gft_mlp_desc {
+    int n_in;        /* inputs (8)                                     */
+    int width;       /* hidden width, a multiple of 16 (256)            */
+    int n_hidden;    /* hidden layers (4)                               */
+    int n_heads;     /* output heads (4)                                */
+    int n_out;       /* outputs per head, a multiple of 16 (256)        */
+    int max_batch;   /* largest batch, a multiple of 16                 */
+} gft_mlp_desc;
+typedef struct gft_mlp gft_mlp;
+int         gft_mlp_create(const gft_mlp_desc* desc, gft_mlp** out);
+void        gft_mlp_destroy(gft_mlp* mlp);
+const char* gft_mlp_last_error(const gft_mlp* mlp);   /* mlp may be NULL: last create error */
+long long   gft_mlp_num_params(const gft_mlp* mlp);
+int         gft_mlp_set_params(gft_mlp* mlp, const float* params);   /* also resets the Adam moments */
+int         gft_mlp_get_params(gft_mlp* mlp, float* params);
+/* y[s][head][o] for x[s][n_in], s < n (any n). */
+int         gft_mlp_forward(gft_mlp* mlp, const float* x, int n, float* y);
+/* Loss and its gradient (flat, parameter order) for one batch (multiple of 16). */
+int         gft_mlp_gradients(gft_mlp* mlp, const float* x, const float* t, const float* w, int batch, float* loss,
+                              float* grad);
+/* Adam (beta 0.9, 0.999) on n samples, seeded minibatch shuffle per epoch;
+ * epoch_loss[e] = mean minibatch loss of epoch e (may be NULL). */
+int         gft_mlp_train(gft_mlp* mlp, const float* x, const float* t, const float* w, int n, int batch, int epochs,
+                          float lr, unsigned seed, float* epoch_loss);
+
 /* Wait for all queued work on the context's stream. */
 int gft_sync(gft_ctx* ctx);
 
