@@ -47,7 +47,7 @@ def grf_component(n, d, rng, slope=-1.0, kmax=None):
     amp = np.zeros_like(kk)
     m = (kk > 0) & (kk <= kmax)
     amp[m] = kk[m] ** (slope / 2.0)
-    f = np.fft.irfftn(wk * amp, s=n)
+    f = np.fft.irfftn(wk * amp, s=n, axes=(0, 1, 2))
     f -= f.mean()
     return f / np.sqrt((f ** 2).mean())
 
