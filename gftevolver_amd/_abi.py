@@ -83,3 +83,8 @@ TWO_HDM_POTENTIAL = dict(m1=0.25, m2=0.25, l1=0.0325, l2=0.0325, l3=0.05, break_
 class GftMlpDesc(C.Structure):
     _fields_ = [("n_in", C.c_int), ("width", C.c_int), ("n_hidden", C.c_int), ("n_heads", C.c_int),
                 ("n_out", C.c_int), ("max_batch", C.c_int)]
+
+
+class GftGrfDesc(C.Structure):
+    _fields_ = [("seed", C.c_ulonglong), ("rms", C.c_double), ("slope", C.c_double), ("kmax", C.c_double),
+                ("vev", C.c_double * 2)]

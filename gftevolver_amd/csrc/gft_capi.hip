@@ -17,6 +17,7 @@
 #include "gft_twoslot.hip"
 #include "gft_momentum.hip"
 #include "gft_mlp.hip"
+#include "gft_grf.hip"
 
 using namespace gft;
 
@@ -739,3 +740,148 @@ int gft_ensemble_last_step_ms(gft_ensemble* ens, double* avg_ms) {
 }
 
 }  // extern "C"
+
+// ---- Gaussian-random-field initial states on the GPU ----------------------------
+
+namespace gft {
+
+__global__ void __launch_bounds__(256) grf_sumsq4(const double* __restrict__ f, long long n, double* __restrict__ partial) {
+    __shared__ double lds[4 * 1];
+    const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double v[1] = {0.0};
+    if (s < n)
+        for (int c = 0; c < 4; c++) v[0] += f[c * n + s] * f[c * n + s];
+    block_reduce_store<1, 0>(v, nullptr, lds, partial, blockIdx.x);
+}
+
+__device__ inline unsigned grf_bf16_bits(double v) {   // round to nearest even, as the host import does
+    const float fv = (float)v;
+    unsigned u = __float_as_uint(fv);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return u >> 16;
+}
+
+// phi_k = (0, 0, vev_k, 0) + scale_k * comps, with scale_k = rms / sqrt(<sum_c f_c^2>)
+// from sq[k] (sum over sites); P = 0, E = 0, links identity.
+template <class R>
+__global__ void __launch_bounds__(256) grf_assemble_mom(MomFields<R> f, const double* __restrict__ comps, long long n,
+                                                        int nd, double rms, double vev0, double vev1,
+                                                        const double* __restrict__ sq) {
+    using V4 = typename V4T<R>::type;
+    const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    for (int k = 0; k < nd; k++) {
+        const double scale = rms / sqrt(sq[k] / (double)n);
+        double v[4];
+        for (int m = 0; m < 4; m++) v[m] = (m == 2 ? (k ? vev1 : vev0) : 0.0) + scale * comps[(4 * k + m) * n + s];
+        V4 hi;
+        hi.x = (R)v[0]; hi.y = (R)v[1]; hi.z = (R)v[2]; hi.w = (R)v[3];
+        f.phi[0][k][s] = hi;
+        V4 z;
+        z.x = z.y = z.z = z.w = R(0);
+        f.P[k][s] = z;
+        if (f.lo_s[k]) {
+            unsigned b[4];
+            for (int m = 0; m < 4; m++) b[m] = grf_bf16_bits(v[m] - (double)((&hi.x)[m]));
+            f.lo_s[k][s] = make_uint4(b[0] | (b[1] << 16), b[2] | (b[3] << 16), 0u, 0u);
+        }
+    }
+    for (int i = 0; i < 3; i++) {
+        V4 e, q;
+        e.x = e.y = e.z = e.w = R(0);
+        q.x = R(1); q.y = q.z = q.w = R(0);
+        f.E[i][s] = e;
+        f.q[0][i][s] = q;
+        f.y[0][i][s] = R(0);
+        if (f.lo_E[i]) f.lo_E[i][s] = make_uint2(0u, 0u);
+    }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) grf_assemble_two_slot(T* __restrict__ S, T* __restrict__ V,
+                                                             const double* __restrict__ comps, long long n, int nd,
+                                                             double rms, double vev0, double vev1,
+                                                             const double* __restrict__ sq) {
+    const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int ns = 4 * nd;
+    for (int k = 0; k < nd; k++) {
+        const double scale = rms / sqrt(sq[k] / (double)n);
+        for (int m = 0; m < 4; m++) {
+            const T v = (T)((m == 2 ? (k ? vev1 : vev0) : 0.0) + scale * comps[(4 * k + m) * n + s]);
+            S[s * ns + 4 * k + m] = v;
+            S[(n + s) * ns + 4 * k + m] = v;
+        }
+    }
+    for (int slot = 0; slot < 2; slot++)
+        for (int i = 0; i < 3; i++)
+            for (int c = 0; c < 5; c++) V[(slot * n + s) * 15 + 5 * i + c] = (T)(c == 1 ? 1.0 : 0.0);
+}
+
+}  // namespace gft
+
+extern "C" int gft_init_grf(gft_ctx* ctx, const gft_grf_desc* g) {
+    if (!ctx || !g) return GFT_EINVAL;
+    const Coef& c = ctx->coef;
+    FftAxis ax[3];
+    if (!grf_plan(c.nz, ax[0]) || !grf_plan(c.ny, ax[1]) || !grf_plan(c.nx, ax[2]))
+        return fail(ctx, GFT_EINVAL, "gft_init_grf: every side must be 2^a 3^b and at most 256");
+    if (!(g->rms >= 0.0)) return fail(ctx, GFT_EINVAL, "gft_init_grf: rms must be non-negative");
+    const long long N = ctx->N;
+    ax[0].es = 1; ax[0].inner = 1; ax[0].outer = c.nz; ax[0].inner_stride = 0;                 // z lines
+    ax[1].es = c.nz; ax[1].inner = c.nz; ax[1].outer = (long long)c.ny * c.nz; ax[1].inner_stride = 1;   // y lines
+    ax[2].es = (long long)c.ny * c.nz; ax[2].inner = (long long)c.ny * c.nz; ax[2].outer = 0; ax[2].inner_stride = 1;  // x lines
+    const long long lines[3] = {(long long)c.nx * c.ny, (long long)c.nx * c.nz, (long long)c.ny * c.nz};
+    const gft_desc& dd = ctx->desc;
+    const double kmax = g->kmax > 0 ? g->kmax : 3.14159265358979323846 / (2.0 * dd.dx);
+    const int nc = 4 * c.nd;
+    const long long nb = (N + 255) / 256;
+    int rc = ensure_partial(ctx, (size_t)nb * 2);
+    if (rc) return rc;
+    double2* w = nullptr;
+    double* comps = nullptr;
+    auto cleanup = [&]() {
+        if (w) (void)hipFree(w);
+        if (comps) (void)hipFree(comps);
+    };
+    if (hipMalloc(&w, N * sizeof(double2)) != hipSuccess || hipMalloc(&comps, nc * N * sizeof(double)) != hipSuccess) {
+        cleanup();
+        return fail(ctx, GFT_ENOMEM, "gft_init_grf: work arrays");
+    }
+    hipStream_t st = ctx->stream;
+    for (int comp = 0; comp < nc; comp++) {
+        grf_noise<<<nb, 256, 0, st>>>(w, N, g->seed, (unsigned)comp);
+        for (int a = 0; a < 3; a++) grf_fft_lines<<<(unsigned)lines[a], 256, 0, st>>>(w, ax[a], -1);
+        grf_filter<<<nb, 256, 0, st>>>(w, c.nx, c.ny, c.nz, dd.dx, dd.dy, dd.dz, g->slope, kmax);
+        for (int a = 0; a < 3; a++) grf_fft_lines<<<(unsigned)lines[a], 256, 0, st>>>(w, ax[a], +1);
+        grf_real_stats<<<nb, 256, 0, st>>>(w, comps + comp * N, N, ctx->partial);
+        finalize_partials<<<1, kFinThreads, 0, st>>>(ctx->partial, nb, 2, 0, ctx->obs_dev);
+        grf_normalise<<<nb, 256, 0, st>>>(comps + comp * N, N, ctx->obs_dev, ctx->partial);
+    }
+    for (int k = 0; k < c.nd; k++) {
+        grf_sumsq4<<<nb, 256, 0, st>>>(comps + 4 * k * N, N, ctx->partial);
+        finalize_partials<<<1, kFinThreads, 0, st>>>(ctx->partial, nb, 1, 0, ctx->obs_dev + 2 + k);
+    }
+    const double vev0 = g->vev[0], vev1 = g->vev[1];
+    if (is_two_slot(ctx->form)) {
+        if (is_f64(ctx->form))
+            grf_assemble_two_slot<double><<<nb, 256, 0, st>>>((double*)ctx->S, (double*)ctx->V, comps, N, c.nd, g->rms,
+                                                                vev0, vev1, ctx->obs_dev + 2);
+        else
+            grf_assemble_two_slot<float><<<nb, 256, 0, st>>>((float*)ctx->S, (float*)ctx->V, comps, N, c.nd, g->rms,
+                                                               vev0, vev1, ctx->obs_dev + 2);
+    } else {
+        ctx->mom.cur = 0;
+        ctx->mom.prev_valid = 0;
+        if (ctx->mom.f64)
+            grf_assemble_mom<double><<<nb, 256, 0, st>>>(ctx->mom.fd, comps, N, c.nd, g->rms, vev0, vev1, ctx->obs_dev + 2);
+        else
+            grf_assemble_mom<float><<<nb, 256, 0, st>>>(ctx->mom.ff, comps, N, c.nd, g->rms, vev0, vev1, ctx->obs_dev + 2);
+    }
+    const bool ok = hipGetLastError() == hipSuccess && hipStreamSynchronize(st) == hipSuccess;
+    cleanup();
+    if (!ok) return fail(ctx, GFT_EHIP, "gft_init_grf: kernel failure");
+    ctx->time_iter = 0;
+    ctx->imported = true;
+    return GFT_OK;
+}

@@ -12,7 +12,7 @@ import os
 import numpy as np
 
 from ._abi import (GFT_FORM_MOMENTUM_F32, GFT_FORM_MOMENTUM_F64, GFT_FORM_TWO_SLOT_F32, GFT_FORM_TWO_SLOT_F64, NV,
-                   GftDesc, GftLoops, GftMlpDesc, GftObs)
+                   GftDesc, GftGrfDesc, GftLoops, GftMlpDesc, GftObs)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libgft.so")
@@ -48,6 +48,7 @@ _L.gft_ensemble_member.argtypes = [C.c_void_p, C.c_int, _P(C.c_void_p)]
 _L.gft_ensemble_set_launch.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
 _L.gft_ensemble_evolve.argtypes = [C.c_void_p, C.c_uint, C.c_int]
 _L.gft_ensemble_last_step_ms.argtypes = [C.c_void_p, _P(C.c_double)]
+_L.gft_init_grf.argtypes = [C.c_void_p, _P(GftGrfDesc)]
 _F = _P(C.c_float)
 _L.gft_mlp_create.argtypes = [_P(GftMlpDesc), _P(C.c_void_p)]
 _L.gft_mlp_destroy.argtypes = [C.c_void_p]
@@ -139,6 +140,12 @@ class Evolver:
         T = C.c_float if dt == np.float32 else C.c_double
         fn = _L.gft_import_two_slot_f32 if dt == np.float32 else _L.gft_import_two_slot_f64
         self._check(fn(self._h, s.ctypes.data_as(C.POINTER(T)), v.ctypes.data_as(C.POINTER(T)), time_iter))
+
+    def init_grf(self, seed: int, rms: float = 0.1, slope: float = -1.0, kmax: float = 0.0, vevs=(1.0, 1.0)):
+        """Gaussian-random-field state generated on the GPU (gft_init_grf): time 0,
+        both slots equal, identity links, E = 0."""
+        g = GftGrfDesc(seed, rms, slope, kmax, (C.c_double * 2)(*vevs))
+        self._check(_L.gft_init_grf(self._h, C.byref(g)))
 
     def export_two_slot(self, dtype=np.float64):
         s = np.zeros((2, self.nsites, self.ns), dtype)

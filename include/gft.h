@@ -204,6 +204,24 @@ int         gft_mlp_gradients(gft_mlp* mlp, const float* x, const float* t, cons
 int         gft_mlp_train(gft_mlp* mlp, const float* x, const float* t, const float* w, int n, int batch, int epochs,
                           float lr, unsigned seed, float* epoch_loss);
 
+/* Gaussian-random-field initial state generated on the GPU (no reference
+ * counterpart: the reference's initial conditions are RandomUniform and zero,
+ * InitialConditions/Source/RandomUniform.cpp:60-77).  Each real component of
+ * each doublet is white noise from Philox4x32-10 (counter = (site, component,
+ * 0, 0), key = seed; Box-Muller on two 53-bit uniforms) filtered by
+ * |k|^(slope/2) for 0 < |k| <= kmax, zero mean and unit rms; each doublet is
+ * then (0, 0, vev[k], 0) plus those four fields scaled to a total rms `rms`.
+ * The state is at time_iter 0 with both time slots equal (zero scalar momentum),
+ * identity links and E = 0.  Every side must be 2^a 3^b and at most 256. */
+typedef struct gft_grf_desc {
+    unsigned long long seed;
+    double rms;        /* sqrt(<sum_c dphi_c^2>) over the four components of a doublet */
+    double slope;      /* P(k) ~ |k|^slope                                               */
+    double kmax;       /* cut-off; <= 0 selects pi / (2 dx)                              */
+    double vev[2];     /* vev of doublet 0 and 1 (component 2)                           */
+} gft_grf_desc;
+int gft_init_grf(gft_ctx* ctx, const gft_grf_desc* grf);
+
 /* Wait for all queued work on the context's stream. */
 int gft_sync(gft_ctx* ctx);
 
