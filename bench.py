@@ -43,21 +43,22 @@ def gauss_rel(row):
     return max(row.gauss_int[m] / row.gauss_scale[m] for m in range(4) if row.gauss_scale[m] > 0)
 
 
-def run_form(rt, desc, s, v, steps, warmup, launch=None, tune=False, occupancy=False):
-    """Imports the state, optionally autotunes the launch shape (untimed), runs
-    `warmup` untimed steps, then times exactly `steps` steps."""
+def run_form(rt, desc, init, steps, warmup, launch=None, tune=False, occupancy=False):
+    """Builds the initial state with init(ev), optionally autotunes the launch
+    shape (untimed), runs `warmup` untimed steps, then times exactly `steps` steps."""
     info = {}
     with rt.Evolver(desc) as ev:
-        ev.import_two_slot(s, v, 0)
-        e0 = ev.observe().energy[0] if desc.form >= 2 else None
+        init(ev)
         if launch:
             ev.set_launch(*launch)
             info["launch"] = list(launch)
         elif tune and desc.form >= 2:
             from gftevolver_amd.tune import tune as autotune
-            best, timing = autotune(ev, fp32_sm=(desc.form == 2 and desc.model == 0), occupancy=occupancy)
+            best, timing = autotune(ev, fp32_sm=(desc.form in (2, 4) and desc.model == 0), occupancy=occupancy)
             info["launch"] = list(best)
             info["tune_ms"] = {",".join(map(str, k)): round(v, 4) for k, v in sorted(timing.items(), key=lambda kv: kv[1])[:6]}
+            init(ev)   # tuning advanced the state: start again from t = 0
+        e0 = ev.observe().energy[0] if desc.form >= 2 else None
         ev.evolve(warmup)
         ev.sync()
         t0 = time.perf_counter()
@@ -170,7 +171,6 @@ def main():
     from gftevolver_amd import runtime as rt
     from gftevolver_amd._abi import (GFT_FORM_MOMENTUM_F32, GFT_FORM_MOMENTUM_F32_PLAIN, GFT_FORM_MOMENTUM_F64,
                                      GFT_FORM_TWO_SLOT_F64, make_desc)
-    from gftevolver_amd.ic import grf_state
 
     name, cus = rt.device_info()
     copy_gbps = None
@@ -182,12 +182,14 @@ def main():
         log(f"copy kernel: 1 GiB in {cms:.3f} ms = {copy_gbps:.0f} GB/s (read + write)")
     n = (args.n,) * 3
     N = args.n ** 3
-    log(f"device {name}, {cus} CUs; building GRF initial state {n}")
-    s, v = grf_state(n, d=(0.5, 0.5, 0.5), rms=0.1, vev=1.0, seed=20261203, dtype=np.float32)
+    log(f"device {name}, {cus} CUs; GRF initial state {n} generated on the GPU")
+
+    def init(ev):
+        ev.init_grf(seed=20261203, rms=0.1, slope=-1.0)
     launch = tuple(int(x) for x in args.launch.split(",")) if args.launch else None
 
     d32 = make_desc(form=GFT_FORM_MOMENTUM_F32, n=n)
-    r32 = run_form(rt, d32, s, v, args.steps, args.warmup, launch, tune=not args.no_tune,
+    r32 = run_form(rt, d32, init, args.steps, args.warmup, launch, tune=not args.no_tune,
                    occupancy=args.occupancy_variants)
     log(f"FP32 launch {r32.get('launch')}; tuner {r32.get('tune_ms')}")
     value = N * args.steps / r32["wall_s"]
@@ -199,7 +201,7 @@ def main():
     extra = {}
     if not args.only_f32:
         dpl = make_desc(form=GFT_FORM_MOMENTUM_F32_PLAIN, n=n)
-        rpl = run_form(rt, dpl, s, v, args.plain_steps, args.warmup, launch, tune=not args.no_tune)
+        rpl = run_form(rt, dpl, init, args.plain_steps, args.warmup, launch, tune=not args.no_tune)
         vpl = N * args.plain_steps / rpl["wall_s"]
         log(f"FP32 plain storage: kernel {rpl['kernel_ms']:.3f} ms, {vpl:.3e} site-updates/s, "
             f"Gauss rel {rpl['gauss_rel']:.2e} after {args.plain_steps + args.warmup + 1} steps")
@@ -207,12 +209,12 @@ def main():
                                        "achieved_GBps": BYTES_F32 * N / (rpl["kernel_ms"] * 1e-3) / 1e9,
                                        "gauss_rel": rpl["gauss_rel"], "launch": rpl.get("launch")}
         d64 = make_desc(form=GFT_FORM_MOMENTUM_F64, n=n)
-        r64 = run_form(rt, d64, s, v, args.fp64_steps, 2, tune=not args.no_tune)
+        r64 = run_form(rt, d64, init, args.fp64_steps, 2, tune=not args.no_tune)
         v64 = N * args.fp64_steps / r64["wall_s"]
         log(f"FP64 momentum: kernel {r64['kernel_ms']:.3f} ms, {v64:.3e} site-updates/s, "
             f"energy drift {(r64['e1'] - r64['e0']) / abs(r64['e0']):.2e}")
         dts = make_desc(form=GFT_FORM_TWO_SLOT_F64, n=n)
-        rts = run_form(rt, dts, s, v, args.twoslot_steps, 2)
+        rts = run_form(rt, dts, init, args.twoslot_steps, 2)
         vts = N * args.twoslot_steps / rts["wall_s"]
         log(f"FP64 two-slot: kernel {rts['kernel_ms']:.3f} ms, {vts:.3e} site-updates/s")
         extra.update({

@@ -22,7 +22,6 @@ import time
 import numpy as np
 
 from ._abi import GFT_FORM_MOMENTUM_F32, GFT_FORM_MOMENTUM_F64, make_desc
-from .ic import grf_state
 
 OBSERVABLES = ("phi_sq", "e_frac", "b_frac", "plaq_su2", "plaq_u1")
 GROUPS = ((16, 32), (24, 48), (16, 64), (8, 96))
@@ -39,8 +38,9 @@ def members(groups=GROUPS, seed=SEED, rms_range=(0.05, 0.2)):
     return out
 
 
-def member_state(n, rms, seed):
-    return grf_state(n, d=(0.5, 0.5, 0.5), rms=rms, vev=1.0, seed=seed, dtype=np.float32)
+def init_member(ev, rms, seed):
+    """The member's GRF state, generated on the GPU (gft_init_grf)."""
+    ev.init_grf(seed=seed, rms=rms, slope=-1.0)
 
 
 def observe(ev):
@@ -79,20 +79,19 @@ def run_fp32(rt, specs, steps, measure_every, tune=True, use_graph=True, log=Non
     (observables [n_meas][n_members][5], timed seconds, avg kernel ms per step,
     tiles).  Timing excludes tuning and measurements: only the evolve calls."""
     descs = [make_desc(form=GFT_FORM_MOMENTUM_F32, n=n) for n, _, _ in specs]
-    states = [member_state(*s) for s in specs]
     sizes = sorted({n for n, _, _ in specs})
     groups_idx = [[i for i, s in enumerate(specs) if s[0] == n] for n in sizes]
     with rt.Ensemble(descs) as ens:
         tiles = None
         if tune:
-            # Tune on a throw-away copy of the state, then import afresh.
-            for m, (s, v) in zip(ens.members, states):
-                m.import_two_slot(s, v, 0)
+            # Tune on a throw-away copy of the state, then start afresh.
+            for m, (_, rms, seed) in zip(ens.members, specs):
+                init_member(m, rms, seed)
             tiles = tune_tiles(ens, groups_idx)
             if log:
                 log(f"ensemble tiles per size {dict(zip(sizes, tiles.values()))}")
-        for m, (s, v) in zip(ens.members, states):
-            m.import_two_slot(s, v, 0)
+        for m, (_, rms, seed) in zip(ens.members, specs):
+            init_member(m, rms, seed)
         obs = [[observe(m) for m in ens.members]]
         timed = 0.0
         kms = []
@@ -112,9 +111,8 @@ def run_fp64_sequential(rt, specs, steps, measure_every):
     """The same members one after another in the FP64 momentum form."""
     out = []
     for n, rms, seed in specs:
-        s, v = member_state(n, rms, seed)
         with rt.Evolver(make_desc(form=GFT_FORM_MOMENTUM_F64, n=n)) as ev:
-            ev.import_two_slot(s.astype(np.float64), v.astype(np.float64), 0)
+            init_member(ev, rms, seed)
             rows = [observe(ev)]
             done = 0
             while done < steps:

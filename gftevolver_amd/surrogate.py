@@ -28,7 +28,6 @@ import time
 import numpy as np
 
 from ._abi import GFT_FORM_MOMENTUM_F32, make_desc
-from .ic import grf_state
 
 PARAMS = ("g", "gp", "lam", "rms", "slope", "kmax_frac", "dx", "dt")
 RANGES = np.array([[0.5, 0.8], [0.25, 0.45], [0.08, 0.20], [0.05, 0.20], [-2.0, 0.0], [0.25, 1.0], [0.4, 0.6],
@@ -59,10 +58,10 @@ def _desc(p, L):
     return make_desc(form=GFT_FORM_MOMENTUM_F32, n=(L, L, L), d=(dx, dx, dx), dt=dt, g=g, gp=gp, lam=lam)
 
 
-def _state(p, L, seed):
+def _init(ev, p, seed):
+    """The point's GRF state, generated on the GPU (gft_init_grf)."""
     _, _, _, rms, slope, kfrac, dx, _ = (float(v) for v in p)
-    return grf_state((L, L, L), d=(dx, dx, dx), rms=rms, vev=1.0, seed=seed, slope=slope,
-                     kmax=kfrac * np.pi / (2 * dx), dtype=np.float32)
+    ev.init_grf(seed=seed, rms=rms, slope=slope, kmax=kfrac * np.pi / (2 * dx))
 
 
 def _observe(m):
@@ -79,32 +78,25 @@ def run_points(rt, points, L=64, steps=2000, members=64, seed=20261203, log=None
     `seed`), filtered and scaled by its own spectrum and amplitude: with this
     common random number the observables are smooth functions of the
     parameters instead of carrying each run's own realisation noise."""
-    from concurrent.futures import ThreadPoolExecutor
     tp = time_points(steps)
     n = len(points)
     Y = np.zeros((n, N_TIMES, len(OBSERVABLES)))
     t_evolve = 0.0
     t0 = time.perf_counter()
     batches = [range(b0, min(n, b0 + members)) for b0 in range(0, n, members)]
-    with ThreadPoolExecutor(max_workers=8) as pool:
-        # Initial states of the next batch are built on the host while the GPU evolves this one.
-        nxt = [pool.submit(_state, points[i], L, seed) for i in batches[0]] if batches else []
-        for bi, idx in enumerate(batches):
-            states = [f.result() for f in nxt]
-            if bi + 1 < len(batches):
-                nxt = [pool.submit(_state, points[i], L, seed) for i in batches[bi + 1]]
-            with rt.Ensemble([_desc(points[i], L) for i in idx]) as ens:
-                for m, (s, v) in zip(ens.members, states):
-                    m.import_two_slot(s, v, 0)
-                for k in range(N_TIMES):
-                    if k:
-                        t1 = time.perf_counter()
-                        ens.evolve(int(tp[k] - tp[k - 1]))
-                        t_evolve += time.perf_counter() - t1
-                    for j, m in enumerate(ens.members):
-                        Y[idx[j], k] = _observe(m)
-            if log and (bi % 8 == 7 or bi + 1 == len(batches)):
-                log(f"  evolved points 0..{idx[-1]} ({time.perf_counter() - t0:.1f} s)")
+    for bi, idx in enumerate(batches):
+        with rt.Ensemble([_desc(points[i], L) for i in idx]) as ens:
+            for m, i in zip(ens.members, idx):
+                _init(m, points[i], seed)
+            for k in range(N_TIMES):
+                if k:
+                    t1 = time.perf_counter()
+                    ens.evolve(int(tp[k] - tp[k - 1]))
+                    t_evolve += time.perf_counter() - t1
+                for j, m in enumerate(ens.members):
+                    Y[idx[j], k] = _observe(m)
+        if log and (bi % 8 == 7 or bi + 1 == len(batches)):
+            log(f"  evolved points 0..{idx[-1]} ({time.perf_counter() - t0:.1f} s)")
     return Y, t_evolve, time.perf_counter() - t0
 
 

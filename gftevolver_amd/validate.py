@@ -27,7 +27,6 @@ import time
 import numpy as np
 
 from ._abi import GFT_FORM_MOMENTUM_F32, GFT_FORM_MOMENTUM_F64, GFT_MODEL_2HDM, TWO_HDM_POTENTIAL, make_desc
-from .ic import grf_state
 
 
 def log(*a):
@@ -43,12 +42,12 @@ def _loops(ev):
     return [w.w11_su2, w.w11_u1, w.w22_su2, w.w22_u1, w.phi_sq]
 
 
-def _run(rt, desc, s, v, steps, every, tag):
+def _run(rt, desc, init, steps, every, tag):
     """Evolves in chunks of `every` steps; after each chunk measures the energy,
     the Gauss law (gft_observe) and the Wilson loops of the current state."""
     series = []
     with rt.Evolver(desc) as ev:
-        ev.import_two_slot(s, v, 0)
+        init(ev)
         e0 = ev.observe().energy[:]
         series.append({"step": 0, "energy": list(e0), "loops": _loops(ev), "gauss_rel": 0.0})
         done, wall = 0, 0.0
@@ -107,22 +106,18 @@ def _summary(name, steps, s32, s64, o32, o64, w32, w64, nsites):
 
 def sm256(rt, steps=10000, every=100, n=256):
     N = n ** 3
-    log(f"sm{n}: building GRF state")
-    s, v = grf_state((n, n, n), rms=0.1, seed=20261203, dtype=np.float32)
-    s32, o32, w32 = _run(rt, make_desc(form=GFT_FORM_MOMENTUM_F32, n=(n, n, n)), s, v, steps, every, "fp32")
-    s64, o64, w64 = _run(rt, make_desc(form=GFT_FORM_MOMENTUM_F64, n=(n, n, n)), s.astype(np.float64),
-                         v.astype(np.float64), steps, every, "fp64")
+    init = lambda ev: ev.init_grf(seed=20261203, rms=0.1, slope=-1.0)   # GRF state generated on the GPU
+    s32, o32, w32 = _run(rt, make_desc(form=GFT_FORM_MOMENTUM_F32, n=(n, n, n)), init, steps, every, "fp32")
+    s64, o64, w64 = _run(rt, make_desc(form=GFT_FORM_MOMENTUM_F64, n=(n, n, n)), init, steps, every, "fp64")
     return _summary(f"SM {n}^3 GRF", steps, s32, s64, o32, o64, w32, w64, N)
 
 
 def twohdm192(rt, steps=5000, every=50, n=192):
     N = n ** 3
-    log(f"2hdm{n}: building GRF state")
-    s, v = grf_state((n, n, n), rms=0.1, seed=20261204, ndoublets=2, vevs=(0.8, 0.6), dtype=np.float32)
+    init = lambda ev: ev.init_grf(seed=20261204, rms=0.1, slope=-1.0, vevs=(0.8, 0.6))
     mk = lambda form: make_desc(model=GFT_MODEL_2HDM, form=form, n=(n, n, n), pot2=TWO_HDM_POTENTIAL)
-    s32, o32, w32 = _run(rt, mk(GFT_FORM_MOMENTUM_F32), s, v, steps, every, "fp32")
-    s64, o64, w64 = _run(rt, mk(GFT_FORM_MOMENTUM_F64), s.astype(np.float64), v.astype(np.float64), steps, every,
-                         "fp64")
+    s32, o32, w32 = _run(rt, mk(GFT_FORM_MOMENTUM_F32), init, steps, every, "fp32")
+    s64, o64, w64 = _run(rt, mk(GFT_FORM_MOMENTUM_F64), init, steps, every, "fp64")
     out = _summary(f"2HDM {n}^3 GRF", steps, s32, s64, o32, o64, w32, w64, N)
     out["series_every"] = every
     out["loops_series_fp32"] = [r["loops"] for r in s32]
