@@ -428,42 +428,6 @@ __device__ __forceinline__ void mom_site(const StepArgs<R>& a, const Coef& c, in
         }
     }
 
-    // Kick and drift of the doublets, before the staples: F and the doublet
-    // values are dead afterwards, which keeps fewer registers live through the
-    // staple products.
-#pragma unroll
-    for (int k = 0; k < ND; k++) {
-        R p[4];
-        unpack(ld_nt(ptr_at(a.P[k], s)), p);
-        V4 pn, fn;
-        if constexpr (CMP) {
-            const uint4 L = ld_nt(ptr_at(a.lo_s[k], s));
-            const float fl[4] = {bf16_lo(L.x), bf16_hi(L.x), bf16_lo(L.y), bf16_hi(L.y)};
-            const float pl[4] = {bf16_lo(L.z), bf16_hi(L.z), bf16_lo(L.w), bf16_hi(L.w)};
-            float ph[4], pr[4], fh[4], fr[4];
-#pragma unroll
-            for (int m = 0; m < 4; m++) ph[m] = cmp_update(p[m], pl[m], a.keep, a.dt2 * F[4 * k + m], &pr[m]);
-            // phi(t+dt) = phi(t) + P(t+dt) with P as stored (its residual rounded).
-            const unsigned pw0 = bf16_pack2(pr[0], pr[1]), pw1 = bf16_pack2(pr[2], pr[3]);
-            const float prr[4] = {bf16_lo(pw0), bf16_hi(pw0), bf16_lo(pw1), bf16_hi(pw1)};
-#pragma unroll
-            for (int m = 0; m < 4; m++) fh[m] = cmp_add(phi[4 * k + m], fl[m], ph[m], prr[m], &fr[m]);
-            pn = make_float4(ph[0], ph[1], ph[2], ph[3]);
-            fn = make_float4(fh[0], fh[1], fh[2], fh[3]);
-            st_nt(make_uint4(bf16_pack2(fr[0], fr[1]), bf16_pack2(fr[2], fr[3]), pw0, pw1), ptr_at(a.lo_s[k], s));
-        } else {
-            pn.x = a.keep * p[0] + a.dt2 * F[4 * k + 0];
-            pn.y = a.keep * p[1] + a.dt2 * F[4 * k + 1];
-            pn.z = a.keep * p[2] + a.dt2 * F[4 * k + 2];
-            pn.w = a.keep * p[3] + a.dt2 * F[4 * k + 3];
-            fn.x = phi[4 * k + 0] + pn.x;
-            fn.y = phi[4 * k + 1] + pn.y;
-            fn.z = phi[4 * k + 2] + pn.z;
-            fn.w = phi[4 * k + 3] + pn.w;
-        }
-        st_nt(pn, ptr_at(a.P[k], s));
-        st_nt(fn, ptr_at(a.phi_out[k], s));
-    }
     // Magnetic force (staples, WilsonLoops/Source/StandardModel.cpp:324-409).
     // Each plaquette of the plane (lo, hi), lo < hi, based at corner b enters the
     // forces on its four links through one rounded quaternion and one rounded
@@ -478,12 +442,9 @@ __device__ __forceinline__ void mom_site(const StepArgs<R>& a, const Coef& c, in
     // the rounded K_b and theta_b makes the contributions of one plaquette cancel in
     // the Gauss law up to the rounding of the rotations, which is relative to the
     // plaquette's small deviation from 1 instead of to the size of the links.
-    // M starts as the current term g_a^2 J and accumulates the staples.
     R M[3][4];
 #pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int m = 0; m < 4; m++) M[i][m] = a.gsq[m] * J[i][m];
+    for (int i = 0; i < 3; i++) M[i][0] = M[i][1] = M[i][2] = M[i][3] = R(0);
     // Plaquettes based at x (b = x, lo = i, hi = j).
 #pragma unroll
     for (int i = 0; i < 3; i++) {
@@ -542,7 +503,40 @@ __device__ __forceinline__ void mom_site(const StepArgs<R>& a, const Coef& c, in
         }
     }
 
-    // Kick of E and drift of the links.
+    // Kick and drift.
+#pragma unroll
+    for (int k = 0; k < ND; k++) {
+        R p[4];
+        unpack(ld_nt(ptr_at(a.P[k], s)), p);
+        V4 pn, fn;
+        if constexpr (CMP) {
+            const uint4 L = ld_nt(ptr_at(a.lo_s[k], s));
+            const float fl[4] = {bf16_lo(L.x), bf16_hi(L.x), bf16_lo(L.y), bf16_hi(L.y)};
+            const float pl[4] = {bf16_lo(L.z), bf16_hi(L.z), bf16_lo(L.w), bf16_hi(L.w)};
+            float ph[4], pr[4], fh[4], fr[4];
+#pragma unroll
+            for (int m = 0; m < 4; m++) ph[m] = cmp_update(p[m], pl[m], a.keep, a.dt2 * F[4 * k + m], &pr[m]);
+            // phi(t+dt) = phi(t) + P(t+dt) with P as stored (its residual rounded).
+            const unsigned pw0 = bf16_pack2(pr[0], pr[1]), pw1 = bf16_pack2(pr[2], pr[3]);
+            const float prr[4] = {bf16_lo(pw0), bf16_hi(pw0), bf16_lo(pw1), bf16_hi(pw1)};
+#pragma unroll
+            for (int m = 0; m < 4; m++) fh[m] = cmp_add(phi[4 * k + m], fl[m], ph[m], prr[m], &fr[m]);
+            pn = make_float4(ph[0], ph[1], ph[2], ph[3]);
+            fn = make_float4(fh[0], fh[1], fh[2], fh[3]);
+            st_nt(make_uint4(bf16_pack2(fr[0], fr[1]), bf16_pack2(fr[2], fr[3]), pw0, pw1), ptr_at(a.lo_s[k], s));
+        } else {
+            pn.x = a.keep * p[0] + a.dt2 * F[4 * k + 0];
+            pn.y = a.keep * p[1] + a.dt2 * F[4 * k + 1];
+            pn.z = a.keep * p[2] + a.dt2 * F[4 * k + 2];
+            pn.w = a.keep * p[3] + a.dt2 * F[4 * k + 3];
+            fn.x = phi[4 * k + 0] + pn.x;
+            fn.y = phi[4 * k + 1] + pn.y;
+            fn.z = phi[4 * k + 2] + pn.z;
+            fn.w = phi[4 * k + 3] + pn.w;
+        }
+        st_nt(pn, ptr_at(a.P[k], s));
+        st_nt(fn, ptr_at(a.phi_out[k], s));
+    }
     unsigned bad = 0;
 #pragma unroll
     for (int i = 0; i < 3; i++) {
@@ -555,14 +549,14 @@ __device__ __forceinline__ void mom_site(const StepArgs<R>& a, const Coef& c, in
             float eh[4], er[4];
 #pragma unroll
             for (int m = 0; m < 4; m++)
-                eh[m] = cmp_update(e[m], el[m], a.keep, a.dt2 * M[i][m], &er[m]);
+                eh[m] = cmp_update(e[m], el[m], a.keep, a.dt2 * (M[i][m] + a.gsq[m] * J[i][m]), &er[m]);
             en = make_float4(eh[0], eh[1], eh[2], eh[3]);
             st_nt(make_uint2(bf16_pack2(er[0], er[1]), bf16_pack2(er[2], er[3])), ptr_at(a.lo_E[i], s));
         } else {
-            en.x = a.keep * e[0] + a.dt2 * M[i][0];
-            en.y = a.keep * e[1] + a.dt2 * M[i][1];
-            en.z = a.keep * e[2] + a.dt2 * M[i][2];
-            en.w = a.keep * e[3] + a.dt2 * M[i][3];
+            en.x = a.keep * e[0] + a.dt2 * (M[i][0] + a.gsq[0] * J[i][0]);
+            en.y = a.keep * e[1] + a.dt2 * (M[i][1] + a.gsq[1] * J[i][1]);
+            en.z = a.keep * e[2] + a.dt2 * (M[i][2] + a.gsq[2] * J[i][2]);
+            en.w = a.keep * e[3] + a.dt2 * (M[i][3] + a.gsq[3] * J[i][3]);
         }
         const R e2 = en.y * en.y + en.z * en.z + en.w * en.w;
         if (!(e2 < R(1)) || !(fabs(en.x) < R(1))) bad = 1;
