@@ -594,7 +594,7 @@ static int ens_prepare(gft_ensemble* ens) {
             host[p].push_back(em);
         }
         c->mom.cur = saved;
-        const std::vector<int4> t = build_tiles((int)i, c->coef.nx, c->coef.ny, c->coef.nz, c->mom.bz, c->mom.by, 0);
+        const std::vector<int4> t = build_tiles_banded((int)i, c->coef.nx, c->coef.ny, c->coef.nz, c->mom.bz, c->mom.by);
         work.insert(work.end(), t.begin(), t.end());
     }
     for (int p = 0; p < 2; p++) {

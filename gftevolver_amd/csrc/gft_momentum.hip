@@ -926,6 +926,23 @@ inline std::vector<int4> build_tiles(int member, int nx, int ny, int nz, int bz,
     return t;
 }
 
+// Tiles of one member in band-major order: 8 bands of y-tiles, and inside a
+// band x-planes in order with the band's (y, z) tiles inside a plane.  Split
+// into 8 contiguous ranges (the ensemble kernel's XCD assignment), a large
+// member gives each XCD one y-band to sweep along x, as block order 2 does
+// for a single lattice, so the x +- 1 planes a tile reads stay in that XCD's L2.
+inline std::vector<int4> build_tiles_banded(int member, int nx, int ny, int nz, int bz, int by) {
+    const int nbz = (nz + bz - 1) / bz, nby = (ny + by - 1) / by;
+    const int br = (nby + 7) / 8;
+    std::vector<int4> t;
+    t.reserve((size_t)nbz * nby * nx);
+    for (int b0 = 0; b0 < nby; b0 += br)
+        for (int x = 0; x < nx; x++)
+            for (int ty = b0; ty < std::min(nby, b0 + br); ty++)
+                for (int tz = 0; tz < nbz; tz++) t.push_back(make_int4(member, x, ty, tz));
+    return t;
+}
+
 inline int ensure_tiles(MomentumState& m, const Coef& c, hipStream_t st, std::string& msg) {
     if (m.tiles && m.tiles_key[0] == m.bz && m.tiles_key[1] == m.by && m.tiles_key[2] == m.order) return GFT_OK;
     const std::vector<int4> t = build_tiles(0, c.nx, c.ny, c.nz, m.bz, m.by, m.order);

@@ -109,3 +109,24 @@ def test_ensemble_workload_observables_match_fp64_sequential(rt):
     print(f"{14450688 * 2000 / sec:.3e} site-updates/s ({kms:.3f} ms per step by HIP events); "
           f"max per-member |diff| {dict(zip(E.OBSERVABLES, per_member))}")
     assert np.abs(m32 - m64).max() <= 1e-4
+
+
+def test_ensemble_graph_replays_with_eager_work_between(rt):
+    """One two-step graph replay per call, with eager work (observables,
+    exports) between the replays: after every replay the members equal the
+    stream-launched run bitwise."""
+    descs = [make_desc(form=GFT_FORM_MOMENTUM_F32, n=n) for n in SIZES[:3]]
+    states = _states()[:3]
+    with rt.Ensemble(descs) as eg, rt.Ensemble(descs) as es:
+        for ens in (eg, es):
+            for m, (s, v) in zip(ens.members, states):
+                m.import_two_slot(s.astype(np.float32), v.astype(np.float32), 0)
+        for rep in range(4):
+            eg.evolve(2, use_graph=True)
+            es.evolve(2, use_graph=False)
+            for a, b in zip(eg.members, es.members):
+                a.observe()
+                sa, va, ia = a.export_two_slot(np.float32)
+                sb, vb, ib = b.export_two_slot(np.float32)
+                assert ia == ib == 2 * (rep + 1)
+                assert np.array_equal(sa, sb) and np.array_equal(va, vb), f"replay {rep}"
