@@ -163,7 +163,7 @@ def main():
                     help="first copy 1 GiB device-to-device (a kernel of known traffic for PMC scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build runs every workload on a single MI355X "
+        print(f"bench.py measures one GPU: the library runs every workload on a single MI355X "
               f"(--gpus {args.gpus} is not supported).", file=sys.stderr)
         sys.exit(2)
 

@@ -1,4 +1,4 @@
-"""Learned surrogate for parameter sweeps, with the evolver as fallback and judge.
+"""Learned surrogate for parameter sweeps, with the evolver as fallback and as the measure of its error.
 
 Inputs (8, sampled by Latin hypercube over RANGES): g, g', lambda, the rms
 amplitude of the GRF Higgs fluctuations, the spectral slope of P(k) ~ |k|^slope,
