@@ -155,7 +155,7 @@ def main():
                     help="steps of the 64-member ensemble workload (0 skips it)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only-f32", action="store_true")
-    ap.add_argument("--launch", type=str, default="", help="bz,by,order,min_waves (skips the autotuner)")
+    ap.add_argument("--launch", type=str, default="", help="bz,by,order,min_waves[,stage] (skips the autotuner)")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--occupancy-variants", action="store_true",
                     help="let the tuner try the 3/4-wave kernels (last-bit different results)")
@@ -251,7 +251,7 @@ def main():
                         f"(P(k)~1/k, k<=pi/(2dx), rms 0.1v), links identity, E=0; dx=0.5 dt=0.05 g=0.65 "
                         f"g'=0.35 lambda=0.13 v=1",
             "lattice": list(n),
-            "launch": r32.get("launch", [64, 4, 2, 3]),
+            "launch": r32.get("launch", [64, 4, 2, 3, 1]),
         },
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,

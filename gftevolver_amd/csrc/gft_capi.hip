@@ -433,8 +433,9 @@ extern "C" int gft_evolve(gft_ctx* ctx, unsigned nsteps, unsigned energy_freq, u
     return GFT_OK;
 }
 
-extern "C" int gft_set_launch(gft_ctx* ctx, int bz, int by, int order, int min_waves) {
+extern "C" int gft_set_launch(gft_ctx* ctx, int bz, int by, int order, int min_waves, int stage) {
     if (!ctx) return GFT_EINVAL;
+    if (stage != 0 && stage != 1) return fail(ctx, GFT_EINVAL, "gft_set_launch: stage must be 0 or 1");
     if (min_waves != 0 && min_waves != 3 && min_waves != 4)
         return fail(ctx, GFT_EINVAL, "gft_set_launch: min_waves must be 0, 3 or 4");
     if (order < 0 || order > 2) return fail(ctx, GFT_EINVAL, "gft_set_launch: order must be 0, 1 or 2");
@@ -444,6 +445,7 @@ extern "C" int gft_set_launch(gft_ctx* ctx, int bz, int by, int order, int min_w
     ctx->mom.by = by;
     ctx->mom.order = order;
     ctx->mom.min_waves = min_waves;
+    ctx->mom.stage = stage;
     return GFT_OK;
 }
 
@@ -545,6 +547,7 @@ struct gft_ensemble {
     int4* work = nullptr;
     long long nwork = 0;
     int grid = 0, blocks_per_xcd = 0;
+    size_t lds = 0;          // dynamic LDS per block of the staged kernel (0: unstaged)
     unsigned* err_dev = nullptr;
     bool dirty = true;
     hipGraph_t graph = nullptr;
@@ -582,6 +585,15 @@ static int ens_prepare(gft_ensemble* ens) {
     const size_t nm = ens->members.size();
     std::vector<EnsMember> host[2];
     std::vector<int4> work;
+    // Staged when every member asks for it and the largest tile's LDS image fits.
+    bool stage = true;
+    size_t lds = 0;
+    for (size_t i = 0; i < nm; i++) {
+        gft_ctx* c = ens->members[i];
+        stage = stage && c->mom.stage;
+        lds = std::max(lds, plane_lds_bytes(c->mom.bz, c->mom.by, 1));
+    }
+    if (lds > kPlaneLdsMax) stage = false;
     for (size_t i = 0; i < nm; i++) {
         gft_ctx* c = ens->members[i];
         const int saved = c->mom.cur;
@@ -607,10 +619,16 @@ static int ens_prepare(gft_ensemble* ens) {
     EK(hipMemcpyAsync(ens->work, work.data(), work.size() * sizeof(int4), hipMemcpyHostToDevice, ens->stream));
     EK(hipStreamSynchronize(ens->stream));
     ens->nwork = (long long)work.size();
-    int cus = 0;
-    EK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0));
-    // Three 256-thread blocks per CU (the kernel's occupancy), a multiple of 8.
-    long long grid = std::min<long long>(ens->nwork, 3LL * cus);
+    int cus = 0, lds_cu = 0, dev = 0;
+    EK(hipGetDevice(&dev));
+    EK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    EK(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
+    ens->lds = stage ? lds : 0;
+    // Three 256-thread blocks per CU (the kernel's register occupancy) or as
+    // many as the CU's LDS holds, a multiple of 8.
+    long long per_cu = 3;
+    if (ens->lds) per_cu = std::max<long long>(1, std::min<long long>(3, (long long)lds_cu / (long long)ens->lds));
+    long long grid = std::min<long long>(ens->nwork, per_cu * cus);
     grid = std::max<long long>(8, (grid / 8) * 8);
     ens->grid = (int)grid;
     ens->blocks_per_xcd = (int)(grid / 8);
@@ -674,7 +692,7 @@ int gft_ensemble_set_launch(gft_ensemble* ens, int member, int bz, int by) {
     gft_ctx* c = nullptr;
     int rc = gft_ensemble_member(ens, member, &c);
     if (rc) return rc;
-    rc = gft_set_launch(c, bz, by, c->mom.order, c->mom.min_waves);
+    rc = gft_set_launch(c, bz, by, c->mom.order, c->mom.min_waves, c->mom.stage);
     if (rc) return efail(ens, rc, c->err);
     ens->dirty = true;
     return GFT_OK;
@@ -694,7 +712,8 @@ int gft_ensemble_evolve(gft_ensemble* ens, unsigned nsteps, int use_graph) {
     if (rc) return rc;
     const dim3 grid(ens->grid), block(256);
     auto launch = [&](int p) {
-        ens_step<3><<<grid, block, 0, ens->stream>>>(ens->args[p], ens->work, ens->nwork, ens->blocks_per_xcd);
+        if (ens->lds) ens_step<3, true><<<grid, block, ens->lds, ens->stream>>>(ens->args[p], ens->work, ens->nwork, ens->blocks_per_xcd);
+        else ens_step<3, false><<<grid, block, 0, ens->stream>>>(ens->args[p], ens->work, ens->nwork, ens->blocks_per_xcd);
     };
     if (use_graph && nsteps >= 2 && ens->graph_parity != parity) {
         ens_drop_graph(ens);

@@ -70,6 +70,7 @@ struct MomentumState {
     int bz = 64, by = 4;     // tile shape of the step kernel
     int order = 2;           // block order (see decode_block)
     int min_waves = 3;       // occupancy target of the FP32 SM step kernel (0 = unconstrained, 3, 4)
+    int stage = 1;           // FP32 step kernels: in-plane neighbours staged in LDS (bitwise the same fields)
     int4* tiles = nullptr;   // device tile table of the current launch shape
     int tiles_key[3] = {0, 0, -1};
 };
@@ -356,10 +357,97 @@ GFT_HD void decode_block(long long b, long long nb, int order, int nx, int nby, 
     *x = static_cast<int>(t / rows);
 }
 
+// In-plane staging of a block's tile in LDS: q[3], y[3] and phi[ND] of the
+// bz x by sites of plane x plus a one-site ring in y and z, one array per field,
+// slot (ty + 1) * w + (tz + 1) for tile coordinates (ty, tz), w = bz + 2.  Every
+// neighbour of a site inside its own x-plane (x +- e_y, x +- e_z, x + e_y - e_z,
+// x + e_z - e_y) is then an LDS read; neighbours in the planes x +- 1 stay global.
+struct PlaneLds {
+    float4* q[3];
+    float* y[3];
+    float4* phi[2];
+    int w;        // slots per tile row (bz + 2)
+    int r0;       // this thread's slot
+};
+
+// Slots per field, rounded up to whole 64-slot chunks (one wave fills one chunk
+// of one field per instruction).
+__host__ __device__ inline int plane_slots(int bz, int by) { return ((bz + 2) * (by + 2) + 63) / 64 * 64; }
+__host__ __device__ inline size_t plane_lds_bytes(int bz, int by, int nd) {
+    return (size_t)plane_slots(bz, by) * (3 * 16 + 3 * 4 + nd * 16);
+}
+
+template <int ND>
+__device__ __forceinline__ PlaneLds plane_layout(char* base, int bz, int by) {
+    const int nsp = plane_slots(bz, by);
+    PlaneLds p;
+    float4* q = reinterpret_cast<float4*>(base);
+#pragma unroll
+    for (int f = 0; f < 3; f++) p.q[f] = q + f * nsp;
+#pragma unroll
+    for (int k = 0; k < 2; k++) p.phi[k] = q + (3 + (k < ND ? k : 0)) * nsp;
+    float* y = reinterpret_cast<float*>(q + (3 + ND) * nsp);
+#pragma unroll
+    for (int f = 0; f < 3; f++) p.y[f] = y + f * nsp;
+    p.w = bz + 2;
+    p.r0 = 0;
+    return p;
+}
+
+// v mod n for v >= -n; `general` false promises v in [-1, 2n).
+__device__ __forceinline__ int wrap_index(int v, int n, bool general) {
+    if (general) return (v % n + n) % n;
+    v = v < 0 ? v + n : v;
+    return v >= n ? v - n : v;
+}
+
+// Whole block: gathers the tile and ring of plane x (tile origin y0, z0) straight
+// into LDS with global_load_lds (no register round trip), then waits for its own
+// transfers and meets the other waves at the barrier.
+
+template <int ND>
+__device__ __forceinline__ void plane_fill(const StepArgs<float>& a, const PlaneLds& p, int x, int y0, int z0, int bz,
+                                           int by) {
+    const int w = bz + 2, ns = w * (by + 2);
+    const int lane = static_cast<int>(threadIdx.x) & 63, wave = static_cast<int>(threadIdx.x) >> 6;
+    const int sx = a.ny * a.nz;
+    const float rcp_w = __builtin_amdgcn_rcpf(static_cast<float>(w));
+    // Ring indices lie in [-1, n - 1 + b]: one conditional add or subtract wraps
+    // them unless the tile is longer than the lattice side (b > n).
+    const bool gen_y = by > a.ny, gen_z = bz > a.nz;
+    for (int c = wave; c * 64 < ns; c += 4) {
+        const int r = min(c * 64 + lane, ns - 1);
+        // r / w from the float reciprocal ((r + 1/2) / w is at least 1/(2w) from an
+        // integer, far above the rounding error), with a one-step correction.
+        int ty = static_cast<int>((static_cast<float>(r) + 0.5f) * rcp_w), tz = r - ty * w;
+        if (tz < 0) { ty--; tz += w; }
+        if (tz >= w) { ty++; tz -= w; }
+        const int yy = wrap_index(y0 + ty - 1, a.ny, gen_y);
+        const int zz = wrap_index(z0 + tz - 1, a.nz, gen_z);
+        const unsigned s = static_cast<unsigned>(x * sx + yy * a.nz + zz);
+#pragma unroll
+        for (int f = 0; f < 3; f++)
+            __builtin_amdgcn_global_load_lds(&at(a.q[f], s), (__attribute__((address_space(3))) void*)(p.q[f] + c * 64),
+                                             16, 0, 0);
+#pragma unroll
+        for (int f = 0; f < 3; f++)
+            __builtin_amdgcn_global_load_lds(&at(a.y[f], s), (__attribute__((address_space(3))) void*)(p.y[f] + c * 64),
+                                             4, 0, 0);
+#pragma unroll
+        for (int k = 0; k < ND; k++)
+            __builtin_amdgcn_global_load_lds(&at(a.phi[k], s),
+                                             (__attribute__((address_space(3))) void*)(p.phi[k] + c * 64), 16, 0, 0);
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+}
+
 // One site-update at (x, y, z): forces, kick and drift (shared by the
-// single-lattice kernel and the ensemble kernel).
-template <class R, int ND, bool CMP>
-__device__ __forceinline__ void mom_site(const StepArgs<R>& a, const Coef& c, int x, int y, int z) {
+// single-lattice kernel and the ensemble kernel).  PL: in-plane neighbours and
+// own q, y, phi come from the block's LDS tile `pl` (FP32 only).
+template <class R, int ND, bool CMP, bool PL = false>
+__device__ __forceinline__ void mom_site(const StepArgs<R>& a, const Coef& c, int x, int y, int z,
+                                         const PlaneLds& pl = PlaneLds{}) {
     using V4 = typename V4T<R>::type;
 
     // 32-bit site offsets (the host limits a lattice to 2^28 sites): addresses
@@ -375,19 +463,52 @@ __device__ __forceinline__ void mom_site(const StepArgs<R>& a, const Coef& c, in
     dp[2] = (z == a.nz - 1) ? -(a.nz - 1) : 1;
     dm[2] = (z == 0) ? (a.nz - 1) : -1;
 
+    // Site x + si e_i + sj e_j (si, sj in {-1, 0, 1}; all compile-time after unrolling).
+    auto gsite = [&](int i, int si, int j, int sj) -> unsigned {
+        int o = 0;
+        if (si > 0) o += dp[i];
+        else if (si < 0) o += dm[i];
+        if (sj > 0) o += dp[j];
+        else if (sj < 0) o += dm[j];
+        return s + o;
+    };
+    auto in_plane = [&](int i, int si, int j, int sj) { return PL && !(si != 0 && i == 0) && !(sj != 0 && j == 0); };
+    auto slot = [&](int i, int si, int j, int sj) {
+        const int step[3] = {0, pl.w, 1};
+        return pl.r0 + si * step[i] + sj * step[j];
+    };
+    auto Q = [&](int f, int i, int si, int j, int sj) -> Qt<R> {
+        if constexpr (PL) {
+            if (in_plane(i, si, j, sj)) return toq<float4, R>(pl.q[f][slot(i, si, j, sj)]);
+        }
+        return toq<V4, R>(at(a.q[f], gsite(i, si, j, sj)));
+    };
+    auto Y = [&](int f, int i, int si, int j, int sj) -> R {
+        if constexpr (PL) {
+            if (in_plane(i, si, j, sj)) return pl.y[f][slot(i, si, j, sj)];
+        }
+        return at(a.y[f], gsite(i, si, j, sj));
+    };
+    auto PHI = [&](int k, int i, int si, R* o) {
+        if constexpr (PL) {
+            if (in_plane(i, si, 0, 0)) { unpack(pl.phi[k][slot(i, si, 0, 0)], o); return; }
+        }
+        unpack(at(a.phi[k], gsite(i, si, 0, 0)), o);
+    };
+
     // Own links and their phases.
     Qt<R> U[3];
     R cy[3], sy_[3], yv[3];
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-        U[i] = toq<V4, R>(at(a.q[i], s));
-        yv[i] = at(a.y[i], s);
+        U[i] = Q(i, 0, 0, 0, 0);
+        yv[i] = Y(i, 0, 0, 0, 0);
         sincos_fast(yv[i], &sy_[i], &cy[i]);
     }
 
     R phi[4 * ND];
 #pragma unroll
-    for (int k = 0; k < ND; k++) unpack(at(a.phi[k], s), phi + 4 * k);
+    for (int k = 0; k < ND; k++) PHI(k, 0, 0, phi + 4 * k);
 
     // Scalar force and currents.
     R F[4 * ND];
@@ -397,10 +518,9 @@ __device__ __forceinline__ void mom_site(const StepArgs<R>& a, const Coef& c, in
     R J[3][4];
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-        const unsigned sp = s + dp[i], sm = s + dm[i];
-        const Qt<R> Um = toq<V4, R>(at(a.q[i], sm));
+        const Qt<R> Um = Q(i, i, -1, 0, 0);
         R cm, smn;
-        sincos_fast(at(a.y[i], sm), &smn, &cm);
+        sincos_fast(Y(i, i, -1, 0, 0), &smn, &cm);
         J[i][0] = J[i][1] = J[i][2] = J[i][3] = R(0);
 #pragma unroll
         for (int k = 0; k < ND; k++) {
@@ -412,8 +532,8 @@ __device__ __forceinline__ void mom_site(const StepArgs<R>& a, const Coef& c, in
             // so the current on a link and the charges it moves at its two ends
             // are computed from the same small differences, not from O(1) terms.
             R fp[4], fm[4], tx[4], tm[4], d[4], dmv[4], dpv[4];
-            unpack(at(a.phi[k], sp), fp);
-            unpack(at(a.phi[k], sm), fm);
+            PHI(k, i, 1, fp);
+            PHI(k, i, -1, fm);
             transport(U[i], cy[i], sy_[i], false, phi + 4 * k, tx);  // T_i(x) phi(x)
             transport(Um, cm, smn, false, fm, tm);                    // T_i(x-i) phi(x-i)
 #pragma unroll
@@ -450,9 +570,9 @@ __device__ __forceinline__ void mom_site(const StepArgs<R>& a, const Coef& c, in
     for (int i = 0; i < 3; i++) {
 #pragma unroll
         for (int j = i + 1; j < 3; j++) {
-            const Qt<R> Ujxpi = toq<V4, R>(at(a.q[j], s + dp[i]));
-            const Qt<R> Uixpj = toq<V4, R>(at(a.q[i], s + dp[j]));
-            const R th = (at(a.y[i], s + dp[j]) - yv[i]) + (yv[j] - at(a.y[j], s + dp[i]));
+            const Qt<R> Ujxpi = Q(j, i, 1, 0, 0);
+            const Qt<R> Uixpj = Q(i, j, 1, 0, 0);
+            const R th = (Y(i, j, 1, 0, 0) - yv[i]) + (yv[j] - Y(j, i, 1, 0, 0));
             const Qt<R> A = qmul(qconj(Ujxpi), Uixpj);
             const Qt<R> C = qmul(U[j], qconj(U[i]));
             R v[3], w[3];
@@ -472,16 +592,14 @@ __device__ __forceinline__ void mom_site(const StepArgs<R>& a, const Coef& c, in
     // (link lo at b+hi), i > j makes x = b + lo (link hi at b+lo).
 #pragma unroll
     for (int j = 0; j < 3; j++) {
-        const unsigned smj = s + dm[j];
-        const Qt<R> Ujm = toq<V4, R>(at(a.q[j], smj));
-        const R yjm = at(a.y[j], smj);
+        const Qt<R> Ujm = Q(j, j, -1, 0, 0);
+        const R yjm = Y(j, j, -1, 0, 0);
 #pragma unroll
         for (int i = 0; i < 3; i++) {
             if (i == j) continue;
-            const unsigned sd = s + dp[i] + dm[j];
-            const Qt<R> Uim = toq<V4, R>(at(a.q[i], smj));
-            const Qt<R> Ujd = toq<V4, R>(at(a.q[j], sd));
-            const R yim = at(a.y[i], smj), yjd = at(a.y[j], sd);
+            const Qt<R> Uim = Q(i, j, -1, 0, 0);
+            const Qt<R> Ujd = Q(j, i, 1, j, -1);
+            const R yim = Y(i, j, -1, 0, 0), yjd = Y(j, i, 1, j, -1);
             R v[3], w[3];
             if (i < j) {
                 // lo = i, hi = j: A_b = U_j(x+i-j)^dag U_i(x), C_b = U_j(x-j) U_i(x-j)^dag.
@@ -525,10 +643,11 @@ __device__ __forceinline__ void mom_site(const StepArgs<R>& a, const Coef& c, in
             fn = make_float4(fh[0], fh[1], fh[2], fh[3]);
             st_nt(make_uint4(bf16_pack2(fr[0], fr[1]), bf16_pack2(fr[2], fr[3]), pw0, pw1), ptr_at(a.lo_s[k], s));
         } else {
-            pn.x = a.keep * p[0] + a.dt2 * F[4 * k + 0];
-            pn.y = a.keep * p[1] + a.dt2 * F[4 * k + 1];
-            pn.z = a.keep * p[2] + a.dt2 * F[4 * k + 2];
-            pn.w = a.keep * p[3] + a.dt2 * F[4 * k + 3];
+            // Explicit FMAs fix which product is fused (exact keep * p for keep = 1).
+            pn.x = fma(a.dt2, F[4 * k + 0], a.keep * p[0]);
+            pn.y = fma(a.dt2, F[4 * k + 1], a.keep * p[1]);
+            pn.z = fma(a.dt2, F[4 * k + 2], a.keep * p[2]);
+            pn.w = fma(a.dt2, F[4 * k + 3], a.keep * p[3]);
             fn.x = phi[4 * k + 0] + pn.x;
             fn.y = phi[4 * k + 1] + pn.y;
             fn.z = phi[4 * k + 2] + pn.z;
@@ -553,17 +672,17 @@ __device__ __forceinline__ void mom_site(const StepArgs<R>& a, const Coef& c, in
             en = make_float4(eh[0], eh[1], eh[2], eh[3]);
             st_nt(make_uint2(bf16_pack2(er[0], er[1]), bf16_pack2(er[2], er[3])), ptr_at(a.lo_E[i], s));
         } else {
-            en.x = a.keep * e[0] + a.dt2 * (M[i][0] + a.gsq[0] * J[i][0]);
-            en.y = a.keep * e[1] + a.dt2 * (M[i][1] + a.gsq[1] * J[i][1]);
-            en.z = a.keep * e[2] + a.dt2 * (M[i][2] + a.gsq[2] * J[i][2]);
-            en.w = a.keep * e[3] + a.dt2 * (M[i][3] + a.gsq[3] * J[i][3]);
+            en.x = fma(a.dt2, fma(a.gsq[0], J[i][0], M[i][0]), a.keep * e[0]);
+            en.y = fma(a.dt2, fma(a.gsq[1], J[i][1], M[i][1]), a.keep * e[1]);
+            en.z = fma(a.dt2, fma(a.gsq[2], J[i][2], M[i][2]), a.keep * e[2]);
+            en.w = fma(a.dt2, fma(a.gsq[3], J[i][3], M[i][3]), a.keep * e[3]);
         }
-        const R e2 = en.y * en.y + en.z * en.z + en.w * en.w;
+        const R e2 = fma(en.w, en.w, fma(en.z, en.z, en.y * en.y));
         if (!(e2 < R(1)) || !(fabs(en.x) < R(1))) bad = 1;
         const R yn = wrap_pi(yv[i] + asin_small(en.x));
         const Qt<R> u{sqrt(R(1) - e2), en.y, en.z, en.w};
         Qt<R> qn = qmul(u, U[i]);
-        const R inv = rsqrt_r(qn.w * qn.w + qn.x * qn.x + qn.y * qn.y + qn.z * qn.z);
+        const R inv = rsqrt_r(fma(qn.z, qn.z, fma(qn.y, qn.y, fma(qn.x, qn.x, qn.w * qn.w))));
         V4 qo;
         qo.x = qn.w * inv;
         qo.y = qn.x * inv;
@@ -578,13 +697,24 @@ __device__ __forceinline__ void mom_site(const StepArgs<R>& a, const Coef& c, in
 
 // MINW: minimum waves per SIMD the register allocation must allow (0: the
 // compiler's choice).  Higher occupancy hides more memory latency but may spill.
-template <class R, int ND, int MINW, bool CMP>
+template <class R, int ND, int MINW, bool CMP, bool PL = false>
 __global__ void __launch_bounds__(256, MINW > 0 ? MINW : 1) mom_step(StepArgs<R> a, Coef c) {
     const int4 t = a.tiles[blockIdx.x];
-    const int z = t.w * a.bz + (static_cast<int>(threadIdx.x) & (a.bz - 1));
-    const int y = t.z * a.by + (static_cast<int>(threadIdx.x) >> a.bz_shift);
-    if (z >= a.nz || y >= a.ny) return;
-    mom_site<R, ND, CMP>(a, c, t.y, y, z);
+    const int tz = static_cast<int>(threadIdx.x) & (a.bz - 1), ty = static_cast<int>(threadIdx.x) >> a.bz_shift;
+    const int z = t.w * a.bz + tz;
+    const int y = t.z * a.by + ty;
+    if constexpr (PL) {
+        static_assert(sizeof(R) == 4, "plane staging is FP32 only");
+        extern __shared__ float4 gft_plane_lds[];
+        PlaneLds pl = plane_layout<ND>(reinterpret_cast<char*>(gft_plane_lds), a.bz, a.by);
+        plane_fill<ND>(a, pl, t.y, t.z * a.by, t.w * a.bz, a.bz, a.by);
+        pl.r0 = (ty + 1) * pl.w + tz + 1;
+        if (z >= a.nz || y >= a.ny) return;
+        mom_site<R, ND, CMP, true>(a, c, t.y, y, z, pl);
+    } else {
+        if (z >= a.nz || y >= a.ny) return;
+        mom_site<R, ND, CMP>(a, c, t.y, y, z);
+    }
 }
 
 // One member of a batched ensemble: its step arguments and coefficients.
@@ -598,7 +728,9 @@ struct EnsMember {
 // another; XCD k (hardware blocks b with b % 8 == k) takes the k-th eighth of
 // the list and its blocks stride through it, so each XCD works on a compact
 // region of a few members.  Every block reaches the end of its range and exits.
-template <int MINW>
+// PL: each tile's plane is staged in LDS as in mom_step (the dynamic LDS holds
+// the largest member tile).
+template <int MINW, bool PL>
 __global__ void __launch_bounds__(256, MINW) ens_step(const EnsMember* __restrict__ members, const int4* __restrict__ work,
                                                       long long nwork, int blocks_per_xcd) {
     const long long xcd = blockIdx.x % 8, idx = blockIdx.x / 8;
@@ -610,9 +742,19 @@ __global__ void __launch_bounds__(256, MINW) ens_step(const EnsMember* __restric
         // compiles to the same instructions as mom_step's.
         const StepArgs<float> a = members[t.x].a;
         const Coef c = members[t.x].c;
-        const int z = t.w * a.bz + (static_cast<int>(threadIdx.x) & (a.bz - 1));
-        const int y = t.z * a.by + (static_cast<int>(threadIdx.x) >> a.bz_shift);
-        if (z < a.nz && y < a.ny) mom_site<float, 1, true>(a, c, t.y, y, z);
+        const int tz = static_cast<int>(threadIdx.x) & (a.bz - 1), ty = static_cast<int>(threadIdx.x) >> a.bz_shift;
+        const int z = t.w * a.bz + tz;
+        const int y = t.z * a.by + ty;
+        if constexpr (PL) {
+            extern __shared__ float4 gft_plane_lds[];
+            __syncthreads();   // every wave is done with the previous tile's LDS image
+            PlaneLds pl = plane_layout<1>(reinterpret_cast<char*>(gft_plane_lds), a.bz, a.by);
+            plane_fill<1>(a, pl, t.y, t.z * a.by, t.w * a.bz, a.bz, a.by);
+            pl.r0 = (ty + 1) * pl.w + tz + 1;
+            if (z < a.nz && y < a.ny) mom_site<float, 1, true, true>(a, c, t.y, y, z, pl);
+        } else {
+            if (z < a.nz && y < a.ny) mom_site<float, 1, true>(a, c, t.y, y, z);
+        }
     }
 }
 
@@ -1176,6 +1318,18 @@ StepArgs<R> make_step_args(MomentumState& m, MomFields<R>& f, const Coef& c, dou
     return a;
 }
 
+// Largest LDS tile a step block may request (the fixed LDS limit of a gfx9 workgroup).
+constexpr size_t kPlaneLdsMax = 65536;
+
+template <bool CMP, bool PL>
+void launch_step_f32(const StepArgs<float>& a, const Coef& c, int min_waves, long long nb, hipStream_t st) {
+    const size_t lds = PL ? plane_lds_bytes(a.bz, a.by, c.nd) : 0;
+    if (c.nd == 2) mom_step<float, 2, 0, CMP, PL><<<nb, 256, lds, st>>>(a, c);
+    else if (min_waves == 3) mom_step<float, 1, 3, CMP, PL><<<nb, 256, lds, st>>>(a, c);
+    else if (min_waves == 4) mom_step<float, 1, 4, CMP, PL><<<nb, 256, lds, st>>>(a, c);
+    else mom_step<float, 1, 0, CMP, PL><<<nb, 256, lds, st>>>(a, c);
+}
+
 inline int momentum_step(MomentumState& m, const Coef& c, double damping_step, hipStream_t st, unsigned* err,
                          std::string& msg) {
     if (m.bz * m.by != 256 || m.bz < 1 || m.by < 1) { msg = "momentum step: tile must hold 256 sites"; return GFT_EINVAL; }
@@ -1187,16 +1341,15 @@ inline int momentum_step(MomentumState& m, const Coef& c, double damping_step, h
         else mom_step<double, 2, 0, false><<<nb, 256, 0, st>>>(a, c);
     } else {
         auto a = make_step_args<float>(m, m.ff, c, damping_step, err);
+        // Staged and unstaged kernels give bitwise the same fields; a tile whose
+        // LDS image exceeds the workgroup limit runs unstaged.
+        const bool pl = m.stage && plane_lds_bytes(m.bz, m.by, c.nd) <= kPlaneLdsMax;
         if (m.cmp) {
-            if (c.nd == 2) mom_step<float, 2, 0, true><<<nb, 256, 0, st>>>(a, c);
-            else if (m.min_waves == 3) mom_step<float, 1, 3, true><<<nb, 256, 0, st>>>(a, c);
-            else if (m.min_waves == 4) mom_step<float, 1, 4, true><<<nb, 256, 0, st>>>(a, c);
-            else mom_step<float, 1, 0, true><<<nb, 256, 0, st>>>(a, c);
+            if (pl) launch_step_f32<true, true>(a, c, m.min_waves, nb, st);
+            else launch_step_f32<true, false>(a, c, m.min_waves, nb, st);
         } else {
-            if (c.nd == 2) mom_step<float, 2, 0, false><<<nb, 256, 0, st>>>(a, c);
-            else if (m.min_waves == 3) mom_step<float, 1, 3, false><<<nb, 256, 0, st>>>(a, c);
-            else if (m.min_waves == 4) mom_step<float, 1, 4, false><<<nb, 256, 0, st>>>(a, c);
-            else mom_step<float, 1, 0, false><<<nb, 256, 0, st>>>(a, c);
+            if (pl) launch_step_f32<false, true>(a, c, m.min_waves, nb, st);
+            else launch_step_f32<false, false>(a, c, m.min_waves, nb, st);
         }
     }
     if (hipGetLastError() != hipSuccess) { msg = "momentum step launch failed"; return GFT_EHIP; }

@@ -36,7 +36,7 @@ _L.gft_evolve.argtypes = [C.c_void_p, C.c_uint, C.c_uint, C.c_uint, _P(GftObs), 
 _L.gft_observe.argtypes = [C.c_void_p, _P(GftObs)]
 _L.gft_wilson_loops.argtypes = [C.c_void_p, _P(GftLoops)]
 _L.gft_sync.argtypes = [C.c_void_p]
-_L.gft_set_launch.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]
+_L.gft_set_launch.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
 _L.gft_bench_copy.argtypes = [C.c_size_t, C.c_int, _P(C.c_double)]
 _L.gft_last_step_ms.argtypes = [C.c_void_p, _P(C.c_double), _P(C.c_double), _P(C.c_uint)]
 _L.gft_ensemble_create.argtypes = [_P(GftDesc), C.c_int, _P(C.c_void_p)]
@@ -174,8 +174,8 @@ class Evolver:
         self._check(_L.gft_wilson_loops(self._h, C.byref(o)))
         return o
 
-    def set_launch(self, bz: int, by: int, order: int = 2, min_waves: int = 3):
-        self._check(_L.gft_set_launch(self._h, bz, by, int(order), min_waves))
+    def set_launch(self, bz: int, by: int, order: int = 2, min_waves: int = 3, stage: int = 1):
+        self._check(_L.gft_set_launch(self._h, bz, by, int(order), min_waves, int(stage)))
 
     def sync(self):
         self._check(_L.gft_sync(self._h))

@@ -141,10 +141,14 @@ int gft_wilson_loops(gft_ctx* ctx, gft_loops* out);
  * contiguous ranges; 2, the default: each XCD sweeps the x-planes of its own
  * y-band); and the occupancy target of the FP32 SM kernel (min_waves 3 or 4
  * waves per SIMD, the default being 3; 0 leaves the register allocation
- * unconstrained).  Every tile shape and order gives bitwise the same fields;
+ * unconstrained); and `stage` (1, the default, for the FP32 forms: each block
+ * gathers q, y and phi of its tile and a one-site ring in y and z into LDS, so
+ * that only the neighbours in the planes x +- 1 are read from global memory;
+ * 0 reads every neighbour from global memory; FP64 forms always run unstaged).
+ * Every tile shape, order and staging choice gives bitwise the same fields;
  * the autotuner (gftevolver_amd/tune.py) picks the fastest.  No reference
  * counterpart (the reference has no launch configuration). */
-int gft_set_launch(gft_ctx* ctx, int bz, int by, int order, int min_waves);
+int gft_set_launch(gft_ctx* ctx, int bz, int by, int order, int min_waves, int stage);
 
 /* Measurement helper (no reference counterpart): copies `bytes` between two
  * device buffers `reps` times with a plain streaming copy kernel
