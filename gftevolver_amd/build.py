@@ -18,6 +18,11 @@ HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     # Packed FP32 instructions stay off: the compiler must not emit them.
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # Contraction per source expression only (HIP's default also lets the
+    # backend fuse across expressions, where the product it picks depends on
+    # the surrounding code): the same expression rounds the same way in every
+    # kernel variant, which the bitwise launch-shape and staging tests rely on.
+    "-ffp-contract=on",
     "-Wno-unused-result", "-Wno-unused-value",
 ]
 

@@ -435,7 +435,7 @@ extern "C" int gft_evolve(gft_ctx* ctx, unsigned nsteps, unsigned energy_freq, u
 
 extern "C" int gft_set_launch(gft_ctx* ctx, int bz, int by, int order, int min_waves, int stage) {
     if (!ctx) return GFT_EINVAL;
-    if (stage != 0 && stage != 1) return fail(ctx, GFT_EINVAL, "gft_set_launch: stage must be 0 or 1");
+    if (stage < 0 || stage > 2) return fail(ctx, GFT_EINVAL, "gft_set_launch: stage must be 0, 1 or 2");
     if (min_waves != 0 && min_waves != 3 && min_waves != 4)
         return fail(ctx, GFT_EINVAL, "gft_set_launch: min_waves must be 0, 3 or 4");
     if (order < 0 || order > 2) return fail(ctx, GFT_EINVAL, "gft_set_launch: order must be 0, 1 or 2");
@@ -548,6 +548,7 @@ struct gft_ensemble {
     long long nwork = 0;
     int grid = 0, blocks_per_xcd = 0;
     size_t lds = 0;          // dynamic LDS per block of the staged kernel (0: unstaged)
+    int stage = 0;           // staging mode of the ensemble kernel
     unsigned* err_dev = nullptr;
     bool dirty = true;
     hipGraph_t graph = nullptr;
@@ -585,15 +586,15 @@ static int ens_prepare(gft_ensemble* ens) {
     const size_t nm = ens->members.size();
     std::vector<EnsMember> host[2];
     std::vector<int4> work;
-    // Staged when every member asks for it and the largest tile's LDS image fits.
-    bool stage = true;
-    size_t lds = 0;
+    // The staging mode every member can run (the lowest over the members).
+    int stage = 2;
     for (size_t i = 0; i < nm; i++) {
         gft_ctx* c = ens->members[i];
-        stage = stage && c->mom.stage;
-        lds = std::max(lds, plane_lds_bytes(c->mom.bz, c->mom.by, 1));
+        stage = std::min(stage, step_stage_mode(c->mom.stage, c->mom.bz, c->mom.by, 1));
     }
-    if (lds > kPlaneLdsMax) stage = false;
+    size_t lds = 0;
+    for (size_t i = 0; i < nm; i++)
+        lds = std::max(lds, step_lds_bytes(ens->members[i]->mom.bz, ens->members[i]->mom.by, 1, stage));
     for (size_t i = 0; i < nm; i++) {
         gft_ctx* c = ens->members[i];
         const int saved = c->mom.cur;
@@ -623,7 +624,8 @@ static int ens_prepare(gft_ensemble* ens) {
     EK(hipGetDevice(&dev));
     EK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     EK(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
-    ens->lds = stage ? lds : 0;
+    ens->lds = lds;
+    ens->stage = stage;
     // Three 256-thread blocks per CU (the kernel's register occupancy) or as
     // many as the CU's LDS holds, a multiple of 8.
     long long per_cu = 3;
@@ -712,8 +714,9 @@ int gft_ensemble_evolve(gft_ensemble* ens, unsigned nsteps, int use_graph) {
     if (rc) return rc;
     const dim3 grid(ens->grid), block(256);
     auto launch = [&](int p) {
-        if (ens->lds) ens_step<3, true><<<grid, block, ens->lds, ens->stream>>>(ens->args[p], ens->work, ens->nwork, ens->blocks_per_xcd);
-        else ens_step<3, false><<<grid, block, 0, ens->stream>>>(ens->args[p], ens->work, ens->nwork, ens->blocks_per_xcd);
+        if (ens->stage == 2) ens_step<3, 2><<<grid, block, ens->lds, ens->stream>>>(ens->args[p], ens->work, ens->nwork, ens->blocks_per_xcd);
+        else if (ens->stage == 1) ens_step<3, 1><<<grid, block, ens->lds, ens->stream>>>(ens->args[p], ens->work, ens->nwork, ens->blocks_per_xcd);
+        else ens_step<3, 0><<<grid, block, 0, ens->stream>>>(ens->args[p], ens->work, ens->nwork, ens->blocks_per_xcd);
     };
     if (use_graph && nsteps >= 2 && ens->graph_parity != parity) {
         ens_drop_graph(ens);

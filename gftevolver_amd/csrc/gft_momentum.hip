@@ -70,7 +70,7 @@ struct MomentumState {
     int bz = 64, by = 4;     // tile shape of the step kernel
     int order = 2;           // block order (see decode_block)
     int min_waves = 3;       // occupancy target of the FP32 SM step kernel (0 = unconstrained, 3, 4)
-    int stage = 1;           // FP32 step kernels: in-plane neighbours staged in LDS (bitwise the same fields)
+    int stage = 2;           // FP32 step kernels: 0 global loads, 1 LDS plane tile, 2 + shared plaquettes
     int4* tiles = nullptr;   // device tile table of the current launch shape
     int tiles_key[3] = {0, 0, -1};
 };
@@ -442,6 +442,116 @@ __device__ __forceinline__ void plane_fill(const StepArgs<float>& a, const Plane
     __syncthreads();
 }
 
+// Kick and drift of site s from its forces: F on the doublet(s), M + g^2 J on
+// the links (shared by every step kernel variant).
+// The site's own momenta and electric fields (with residuals), read once by
+// the kick; loadable ahead of it.
+template <class R, int ND>
+struct OwnState {
+    typename V4T<R>::type P[ND];
+    uint4 ls[ND];
+    typename V4T<R>::type E[3];
+    uint2 lE[3];
+};
+template <class R, int ND, bool CMP>
+__device__ __forceinline__ void load_own(const StepArgs<R>& a, unsigned s, OwnState<R, ND>& o) {
+#pragma unroll
+    for (int k = 0; k < ND; k++) {
+        o.P[k] = ld_nt(ptr_at(a.P[k], s));
+        if constexpr (CMP) o.ls[k] = ld_nt(ptr_at(a.lo_s[k], s));
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        o.E[i] = ld_nt(ptr_at(a.E[i], s));
+        if constexpr (CMP) o.lE[i] = ld_nt(ptr_at(a.lo_E[i], s));
+    }
+}
+
+// PRE: own state already loaded into o (else loaded here, next to its use).
+template <class R, int ND, bool CMP, bool PRE = false>
+__device__ __forceinline__ void kick_drift(const StepArgs<R>& a, unsigned s, const Qt<R>* U, const R* yv,
+                                           const R* phi, const R* F, const R (*J)[4], const R (*M)[4],
+                                           const OwnState<R, ND>* o = nullptr) {
+    using V4 = typename V4T<R>::type;
+#pragma unroll
+    for (int k = 0; k < ND; k++) {
+        R p[4];
+        if constexpr (PRE) unpack(o->P[k], p);
+        else unpack(ld_nt(ptr_at(a.P[k], s)), p);
+        V4 pn, fn;
+        if constexpr (CMP) {
+            uint4 L;
+            if constexpr (PRE) L = o->ls[k];
+            else L = ld_nt(ptr_at(a.lo_s[k], s));
+            const float fl[4] = {bf16_lo(L.x), bf16_hi(L.x), bf16_lo(L.y), bf16_hi(L.y)};
+            const float pl[4] = {bf16_lo(L.z), bf16_hi(L.z), bf16_lo(L.w), bf16_hi(L.w)};
+            float ph[4], pr[4], fh[4], fr[4];
+#pragma unroll
+            for (int m = 0; m < 4; m++) ph[m] = cmp_update(p[m], pl[m], a.keep, a.dt2 * F[4 * k + m], &pr[m]);
+            // phi(t+dt) = phi(t) + P(t+dt) with P as stored (its residual rounded).
+            const unsigned pw0 = bf16_pack2(pr[0], pr[1]), pw1 = bf16_pack2(pr[2], pr[3]);
+            const float prr[4] = {bf16_lo(pw0), bf16_hi(pw0), bf16_lo(pw1), bf16_hi(pw1)};
+#pragma unroll
+            for (int m = 0; m < 4; m++) fh[m] = cmp_add(phi[4 * k + m], fl[m], ph[m], prr[m], &fr[m]);
+            pn = make_float4(ph[0], ph[1], ph[2], ph[3]);
+            fn = make_float4(fh[0], fh[1], fh[2], fh[3]);
+            st_nt(make_uint4(bf16_pack2(fr[0], fr[1]), bf16_pack2(fr[2], fr[3]), pw0, pw1), ptr_at(a.lo_s[k], s));
+        } else {
+            // Explicit FMAs fix which product is fused (exact keep * p for keep = 1).
+            pn.x = fma(a.dt2, F[4 * k + 0], a.keep * p[0]);
+            pn.y = fma(a.dt2, F[4 * k + 1], a.keep * p[1]);
+            pn.z = fma(a.dt2, F[4 * k + 2], a.keep * p[2]);
+            pn.w = fma(a.dt2, F[4 * k + 3], a.keep * p[3]);
+            fn.x = phi[4 * k + 0] + pn.x;
+            fn.y = phi[4 * k + 1] + pn.y;
+            fn.z = phi[4 * k + 2] + pn.z;
+            fn.w = phi[4 * k + 3] + pn.w;
+        }
+        st_nt(pn, ptr_at(a.P[k], s));
+        st_nt(fn, ptr_at(a.phi_out[k], s));
+    }
+    unsigned bad = 0;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        R e[4];
+        if constexpr (PRE) unpack(o->E[i], e);
+        else unpack(ld_nt(ptr_at(a.E[i], s)), e);
+        V4 en;
+        if constexpr (CMP) {
+            uint2 L;
+            if constexpr (PRE) L = o->lE[i];
+            else L = ld_nt(ptr_at(a.lo_E[i], s));
+            const float el[4] = {bf16_lo(L.x), bf16_hi(L.x), bf16_lo(L.y), bf16_hi(L.y)};
+            float eh[4], er[4];
+#pragma unroll
+            for (int m = 0; m < 4; m++)
+                eh[m] = cmp_update(e[m], el[m], a.keep, a.dt2 * (M[i][m] + a.gsq[m] * J[i][m]), &er[m]);
+            en = make_float4(eh[0], eh[1], eh[2], eh[3]);
+            st_nt(make_uint2(bf16_pack2(er[0], er[1]), bf16_pack2(er[2], er[3])), ptr_at(a.lo_E[i], s));
+        } else {
+            en.x = fma(a.dt2, fma(a.gsq[0], J[i][0], M[i][0]), a.keep * e[0]);
+            en.y = fma(a.dt2, fma(a.gsq[1], J[i][1], M[i][1]), a.keep * e[1]);
+            en.z = fma(a.dt2, fma(a.gsq[2], J[i][2], M[i][2]), a.keep * e[2]);
+            en.w = fma(a.dt2, fma(a.gsq[3], J[i][3], M[i][3]), a.keep * e[3]);
+        }
+        const R e2 = fma(en.w, en.w, fma(en.z, en.z, en.y * en.y));
+        if (!(e2 < R(1)) || !(fabs(en.x) < R(1))) bad = 1;
+        const R yn = wrap_pi(yv[i] + asin_small(en.x));
+        const Qt<R> u{sqrt(R(1) - e2), en.y, en.z, en.w};
+        Qt<R> qn = qmul(u, U[i]);
+        const R inv = rsqrt_r(fma(qn.z, qn.z, fma(qn.y, qn.y, fma(qn.x, qn.x, qn.w * qn.w))));
+        V4 qo;
+        qo.x = qn.w * inv;
+        qo.y = qn.x * inv;
+        qo.z = qn.y * inv;
+        qo.w = qn.z * inv;
+        st_nt(en, ptr_at(a.E[i], s));
+        st_nt(yn, ptr_at(a.y_out[i], s));
+        st_nt(qo, ptr_at(a.q_out[i], s));
+    }
+    if (bad) atomicOr(a.err, 1u);
+}
+
 // One site-update at (x, y, z): forces, kick and drift (shared by the
 // single-lattice kernel and the ensemble kernel).  PL: in-plane neighbours and
 // own q, y, phi come from the block's LDS tile `pl` (FP32 only).
@@ -621,98 +731,312 @@ __device__ __forceinline__ void mom_site(const StepArgs<R>& a, const Coef& c, in
         }
     }
 
-    // Kick and drift.
+    kick_drift<R, ND, CMP>(a, s, U, yv, phi, F, J, M);
+}
+
+// ---- shared plaquettes and covariant differences (FP32 SM, staged) ----------
+// Every plaquette is needed by the three sites whose links it touches and every
+// covariant difference d_i(x) = phi(x+i) - T_i(x) phi(x) by two sites.  Here a
+// block evaluates each in-plane one once: in phase A every site computes its
+// own three plaquettes (base b = x) and its own d_i, and the sites of the ring
+// row y0 - 1 and ring column z0 - 1 compute theirs; after a barrier they go to
+// LDS (over the plane tile, no longer read) and in phase B each site reads the
+// plaquettes based at x - e_y and x - e_z and the differences d_y(x - e_y),
+// d_z(x - e_z) instead of recomputing them.  Plaquettes based at x - e_x and
+// d_x(x - e_x) lie in the plane x - 1 and are still computed per site.  Every
+// shared value is produced by the same expressions, in the same operation
+// order, as the per-site evaluation in mom_site, and the forces accumulate in
+// the same order: the fields are bitwise those of mom_site.
+
+// One plaquette of the plane (lo, hi) based at b from its four links and
+// phases (U_hi(b+lo) = Uhi_lo, U_lo(b+hi) = Ulo_hi):
+//   K_b = A_b C_b, A_b = U_hi(b+lo)^dag U_lo(b+hi), C_b = U_hi(b) U_lo(b)^dag,
+//   theta_b = (y_lo(b+hi) - y_lo(b)) + (y_hi(b) - y_hi(b+lo));
+// v = vec K_b, wc = rot(C_b) v, wu = rot(U_hi(b+lo)) v, st = sin theta_b.
+template <class R>
+struct Plaq {
+    R v[3], wc[3], wu[3], st;
+};
+template <class R>
+__device__ __forceinline__ void plaquette(const Qt<R>& Ulo, const Qt<R>& Uhi, const Qt<R>& Uhi_lo, const Qt<R>& Ulo_hi,
+                                          R ylo, R yhi, R yhi_lo, R ylo_hi, Plaq<R>& p) {
+    const R th = (ylo_hi - ylo) + (yhi - yhi_lo);
+    const Qt<R> A = qmul(qconj(Uhi_lo), Ulo_hi);
+    const Qt<R> C = qmul(Uhi, qconj(Ulo));
+    qvec(A, C, p.v);
+    qrot(C, p.v, p.wc);
+    qrot(Uhi_lo, p.v, p.wu);
+    p.st = sin_fast(th);
+}
+
+// Shared values in LDS, one float4 per site of the tile plus ring row and ring
+// column: index (ty + 1) * (bz + 1) + (tz + 1) for ty in [-1, by), tz in [-1, bz).
+//   d1, d2   covariant differences d_y, d_z at the site
+//   p01, p02, p12   (rot(U_hi(b+lo)) v, sin theta) of the plaquette based there
+struct ShareLds {
+    float4* d1;
+    float4* d2;
+    float4* p01;
+    float4* p02;
+    float4* p12;
+    int w;   // bz + 1
+};
+__host__ __device__ inline size_t share_lds_bytes(int bz, int by) { return (size_t)(bz + 1) * (by + 1) * 5 * 16; }
+__device__ __forceinline__ ShareLds share_layout(char* base, int bz, int by) {
+    const int n = (bz + 1) * (by + 1);
+    float4* f = reinterpret_cast<float4*>(base);
+    return ShareLds{f, f + n, f + 2 * n, f + 3 * n, f + 4 * n, bz + 1};
+}
+__device__ __forceinline__ float4 pack_wu(const Plaq<float>& p) { return make_float4(p.wu[0], p.wu[1], p.wu[2], p.st); }
+
+// Whole block, after plane_fill: one site-update per valid thread (tile
+// coordinates ty, tz; site (x, y0 + ty, z0 + tz)) with the in-plane plaquettes
+// and differences shared.  Needs bz + by <= 256 (one ring item per thread).
+template <bool CMP>
+__device__ __forceinline__ void mom_block_shared(const StepArgs<float>& a, const Coef& c, int x, int y0, int z0, int ty,
+                                                 int tz, bool valid, const PlaneLds& pl, char* lds) {
+    using R = float;
+    const int bz = a.bz, W = pl.w;
+    const int sx = a.ny * a.nz;
+    const int xp = (x == a.nx - 1) ? 0 : x + 1, xm = (x == 0) ? a.nx - 1 : x - 1;
+
+    // ---- phase A: own site ----
+    const int y = y0 + ty, z = z0 + tz;
+    const unsigned s = static_cast<unsigned>(x * sx + y * a.nz + z);
+    const int r0 = pl.r0;
+    const unsigned sm0 = static_cast<unsigned>(xm * sx + y * a.nz + z);   // x - e_0
+    Qt<R> U[3];
+    R cy[3], sy_[3], yv[3], phi[4], F[4], J[3][4], M[3][4], dpv[3][4], dsh[3][4];
+    Plaq<R> P[3];   // at x: (0,1), (0,2), (1,2)
+    OwnState<R, 1> own;
+    if (valid) {
+        load_own<R, 1, CMP>(a, s, own);   // issued early: its latency overlaps phase A
 #pragma unroll
-    for (int k = 0; k < ND; k++) {
-        R p[4];
-        unpack(ld_nt(ptr_at(a.P[k], s)), p);
-        V4 pn, fn;
-        if constexpr (CMP) {
-            const uint4 L = ld_nt(ptr_at(a.lo_s[k], s));
-            const float fl[4] = {bf16_lo(L.x), bf16_hi(L.x), bf16_lo(L.y), bf16_hi(L.y)};
-            const float pl[4] = {bf16_lo(L.z), bf16_hi(L.z), bf16_lo(L.w), bf16_hi(L.w)};
-            float ph[4], pr[4], fh[4], fr[4];
-#pragma unroll
-            for (int m = 0; m < 4; m++) ph[m] = cmp_update(p[m], pl[m], a.keep, a.dt2 * F[4 * k + m], &pr[m]);
-            // phi(t+dt) = phi(t) + P(t+dt) with P as stored (its residual rounded).
-            const unsigned pw0 = bf16_pack2(pr[0], pr[1]), pw1 = bf16_pack2(pr[2], pr[3]);
-            const float prr[4] = {bf16_lo(pw0), bf16_hi(pw0), bf16_lo(pw1), bf16_hi(pw1)};
-#pragma unroll
-            for (int m = 0; m < 4; m++) fh[m] = cmp_add(phi[4 * k + m], fl[m], ph[m], prr[m], &fr[m]);
-            pn = make_float4(ph[0], ph[1], ph[2], ph[3]);
-            fn = make_float4(fh[0], fh[1], fh[2], fh[3]);
-            st_nt(make_uint4(bf16_pack2(fr[0], fr[1]), bf16_pack2(fr[2], fr[3]), pw0, pw1), ptr_at(a.lo_s[k], s));
-        } else {
-            // Explicit FMAs fix which product is fused (exact keep * p for keep = 1).
-            pn.x = fma(a.dt2, F[4 * k + 0], a.keep * p[0]);
-            pn.y = fma(a.dt2, F[4 * k + 1], a.keep * p[1]);
-            pn.z = fma(a.dt2, F[4 * k + 2], a.keep * p[2]);
-            pn.w = fma(a.dt2, F[4 * k + 3], a.keep * p[3]);
-            fn.x = phi[4 * k + 0] + pn.x;
-            fn.y = phi[4 * k + 1] + pn.y;
-            fn.z = phi[4 * k + 2] + pn.z;
-            fn.w = phi[4 * k + 3] + pn.w;
+        for (int i = 0; i < 3; i++) {
+            U[i] = toq<float4, R>(pl.q[i][r0]);
+            yv[i] = pl.y[i][r0];
+            sincos_fast(yv[i], &sy_[i], &cy[i]);
         }
-        st_nt(pn, ptr_at(a.P[k], s));
-        st_nt(fn, ptr_at(a.phi_out[k], s));
-    }
-    unsigned bad = 0;
+        unpack(pl.phi[0][r0], phi);
+        potential_derivative<R>(c, phi, F);
 #pragma unroll
-    for (int i = 0; i < 3; i++) {
-        R e[4];
-        unpack(ld_nt(ptr_at(a.E[i], s)), e);
-        V4 en;
-        if constexpr (CMP) {
-            const uint2 L = ld_nt(ptr_at(a.lo_E[i], s));
-            const float el[4] = {bf16_lo(L.x), bf16_hi(L.x), bf16_lo(L.y), bf16_hi(L.y)};
-            float eh[4], er[4];
+        for (int k = 0; k < 4; k++) F[k] = -F[k];
+        // Neighbours x + e_i: plane x + 1 (global) for i = 0, LDS for i = 1, 2.
+        const unsigned sp0 = static_cast<unsigned>(xp * sx + y * a.nz + z);
+        const int step[3] = {0, W, 1};
 #pragma unroll
-            for (int m = 0; m < 4; m++)
-                eh[m] = cmp_update(e[m], el[m], a.keep, a.dt2 * (M[i][m] + a.gsq[m] * J[i][m]), &er[m]);
-            en = make_float4(eh[0], eh[1], eh[2], eh[3]);
-            st_nt(make_uint2(bf16_pack2(er[0], er[1]), bf16_pack2(er[2], er[3])), ptr_at(a.lo_E[i], s));
-        } else {
-            en.x = fma(a.dt2, fma(a.gsq[0], J[i][0], M[i][0]), a.keep * e[0]);
-            en.y = fma(a.dt2, fma(a.gsq[1], J[i][1], M[i][1]), a.keep * e[1]);
-            en.z = fma(a.dt2, fma(a.gsq[2], J[i][2], M[i][2]), a.keep * e[2]);
-            en.w = fma(a.dt2, fma(a.gsq[3], J[i][3], M[i][3]), a.keep * e[3]);
+        for (int i = 0; i < 3; i++) {
+            R fp[4], tx[4];
+            if (i == 0) unpack(at(a.phi[0], sp0), fp);
+            else unpack(pl.phi[0][r0 + step[i]], fp);
+            transport(U[i], cy[i], sy_[i], false, phi, tx);
+            J[i][0] = J[i][1] = J[i][2] = J[i][3] = R(0);
+#pragma unroll
+            for (int m = 0; m < 4; m++) dsh[i][m] = fp[m] - tx[m];
+            transport(U[i], cy[i], sy_[i], true, dsh[i], dpv[i]);
+            bilinears_add(tx, dsh[i], J[i]);
         }
-        const R e2 = fma(en.w, en.w, fma(en.z, en.z, en.y * en.y));
-        if (!(e2 < R(1)) || !(fabs(en.x) < R(1))) bad = 1;
-        const R yn = wrap_pi(yv[i] + asin_small(en.x));
-        const Qt<R> u{sqrt(R(1) - e2), en.y, en.z, en.w};
-        Qt<R> qn = qmul(u, U[i]);
-        const R inv = rsqrt_r(fma(qn.z, qn.z, fma(qn.y, qn.y, fma(qn.x, qn.x, qn.w * qn.w))));
-        V4 qo;
-        qo.x = qn.w * inv;
-        qo.y = qn.x * inv;
-        qo.z = qn.y * inv;
-        qo.w = qn.z * inv;
-        st_nt(en, ptr_at(a.E[i], s));
-        st_nt(yn, ptr_at(a.y_out[i], s));
-        st_nt(qo, ptr_at(a.q_out[i], s));
+        // d_x(x - e_x) from the plane x - 1; the scalar force's i = 0 term.
+        {
+            const Qt<R> Um = toq<float4, R>(at(a.q[0], sm0));
+            R cm, smn, fm[4], tm[4], dmv0[4];
+            sincos_fast(at(a.y[0], sm0), &smn, &cm);
+            unpack(at(a.phi[0], sm0), fm);
+            transport(Um, cm, smn, false, fm, tm);
+#pragma unroll
+            for (int m = 0; m < 4; m++) dmv0[m] = phi[m] - tm[m];
+#pragma unroll
+            for (int m = 0; m < 4; m++) F[m] += a.inv_sp[0] * (dpv[0][m] - dmv0[m]);
+        }
+        // Plaquettes based at x and their contributions to the links at x.
+#pragma unroll
+        for (int i = 0; i < 3; i++) M[i][0] = M[i][1] = M[i][2] = M[i][3] = R(0);
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+#pragma unroll
+            for (int j = i + 1; j < 3; j++) {
+                Plaq<R>& pq = P[i + j - 1];
+                Qt<R> Uhi_lo, Ulo_hi;
+                R yhi_lo, ylo_hi;
+                if (i == 0) {   // U_j(x + e_0): plane x + 1
+                    Uhi_lo = toq<float4, R>(at(a.q[j], sp0));
+                    yhi_lo = at(a.y[j], sp0);
+                } else {
+                    Uhi_lo = toq<float4, R>(pl.q[j][r0 + step[i]]);
+                    yhi_lo = pl.y[j][r0 + step[i]];
+                }
+                Ulo_hi = toq<float4, R>(pl.q[i][r0 + step[j]]);
+                ylo_hi = pl.y[i][r0 + step[j]];
+                plaquette(U[i], U[j], Uhi_lo, Ulo_hi, yv[i], yv[j], yhi_lo, ylo_hi, pq);
+                M[i][0] += a.inv_sp[j] * pq.st;
+                M[j][0] -= a.inv_sp[i] * pq.st;
+#pragma unroll
+                for (int m = 0; m < 3; m++) {
+                    M[i][1 + m] += a.inv_sp[j] * pq.v[m];
+                    M[j][1 + m] -= a.inv_sp[i] * pq.wc[m];
+                }
+            }
+        }
+        // Plaquettes based at x - e_0 (plane x - 1): the j = 0 terms of links 1
+        // and 2, which follow the plaquettes at x in their force sums.
+        {
+            const Qt<R> U0m = toq<float4, R>(at(a.q[0], sm0));
+            const R y0m = at(a.y[0], sm0);
+            const unsigned sm0p1 = static_cast<unsigned>(xm * sx + (y == a.ny - 1 ? 0 : y + 1) * a.nz + z);
+            const unsigned sm0p2 = static_cast<unsigned>(xm * sx + y * a.nz + (z == a.nz - 1 ? 0 : z + 1));
+#pragma unroll
+            for (int i = 1; i < 3; i++) {
+                const unsigned sd = i == 1 ? sm0p1 : sm0p2;       // x + e_i - e_0
+                Plaq<R> pq;
+                plaquette(U0m, toq<float4, R>(at(a.q[i], sm0)), U[i], toq<float4, R>(at(a.q[0], sd)), y0m,
+                          at(a.y[i], sm0), yv[i], at(a.y[0], sd), pq);
+                M[i][0] += a.inv_sp[0] * pq.st;
+#pragma unroll
+                for (int m = 0; m < 3; m++) M[i][1 + m] += a.inv_sp[0] * pq.wu[m];
+            }
+        }
     }
-    if (bad) atomicOr(a.err, 1u);
+
+    // ---- phase A: ring item (y-ring site for k < bz, z-ring site after) ----
+    const int k = static_cast<int>(threadIdx.x);
+    const bool ring = k < bz + a.by;
+    const int rdir = k < bz ? 1 : 2;                      // the ring's direction
+    const int rt = k - bz;                                // z-ring: tile row
+    const bool ring_valid = ring && (rdir == 1 ? z0 + k < a.nz : y0 + rt < a.ny);
+    float4 ring_d, ring_p0, ring_p12;
+    if (ring_valid) {
+        const int rb = rdir == 1 ? k + 1 : (rt + 1) * W;      // plane slot of the ring site b
+        const int rbr = rdir == 1 ? rb + W : rb + 1;          // b + e_r (a tile site)
+        const int yy = rdir == 1 ? (y0 == 0 ? a.ny - 1 : y0 - 1) : y0 + rt;
+        const int zz = rdir == 1 ? z0 + k : (z0 == 0 ? a.nz - 1 : z0 - 1);
+        const unsigned sbp0 = static_cast<unsigned>(xp * sx + yy * a.nz + zz);   // b + e_0
+        Qt<R> Ub[3];
+        R yb[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            Ub[i] = toq<float4, R>(pl.q[i][rb]);
+            yb[i] = pl.y[i][rb];
+        }
+        // d_r(b) = phi(b + e_r) - T_r(b) phi(b).
+        {
+            const Qt<R> Ur = rdir == 1 ? Ub[1] : Ub[2];
+            const R yr = rdir == 1 ? yb[1] : yb[2];
+            R cr, sr, fb[4], fn[4], tx[4], d[4];
+            sincos_fast(yr, &sr, &cr);
+            unpack(pl.phi[0][rb], fb);
+            unpack(pl.phi[0][rbr], fn);
+            transport(Ur, cr, sr, false, fb, tx);
+#pragma unroll
+            for (int m = 0; m < 4; m++) d[m] = fn[m] - tx[m];
+            ring_d = make_float4(d[0], d[1], d[2], d[3]);
+        }
+        // Plaquette (0, r) based at b: U_r(b + e_0) from the plane x + 1.
+        {
+            Plaq<R> pq;
+            const Qt<R> Ur = rdir == 1 ? Ub[1] : Ub[2];
+            const R yr = rdir == 1 ? yb[1] : yb[2];
+            const float4 qrn = rdir == 1 ? at(a.q[1], sbp0) : at(a.q[2], sbp0);
+            const R yrn = rdir == 1 ? at(a.y[1], sbp0) : at(a.y[2], sbp0);
+            plaquette(Ub[0], Ur, toq<float4, R>(qrn), toq<float4, R>(pl.q[0][rbr]), yb[0], yr, yrn, pl.y[0][rbr], pq);
+            ring_p0 = pack_wu(pq);
+        }
+        // Plaquette (1, 2) based at b: U_2(b + e_1), U_1(b + e_2) in the plane.
+        {
+            Plaq<R> pq;
+            plaquette(Ub[1], Ub[2], toq<float4, R>(pl.q[2][rb + W]), toq<float4, R>(pl.q[1][rb + 1]), yb[1], yb[2],
+                      pl.y[2][rb + W], pl.y[1][rb + 1], pq);
+            ring_p12 = pack_wu(pq);
+        }
+    }
+
+    // ---- exchange through LDS (over the plane tile) ----
+    __syncthreads();
+    const ShareLds sh = share_layout(lds, bz, a.by);
+    const int si = (ty + 1) * sh.w + tz + 1;
+    if (valid) {
+        sh.d1[si] = make_float4(dsh[1][0], dsh[1][1], dsh[1][2], dsh[1][3]);
+        sh.d2[si] = make_float4(dsh[2][0], dsh[2][1], dsh[2][2], dsh[2][3]);
+        sh.p01[si] = pack_wu(P[0]);
+        sh.p02[si] = pack_wu(P[1]);
+        sh.p12[si] = pack_wu(P[2]);
+    }
+    if (ring_valid) {
+        const int ri = rdir == 1 ? k + 1 : (rt + 1) * sh.w;
+        if (rdir == 1) {
+            sh.d1[ri] = ring_d;
+            sh.p01[ri] = ring_p0;
+        } else {
+            sh.d2[ri] = ring_d;
+            sh.p02[ri] = ring_p0;
+        }
+        sh.p12[ri] = ring_p12;
+    }
+    __syncthreads();
+    if (!valid) return;
+
+    // ---- phase B: the scalar force's i = 1, 2 terms ----
+    {
+        const float4 d1m = sh.d1[si - sh.w], d2m = sh.d2[si - 1];
+        const R dm1[4] = {d1m.x, d1m.y, d1m.z, d1m.w}, dm2[4] = {d2m.x, d2m.y, d2m.z, d2m.w};
+#pragma unroll
+        for (int m = 0; m < 4; m++) F[m] += a.inv_sp[1] * (dpv[1][m] - dm1[m]);
+#pragma unroll
+        for (int m = 0; m < 4; m++) F[m] += a.inv_sp[2] * (dpv[2][m] - dm2[m]);
+    }
+    // Plaquettes based at x - e_1 and x - e_2 from LDS, in the order j = 1, 2.
+    {
+        const float4 p01 = sh.p01[si - sh.w], p12a = sh.p12[si - sh.w];   // based at x - e_1
+        const float4 p02 = sh.p02[si - 1], p12b = sh.p12[si - 1];         // based at x - e_2
+        // j = 1: link 0 (i < j), link 2 (i > j).
+        M[0][0] -= a.inv_sp[1] * p01.w;
+        M[0][1] -= a.inv_sp[1] * p01.x;
+        M[0][2] -= a.inv_sp[1] * p01.y;
+        M[0][3] -= a.inv_sp[1] * p01.z;
+        M[2][0] += a.inv_sp[1] * p12a.w;
+        M[2][1] += a.inv_sp[1] * p12a.x;
+        M[2][2] += a.inv_sp[1] * p12a.y;
+        M[2][3] += a.inv_sp[1] * p12a.z;
+        // j = 2: links 0 and 1 (i < j).
+        M[0][0] -= a.inv_sp[2] * p02.w;
+        M[0][1] -= a.inv_sp[2] * p02.x;
+        M[0][2] -= a.inv_sp[2] * p02.y;
+        M[0][3] -= a.inv_sp[2] * p02.z;
+        M[1][0] -= a.inv_sp[2] * p12b.w;
+        M[1][1] -= a.inv_sp[2] * p12b.x;
+        M[1][2] -= a.inv_sp[2] * p12b.y;
+        M[1][3] -= a.inv_sp[2] * p12b.z;
+    }
+    kick_drift<R, 1, CMP, true>(a, s, U, yv, phi, F, J, M, &own);
 }
 
 // MINW: minimum waves per SIMD the register allocation must allow (0: the
 // compiler's choice).  Higher occupancy hides more memory latency but may spill.
-template <class R, int ND, int MINW, bool CMP, bool PL = false>
+// PL: 0 every neighbour from global memory; 1 in-plane neighbours from the
+// block's LDS plane tile; 2 as 1, with the in-plane plaquettes and covariant
+// differences shared (FP32 SM, bz + by <= 256).  All give bitwise the same fields.
+template <class R, int ND, int MINW, bool CMP, int PL = 0>
 __global__ void __launch_bounds__(256, MINW > 0 ? MINW : 1) mom_step(StepArgs<R> a, Coef c) {
     const int4 t = a.tiles[blockIdx.x];
     const int tz = static_cast<int>(threadIdx.x) & (a.bz - 1), ty = static_cast<int>(threadIdx.x) >> a.bz_shift;
     const int z = t.w * a.bz + tz;
     const int y = t.z * a.by + ty;
-    if constexpr (PL) {
+    const bool valid = z < a.nz && y < a.ny;
+    if constexpr (PL > 0) {
         static_assert(sizeof(R) == 4, "plane staging is FP32 only");
+        static_assert(PL == 1 || ND == 1, "shared plaquettes are for one doublet");
         extern __shared__ float4 gft_plane_lds[];
-        PlaneLds pl = plane_layout<ND>(reinterpret_cast<char*>(gft_plane_lds), a.bz, a.by);
+        char* lds = reinterpret_cast<char*>(gft_plane_lds);
+        PlaneLds pl = plane_layout<ND>(lds, a.bz, a.by);
         plane_fill<ND>(a, pl, t.y, t.z * a.by, t.w * a.bz, a.bz, a.by);
         pl.r0 = (ty + 1) * pl.w + tz + 1;
-        if (z >= a.nz || y >= a.ny) return;
-        mom_site<R, ND, CMP, true>(a, c, t.y, y, z, pl);
+        if constexpr (PL == 2) {
+            mom_block_shared<CMP>(a, c, t.y, t.z * a.by, t.w * a.bz, ty, tz, valid, pl, lds);
+        } else {
+            if (!valid) return;
+            mom_site<R, ND, CMP, true>(a, c, t.y, y, z, pl);
+        }
     } else {
-        if (z >= a.nz || y >= a.ny) return;
+        if (!valid) return;
         mom_site<R, ND, CMP>(a, c, t.y, y, z);
     }
 }
@@ -730,7 +1054,7 @@ struct EnsMember {
 // region of a few members.  Every block reaches the end of its range and exits.
 // PL: each tile's plane is staged in LDS as in mom_step (the dynamic LDS holds
 // the largest member tile).
-template <int MINW, bool PL>
+template <int MINW, int PL>
 __global__ void __launch_bounds__(256, MINW) ens_step(const EnsMember* __restrict__ members, const int4* __restrict__ work,
                                                       long long nwork, int blocks_per_xcd) {
     const long long xcd = blockIdx.x % 8, idx = blockIdx.x / 8;
@@ -745,15 +1069,18 @@ __global__ void __launch_bounds__(256, MINW) ens_step(const EnsMember* __restric
         const int tz = static_cast<int>(threadIdx.x) & (a.bz - 1), ty = static_cast<int>(threadIdx.x) >> a.bz_shift;
         const int z = t.w * a.bz + tz;
         const int y = t.z * a.by + ty;
-        if constexpr (PL) {
+        const bool valid = z < a.nz && y < a.ny;
+        if constexpr (PL > 0) {
             extern __shared__ float4 gft_plane_lds[];
+            char* lds = reinterpret_cast<char*>(gft_plane_lds);
             __syncthreads();   // every wave is done with the previous tile's LDS image
-            PlaneLds pl = plane_layout<1>(reinterpret_cast<char*>(gft_plane_lds), a.bz, a.by);
+            PlaneLds pl = plane_layout<1>(lds, a.bz, a.by);
             plane_fill<1>(a, pl, t.y, t.z * a.by, t.w * a.bz, a.bz, a.by);
             pl.r0 = (ty + 1) * pl.w + tz + 1;
-            if (z < a.nz && y < a.ny) mom_site<float, 1, true, true>(a, c, t.y, y, z, pl);
+            if constexpr (PL == 2) mom_block_shared<true>(a, c, t.y, t.z * a.by, t.w * a.bz, ty, tz, valid, pl, lds);
+            else if (valid) mom_site<float, 1, true, true>(a, c, t.y, y, z, pl);
         } else {
-            if (z < a.nz && y < a.ny) mom_site<float, 1, true>(a, c, t.y, y, z);
+            if (valid) mom_site<float, 1, true>(a, c, t.y, y, z);
         }
     }
 }
@@ -1321,13 +1648,33 @@ StepArgs<R> make_step_args(MomentumState& m, MomFields<R>& f, const Coef& c, dou
 // Largest LDS tile a step block may request (the fixed LDS limit of a gfx9 workgroup).
 constexpr size_t kPlaneLdsMax = 65536;
 
-template <bool CMP, bool PL>
+// LDS bytes of a step block with staging mode pl (0, 1, 2).
+inline size_t step_lds_bytes(int bz, int by, int nd, int pl) {
+    if (pl == 0) return 0;
+    const size_t b = plane_lds_bytes(bz, by, nd);
+    return pl == 2 ? std::max(b, share_lds_bytes(bz, by)) : b;
+}
+
+// The staging mode a launch runs with: the requested one where it applies.
+inline int step_stage_mode(int stage, int bz, int by, int nd) {
+    if (stage >= 2 && (nd != 1 || bz + by > 256 || step_lds_bytes(bz, by, nd, 2) > kPlaneLdsMax)) stage = 1;
+    if (stage >= 1 && step_lds_bytes(bz, by, nd, 1) > kPlaneLdsMax) stage = 0;
+    return stage;
+}
+
+template <bool CMP, int PL>
 void launch_step_f32(const StepArgs<float>& a, const Coef& c, int min_waves, long long nb, hipStream_t st) {
-    const size_t lds = PL ? plane_lds_bytes(a.bz, a.by, c.nd) : 0;
-    if (c.nd == 2) mom_step<float, 2, 0, CMP, PL><<<nb, 256, lds, st>>>(a, c);
-    else if (min_waves == 3) mom_step<float, 1, 3, CMP, PL><<<nb, 256, lds, st>>>(a, c);
-    else if (min_waves == 4) mom_step<float, 1, 4, CMP, PL><<<nb, 256, lds, st>>>(a, c);
-    else mom_step<float, 1, 0, CMP, PL><<<nb, 256, lds, st>>>(a, c);
+    const size_t lds = step_lds_bytes(a.bz, a.by, c.nd, PL);
+    if constexpr (PL == 2) {
+        if (min_waves == 3) mom_step<float, 1, 3, CMP, 2><<<nb, 256, lds, st>>>(a, c);
+        else if (min_waves == 4) mom_step<float, 1, 4, CMP, 2><<<nb, 256, lds, st>>>(a, c);
+        else mom_step<float, 1, 0, CMP, 2><<<nb, 256, lds, st>>>(a, c);
+    } else {
+        if (c.nd == 2) mom_step<float, 2, 0, CMP, PL><<<nb, 256, lds, st>>>(a, c);
+        else if (min_waves == 3) mom_step<float, 1, 3, CMP, PL><<<nb, 256, lds, st>>>(a, c);
+        else if (min_waves == 4) mom_step<float, 1, 4, CMP, PL><<<nb, 256, lds, st>>>(a, c);
+        else mom_step<float, 1, 0, CMP, PL><<<nb, 256, lds, st>>>(a, c);
+    }
 }
 
 inline int momentum_step(MomentumState& m, const Coef& c, double damping_step, hipStream_t st, unsigned* err,
@@ -1341,15 +1688,17 @@ inline int momentum_step(MomentumState& m, const Coef& c, double damping_step, h
         else mom_step<double, 2, 0, false><<<nb, 256, 0, st>>>(a, c);
     } else {
         auto a = make_step_args<float>(m, m.ff, c, damping_step, err);
-        // Staged and unstaged kernels give bitwise the same fields; a tile whose
-        // LDS image exceeds the workgroup limit runs unstaged.
-        const bool pl = m.stage && plane_lds_bytes(m.bz, m.by, c.nd) <= kPlaneLdsMax;
+        // Every staging mode gives bitwise the same fields; a launch runs the
+        // requested mode where it applies (see step_stage_mode), else a lower one.
+        const int pl = step_stage_mode(m.stage, m.bz, m.by, c.nd);
         if (m.cmp) {
-            if (pl) launch_step_f32<true, true>(a, c, m.min_waves, nb, st);
-            else launch_step_f32<true, false>(a, c, m.min_waves, nb, st);
+            if (pl == 2) launch_step_f32<true, 2>(a, c, m.min_waves, nb, st);
+            else if (pl == 1) launch_step_f32<true, 1>(a, c, m.min_waves, nb, st);
+            else launch_step_f32<true, 0>(a, c, m.min_waves, nb, st);
         } else {
-            if (pl) launch_step_f32<false, true>(a, c, m.min_waves, nb, st);
-            else launch_step_f32<false, false>(a, c, m.min_waves, nb, st);
+            if (pl == 2) launch_step_f32<false, 2>(a, c, m.min_waves, nb, st);
+            else if (pl == 1) launch_step_f32<false, 1>(a, c, m.min_waves, nb, st);
+            else launch_step_f32<false, 0>(a, c, m.min_waves, nb, st);
         }
     }
     if (hipGetLastError() != hipSuccess) { msg = "momentum step launch failed"; return GFT_EHIP; }

@@ -17,7 +17,7 @@ ORDERS = (2, 1)
 
 def candidates(occupancy: bool = False):
     return [(bz, by, order, mw, st) for (bz, by) in TILES for order in ORDERS
-            for mw in ((3, 0, 4) if occupancy else (3,)) for st in ((1, 0) if occupancy else (1,))]
+            for mw in ((3, 0, 4) if occupancy else (3,)) for st in ((2, 1, 0) if occupancy else (2,))]
 
 
 def tune(ev, candidates_=None, steps: int = 6, rounds: int = 2, fp32_sm: bool = True, occupancy: bool = False):
