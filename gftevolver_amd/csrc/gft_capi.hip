@@ -5,6 +5,7 @@
 // stream created at the default priority.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -551,6 +552,7 @@ struct gft_ensemble {
     int stage = 0;           // staging mode of the ensemble kernel
     unsigned* err_dev = nullptr;
     bool dirty = true;
+    std::vector<int4> launch_key;   // (bz, by, stage) of every member at the last prepare
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int graph_parity = -1;
@@ -581,7 +583,16 @@ static void ens_drop_graph(gft_ensemble* ens) {
 
 // Uploads the per-member step arguments (both parities) and the work list.
 static int ens_prepare(gft_ensemble* ens) {
+    // A member's launch shape may also change through its own context (gft_set_launch).
+    std::vector<int4> key;
+    for (auto c : ens->members) key.push_back(make_int4(c->mom.bz, c->mom.by, c->mom.stage, 0));
+    const bool same = key.size() == ens->launch_key.size() &&
+                      std::equal(key.begin(), key.end(), ens->launch_key.begin(), [](const int4& p, const int4& q) {
+                          return p.x == q.x && p.y == q.y && p.z == q.z;
+                      });
+    if (!same) ens->dirty = true;
     if (!ens->dirty) return GFT_OK;
+    ens->launch_key = key;
     ens_drop_graph(ens);
     const size_t nm = ens->members.size();
     std::vector<EnsMember> host[2];

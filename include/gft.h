@@ -166,7 +166,10 @@ int gft_bench_copy(size_t bytes, int reps, double* avg_ms);
  * list of 256-site tiles of all members, optionally replayed from a hipGraph
  * that holds two steps.  Each member is an ordinary context (gft_ensemble_member)
  * for import, export, gft_observe and gft_wilson_loops; its fields are bitwise
- * those of the same member evolved alone. */
+ * those of the same member evolved alone.  A member's tile shape comes from
+ * gft_ensemble_set_launch or gft_set_launch on the member context (whose
+ * `stage` also applies: the ensemble kernel runs the lowest staging mode any
+ * member asks for or supports); block order and min_waves do not apply. */
 typedef struct gft_ensemble gft_ensemble;
 int         gft_ensemble_create(const gft_desc* members, int nmembers, gft_ensemble** out);
 void        gft_ensemble_destroy(gft_ensemble* ens);

@@ -4,10 +4,11 @@ per step, with and without a captured two-step hipGraph.
 
 Each member must be bitwise the same lattice evolved alone by the same kernel
 (a one-member ensemble with the same tile shape): any difference is a bug in
-the work list, the member table or the parity bookkeeping.  Against the
-single-lattice kernel (gft_evolve) the per-site arithmetic is the same but the
-compiler may contract a different product of a sum into an FMA, so that
-comparison allows a few ulps of the field scale."""
+the work list, the member table or the parity bookkeeping.  The single-lattice
+kernel (gft_evolve) evaluates the same expressions (FP contraction is per
+expression, -ffp-contract=on), so it must agree bitwise as well, and so must
+the ensemble kernel's staging modes (global loads, LDS plane tile, shared
+plaquettes)."""
 from __future__ import annotations
 
 import numpy as np
@@ -75,7 +76,30 @@ def test_ensemble_members_bitwise_equal_single_runs(rt, use_graph, steps):
         ks, kv, _ = _run_single_kernel(rt, d, s, v, steps, TILES[i])
         ds, dv = np.abs(gs - ks).max(), np.abs(gv - kv).max()
         print(f"  vs gft_evolve: scalar {ds:.3e} links {dv:.3e}")
-        assert ds <= 8 * 2.0**-24 * np.abs(ks).max() and dv <= 8 * 2.0**-24 * max(np.abs(kv).max(), 1.0)
+        assert np.array_equal(gs, ks) and np.array_equal(gv, kv)
+
+
+def test_ensemble_staging_modes_bitwise_equal(rt):
+    """The ensemble kernel with every neighbour from global memory (stage 0),
+    with the LDS plane tile (1) and with shared plaquettes (2): the same fields.
+    The ensemble runs the lowest stage any member asks for."""
+    tiles = [(64, 4), (16, 16), (32, 8), (64, 4), (128, 2)]
+    descs = [make_desc(form=GFT_FORM_MOMENTUM_F32, n=n) for n in SIZES]
+    states = _states()
+    out = {}
+    for stage in (2, 1, 0):
+        with rt.Ensemble(descs) as ens:
+            for i, (m, (s, v)) in enumerate(zip(ens.members, states)):
+                m.import_two_slot(s.astype(np.float32), v.astype(np.float32), 0)
+                m.set_launch(tiles[i][0], tiles[i][1], 2, 3, stage)
+            ens.evolve(9)
+            out[stage] = [m.export_two_slot(np.float32) for m in ens.members]
+    for stage in (1, 0):
+        for i in range(len(SIZES)):
+            a, b = out[2][i], out[stage][i]
+            print(f"stage {stage} member {i}: max |diff| scalar {np.abs(a[0] - b[0]).max():.3e} "
+                  f"links {np.abs(a[1] - b[1]).max():.3e}")
+            assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
 
 
 def test_ensemble_rejects_mixed_steps(rt):
