@@ -51,26 +51,27 @@ def observe(ev):
     return np.array([w.phi_sq, o.energy[5] / tot, o.energy[4] / tot, w.w11_su2, w.w11_u1])
 
 
-def tune_tiles(ens, groups_idx, tiles=None, steps=4):
+def tune_tiles(ens, groups_idx, tiles=None, steps=20, passes=2):
     """Per-group tile shapes by coordinate descent on the whole ensemble's step
     time: for each group of equal-size members, try every tile shape with the
-    other groups fixed and keep the fastest.  Every shape gives bitwise the
-    same fields, so tuning advances the state but never changes results.
-    Returns {group: (bz, by)}."""
+    other groups fixed and keep the fastest; `passes` sweeps over the groups.
+    Every shape gives bitwise the same fields, so tuning advances the state but
+    never changes results.  Returns {group: (bz, by)}."""
     from .tune import TILES
     tiles = tiles or TILES
-    best = {}
-    for g, idx in enumerate(groups_idx):
-        timing = {}
-        for t in tiles:
+    best = {g: tiles[0] for g in range(len(groups_idx))}
+    for _ in range(passes):
+        for g, idx in enumerate(groups_idx):
+            timing = {}
+            for t in tiles:
+                for i in idx:
+                    ens.set_launch(i, *t)
+                ens.evolve(2)
+                ens.evolve(steps)
+                timing[t] = ens.last_step_ms()
+            best[g] = min(timing, key=timing.get)
             for i in idx:
-                ens.set_launch(i, *t)
-            ens.evolve(2)
-            ens.evolve(steps)
-            timing[t] = ens.last_step_ms()
-        best[g] = min(timing, key=timing.get)
-        for i in idx:
-            ens.set_launch(i, *best[g])
+                ens.set_launch(i, *best[g])
     return best
 
 
