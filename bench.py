@@ -198,6 +198,18 @@ def main():
         f"{value:.3e} site-updates/s, {achieved:.0f} GB/s algorithmic, Gauss rel {r32['gauss_rel']:.2e} "
         f"after {args.steps + args.warmup + 1} steps")
 
+    stream_ms = None
+    if args.calibrate_copy or not args.only_f32:
+        # The HBM floor of this data set: the same arrays streamed once with
+        # trivial arithmetic (gft_bench_streams), on a context of its own.
+        with rt.Evolver(d32) as ev:
+            if r32.get("launch"):
+                ev.set_launch(*r32["launch"])
+            stream_ms = ev.bench_streams(20)
+        log(f"stream floor of the step's arrays: {stream_ms:.3f} ms = "
+            f"{STORED_F32C * N / (stream_ms * 1e-3) / 1e9:.0f} GB/s; step kernel at "
+            f"{stream_ms / r32['kernel_ms'] * 100:.1f}% of it")
+
     extra = {}
     if not args.only_f32:
         dpl = make_desc(form=GFT_FORM_MOMENTUM_F32_PLAIN, n=n)
@@ -257,7 +269,9 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_site_update": BYTES_F32,
                      "stored_bytes_per_site_update": STORED_F32C,
-                     "kernel_ms": r32["kernel_ms"], "copy_kernel_GBps": copy_gbps},
+                     "kernel_ms": r32["kernel_ms"], "copy_kernel_GBps": copy_gbps,
+                     "stream_floor_ms": stream_ms,
+                     "stream_floor_GBps": STORED_F32C * N / (stream_ms * 1e-3) / 1e9 if stream_ms else None},
         "cpu_baseline": cpu,
         "gauss_rel_violation": r32["gauss_rel"],
         "gauss_measured_after_steps": args.steps + args.warmup + 1,

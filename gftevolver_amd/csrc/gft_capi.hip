@@ -539,6 +539,78 @@ extern "C" int gft_bench_copy(size_t bytes, int reps, double* avg_ms) {
     return rc;
 }
 
+// Streams the arrays of one FP32 SM momentum-form step, own site only, with
+// the step kernel's access kinds (cached loads of q, y, phi; streaming loads
+// of P, E and residuals; streaming stores), its tiles and block order, and
+// trivial arithmetic: the HBM floor of the step's data set without its stencil.
+template <bool CMP>
+__global__ void __launch_bounds__(256) gft_stream_kernel(StepArgs<float> a) {
+    const int4 t = a.tiles[blockIdx.x];
+    const int z = t.w * a.bz + (static_cast<int>(threadIdx.x) & (a.bz - 1));
+    const int y = t.z * a.by + (static_cast<int>(threadIdx.x) >> a.bz_shift);
+    if (z >= a.nz || y >= a.ny) return;
+    const unsigned s = static_cast<unsigned>((t.y * a.ny + y) * a.nz + z);
+    float4 q[3], f = at(a.phi[0], s), p = ld_nt(ptr_at(a.P[0], s)), e[3];
+    float y[3];
+    uint4 ls = make_uint4(0, 0, 0, 0);
+    uint2 le[3] = {};
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        q[k] = at(a.q[k], s);
+        y[k] = at(a.y[k], s);
+        e[k] = ld_nt(ptr_at(a.E[k], s));
+        if (CMP) le[k] = ld_nt(ptr_at(a.lo_E[k], s));
+    }
+    if (CMP) ls = ld_nt(ptr_at(a.lo_s[0], s));
+    const float t = f.x + p.x + y[0] + y[1] + y[2];
+    p.x += t;
+    f.x += t;
+    st_nt(p, ptr_at(a.P[0], s));
+    st_nt(f, ptr_at(a.phi_out[0], s));
+    if (CMP) st_nt(make_uint4(ls.x + 1u, ls.y, ls.z, ls.w), ptr_at(a.lo_s[0], s));
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        e[k].x += t;
+        q[k].x += t;
+        st_nt(e[k], ptr_at(a.E[k], s));
+        if (CMP) st_nt(make_uint2(le[k].x + 1u, le[k].y), ptr_at(a.lo_E[k], s));
+        st_nt(y[k] + t, ptr_at(a.y_out[k], s));
+        st_nt(q[k], ptr_at(a.q_out[k], s));
+    }
+}
+
+extern "C" int gft_bench_streams(gft_ctx* ctx, int reps, double* avg_ms) {
+    if (!ctx || reps < 1 || !avg_ms) return GFT_EINVAL;
+    if ((ctx->form != GFT_FORM_MOMENTUM_F32 && ctx->form != GFT_FORM_MOMENTUM_F32_PLAIN) || ctx->coef.nd != 1)
+        return fail(ctx, GFT_EINVAL, "gft_bench_streams: FP32 momentum-form SM contexts only");
+    int rc = GFT_OK;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = GFT_EHIP;
+    std::string msg;
+    if (!rc && ensure_tiles(ctx->mom, ctx->coef, ctx->stream, msg) != GFT_OK) rc = fail(ctx, GFT_EHIP, msg);
+    if (!rc) {
+        const StepArgs<float> a = make_step_args<float>(ctx->mom, ctx->mom.ff, ctx->coef, 0.0, ctx->err_dev);
+        const long long nb = mom_nblocks(ctx->mom, ctx->coef);
+        auto launch = [&]() {
+            if (ctx->mom.cmp) gft_stream_kernel<true><<<nb, 256, 0, ctx->stream>>>(a);
+            else gft_stream_kernel<false><<<nb, 256, 0, ctx->stream>>>(a);
+        };
+        launch();
+        hipEventRecord(e0, ctx->stream);
+        for (int r = 0; r < reps; r++) launch();
+        hipEventRecord(e1, ctx->stream);
+        float ms = 0.f;
+        if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+            rc = fail(ctx, GFT_EHIP, "gft_bench_streams: launch failed");
+        *avg_ms = ms / reps;
+    }
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    ctx->imported = false;   // the state now holds the kernel's scratch values
+    return rc;
+}
+
 // ---- batched ensembles -----------------------------------------------------------
 
 struct gft_ensemble {

@@ -38,6 +38,7 @@ _L.gft_wilson_loops.argtypes = [C.c_void_p, _P(GftLoops)]
 _L.gft_sync.argtypes = [C.c_void_p]
 _L.gft_set_launch.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
 _L.gft_bench_copy.argtypes = [C.c_size_t, C.c_int, _P(C.c_double)]
+_L.gft_bench_streams.argtypes = [C.c_void_p, C.c_int, _P(C.c_double)]
 _L.gft_last_step_ms.argtypes = [C.c_void_p, _P(C.c_double), _P(C.c_double), _P(C.c_uint)]
 _L.gft_ensemble_create.argtypes = [_P(GftDesc), C.c_int, _P(C.c_void_p)]
 _L.gft_ensemble_destroy.argtypes = [C.c_void_p]
@@ -179,6 +180,13 @@ class Evolver:
 
     def sync(self):
         self._check(_L.gft_sync(self._h))
+
+    def bench_streams(self, reps: int = 20) -> float:
+        """Average ms of streaming exactly the arrays one step reads and writes
+        (gft_bench_streams); overwrites the state."""
+        ms = C.c_double()
+        self._check(_L.gft_bench_streams(self._h, reps, C.byref(ms)))
+        return ms.value
 
     def last_step_ms(self):
         avg, tot, n = C.c_double(), C.c_double(), C.c_uint()

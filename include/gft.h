@@ -160,6 +160,16 @@ int gft_set_launch(gft_ctx* ctx, int bz, int by, int order, int min_waves, int s
  * and WRITE_SIZE of this kernel calibrate the same counters of the step kernel. */
 int gft_bench_copy(size_t bytes, int reps, double* avg_ms);
 
+/* Measurement helper (no reference counterpart): the HBM floor of the step's
+ * data set.  Streams exactly the arrays one FP32 SM momentum-form step of this
+ * context reads and writes (own site only: q, y, phi, P, E and residuals in,
+ * the same out), with the step kernel's load and store kinds, tiles and block
+ * order (gft_set_launch) and trivial arithmetic, `reps` times, and returns the
+ * average launch duration.  It
+ * overwrites the context's state: import or generate a state again before
+ * evolving (gft_ensemble_evolve refuses a member until then). */
+int gft_bench_streams(gft_ctx* ctx, int reps, double* avg_ms);
+
 /* Batched ensembles (no reference counterpart: the reference evolves one
  * lattice per process).  Independent FP32 momentum-form SM lattices of any
  * sizes advance together: one persistent kernel launch per step over a work

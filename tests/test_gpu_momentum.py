@@ -284,3 +284,28 @@ def test_momentum_domain_error_is_reported(rt):
         with pytest.raises(rt.GftError) as e:
             ev.evolve(3)
         assert e.value.code == -4
+
+
+def test_bench_streams_measures_and_guards(rt):
+    """gft_bench_streams: a positive duration for the FP32 SM forms, refused for
+    FP64, and the overwritten state must be generated again before an ensemble
+    evolves it."""
+    from gftevolver_amd._abi import GFT_FORM_MOMENTUM_F64
+    from gftevolver_amd.runtime import GftError
+    for form in (GFT_FORM_MOMENTUM_F32, GFT_FORM_MOMENTUM_F32_PLAIN):
+        with rt.Evolver(make_desc(form=form, n=(32, 32, 32))) as ev:
+            ev.init_grf(seed=3)
+            ms = ev.bench_streams(3)
+            print(f"form {form}: stream floor {ms * 1e3:.1f} us for 32^3")
+            assert 0.0 < ms < 100.0
+    with rt.Evolver(make_desc(form=GFT_FORM_MOMENTUM_F64, n=(8, 8, 8))) as ev:
+        with pytest.raises(GftError):
+            ev.bench_streams(1)
+    with rt.Ensemble([make_desc(form=GFT_FORM_MOMENTUM_F32, n=(16, 16, 16))]) as ens:
+        m = ens.members[0]
+        m.init_grf(seed=3)
+        m.bench_streams(1)
+        with pytest.raises(GftError):
+            ens.evolve(1)
+        m.init_grf(seed=3)
+        ens.evolve(1)
