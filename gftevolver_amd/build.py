@@ -36,7 +36,14 @@ def build_library(verbose: bool = False):
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     tmp = LIB + ".tmp"
     cmd = [hipcc, *HIPCC_FLAGS, "-o", tmp, os.path.join(CSRC, "gft_capi.hip")]
-    _run(cmd, stderr=None if verbose else subprocess.DEVNULL)
+    print("+", " ".join(cmd), flush=True)
+    # The compiler's warnings (the host pass does not know the packed-FP32
+    # target feature) stay quiet unless verbose; its errors are always shown.
+    r = subprocess.run(cmd, stderr=None if verbose else subprocess.PIPE, text=True)
+    if r.returncode != 0:
+        if not verbose:
+            sys.stderr.write(r.stderr)
+        raise RuntimeError(f"hipcc failed ({r.returncode}) building {LIB}")
     os.replace(tmp, LIB)
 
 

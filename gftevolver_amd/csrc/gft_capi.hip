@@ -545,15 +545,15 @@ extern "C" int gft_bench_copy(size_t bytes, int reps, double* avg_ms) {
 // trivial arithmetic: the HBM floor of the step's data set without its stencil.
 template <bool CMP>
 __global__ void __launch_bounds__(256) gft_stream_kernel(StepArgs<float> a) {
-    const int4 t = a.tiles[blockIdx.x];
-    const int z = t.w * a.bz + (static_cast<int>(threadIdx.x) & (a.bz - 1));
-    const int y = t.z * a.by + (static_cast<int>(threadIdx.x) >> a.bz_shift);
-    if (z >= a.nz || y >= a.ny) return;
-    const unsigned s = static_cast<unsigned>((t.y * a.ny + y) * a.nz + z);
+    const int4 tile = a.tiles[blockIdx.x];
+    const int zs = tile.w * a.bz + (static_cast<int>(threadIdx.x) & (a.bz - 1));
+    const int ys = tile.z * a.by + (static_cast<int>(threadIdx.x) >> a.bz_shift);
+    if (zs >= a.nz || ys >= a.ny) return;
+    const unsigned s = static_cast<unsigned>((tile.y * a.ny + ys) * a.nz + zs);
     float4 q[3], f = at(a.phi[0], s), p = ld_nt(ptr_at(a.P[0], s)), e[3];
     float y[3];
-    uint4 ls = make_uint4(0, 0, 0, 0);
-    uint2 le[3] = {};
+    uint2 ls = make_uint2(0, 0);
+    unsigned le[3] = {0u, 0u, 0u};
 #pragma unroll
     for (int k = 0; k < 3; k++) {
         q[k] = at(a.q[k], s);
@@ -567,13 +567,13 @@ __global__ void __launch_bounds__(256) gft_stream_kernel(StepArgs<float> a) {
     f.x += t;
     st_nt(p, ptr_at(a.P[0], s));
     st_nt(f, ptr_at(a.phi_out[0], s));
-    if (CMP) st_nt(make_uint4(ls.x + 1u, ls.y, ls.z, ls.w), ptr_at(a.lo_s[0], s));
+    if (CMP) st_nt(make_uint2(ls.x + 1u, ls.y), ptr_at(a.lo_s[0], s));
 #pragma unroll
     for (int k = 0; k < 3; k++) {
         e[k].x += t;
         q[k].x += t;
         st_nt(e[k], ptr_at(a.E[k], s));
-        if (CMP) st_nt(make_uint2(le[k].x + 1u, le[k].y), ptr_at(a.lo_E[k], s));
+        if (CMP) st_nt(le[k] + 1u, ptr_at(a.lo_E[k], s));
         st_nt(y[k] + t, ptr_at(a.y_out[k], s));
         st_nt(q[k], ptr_at(a.q_out[k], s));
     }
@@ -859,11 +859,10 @@ __global__ void __launch_bounds__(256) grf_sumsq4(const double* __restrict__ f, 
     block_reduce_store<1, 0>(v, nullptr, lds, partial, blockIdx.x);
 }
 
-__device__ inline unsigned grf_bf16_bits(double v) {   // round to nearest even, as the host import does
-    const float fv = (float)v;
-    unsigned u = __float_as_uint(fv);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return u >> 16;
+// 8-bit residual of v next to its FP32 hi, as the host import encodes it.
+__device__ inline unsigned grf_res_q(double v, float hi) {
+    const double q = rint(ldexp(v - (double)hi, 157 - res_exp(hi)));
+    return static_cast<unsigned>(static_cast<int>(fmin(127.0, fmax(-127.0, q)))) & 0xffu;
 }
 
 // phi_k = (0, 0, vev_k, 0) + scale_k * comps, with scale_k = rms / sqrt(<sum_c f_c^2>)
@@ -886,9 +885,9 @@ __global__ void __launch_bounds__(256) grf_assemble_mom(MomFields<R> f, const do
         z.x = z.y = z.z = z.w = R(0);
         f.P[k][s] = z;
         if (f.lo_s[k]) {
-            unsigned b[4];
-            for (int m = 0; m < 4; m++) b[m] = grf_bf16_bits(v[m] - (double)((&hi.x)[m]));
-            f.lo_s[k][s] = make_uint4(b[0] | (b[1] << 16), b[2] | (b[3] << 16), 0u, 0u);
+            unsigned w = 0;
+            for (int m = 0; m < 4; m++) w |= grf_res_q(v[m], (float)((&hi.x)[m])) << (8 * m);
+            f.lo_s[k][s] = make_uint2(w, 0u);
         }
     }
     for (int i = 0; i < 3; i++) {
@@ -898,7 +897,7 @@ __global__ void __launch_bounds__(256) grf_assemble_mom(MomFields<R> f, const do
         f.E[i][s] = e;
         f.q[0][i][s] = q;
         f.y[0][i][s] = R(0);
-        if (f.lo_E[i]) f.lo_E[i][s] = make_uint2(0u, 0u);
+        if (f.lo_E[i]) f.lo_E[i][s] = 0u;
     }
 }
 

@@ -52,10 +52,10 @@ struct MomFields {
     R* y[2][3];      // [buffer][direction]
     V4* q[2][3];     // [buffer][direction]
     V4* E[3];        // [direction]
-    // Compensated FP32 storage only: BF16 residuals of (phi, P) per doublet
-    // (phi_lo[4], P_lo[4]) and of E per direction; null otherwise.
-    uint4* lo_s[2];  // [doublet]
-    uint2* lo_E[3];  // [direction]
+    // Compensated FP32 storage only: 8-bit residuals of (phi, P) per doublet
+    // (x: phi_lo[4], y: P_lo[4], one byte each) and of E per direction; null otherwise.
+    uint2* lo_s[2];      // [doublet]
+    unsigned* lo_E[3];   // [direction]
 };
 
 struct MomentumState {
@@ -162,6 +162,8 @@ __device__ __forceinline__ void st_nt(const uint2& v, uint2* p) {
     __builtin_nontemporal_store(n, reinterpret_cast<u2n*>(p));
 }
 __device__ __forceinline__ void st_nt(float v, float* p) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ unsigned ld_nt(const unsigned* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ void st_nt(unsigned v, unsigned* p) { __builtin_nontemporal_store(v, p); }
 __device__ __forceinline__ void st_nt(double v, double* p) { __builtin_nontemporal_store(v, p); }
 
 template <class R>
@@ -234,19 +236,28 @@ __device__ __forceinline__ void qrot(const Qt<R>& q, const R* v, R* o) {
 }
 
 // ---- compensated FP32 storage ------------------------------------------------
-// A stored value is hi + lo: hi the FP32 value the stencil reads, lo a BF16
-// residual kept at the owning site only.  Updates are error-free transformations
-// (TwoSum, exact products by FMA), so the only rounding per step is that of the
-// residual to BF16, 2^-9 of an FP32 half-ulp: the state keeps about 32
-// significant bits while the kernel's arithmetic and neighbour traffic stay FP32.
-typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
-typedef float f2v_t __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ unsigned bf16_pack2(float a, float b) {   // round to nearest even
-    const f2v_t v = {a, b};
-    return __builtin_bit_cast(unsigned, __builtin_convertvector(v, bf16x2_t));
+// A stored value is hi + lo: hi the FP32 value the stencil reads, lo an 8-bit
+// residual kept at the owning site only, in units of hi's ulp:
+//   lo = q ulp(hi) / 2^7,  ulp(hi) = 2^(e - 150) for hi's biased exponent e,
+// with q an integer in [-127, 127] (|lo| <= ulp(hi) / 2 gives |q| <= 64 for a
+// normal hi).  Updates are error-free transformations (TwoSum, exact products
+// by FMA), so the only rounding per step is that of the residual to q, at most
+// 2^-8 ulp(hi): the state keeps about 31 significant bits for one extra byte
+// per value, while the kernel's arithmetic and neighbour traffic stay FP32.
+__device__ __forceinline__ int res_exp(float hi) { return static_cast<int>((__float_as_uint(hi) >> 23) & 0xffu); }
+// q of residual r next to hi (round to nearest even, clamped).
+__device__ __forceinline__ int res_q(float r, float hi) {
+    const int q = static_cast<int>(rintf(ldexpf(r, 157 - res_exp(hi))));
+    return min(max(q, -127), 127);
 }
-__device__ __forceinline__ float bf16_lo(unsigned w) { return __uint_as_float(w << 16); }
-__device__ __forceinline__ float bf16_hi(unsigned w) { return __uint_as_float(w & 0xffff0000u); }
+__device__ __forceinline__ float res_from_q(int q, float hi) { return ldexpf(static_cast<float>(q), res_exp(hi) - 157); }
+// Four q in one word (byte m = value m) and back.
+__device__ __forceinline__ unsigned res_pack4(int q0, int q1, int q2, int q3) {
+    return (static_cast<unsigned>(q0) & 0xffu) | ((static_cast<unsigned>(q1) & 0xffu) << 8) |
+           ((static_cast<unsigned>(q2) & 0xffu) << 16) | (static_cast<unsigned>(q3) << 24);
+}
+__device__ __forceinline__ int res_byte(unsigned w, int m) { return static_cast<int>(w << (24 - 8 * m)) >> 24; }
+__device__ __forceinline__ float res_val(unsigned w, int m, float hi) { return res_from_q(res_byte(w, m), hi); }
 
 // (h, l) <- keep (h + l) + delta; returns the new hi, *r the exact residual.
 __device__ __forceinline__ float cmp_update(float h, float l, float keep, float delta, float* r) {
@@ -257,6 +268,16 @@ __device__ __forceinline__ float cmp_update(float h, float l, float keep, float 
     const float sum = p + t;
     const float bb = sum - p;
     *r = (p - (sum - bb)) + (t - bb);
+    return sum;
+}
+// The same for keep = 1 (no damping): keep h is h and its error 0, so this
+// gives bitwise the values of cmp_update with keep = 1 in fewer operations.
+__device__ __forceinline__ float cmp_update1(float h, float l, float delta, float* r) {
+#pragma clang fp contract(off)
+    const float t = l + delta;
+    const float sum = h + t;
+    const float bb = sum - h;
+    *r = (h - (sum - bb)) + (t - bb);
     return sum;
 }
 // (h1 + l1) + (h2 + l2) with |l1|, |l2| small; returns the hi, *r the residual.
@@ -310,8 +331,8 @@ struct StepArgs {
     const V4* q[3];
     V4* q_out[3];
     V4* E[3];
-    uint4* lo_s[2];          // compensated storage: residuals of (phi, P) and E
-    uint2* lo_E[3];
+    uint2* lo_s[2];          // compensated storage: residuals of (phi, P) and E
+    unsigned* lo_E[3];
     int nx, ny, nz, nbz, nby;
     long long nblocks;
     int bz, by, order, bz_shift;   // bz = 1 << bz_shift
@@ -449,9 +470,9 @@ __device__ __forceinline__ void plane_fill(const StepArgs<float>& a, const Plane
 template <class R, int ND>
 struct OwnState {
     typename V4T<R>::type P[ND];
-    uint4 ls[ND];
+    uint2 ls[ND];
     typename V4T<R>::type E[3];
-    uint2 lE[3];
+    unsigned lE[3];
 };
 template <class R, int ND, bool CMP>
 __device__ __forceinline__ void load_own(const StepArgs<R>& a, unsigned s, OwnState<R, ND>& o) {
@@ -468,10 +489,11 @@ __device__ __forceinline__ void load_own(const StepArgs<R>& a, unsigned s, OwnSt
 }
 
 // PRE: own state already loaded into o (else loaded here, next to its use).
-template <class R, int ND, bool CMP, bool PRE = false>
-__device__ __forceinline__ void kick_drift(const StepArgs<R>& a, unsigned s, const Qt<R>* U, const R* yv,
-                                           const R* phi, const R* F, const R (*J)[4], const R (*M)[4],
-                                           const OwnState<R, ND>* o = nullptr) {
+// U1: no damping (keep = 1), with the kick's keep products dropped (exact).
+template <class R, int ND, bool CMP, bool PRE = false, bool U1 = false>
+__device__ __forceinline__ void kick_drift_t(const StepArgs<R>& a, unsigned s, const Qt<R>* U, const R* yv,
+                                             const R* phi, const R* F, const R (*J)[4], const R (*M)[4],
+                                             const OwnState<R, ND>* o) {
     using V4 = typename V4T<R>::type;
 #pragma unroll
     for (int k = 0; k < ND; k++) {
@@ -480,28 +502,34 @@ __device__ __forceinline__ void kick_drift(const StepArgs<R>& a, unsigned s, con
         else unpack(ld_nt(ptr_at(a.P[k], s)), p);
         V4 pn, fn;
         if constexpr (CMP) {
-            uint4 L;
+            uint2 L;
             if constexpr (PRE) L = o->ls[k];
             else L = ld_nt(ptr_at(a.lo_s[k], s));
-            const float fl[4] = {bf16_lo(L.x), bf16_hi(L.x), bf16_lo(L.y), bf16_hi(L.y)};
-            const float pl[4] = {bf16_lo(L.z), bf16_hi(L.z), bf16_lo(L.w), bf16_hi(L.w)};
             float ph[4], pr[4], fh[4], fr[4];
+            int qp[4];
 #pragma unroll
-            for (int m = 0; m < 4; m++) ph[m] = cmp_update(p[m], pl[m], a.keep, a.dt2 * F[4 * k + m], &pr[m]);
+            for (int m = 0; m < 4; m++) {
+                if constexpr (U1) ph[m] = cmp_update1(p[m], res_val(L.y, m, p[m]), a.dt2 * F[4 * k + m], &pr[m]);
+                else ph[m] = cmp_update(p[m], res_val(L.y, m, p[m]), a.keep, a.dt2 * F[4 * k + m], &pr[m]);
+                qp[m] = res_q(pr[m], ph[m]);
+            }
             // phi(t+dt) = phi(t) + P(t+dt) with P as stored (its residual rounded).
-            const unsigned pw0 = bf16_pack2(pr[0], pr[1]), pw1 = bf16_pack2(pr[2], pr[3]);
-            const float prr[4] = {bf16_lo(pw0), bf16_hi(pw0), bf16_lo(pw1), bf16_hi(pw1)};
 #pragma unroll
-            for (int m = 0; m < 4; m++) fh[m] = cmp_add(phi[4 * k + m], fl[m], ph[m], prr[m], &fr[m]);
+            for (int m = 0; m < 4; m++)
+                fh[m] = cmp_add(phi[4 * k + m], res_val(L.x, m, phi[4 * k + m]), ph[m], res_from_q(qp[m], ph[m]), &fr[m]);
             pn = make_float4(ph[0], ph[1], ph[2], ph[3]);
             fn = make_float4(fh[0], fh[1], fh[2], fh[3]);
-            st_nt(make_uint4(bf16_pack2(fr[0], fr[1]), bf16_pack2(fr[2], fr[3]), pw0, pw1), ptr_at(a.lo_s[k], s));
+            st_nt(make_uint2(res_pack4(res_q(fr[0], fh[0]), res_q(fr[1], fh[1]), res_q(fr[2], fh[2]), res_q(fr[3], fh[3])),
+                             res_pack4(qp[0], qp[1], qp[2], qp[3])),
+                  ptr_at(a.lo_s[k], s));
         } else {
             // Explicit FMAs fix which product is fused (exact keep * p for keep = 1).
-            pn.x = fma(a.dt2, F[4 * k + 0], a.keep * p[0]);
-            pn.y = fma(a.dt2, F[4 * k + 1], a.keep * p[1]);
-            pn.z = fma(a.dt2, F[4 * k + 2], a.keep * p[2]);
-            pn.w = fma(a.dt2, F[4 * k + 3], a.keep * p[3]);
+            const R kp[4] = {U1 ? p[0] : a.keep * p[0], U1 ? p[1] : a.keep * p[1], U1 ? p[2] : a.keep * p[2],
+                             U1 ? p[3] : a.keep * p[3]};
+            pn.x = fma(a.dt2, F[4 * k + 0], kp[0]);
+            pn.y = fma(a.dt2, F[4 * k + 1], kp[1]);
+            pn.z = fma(a.dt2, F[4 * k + 2], kp[2]);
+            pn.w = fma(a.dt2, F[4 * k + 3], kp[3]);
             fn.x = phi[4 * k + 0] + pn.x;
             fn.y = phi[4 * k + 1] + pn.y;
             fn.z = phi[4 * k + 2] + pn.z;
@@ -518,21 +546,22 @@ __device__ __forceinline__ void kick_drift(const StepArgs<R>& a, unsigned s, con
         else unpack(ld_nt(ptr_at(a.E[i], s)), e);
         V4 en;
         if constexpr (CMP) {
-            uint2 L;
+            unsigned L;
             if constexpr (PRE) L = o->lE[i];
             else L = ld_nt(ptr_at(a.lo_E[i], s));
-            const float el[4] = {bf16_lo(L.x), bf16_hi(L.x), bf16_lo(L.y), bf16_hi(L.y)};
             float eh[4], er[4];
 #pragma unroll
             for (int m = 0; m < 4; m++)
-                eh[m] = cmp_update(e[m], el[m], a.keep, a.dt2 * (M[i][m] + a.gsq[m] * J[i][m]), &er[m]);
+                if constexpr (U1) eh[m] = cmp_update1(e[m], res_val(L, m, e[m]), a.dt2 * (M[i][m] + a.gsq[m] * J[i][m]), &er[m]);
+                else eh[m] = cmp_update(e[m], res_val(L, m, e[m]), a.keep, a.dt2 * (M[i][m] + a.gsq[m] * J[i][m]), &er[m]);
             en = make_float4(eh[0], eh[1], eh[2], eh[3]);
-            st_nt(make_uint2(bf16_pack2(er[0], er[1]), bf16_pack2(er[2], er[3])), ptr_at(a.lo_E[i], s));
+            st_nt(res_pack4(res_q(er[0], eh[0]), res_q(er[1], eh[1]), res_q(er[2], eh[2]), res_q(er[3], eh[3])),
+                  ptr_at(a.lo_E[i], s));
         } else {
-            en.x = fma(a.dt2, fma(a.gsq[0], J[i][0], M[i][0]), a.keep * e[0]);
-            en.y = fma(a.dt2, fma(a.gsq[1], J[i][1], M[i][1]), a.keep * e[1]);
-            en.z = fma(a.dt2, fma(a.gsq[2], J[i][2], M[i][2]), a.keep * e[2]);
-            en.w = fma(a.dt2, fma(a.gsq[3], J[i][3], M[i][3]), a.keep * e[3]);
+            en.x = fma(a.dt2, fma(a.gsq[0], J[i][0], M[i][0]), U1 ? e[0] : a.keep * e[0]);
+            en.y = fma(a.dt2, fma(a.gsq[1], J[i][1], M[i][1]), U1 ? e[1] : a.keep * e[1]);
+            en.z = fma(a.dt2, fma(a.gsq[2], J[i][2], M[i][2]), U1 ? e[2] : a.keep * e[2]);
+            en.w = fma(a.dt2, fma(a.gsq[3], J[i][3], M[i][3]), U1 ? e[3] : a.keep * e[3]);
         }
         const R e2 = fma(en.w, en.w, fma(en.z, en.z, en.y * en.y));
         if (!(e2 < R(1)) || !(fabs(en.x) < R(1))) bad = 1;
@@ -550,6 +579,15 @@ __device__ __forceinline__ void kick_drift(const StepArgs<R>& a, unsigned s, con
         st_nt(qo, ptr_at(a.q_out[i], s));
     }
     if (bad) atomicOr(a.err, 1u);
+}
+
+// The kick with a uniform branch on damping: keep = 1 takes the shorter path.
+template <class R, int ND, bool CMP, bool PRE = false>
+__device__ __forceinline__ void kick_drift(const StepArgs<R>& a, unsigned s, const Qt<R>* U, const R* yv,
+                                           const R* phi, const R* F, const R (*J)[4], const R (*M)[4],
+                                           const OwnState<R, ND>* o = nullptr) {
+    if (a.keep == R(1)) kick_drift_t<R, ND, CMP, PRE, true>(a, s, U, yv, phi, F, J, M, o);
+    else kick_drift_t<R, ND, CMP, PRE, false>(a, s, U, yv, phi, F, J, M, o);
 }
 
 // One site-update at (x, y, z): forces, kick and drift (shared by the
@@ -1097,8 +1135,8 @@ struct ObsArgs {
     const V4* q[3];
     const V4* q_old[3];     // links one step earlier (Gauss only)
     const V4* E[3];
-    const uint4* lo_s[2];   // residuals of compensated storage (null otherwise)
-    const uint2* lo_E[3];
+    const uint2* lo_s[2];      // residuals of compensated storage (null otherwise)
+    const unsigned* lo_E[3];
     int nx, ny, nz;
 };
 
@@ -1115,31 +1153,30 @@ __device__ __forceinline__ void load_d(const V4& v, double* o) {
     o[0] = (double)v.x; o[1] = (double)v.y; o[2] = (double)v.z; o[3] = (double)v.w;
 }
 
-// Full-precision values of the compensated FP32 storage (hi + BF16 residual;
+// Full-precision values of the compensated FP32 storage (hi + 8-bit residual;
 // exact in FP64), or the stored values when there are no residuals.
+template <class V4>
+__device__ __forceinline__ void obs_add_res(const V4& hi, unsigned w, double* o) {
+    const float h[4] = {(float)hi.x, (float)hi.y, (float)hi.z, (float)hi.w};
+    for (int m = 0; m < 4; m++) o[m] += ldexp((double)res_byte(w, m), res_exp(h[m]) - 157);
+}
 template <class R>
 __device__ __forceinline__ void obs_phi(const ObsArgs<R>& a, int k, long long s, double* o) {
-    load_d(a.phi[k][s], o);
-    if (a.lo_s[k]) {
-        const uint4 L = a.lo_s[k][s];
-        o[0] += bf16_lo(L.x); o[1] += bf16_hi(L.x); o[2] += bf16_lo(L.y); o[3] += bf16_hi(L.y);
-    }
+    const auto hi = a.phi[k][s];
+    load_d(hi, o);
+    if (a.lo_s[k]) obs_add_res(hi, a.lo_s[k][s].x, o);
 }
 template <class R>
 __device__ __forceinline__ void obs_P(const ObsArgs<R>& a, int k, long long s, double* o) {
-    load_d(a.P[k][s], o);
-    if (a.lo_s[k]) {
-        const uint4 L = a.lo_s[k][s];
-        o[0] += bf16_lo(L.z); o[1] += bf16_hi(L.z); o[2] += bf16_lo(L.w); o[3] += bf16_hi(L.w);
-    }
+    const auto hi = a.P[k][s];
+    load_d(hi, o);
+    if (a.lo_s[k]) obs_add_res(hi, a.lo_s[k][s].y, o);
 }
 template <class R>
 __device__ __forceinline__ void obs_E(const ObsArgs<R>& a, int i, long long s, double* o) {
-    load_d(a.E[i][s], o);
-    if (a.lo_E[i]) {
-        const uint2 L = a.lo_E[i][s];
-        o[0] += bf16_lo(L.x); o[1] += bf16_hi(L.x); o[2] += bf16_lo(L.y); o[3] += bf16_hi(L.y);
-    }
+    const auto hi = a.E[i][s];
+    load_d(hi, o);
+    if (a.lo_E[i]) obs_add_res(hi, a.lo_E[i][s], o);
 }
 
 // 1 - sqrt(1 - x) without cancellation.
@@ -1356,8 +1393,8 @@ inline size_t carve(MomFields<R>* f, char* base, long long N, bool cmp) {
     for (int b = 0; b < 2; b++)
         for (int i = 0; i < 3; i++) { char* p = take(N, sizeof(V4)); if (f) f->q[b][i] = reinterpret_cast<V4*>(p); }
     for (int i = 0; i < 3; i++) { char* p = take(N, sizeof(V4)); if (f) f->E[i] = reinterpret_cast<V4*>(p); }
-    for (int k = 0; k < 2; k++) { char* p = cmp ? take(N, sizeof(uint4)) : nullptr; if (f) f->lo_s[k] = reinterpret_cast<uint4*>(p); }
-    for (int i = 0; i < 3; i++) { char* p = cmp ? take(N, sizeof(uint2)) : nullptr; if (f) f->lo_E[i] = reinterpret_cast<uint2*>(p); }
+    for (int k = 0; k < 2; k++) { char* p = cmp ? take(N, sizeof(uint2)) : nullptr; if (f) f->lo_s[k] = reinterpret_cast<uint2*>(p); }
+    for (int i = 0; i < 3; i++) { char* p = cmp ? take(N, sizeof(unsigned)) : nullptr; if (f) f->lo_E[i] = reinterpret_cast<unsigned*>(p); }
     return off;
 }
 
@@ -1440,23 +1477,25 @@ inline double wrap_host(double y) {
     return y;
 }
 
-// BF16 residuals on the host (round to nearest even, as the kernel's conversion).
-inline unsigned bf16_bits_host(double v) {
-    const float f = (float)v;
+// 8-bit residuals on the host, as the kernels encode them: q = r 2^(157 - e)
+// rounded to nearest even and clamped to [-127, 127], e the biased exponent of
+// the FP32 hi; value q 2^(e - 157).
+inline int res_exp_host(float hi) {
     unsigned u;
-    std::memcpy(&u, &f, 4);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return u >> 16;
+    std::memcpy(&u, &hi, 4);
+    return static_cast<int>((u >> 23) & 0xffu);
 }
-inline double bf16_value_host(unsigned h) {
-    const unsigned u = (h & 0xffffu) << 16;
-    float f;
-    std::memcpy(&f, &u, 4);
-    return (double)f;
+inline unsigned res_q_host(double r, float hi) {
+    const double q = std::nearbyint(std::ldexp(r, 157 - res_exp_host(hi)));
+    return static_cast<unsigned>(static_cast<int>(std::min(127.0, std::max(-127.0, q)))) & 0xffu;
+}
+inline double res_val_host(unsigned w, int m, float hi) {
+    const int q = static_cast<int>(static_cast<signed char>((w >> (8 * m)) & 0xffu));
+    return std::ldexp(static_cast<double>(q), res_exp_host(hi) - 157);
 }
 
 // Two-slot (reference layout) -> momentum form, on the host in FP64.  With
-// compensated storage each value is split into its FP32 hi and BF16 residual.
+// compensated storage each value is split into its FP32 hi and 8-bit residual.
 template <class R, class Src>
 int momentum_import_t(MomentumState& m, MomFields<R>& f, const Coef& c, const Src* S, const Src* V,
                       unsigned time_iter, hipStream_t st, std::string& err) {
@@ -1465,21 +1504,20 @@ int momentum_import_t(MomentumState& m, MomFields<R>& f, const Coef& c, const Sr
     const unsigned now = (time_iter + 1) % 2, past = !now;
     std::vector<R> phi(4 * N * 2, R(0)), P(4 * N * 2, R(0)), y(3 * N), q(12 * N), E(12 * N);
     const bool cmp = f.lo_s[0] != nullptr;
-    std::vector<uint4> lo_s(cmp ? 2 * N : 0, make_uint4(0, 0, 0, 0));
-    std::vector<uint2> lo_E(cmp ? 3 * N : 0, make_uint2(0, 0));
-    auto pair = [](unsigned a, unsigned b) { return a | (b << 16); };
+    std::vector<uint2> lo_s(cmp ? 2 * N : 0, make_uint2(0, 0));
+    std::vector<unsigned> lo_E(cmp ? 3 * N : 0, 0u);
     for (long long s = 0; s < N; s++) {
         for (int k = 0; k < nd; k++) {
-            unsigned lf[4], lp[4];
+            unsigned lf = 0, lp = 0;
             for (int m4 = 0; m4 < 4; m4++) {
                 const double a = (double)S[(now * N + s) * Ns + 4 * k + m4];
                 const double b = (double)S[(past * N + s) * Ns + 4 * k + m4];
                 phi[(k * N + s) * 4 + m4] = (R)a;
                 P[(k * N + s) * 4 + m4] = (R)(a - b);
-                lf[m4] = bf16_bits_host(a - (double)(R)a);
-                lp[m4] = bf16_bits_host((a - b) - (double)(R)(a - b));
+                lf |= res_q_host(a - (double)(float)a, (float)a) << (8 * m4);
+                lp |= res_q_host((a - b) - (double)(float)(a - b), (float)(a - b)) << (8 * m4);
             }
-            if (cmp) lo_s[k * N + s] = make_uint4(pair(lf[0], lf[1]), pair(lf[2], lf[3]), pair(lp[0], lp[1]), pair(lp[2], lp[3]));
+            if (cmp) lo_s[k * N + s] = make_uint2(lf, lp);
         }
         for (int i = 0; i < 3; i++) {
             const Src* ln = V + (now * N + s) * 15 + 5 * i;
@@ -1491,12 +1529,12 @@ int momentum_import_t(MomentumState& m, MomFields<R>& f, const Coef& c, const Sr
             double r[4];
             qmul_full(un, up, r);
             const double ev[4] = {std::sin((double)ln[0] - (double)lp[0]), r[1], r[2], r[3]};
-            unsigned le[4];
+            unsigned le = 0;
             for (int k = 0; k < 4; k++) {
                 E[(i * N + s) * 4 + k] = (R)ev[k];
-                le[k] = bf16_bits_host(ev[k] - (double)(R)ev[k]);
+                le |= res_q_host(ev[k] - (double)(float)ev[k], (float)ev[k]) << (8 * k);
             }
-            if (cmp) lo_E[i * N + s] = make_uint2(pair(le[0], le[1]), pair(le[2], le[3]));
+            if (cmp) lo_E[i * N + s] = le;
         }
     }
     m.cur = 0;
@@ -1513,10 +1551,10 @@ int momentum_import_t(MomentumState& m, MomFields<R>& f, const Coef& c, const Sr
         ok &= cp(f.y[0][i], &y[i * N], N * sizeof(R));
         ok &= cp(f.q[0][i], &q[i * N * 4], N * 4 * sizeof(R));
         ok &= cp(f.E[i], &E[i * N * 4], N * 4 * sizeof(R));
-        if (cmp) ok &= cp(f.lo_E[i], &lo_E[i * N], N * sizeof(uint2));
+        if (cmp) ok &= cp(f.lo_E[i], &lo_E[i * N], N * sizeof(unsigned));
     }
     if (cmp)
-        for (int k = 0; k < nd; k++) ok &= cp(f.lo_s[k], &lo_s[k * N], N * sizeof(uint4));
+        for (int k = 0; k < nd; k++) ok &= cp(f.lo_s[k], &lo_s[k * N], N * sizeof(uint2));
     ok &= hipStreamSynchronize(st) == hipSuccess;
     if (!ok) { err = "momentum import: copy failed"; return GFT_EHIP; }
     return GFT_OK;
@@ -1532,8 +1570,8 @@ int momentum_export_t(MomentumState& m, MomFields<R>& f, const Coef& c, Dst* S, 
     const unsigned now = (time_iter + 1) % 2, past = !now;
     std::vector<R> phi(4 * N * 2), P(4 * N * 2), y(3 * N), q(12 * N), E(12 * N);
     const bool cmp = f.lo_s[0] != nullptr;
-    std::vector<uint4> lo_s(cmp ? 2 * N : 0);
-    std::vector<uint2> lo_E(cmp ? 3 * N : 0);
+    std::vector<uint2> lo_s(cmp ? 2 * N : 0);
+    std::vector<unsigned> lo_E(cmp ? 3 * N : 0);
     auto cp = [&](void* dst, const void* src, size_t bytes) {
         return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st) == hipSuccess;
     };
@@ -1546,26 +1584,21 @@ int momentum_export_t(MomentumState& m, MomFields<R>& f, const Coef& c, Dst* S, 
         ok &= cp(&y[i * N], f.y[m.cur][i], N * sizeof(R));
         ok &= cp(&q[i * N * 4], f.q[m.cur][i], N * 4 * sizeof(R));
         ok &= cp(&E[i * N * 4], f.E[i], N * 4 * sizeof(R));
-        if (cmp) ok &= cp(&lo_E[i * N], f.lo_E[i], N * sizeof(uint2));
+        if (cmp) ok &= cp(&lo_E[i * N], f.lo_E[i], N * sizeof(unsigned));
     }
     if (cmp)
-        for (int k = 0; k < nd; k++) ok &= cp(&lo_s[k * N], f.lo_s[k], N * sizeof(uint4));
+        for (int k = 0; k < nd; k++) ok &= cp(&lo_s[k * N], f.lo_s[k], N * sizeof(uint2));
     ok &= hipStreamSynchronize(st) == hipSuccess;
     if (!ok) { err = "momentum export: copy failed"; return GFT_EHIP; }
-    // Residual m (0..3) of a pair of packed BF16 words.
-    auto res = [](unsigned w0, unsigned w1, int m) {
-        const unsigned w = m < 2 ? w0 : w1;
-        return bf16_value_host((m & 1) ? (w >> 16) : w);
-    };
     for (long long s = 0; s < N; s++) {
         for (int k = 0; k < nd; k++)
             for (int m4 = 0; m4 < 4; m4++) {
                 double a = (double)phi[(k * N + s) * 4 + m4];
                 double pp = (double)P[(k * N + s) * 4 + m4];
                 if (cmp) {
-                    const uint4 L = lo_s[k * N + s];
-                    a += res(L.x, L.y, m4);
-                    pp += res(L.z, L.w, m4);
+                    const uint2 L = lo_s[k * N + s];
+                    a += res_val_host(L.x, m4, (float)phi[(k * N + s) * 4 + m4]);
+                    pp += res_val_host(L.y, m4, (float)P[(k * N + s) * 4 + m4]);
                 }
                 S[(now * N + s) * Ns + 4 * k + m4] = (Dst)a;
                 S[(past * N + s) * Ns + 4 * k + m4] = (Dst)(a - pp);
@@ -1577,7 +1610,7 @@ int momentum_export_t(MomentumState& m, MomFields<R>& f, const Coef& c, Dst* S, 
             double e[4] = {(double)E[(i * N + s) * 4], (double)E[(i * N + s) * 4 + 1],
                            (double)E[(i * N + s) * 4 + 2], (double)E[(i * N + s) * 4 + 3]};
             if (cmp)
-                for (int k = 0; k < 4; k++) e[k] += res(lo_E[i * N + s].x, lo_E[i * N + s].y, k);
+                for (int k = 0; k < 4; k++) e[k] += res_val_host(lo_E[i * N + s], k, (float)E[(i * N + s) * 4 + k]);
             const double un[4] = {(double)q[(i * N + s) * 4], (double)q[(i * N + s) * 4 + 1],
                                   (double)q[(i * N + s) * 4 + 2], (double)q[(i * N + s) * 4 + 3]};
             const double ud[4] = {std::sqrt(std::max(0.0, 1.0 - e[1] * e[1] - e[2] * e[2] - e[3] * e[3])), -e[1], -e[2],
