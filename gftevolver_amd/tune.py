@@ -6,9 +6,8 @@ only picks the fastest: each candidate runs a few steps on the live state and
 is timed by the library's HIP events on its own stream (gft_last_step_ms).
 The FP32 SM kernel is compiled for 3 waves per SIMD by default; the other
 occupancy variants (unconstrained 0, 4) are separately compiled kernels whose
-results may differ in the last bit; they, and the unstaged kernels, are tried
-only with occupancy=True (staging modes 2 and 1 always).  A candidate is
-(bz, by, order, min_waves, stage).
+results may differ in the last bit; they, and staging modes 1 and 0, are
+tried only with occupancy=True.  A candidate is (bz, by, order, min_waves, stage).
 """
 from __future__ import annotations
 
@@ -18,7 +17,7 @@ ORDERS = (2, 1)
 
 def candidates(occupancy: bool = False):
     return [(bz, by, order, mw, st) for (bz, by) in TILES for order in ORDERS
-            for mw in ((3, 0, 4) if occupancy else (3,)) for st in ((2, 1, 0) if occupancy else (2, 1))]
+            for mw in ((3, 0, 4) if occupancy else (3,)) for st in ((2, 1, 0) if occupancy else (2,))]
 
 
 def tune(ev, candidates_=None, steps: int = 6, rounds: int = 2, fp32_sm: bool = True, occupancy: bool = False):
