@@ -62,7 +62,7 @@ struct MomentumState {
     void* base = nullptr;
     size_t bytes = 0;
     bool f64 = false;
-    bool cmp = false;        // FP32 storage with BF16 residuals of phi, P and E
+    bool cmp = false;        // FP32 storage with 8-bit residuals of phi, P and E
     int cur = 0;             // buffer holding (phi, U) at the current time
     int prev_valid = 0;      // buffer !cur holds the state one step earlier
     MomFields<float> ff{};
