@@ -35,6 +35,7 @@ class GftDesc(C.Structure):
         ("m12i", C.c_double), ("l4m5", C.c_double), ("l5i", C.c_double), ("l6i", C.c_double), ("l7i", C.c_double),
         ("pot_norm", C.c_int), ("pot_offset", C.c_double),
         ("damping", C.c_float), ("nt_damped", C.c_uint),
+        ("fixed_faces", C.c_int),
     ]
 
 
@@ -52,10 +53,12 @@ class GftLoops(C.Structure):
 
 
 def make_desc(model=GFT_MODEL_SM, form=GFT_FORM_TWO_SLOT_F64, n=(8, 8, 8), d=(0.5, 0.5, 0.5), dt=0.05,
-              g=0.65, gp=0.35, lam=0.13, eta_sqr=1.0, pot2=None, damping=0.0, nt_damped=0, div_tol=1e-9):
+              g=0.65, gp=0.35, lam=0.13, eta_sqr=1.0, pot2=None, damping=0.0, nt_damped=0, div_tol=1e-9,
+              fixed_faces=0):
     """A gft_desc with the SM couplings of the 32^3 / 256^3 workloads by default.
 
-    pot2: Double_SO_N parameters for the 2HDM (keys as in gft_desc)."""
+    pot2: Double_SO_N parameters for the 2HDM (keys as in gft_desc).
+    fixed_faces: bit 2a / 2a+1 fixes the low / high face of axis a (x, y, z)."""
     ds = GftDesc()
     ds.model, ds.form = model, form
     ds.nx, ds.ny, ds.nz = n
@@ -68,6 +71,7 @@ def make_desc(model=GFT_MODEL_SM, form=GFT_FORM_TWO_SLOT_F64, n=(8, 8, 8), d=(0.
         for k, v in pot2.items():
             setattr(ds, k, v)
     ds.damping, ds.nt_damped = damping, nt_damped
+    ds.fixed_faces = fixed_faces
     return ds
 
 

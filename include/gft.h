@@ -76,6 +76,13 @@ typedef struct gft_desc {
     /* damping (Core/Source/Model.cpp:241-254, 345): gamma for time_iter < nt_damped          */
     float    damping;
     unsigned nt_damped;
+    /* Fixed faces (Lattice.cfg face types, Lattice.cpp:217-356; Fixed.cpp:159-251): bit 2a set
+     * fixes the low face of axis a (x, y, z), bit 2a+1 the high face; a face's pair must match
+     * (the reference requires periodic faces in opposite pairs).  A site in the one-site shell
+     * of a fixed face is frozen: never updated (both time slots keep its initial values, which
+     * must be equal there), still read by its neighbours, and left out of the energy and Gauss
+     * sums.  0 = fully periodic.                                                              */
+    int      fixed_faces;
 } gft_desc;
 
 /* Observables of one measured step.  energy[] columns follow the reference's

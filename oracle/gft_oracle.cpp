@@ -68,6 +68,7 @@ struct Params {
     const gft_desc* d;
     float damping = 0.f;
     unsigned nt_damped = 0;
+    int fixed = 0;        // fixed faces (bit 2a low, 2a+1 high face of axis a)
 };
 
 Params make_params(const gft_desc* d) {
@@ -96,7 +97,23 @@ Params make_params(const gft_desc* d) {
     p.gsq_f[1] = p.gsq_f[2] = p.gsq_f[3] = static_cast<float>(p.g_sqr);
     p.damping = d->damping;
     p.nt_damped = d->nt_damped;
+    p.fixed = d->fixed_faces;
     return p;
+}
+
+// A site in the one-site shell of a fixed face: Fixed::evolve (Fixed.cpp:159-240)
+// leaves it alone and Fixed::postEvolveAnalysis (Fixed.cpp:245-251) and the
+// skipped location analysis leave it out of the diagnostics; regions touching
+// any fixed face are fixed (Lattice.cpp:293-312).
+inline bool frozen(const Params& p, long s) {
+    if (!p.fixed) return false;
+    const long z = s % p.nz, y = (s / p.nz) % p.ny, x = s / (p.nz * p.ny);
+    const long c[3] = {x, y, z}, n[3] = {p.nx, p.ny, p.nz};
+    for (int a = 0; a < 3; a++) {
+        if ((p.fixed >> (2 * a) & 1) && c[a] == 0) return true;
+        if ((p.fixed >> (2 * a + 1) & 1) && c[a] == n[a] - 1) return true;
+    }
+    return false;
 }
 
 // Periodic neighbour of site s along axis a by delta in {-1, +1}
@@ -663,12 +680,14 @@ int evolve(const gft_desc* d, T* S, T* V, unsigned time_iter, unsigned nsteps, u
         if (meas_e) dens.assign(p.N, Dens<A>{});
 #pragma omp parallel for schedule(static)
         for (long s = 0; s < p.N; s++)
-            site_update<T, A>(p, S, V, s, now, meas_e, damping_step, meas_e ? &dens[s] : nullptr, &panics);
+            if (!frozen(p, s))
+                site_update<T, A>(p, S, V, s, now, meas_e, damping_step, meas_e ? &dens[s] : nullptr, &panics);
         if (meas_g) {
             gviol.assign(p.N * 4, A(0));
             gscale.assign(p.N * 4, 0.0);
 #pragma omp parallel for schedule(static)
-            for (long s = 0; s < p.N; s++) site_gauss<T, A>(p, S, V, s, now, &gviol[s * 4], &gscale[s * 4]);
+            for (long s = 0; s < p.N; s++)
+                if (!frozen(p, s)) site_gauss<T, A>(p, S, V, s, now, &gviol[s * 4], &gscale[s * 4]);
         }
         if ((meas_e || meas_g) && row < max_rows) {
             gft_obs* o = rows ? &rows[row] : nullptr;
@@ -680,6 +699,7 @@ int evolve(const gft_desc* d, T* S, T* V, unsigned time_iter, unsigned nsteps, u
                 A acc[6] = {0, 0, 0, 0, 0, 0};
                 Neumaier nm[6];
                 for (long s : order) {
+                    if (frozen(p, s)) continue;
                     const Dens<A>& dd = dens[s];
                     const A tot = dd.v[0] + dd.v[1] + dd.v[2] + dd.v[3] + dd.v[4];
                     acc[0] += tot;
@@ -699,6 +719,7 @@ int evolve(const gft_desc* d, T* S, T* V, unsigned time_iter, unsigned nsteps, u
                 Neumaier ni[4], ns[4];
                 double dmx[4] = {0, 0, 0, 0};
                 for (long s : order) {
+                    if (frozen(p, s)) continue;
                     for (int m = 0; m < 4; m++) {
                         const A v = std::abs(gviol[s * 4 + m]);
                         integ[m] += v;

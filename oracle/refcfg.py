@@ -97,11 +97,12 @@ def write_model(root: str, model: int, g: float, gp: float, *, lam=0.13, eta_sqr
 
 
 def write_lattice(root: str, n, d, nt, dt, ns, *, ic_seed=42, ic_range=(-1.0, 1.0), efreq=1, gfreq=1,
-                  energy=True, gauss=True, progress=False):
+                  energy=True, gauss=True, progress=False, faces=(3, 3, 3, 3, 3, 3)):
     """Lattice.cfg, RandomUniform.cfg and analyser configs for a full program run
-    with a RandomUniform scalar initial state and identity links."""
+    with a RandomUniform scalar initial state and identity links.  faces: the
+    six face boundary types (-x, +x, -y, +y, -z, +z): 1 fixed, 3 periodic."""
     set_values(root, "Core", "Lattice.cfg", [
-        tuple(n), tuple(d), nt, dt, ns, 5, 0, 3.0, (1, 0), 0, int(energy), int(gauss), (3, 3, 3, 3, 3, 3),
+        tuple(n), tuple(d), nt, dt, ns, 5, 0, 3.0, (1, 0), 0, int(energy), int(gauss), tuple(faces),
         int(progress), max(nt, 1),
     ])
     set_values(root, "InitialConditions", "RandomUniform.cfg", [ic_seed, tuple(ic_range), 1])

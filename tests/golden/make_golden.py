@@ -13,6 +13,11 @@ Two kinds of fixture, both produced by the reference's own compiled code:
                        oracle/_ref/gft_reference) on a 6x5x7 lattice with its
                        RandomUniform initial state: Data/global_quantities.dat and
                        Data/global_quantities_gauge.dat as printed (6 digits).
+  program_sm_fixed.npz the same SM program with fixed faces: x and z fixed
+                       (faces 1 1 3 3 1 1) and y fixed (3 3 1 1 3 3), keys
+                       energy_<case>, gauss_<case>.
+
+    python tests/golden/make_golden.py [--fixed]   # --fixed: only the last file
 """
 from __future__ import annotations
 
@@ -92,5 +97,29 @@ def main():
         print(tag, "written")
 
 
+FIXED_CASES = {"xz": (1, 1, 3, 3, 1, 1), "y": (3, 3, 1, 1, 3, 3)}
+
+
+def fixed_faces_program():
+    out = {}
+    for case, faces in FIXED_CASES.items():
+        with tempfile.TemporaryDirectory() as td:
+            refcfg.prepare(td)
+            refcfg.write_model(td, 0, 0.65, 0.35)
+            refcfg.write_lattice(td, PROGRAM_N, (0.5, 0.5, 0.5), PROGRAM_STEPS, 0.05, 4, ic_seed=PROGRAM_SEED,
+                                 ic_range=PROGRAM_RANGE, efreq=PROGRAM_EFREQ, gfreq=PROGRAM_GFREQ, faces=faces)
+            subprocess.run([os.path.join(O.REF_DIR, "gft_reference")], check=True, stdout=subprocess.DEVNULL,
+                           stderr=subprocess.DEVNULL, cwd=td)
+            data = os.path.join(td, "data")
+            out[f"energy_{case}"] = np.loadtxt(os.path.join(data, "global_quantities.dat"), ndmin=2)
+            out[f"gauss_{case}"] = np.loadtxt(os.path.join(data, "global_quantities_gauge.dat"), ndmin=2)
+    np.savez_compressed(os.path.join(HERE, "program_sm_fixed.npz"), **out)
+    print("sm_fixed written")
+
+
 if __name__ == "__main__":
-    main()
+    if "--fixed" in sys.argv[1:]:
+        fixed_faces_program()
+    else:
+        main()
+        fixed_faces_program()

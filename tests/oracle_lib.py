@@ -140,3 +140,16 @@ def gauss_site_f32(desc, scalar, vec, time_iter):
     L.oracle_gauss_site_f32(C.byref(desc), scalar.ctypes.data_as(P), vec.ctypes.data_as(P), time_iter,
                             out.ctypes.data_as(P))
     return out
+
+
+def frozen_mask(n, fixed_faces):
+    """Sites in the one-site shell of a fixed face (bit 2a low, 2a+1 high face of axis a)."""
+    x, y, z = np.meshgrid(*[np.arange(k) for k in n], indexing="ij")
+    c = (x, y, z)
+    m = np.zeros(n, bool)
+    for a in range(3):
+        if fixed_faces >> (2 * a) & 1:
+            m |= c[a] == 0
+        if fixed_faces >> (2 * a + 1) & 1:
+            m |= c[a] == n[a] - 1
+    return m.reshape(-1)

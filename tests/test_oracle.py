@@ -63,6 +63,34 @@ def test_oracle_reproduces_reference_program_output(model, tag, pot2):
     np.testing.assert_allclose(e64, e, rtol=2e-5, atol=1e-3 * np.abs(e).max())
 
 
+@pytest.mark.parametrize("case,mask", [("xz", 1 | 2 | 16 | 32), ("y", 4 | 8)], ids=["xz", "y"])
+def test_oracle_reproduces_reference_program_with_fixed_faces(case, mask):
+    """Fixed faces (Fixed.cpp: frozen one-site shells, left out of the
+    diagnostics): the reference program's printed energies and Gauss
+    violations, digit for digit, for x and z fixed and for y fixed."""
+    g = np.load(os.path.join(GOLD, "program_sm_fixed.npz"))
+    n = (6, 5, 7)
+    N = int(np.prod(n))
+    s = O.random_uniform(2 * N * 4, 7, -0.8, 0.8).reshape(2, N, 4)
+    v = O.identity_links(N)
+    d = make_desc(n=n, fixed_faces=mask)
+    s0 = s.copy()
+    ref, obs, _ = O.evolve(d, s, v, 12, energy_freq=3, gauss_freq=2)
+    fmt = np.vectorize(lambda x: float("%.6g" % x))
+    e = np.array([r[:6] for r, o in zip(ref, obs) if o.has_energy])
+    gs = np.array([r[6:] for r, o in zip(ref, obs) if o.has_gauss])
+    assert np.array_equal(fmt(e), g[f"energy_{case}"])
+    assert np.array_equal(fmt(gs), g[f"gauss_{case}"])
+    # The frozen shell keeps its initial values in both slots.
+    frozen = O.frozen_mask(n, mask)
+    assert frozen.any() and not frozen.all()
+    assert np.array_equal(s[:, frozen], s0[:, frozen])
+    # Fully periodic differs: the boundary condition is exercised.
+    s2 = O.random_uniform(2 * N * 4, 7, -0.8, 0.8).reshape(2, N, 4)
+    ref2, obs2, _ = O.evolve(make_desc(n=n), s2, O.identity_links(N), 12, energy_freq=3, gauss_freq=2)
+    assert not np.array_equal(fmt(np.array([r[:6] for r, o in zip(ref2, obs2) if o.has_energy])), g[f"energy_{case}"])
+
+
 def test_reference_site_order_is_a_permutation_with_shell_first():
     n = (5, 4, 6)
     order = O.site_order(*n)
