@@ -103,6 +103,7 @@ static Coef make_coef(const gft_desc& d) {
     c.break_son = d.break_to_son; c.break_uhalfn = d.break_to_uhalfn;
     c.pot_norm = d.pot_norm; c.pot_offset = d.pot_offset;
     c.dvol = d.dx * d.dy * d.dz;
+    c.fixed = d.fixed_faces;
     return c;
 }
 
@@ -143,6 +144,14 @@ int gft_create(const gft_desc* desc, gft_ctx** out) {
         return fail(nullptr, GFT_EINVAL, "gft_create: at most 2^28 sites per device (32-bit site offsets)");
     if (!(d.dx > 0 && d.dy > 0 && d.dz > 0 && d.dt > 0))
         return fail(nullptr, GFT_EINVAL, "LATTICE:: spacings and time step must be positive");
+    if (d.fixed_faces < 0 || d.fixed_faces > 63)
+        return fail(nullptr, GFT_EINVAL, "LATTICE:: fixed_faces holds six face bits");
+    for (int a = 0; a < 3; a++) {
+        const int pair = (d.fixed_faces >> (2 * a)) & 3;
+        if (pair == 1 || pair == 2)   // Lattice.cpp:245-260
+            return fail(nullptr, GFT_EINVAL, "LATTICE:: Periodic boundary conditions must be assigned to both faces "
+                                             "of an axis (axis " + std::to_string(a) + ").");
+    }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
         return fail(nullptr, GFT_ENODEV, "gft_create: no HIP device");

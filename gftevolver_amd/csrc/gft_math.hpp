@@ -37,7 +37,21 @@ struct Coef {
     int break_son, break_uhalfn, pot_norm;
     double pot_offset;
     double dvol;
+    int fixed;                   // fixed faces: bit 2a low, 2a+1 high face of axis a (0: periodic)
 };
+
+// A site in the one-site shell of a fixed face is frozen: not updated, left out
+// of the energy and Gauss sums (Fixed.cpp:159-251; Lattice.cpp:293-312).
+GFT_HD inline bool site_frozen(const Coef& c, int x, int y, int z) {
+    if (!c.fixed) return false;
+    return ((c.fixed & 1) && x == 0) || ((c.fixed & 2) && x == c.nx - 1) || ((c.fixed & 4) && y == 0) ||
+           ((c.fixed & 8) && y == c.ny - 1) || ((c.fixed & 16) && z == 0) || ((c.fixed & 32) && z == c.nz - 1);
+}
+GFT_HD inline bool site_frozen(const Coef& c, long long s) {
+    if (!c.fixed) return false;
+    const int z = (int)(s % c.nz), y = (int)((s / c.nz) % c.ny), x = (int)(s / ((long long)c.nz * c.ny));
+    return site_frozen(c, x, y, z);
+}
 
 template <class R>
 GFT_HD void qmul_vec(const R* a, const R* b, R* r) {

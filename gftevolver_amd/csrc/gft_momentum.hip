@@ -602,6 +602,16 @@ __device__ __forceinline__ void mom_site(const StepArgs<R>& a, const Coef& c, in
     // become a scalar base plus one 32-bit vector offset, with no 64-bit math.
     const int sy = a.nz, sx = a.ny * a.nz;
     const unsigned s = static_cast<unsigned>(x * sx + y * sy + z);
+    if (site_frozen(c, x, y, z)) {   // shell of a fixed face: carried into the next buffer unchanged
+#pragma unroll
+        for (int k = 0; k < ND; k++) st_nt(at(a.phi[k], s), ptr_at(a.phi_out[k], s));
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            st_nt(at(a.y[i], s), ptr_at(a.y_out[i], s));
+            st_nt(at(a.q[i], s), ptr_at(a.q_out[i], s));
+        }
+        return;
+    }
     // Offsets to the +e_a and -e_a neighbours with periodic wrap.
     int dp[3], dm[3];
     dp[0] = (x == a.nx - 1) ? -(a.nx - 1) * sx : sx;
@@ -1011,6 +1021,17 @@ __device__ __forceinline__ void mom_block_shared(const StepArgs<float>& a, const
     }
     __syncthreads();
     if (!valid) return;
+    if (site_frozen(c, x, y, z)) {
+        // Shell of a fixed face: its plaquettes and differences served the
+        // neighbours above; the site itself is carried over unchanged.
+        st_nt(make_float4(phi[0], phi[1], phi[2], phi[3]), ptr_at(a.phi_out[0], s));
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            st_nt(yv[i], ptr_at(a.y_out[i], s));
+            st_nt(make_float4(U[i].w, U[i].x, U[i].y, U[i].z), ptr_at(a.q_out[i], s));
+        }
+        return;
+    }
 
     // ---- phase B: the scalar force's i = 1, 2 terms ----
     {
@@ -1190,7 +1211,7 @@ __global__ void __launch_bounds__(256) mom_energy(ObsArgs<R> a, Coef c, double* 
     __shared__ double lds[4 * 5];
     const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     double d[5] = {0, 0, 0, 0, 0};
-    if (s < c.nsites) {
+    if (s < c.nsites && !site_frozen(c, s)) {
         double phi[8];
         for (int k = 0; k < ND; k++) obs_phi(a, k, s, phi + 4 * k);
         d[0] = potential_energy(c, phi);
@@ -1268,7 +1289,7 @@ __global__ void __launch_bounds__(256) mom_gauss(ObsArgs<R> a, Coef c, double* _
     __shared__ double lds[4 * 12];
     const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     double sums[8] = {0, 0, 0, 0, 0, 0, 0, 0}, maxs[4] = {0, 0, 0, 0};
-    if (s < c.nsites) {
+    if (s < c.nsites && !site_frozen(c, s)) {
         // rho = 1/4 B(phi(t), phi(t+dt)) = 1/4 B(phi(t), P): the bilinears are
         // antisymmetric, so B(phi, phi) = 0.  Using the stored P avoids the
         // rounding of phi(t+dt) = phi(t) + P, which is large next to the charge.

@@ -124,7 +124,7 @@ __global__ void __launch_bounds__(256) twoslot_step(Coef c, T* __restrict__ S, T
     __shared__ double lds[4 * 5];
     const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     double dens[5] = {0, 0, 0, 0, 0};
-    if (s < c.nsites) {
+    if (s < c.nsites && !site_frozen(c, s)) {
         const int Ns = c.ns, nd = c.nd;
         const long long N = c.nsites;
         const unsigned past = !now;
@@ -290,7 +290,7 @@ __global__ void __launch_bounds__(256) twoslot_gauss(Coef c, const T* __restrict
     __shared__ double lds[4 * 12];
     const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     double sums[8] = {0, 0, 0, 0, 0, 0, 0, 0}, maxs[4] = {0, 0, 0, 0};
-    if (s < c.nsites) {
+    if (s < c.nsites && !site_frozen(c, s)) {
         const long long N = c.nsites;
         const int Ns = c.ns;
         const unsigned fut = !now;

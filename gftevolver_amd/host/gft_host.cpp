@@ -434,13 +434,16 @@ void Lattice::configure(const std::string& path) {
     r.skip_lines(7);
     int faces[6] = {0, 0, 0, 0, 0, 0};
     r.values(faces, 6);
-    for (int f = 0; f < 6; f++) {
+    fixedFaces = 0;
+    for (int f = 0; f < 6; f++) {   // Lattice.cpp:217-260
         if (faces[f] != FIXED && faces[f] != PERIODIC)
             throw std::runtime_error("LATTICE:: Face boundary condition (" + std::to_string(f) + ") has unsupported type "
                                      + std::to_string(faces[f]) + ".");
-        if (faces[f] == FIXED)
-            throw std::runtime_error("LATTICE:: the GPU evolver supports periodic boundaries only");
+        if (faces[f] == FIXED) fixedFaces |= 1 << f;
     }
+    for (int a = 0; a < 3; a++)
+        if ((faces[2 * a] == PERIODIC) != (faces[2 * a + 1] == PERIODIC))
+            throw std::runtime_error("LATTICE:: Periodic boundary conditions must be assigned to both faces of an axis.");
     r.skip_lines(1);
     progressReport = r.value<bool>();
     reportFrequency = r.value<unsigned>();
@@ -475,6 +478,7 @@ void Lattice::evolve() {
     d.form = form;
     d.nx = nx; d.ny = ny; d.nz = nz;
     d.dx = dx; d.dy = dy; d.dz = dz; d.dt = dt;
+    d.fixed_faces = fixedFaces;
     model.fill(d);
     if (!ctx) {
         if (gft_create(&d, &ctx) != GFT_OK) throw std::runtime_error(gft_last_error(nullptr));

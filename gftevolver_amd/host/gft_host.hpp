@@ -251,6 +251,7 @@ public:
     std::vector<float> scalarFields, vectorFields;
     Model model;
     bool progressReport = false;
+    int fixedFaces = 0;          // fixed faces, gft_desc::fixed_faces bits
     unsigned reportFrequency = 1;
     double lastEvolveSeconds = 0;
 

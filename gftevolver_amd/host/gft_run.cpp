@@ -34,7 +34,8 @@ int main(int argc, char** argv) {
                       << "m1 " << d.m1 << "\nm2 " << d.m2 << "\nl1 " << d.l1 << "\nl3 " << d.l3 << "\n"
                       << "m12r " << d.m12r << "\nl4p5 " << d.l4p5 << "\nl4m5 " << d.l4m5 << "\n"
                       << "damping " << d.damping << "\nnt_damped " << d.nt_damped << "\n"
-                      << "progress " << lattice.progressReport << "\n";
+                      << "progress " << lattice.progressReport << "\n"
+                      << "fixed_faces " << lattice.fixedFaces << "\n";
             double s0 = 0;
             for (float x : lattice.scalarFields) s0 += x;
             std::cout << "scalar_sum " << s0 << "\n";
