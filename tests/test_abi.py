@@ -46,3 +46,19 @@ def test_struct_layouts_match_header():
     hdr = re.findall(r"\b(?:int|double|float|unsigned)\s+([^;]+);", body)
     names = [x.strip() for grp in hdr for x in grp.split(",")]
     assert [n if n != "lambda" else "lambda_" for n in names] == fields
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libgft.so not built")
+@pytest.mark.parametrize("faces,msg", [(1, "both faces of an axis (axis 0)"), (4 | 16 | 32, "(axis 1)"),
+                                       (64, "six face bits"), (-1, "six face bits")])
+def test_create_rejects_invalid_face_masks_before_touching_a_device(faces, msg):
+    """gft_create validates the configuration before any HIP call, with the
+    reference's message for an unpaired periodic face (Lattice.cpp:244-259)."""
+    lib = C.CDLL(LIB)
+    lib.gft_last_error.restype = C.c_char_p
+    lib.gft_last_error.argtypes = [C.c_void_p]
+    d = _abi.make_desc(form=_abi.GFT_FORM_MOMENTUM_F32, n=(8, 8, 8), fixed_faces=faces)
+    ctx = C.c_void_p()
+    assert lib.gft_create(C.byref(d), C.byref(ctx)) == -1
+    assert not ctx.value
+    assert msg in lib.gft_last_error(None).decode()
