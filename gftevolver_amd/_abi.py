@@ -36,6 +36,7 @@ class GftDesc(C.Structure):
         ("pot_norm", C.c_int), ("pot_offset", C.c_double),
         ("damping", C.c_float), ("nt_damped", C.c_uint),
         ("fixed_faces", C.c_int),
+        ("gen_rep", C.c_int),
     ]
 
 
@@ -54,11 +55,12 @@ class GftLoops(C.Structure):
 
 def make_desc(model=GFT_MODEL_SM, form=GFT_FORM_TWO_SLOT_F64, n=(8, 8, 8), d=(0.5, 0.5, 0.5), dt=0.05,
               g=0.65, gp=0.35, lam=0.13, eta_sqr=1.0, pot2=None, damping=0.0, nt_damped=0, div_tol=1e-9,
-              fixed_faces=0):
+              fixed_faces=0, gen_rep=0):
     """A gft_desc with the SM couplings of the 32^3 / 256^3 workloads by default.
 
     pot2: Double_SO_N parameters for the 2HDM (keys as in gft_desc).
-    fixed_faces: bit 2a / 2a+1 fixes the low / high face of axis a (x, y, z)."""
+    fixed_faces: bit 2a / 2a+1 fixes the low / high face of axis a (x, y, z).
+    gen_rep: 1 stores SU(2) links as generator components (12 reals per site)."""
     ds = GftDesc()
     ds.model, ds.form = model, form
     ds.nx, ds.ny, ds.nz = n
@@ -72,6 +74,7 @@ def make_desc(model=GFT_MODEL_SM, form=GFT_FORM_TWO_SLOT_F64, n=(8, 8, 8), d=(0.
             setattr(ds, k, v)
     ds.damping, ds.nt_damped = damping, nt_damped
     ds.fixed_faces = fixed_faces
+    ds.gen_rep = gen_rep
     return ds
 
 

@@ -31,6 +31,8 @@ constexpr int kEventPool = 256;
 
 bool is_two_slot(int form) { return form == GFT_FORM_TWO_SLOT_F64 || form == GFT_FORM_TWO_SLOT_F32; }
 bool is_f64(int form) { return form == GFT_FORM_TWO_SLOT_F64 || form == GFT_FORM_MOMENTUM_F64; }
+// Link reals per site of the two-slot arrays: 3 x [y, c0..c3] or 3 x [y, w1..w3].
+long long link_reals(const gft_desc& d) { return d.gen_rep ? 12 : 15; }
 
 }  // namespace
 
@@ -104,6 +106,7 @@ static Coef make_coef(const gft_desc& d) {
     c.pot_norm = d.pot_norm; c.pot_offset = d.pot_offset;
     c.dvol = d.dx * d.dy * d.dz;
     c.fixed = d.fixed_faces;
+    c.tol = d.div_tol;
     return c;
 }
 
@@ -152,6 +155,8 @@ int gft_create(const gft_desc* desc, gft_ctx** out) {
             return fail(nullptr, GFT_EINVAL, "LATTICE:: Periodic boundary conditions must be assigned to both faces "
                                              "of an axis (axis " + std::to_string(a) + ").");
     }
+    if (d.gen_rep != 0 && d.gen_rep != 1)
+        return fail(nullptr, GFT_EINVAL, "gft_create: gen_rep is 0 (quaternion links) or 1 (generator form)");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
         return fail(nullptr, GFT_ENODEV, "gft_create: no HIP device");
@@ -170,7 +175,7 @@ int gft_create(const gft_desc* desc, gft_ctx** out) {
     const long long N = ctx->N;
     if (is_two_slot(d.form)) {
         if (hipMalloc(&ctx->S, 2 * N * ctx->coef.ns * esz) != hipSuccess
-            || hipMalloc(&ctx->V, 2 * N * 15 * esz) != hipSuccess) {
+            || hipMalloc(&ctx->V, 2 * N * link_reals(d) * esz) != hipSuccess) {
             g_create_error = "gft_create: device allocation failed";
             return bail(GFT_ENOMEM);
         }
@@ -224,10 +229,59 @@ int gft_sync(gft_ctx* ctx) {
 
 // ---- two-slot import/export --------------------------------------------------
 
+// Generator-form links [y, w1, w2, w3] to the quaternion form [y, c0, c1, c2, c3]
+// and back, in FP64.  The momentum forms keep quaternions; these run at their
+// import and export only.  The forward map is the reference's
+// (WilsonLoops/Source/StandardModel.cpp:78-111).  The inverse takes
+// |w| = atan2(|c|, c0), equal in exact arithmetic to the reference's
+// asin(|c|) with its pi - asin reflection for c0 < 0 (StandardModel.cpp:114-157)
+// but well conditioned at |w| = pi/2, where asin(|c|) turns a rounding of |c|
+// into an error of eps/|c0| in w (and it also normalises a quaternion that FP32
+// storage has left slightly off the unit sphere).
+template <class Src>
+static void gen_to_quat(const Src* w, double tol, double* q) {
+    q[0] = (double)w[0];
+    double wm = 0.0;
+    for (int k = 1; k < 4; k++) wm += (double)w[k] * (double)w[k];
+    wm = std::sqrt(wm);
+    q[1] = std::cos(wm);
+    q[2] = q[3] = q[4] = 0.0;
+    if (wm > tol) {
+        for (int k = 1; k < 4; k++) q[1 + k] = std::sin(wm) * (double)w[k] / wm;
+    } else {
+        q[2] = std::sin(wm);
+    }
+}
+template <class Dst>
+static void quat_to_gen(const double* q, double tol, Dst* w) {
+    w[0] = (Dst)q[0];
+    double cm = 0.0;
+    for (int k = 2; k < 5; k++) cm += q[k] * q[k];
+    cm = std::sqrt(cm);
+    const double ph = std::atan2(cm, q[1]);
+    if (cm > tol) {
+        for (int k = 0; k < 3; k++) w[1 + k] = (Dst)(ph * q[2 + k] / cm);
+    } else {
+        w[1] = (Dst)ph;
+        w[2] = w[3] = (Dst)0;
+    }
+}
+
 template <class Src>
 static int import_two_slot(gft_ctx* ctx, const Src* scalar, const Src* vec, unsigned time_iter) {
     const long long N = ctx->N;
-    const size_t ns = 2 * N * ctx->coef.ns, nv = 2 * N * 15;
+    const size_t ns = 2 * N * ctx->coef.ns, nv = 2 * N * link_reals(ctx->desc);
+    if (!is_two_slot(ctx->form) && ctx->desc.gen_rep) {
+        // Momentum forms: the links enter as FP64 quaternions, the scalars exactly.
+        std::vector<double> s(scalar, scalar + ns), v(2 * N * 15);
+        for (long long l = 0; l < 2 * N * 3; l++) gen_to_quat(vec + 4 * l, ctx->coef.tol, v.data() + 5 * l);
+        int rc = momentum_import(ctx->mom, ctx->coef, s.data(), v.data(), time_iter, ctx->stream, ctx->err);
+        if (rc) return rc;
+        CK(hipStreamSynchronize(ctx->stream));
+        ctx->time_iter = time_iter;
+        ctx->imported = true;
+        return GFT_OK;
+    }
     if (is_two_slot(ctx->form)) {
         if (is_f64(ctx->form) == (sizeof(Src) == 8)) {
             CK(hipMemcpyAsync(ctx->S, scalar, ns * sizeof(Src), hipMemcpyHostToDevice, ctx->stream));
@@ -258,7 +312,16 @@ static int import_two_slot(gft_ctx* ctx, const Src* scalar, const Src* vec, unsi
 template <class Dst>
 static int export_two_slot(gft_ctx* ctx, Dst* scalar, Dst* vec, unsigned* time_iter) {
     const long long N = ctx->N;
-    const size_t ns = 2 * N * ctx->coef.ns, nv = 2 * N * 15;
+    const size_t ns = 2 * N * ctx->coef.ns, nv = 2 * N * link_reals(ctx->desc);
+    if (!is_two_slot(ctx->form) && ctx->desc.gen_rep) {
+        std::vector<double> s(ns), v(2 * N * 15);
+        int rc = momentum_export(ctx->mom, ctx->coef, s.data(), v.data(), ctx->time_iter, ctx->stream, ctx->err);
+        if (rc) return rc;
+        for (size_t i = 0; i < ns; i++) scalar[i] = (Dst)s[i];
+        for (long long l = 0; l < 2 * N * 3; l++) quat_to_gen(v.data() + 5 * l, ctx->coef.tol, vec + 4 * l);
+        if (time_iter) *time_iter = ctx->time_iter;
+        return GFT_OK;
+    }
     if (is_two_slot(ctx->form)) {
         if (is_f64(ctx->form) == (sizeof(Dst) == 8)) {
             CK(hipMemcpyAsync(scalar, ctx->S, ns * sizeof(Dst), hipMemcpyDeviceToHost, ctx->stream));
@@ -372,6 +435,28 @@ static int collect_row(gft_ctx* ctx, bool e, bool g, gft_obs* row) {
 
 static long long obs_blocks(const gft_ctx* ctx) { return (ctx->N + kBlock - 1) / kBlock; }
 
+// Two-slot kernels by storage type and link width (5: quaternion, 4: generator form).
+template <class T, int VD>
+static void twoslot_step_as(gft_ctx* ctx, long long nblocks, unsigned now, bool me, double damping_step) {
+    twoslot_step<T, VD><<<nblocks, kBlock, 0, ctx->stream>>>(ctx->coef, (T*)ctx->S, (T*)ctx->V, now, me, damping_step,
+                                                              ctx->partial, ctx->err_dev);
+}
+static void twoslot_launch_step(gft_ctx* ctx, long long nblocks, unsigned now, bool me, double damping_step) {
+    const bool f64 = ctx->form == GFT_FORM_TWO_SLOT_F64, gen = ctx->desc.gen_rep != 0;
+    if (f64) (gen ? twoslot_step_as<double, 4> : twoslot_step_as<double, 5>)(ctx, nblocks, now, me, damping_step);
+    else (gen ? twoslot_step_as<float, 4> : twoslot_step_as<float, 5>)(ctx, nblocks, now, me, damping_step);
+}
+template <class T, int VD>
+static void twoslot_gauss_as(gft_ctx* ctx, long long nblocks, unsigned now) {
+    twoslot_gauss<T, VD><<<nblocks, kBlock, 0, ctx->stream>>>(ctx->coef, (const T*)ctx->S, (const T*)ctx->V, now,
+                                                               ctx->partial);
+}
+static void twoslot_launch_gauss(gft_ctx* ctx, long long nblocks, unsigned now) {
+    const bool f64 = ctx->form == GFT_FORM_TWO_SLOT_F64, gen = ctx->desc.gen_rep != 0;
+    if (f64) (gen ? twoslot_gauss_as<double, 4> : twoslot_gauss_as<double, 5>)(ctx, nblocks, now);
+    else (gen ? twoslot_gauss_as<float, 4> : twoslot_gauss_as<float, 5>)(ctx, nblocks, now);
+}
+
 extern "C" int gft_evolve(gft_ctx* ctx, unsigned nsteps, unsigned energy_freq, unsigned gauss_freq, gft_obs* rows,
                           unsigned max_rows, unsigned* nrows) {
     if (!ctx) return GFT_EINVAL;
@@ -399,14 +484,7 @@ extern "C" int gft_evolve(gft_ctx* ctx, unsigned nsteps, unsigned energy_freq, u
         }
         if ((rc = timed_begin(ctx))) return rc;
         if (two) {
-            if (ctx->form == GFT_FORM_TWO_SLOT_F64)
-                twoslot_step<double><<<nblocks, kBlock, 0, ctx->stream>>>(ctx->coef, (double*)ctx->S, (double*)ctx->V,
-                                                                          now, me, damping_step, ctx->partial,
-                                                                          ctx->err_dev);
-            else
-                twoslot_step<float><<<nblocks, kBlock, 0, ctx->stream>>>(ctx->coef, (float*)ctx->S, (float*)ctx->V,
-                                                                         now, me, damping_step, ctx->partial,
-                                                                         ctx->err_dev);
+            twoslot_launch_step(ctx, nblocks, now, me, damping_step);
             CK(hipGetLastError());
         } else {
             if ((rc = momentum_step(ctx->mom, ctx->coef, damping_step, ctx->stream, ctx->err_dev, ctx->err))) return rc;
@@ -414,12 +492,8 @@ extern "C" int gft_evolve(gft_ctx* ctx, unsigned nsteps, unsigned energy_freq, u
         if ((rc = timed_end(ctx))) return rc;
         if (me && two) finalize_partials<<<1, kFinThreads, 0, ctx->stream>>>(ctx->partial, nblocks, 5, 0, ctx->obs_dev);
         if (mg) {
-            if (ctx->form == GFT_FORM_TWO_SLOT_F64)
-                twoslot_gauss<double><<<nblocks, kBlock, 0, ctx->stream>>>(ctx->coef, (const double*)ctx->S,
-                                                                           (const double*)ctx->V, now, ctx->partial);
-            else if (ctx->form == GFT_FORM_TWO_SLOT_F32)
-                twoslot_gauss<float><<<nblocks, kBlock, 0, ctx->stream>>>(ctx->coef, (const float*)ctx->S,
-                                                                          (const float*)ctx->V, now, ctx->partial);
+            if (two)
+                twoslot_launch_gauss(ctx, nblocks, now);
             else if ((rc = momentum_observe(ctx->mom, ctx->coef, MomObs::Gauss, ctx->partial, nblocks, ctx->stream,
                                             ctx->err)))
                 return rc;
@@ -914,7 +988,7 @@ template <class T>
 __global__ void __launch_bounds__(256) grf_assemble_two_slot(T* __restrict__ S, T* __restrict__ V,
                                                              const double* __restrict__ comps, long long n, int nd,
                                                              double rms, double vev0, double vev1,
-                                                             const double* __restrict__ sq) {
+                                                             const double* __restrict__ sq, int vd) {
     const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const int ns = 4 * nd;
@@ -926,9 +1000,11 @@ __global__ void __launch_bounds__(256) grf_assemble_two_slot(T* __restrict__ S, 
             S[(n + s) * ns + 4 * k + m] = v;
         }
     }
+    // Identity links: c0 = 1 in the quaternion form, w = 0 in the generator form
+    // (Lattice.cpp:369-392).
     for (int slot = 0; slot < 2; slot++)
         for (int i = 0; i < 3; i++)
-            for (int c = 0; c < 5; c++) V[(slot * n + s) * 15 + 5 * i + c] = (T)(c == 1 ? 1.0 : 0.0);
+            for (int c = 0; c < vd; c++) V[(slot * n + s) * 3 * vd + vd * i + c] = (T)(vd == 5 && c == 1 ? 1.0 : 0.0);
 }
 
 }  // namespace gft
@@ -979,10 +1055,12 @@ extern "C" int gft_init_grf(gft_ctx* ctx, const gft_grf_desc* g) {
     if (is_two_slot(ctx->form)) {
         if (is_f64(ctx->form))
             grf_assemble_two_slot<double><<<nb, 256, 0, st>>>((double*)ctx->S, (double*)ctx->V, comps, N, c.nd, g->rms,
-                                                                vev0, vev1, ctx->obs_dev + 2);
+                                                                vev0, vev1, ctx->obs_dev + 2,
+                                                                (int)link_reals(ctx->desc) / 3);
         else
             grf_assemble_two_slot<float><<<nb, 256, 0, st>>>((float*)ctx->S, (float*)ctx->V, comps, N, c.nd, g->rms,
-                                                               vev0, vev1, ctx->obs_dev + 2);
+                                                               vev0, vev1, ctx->obs_dev + 2,
+                                                               (int)link_reals(ctx->desc) / 3);
     } else {
         ctx->mom.cur = 0;
         ctx->mom.prev_valid = 0;

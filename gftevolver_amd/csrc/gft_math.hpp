@@ -38,6 +38,7 @@ struct Coef {
     double pot_offset;
     double dvol;
     int fixed;                   // fixed faces: bit 2a low, 2a+1 high face of axis a (0: periodic)
+    double tol;                  // division-by-zero tolerance of the generator-form conversions
 };
 
 // A site in the one-site shell of a fixed face is frozen: not updated, left out

@@ -141,9 +141,12 @@ StandardModel::StandardModel(const std::string& root, unsigned ns, unsigned nv) 
     if (ns != 4)
         throw std::runtime_error("GRADIENTS::STANDARDMODEL:: 4 scalar components are required but " + std::to_string(ns)
                                  + " have been assigned.");
-    if (nv != 15)
-        throw std::runtime_error("GRADIENTS::STANDARDMODEL:: the GPU evolver uses the quaternion representation "
-                                 "(15 vector components) but " + std::to_string(nv) + " have been assigned.");
+    if (nv != 12 && nv != 15)   // StandardModel.cpp:163-169
+        throw std::runtime_error("GRADIENTS::STANDARDMODEL:: Either 12 or 15 vector components are required (depending "
+                                 "on whether the generator or quaternion representation is used) but "
+                                 + std::to_string(nv) + " have been assigned.\n"
+                                 "Note that 3 spatial dimensions are assumed so requested number of components is "
+                                 "multiplied by 3 internally.");
 }
 
 TwoHDM::TwoHDM(const std::string& root, unsigned ns, unsigned nv) {
@@ -153,9 +156,7 @@ TwoHDM::TwoHDM(const std::string& root, unsigned ns, unsigned nv) {
     if (ns != 8)
         throw std::runtime_error("GRADIENTS::TWOHDM:: 8 scalar components are required but " + std::to_string(ns)
                                  + " have been assigned.");
-    if (nv != 15)
-        throw std::runtime_error("GRADIENTS::TWOHDM:: the GPU evolver supports the SM gauge group in the quaternion "
-                                 "representation (15 vector components) but " + std::to_string(nv) + " have been assigned.");
+    (void)nv;   // checked against the representation by WilsonLoops::TwoHDM, which is built first
 }
 }  // namespace Gradients
 
@@ -171,9 +172,7 @@ StandardModel::StandardModel(const std::string& root, unsigned nv) {
         throw std::runtime_error("WILSONLOOPS::STANDARDMODEL:: Either 12 or 15 vector components are required "
                                  "(depending on whether the generator or quaternion representation is used) but "
                                  + std::to_string(nv) + " have been assigned.");
-    usingGeneratorRepresentation = (nv == 12);
-    if (usingGeneratorRepresentation)
-        throw std::runtime_error("WILSONLOOPS::STANDARDMODEL:: the GPU evolver uses the quaternion representation");
+    usingGeneratorRepresentation = (nv == 12);   // StandardModel.cpp:206-209
 }
 
 float StandardModel::getSqrCouplings(unsigned c) const {
@@ -190,9 +189,20 @@ TwoHDM::TwoHDM(const std::string& root, unsigned nv) {
     usingGeneratorRepresentation = r.value<bool>();
     r.skip_lines(1);
     divisionByZeroTolerance = r.value<double>();
-    if (usingGeneratorRepresentation || nv != 15)
-        throw std::runtime_error("WILSONLOOPS::TWOHDM:: the GPU evolver supports the SM gauge group (gaugeNum 4) in "
-                                 "the quaternion representation (15 vector components)");
+    // TwoHDM.cpp:215-242: the representation and the component count fix gaugeNum.
+    if (usingGeneratorRepresentation ? (nv != 12 && nv != 15 && nv != 21) : (nv != 15 && nv != 18 && nv != 27))
+        throw std::runtime_error(usingGeneratorRepresentation
+            ? "WILSONLOOPS::TWOHDM:: When using the generator representation, either 12 (just SM), 15 (SM x U(1)) or "
+              "21 (SM x SU(2)) vector components are required but " + std::to_string(nv) + " have been assigned.\n"
+              "Note that 3 spatial dimensions are assumed so requested number of components is multiplied by 3 "
+              "internally."
+            : "WILSONLOOPS::TWOHDM:: When using the quaternion representation, either 15 (just SM), 18 (SM x U(1)) or "
+              "27 (SM x SU(2)) vector components are required but " + std::to_string(nv) + " have been assigned.\n"
+              "Note that 3 spatial dimensions are assumed so requested number of components is multiplied by 3 "
+              "internally.");
+    if (nv != (usingGeneratorRepresentation ? 12u : 15u))
+        throw std::runtime_error("WILSONLOOPS::TWOHDM:: the GPU evolver supports the SM gauge group (gaugeNum 4); "
+                                 "the Higgs-family gauge fields (gaugeNum 5 and 7) are not built");
 }
 
 float TwoHDM::getSqrCouplings(unsigned c) const {
@@ -259,6 +269,7 @@ void Model::fill(gft_desc& d) const {
     d.g = wilsonLoop->g;
     d.gp = wilsonLoop->gp;
     d.div_tol = wilsonLoop->divisionByZeroTolerance;
+    d.gen_rep = wilsonLoop->isUsingGeneratorRepresentation() ? 1 : 0;
     potential->fill(d);
     d.damping = dampingFactor;
     d.nt_damped = ntDamped;

@@ -35,7 +35,8 @@ int main(int argc, char** argv) {
                       << "m12r " << d.m12r << "\nl4p5 " << d.l4p5 << "\nl4m5 " << d.l4m5 << "\n"
                       << "damping " << d.damping << "\nnt_damped " << d.nt_damped << "\n"
                       << "progress " << lattice.progressReport << "\n"
-                      << "fixed_faces " << lattice.fixedFaces << "\n";
+                      << "fixed_faces " << lattice.fixedFaces << "\n"
+                      << "gen_rep " << d.gen_rep << "\n";
             double s0 = 0;
             for (float x : lattice.scalarFields) s0 += x;
             std::cout << "scalar_sum " << s0 << "\n";

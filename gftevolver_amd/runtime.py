@@ -99,6 +99,7 @@ class Evolver:
     def __init__(self, desc: GftDesc, _handle=None):
         self.desc = desc
         self.ns = 8 if desc.model == 1 else 4
+        self.nv = 12 if desc.gen_rep else NV
         self.n = (desc.nx, desc.ny, desc.nz)
         self.nsites = desc.nx * desc.ny * desc.nz
         self._owned = _handle is None
@@ -135,7 +136,7 @@ class Evolver:
     def import_two_slot(self, scalar: np.ndarray, vec: np.ndarray, time_iter: int = 0):
         dt = scalar.dtype
         assert dt in (np.float32, np.float64) and vec.dtype == dt
-        assert scalar.size == 2 * self.nsites * self.ns and vec.size == 2 * self.nsites * NV
+        assert scalar.size == 2 * self.nsites * self.ns and vec.size == 2 * self.nsites * self.nv
         s = np.ascontiguousarray(scalar)
         v = np.ascontiguousarray(vec)
         T = C.c_float if dt == np.float32 else C.c_double
@@ -150,7 +151,7 @@ class Evolver:
 
     def export_two_slot(self, dtype=np.float64):
         s = np.zeros((2, self.nsites, self.ns), dtype)
-        v = np.zeros((2, self.nsites, NV), dtype)
+        v = np.zeros((2, self.nsites, self.nv), dtype)
         T = C.c_float if dtype == np.float32 else C.c_double
         fn = _L.gft_export_two_slot_f32 if dtype == np.float32 else _L.gft_export_two_slot_f64
         it = C.c_uint(0)

@@ -22,7 +22,8 @@
  *   - Two-slot arrays use the reference layout
  *       scalar[2][nx][ny][nz][Ns], vector[2][nx][ny][nz][Nv]
  *     (Core/Source/Lattice.cpp:358-367, 658-671) with Nv = 15 (three
- *     directions of [y, c0, c1, c2, c3]: U(1) phase then SU(2) quaternion).
+ *     directions of [y, c0, c1, c2, c3]: U(1) phase then SU(2) quaternion),
+ *     or Nv = 12 ([y, w1, w2, w3], generator form) when gft_desc.gen_rep is set.
  *     At step index k the current slot is (k+1)%2 and new values go to slot
  *     k%2 (Lattice.cpp:773-774).
  */
@@ -60,7 +61,7 @@ extern "C" {
 typedef struct gft_desc {
     int      model;            /* GFT_MODEL_*                                                  */
     int      form;             /* GFT_FORM_*                                                   */
-    int      nx, ny, nz;       /* periodic lattice size (Lattice.cfg line 11)                  */
+    int      nx, ny, nz;       /* lattice size (Lattice.cfg line 11)                           */
     double   dx, dy, dz, dt;   /* spacings and time step (Lattice.cfg lines 12, 14)            */
     double   g, gp;            /* isospin / hypercharge couplings (WilsonLoops StandardModel.cfg) */
     double   div_tol;          /* division-by-zero tolerance for 1/g^2 (same cfg), 1e-9        */
@@ -83,6 +84,14 @@ typedef struct gft_desc {
      * must be equal there), still read by its neighbours, and left out of the energy and Gauss
      * sums.  0 = fully periodic.                                                              */
     int      fixed_faces;
+    /* SU(2) link representation of the two-slot arrays (WilsonLoops/Source/StandardModel.cpp:
+     * 78-157, 195-210; Lattice.cfg vector components 5 or 4; TwoHDM.cfg generator flag):
+     * 0 = quaternion, vector[2][N][15] = 3 x [y, c0, c1, c2, c3];
+     * 1 = generator,  vector[2][N][12] = 3 x [y, w1, w2, w3], U = exp(i w.sigma) =
+     *     cos|w| + i sin|w| (w/|w|).sigma (w = 0 is the identity).
+     * The two-slot forms step the stored form as the reference does (convert to a
+     * quaternion, multiply, convert back); the momentum forms convert at import/export. */
+    int      gen_rep;
 } gft_desc;
 
 /* Observables of one measured step.  energy[] columns follow the reference's

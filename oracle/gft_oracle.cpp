@@ -14,6 +14,7 @@
 //   * Gradients/Source/TwoHDM.cpp:72-116, 270-489 the same for two doublets (gaugeNum 4)
 //   * WilsonLoops/Source/StandardModel.cpp:160-514 SU2Product, magnetic/electric
 //                                                 forces and energies, link update, Gauss
+//   * WilsonLoops/Source/StandardModel.cpp:78-157  quaternion and generator link forms
 //   * WilsonLoops/Source/TwoHDM.cpp:407-490      identical to the SM for gaugeNum 4
 //   * Potentials/Source/SO_N.cpp:57-87, Potentials/Source/Double_SO_N.cpp:157-368
 //   * Analysers/Source/Energy.cpp:117-220, GaugeCondition.cpp:117-226 (float accumulators)
@@ -43,7 +44,6 @@
 
 namespace {
 
-constexpr int NV = 15;  // 3 directions x [y, c0, c1, c2, c3]
 
 struct Neumaier {
     double s = 0.0, c = 0.0;
@@ -69,6 +69,9 @@ struct Params {
     float damping = 0.f;
     unsigned nt_damped = 0;
     int fixed = 0;        // fixed faces (bit 2a low, 2a+1 high face of axis a)
+    bool gen = false;     // generator representation of the SU(2) links
+    int vd = 5, nv = 15;  // reals per link direction and per site: [y, c0..c3] or [y, w1..w3]
+    double tol = 1e-9;    // division-by-zero tolerance of the unit-vector conversions
 };
 
 Params make_params(const gft_desc* d) {
@@ -98,6 +101,10 @@ Params make_params(const gft_desc* d) {
     p.damping = d->damping;
     p.nt_damped = d->nt_damped;
     p.fixed = d->fixed_faces;
+    p.gen = d->gen_rep != 0;
+    p.vd = p.gen ? 4 : 5;
+    p.nv = 3 * p.vd;
+    p.tol = d->div_tol;
     return p;
 }
 
@@ -126,12 +133,48 @@ inline long shift(const Params& p, long s, int a, int delta) {
     return (x * p.ny + y) * p.nz + z;
 }
 
-// getSU2Representation, quaternion branch (WilsonLoops/Source/StandardModel.cpp:104-109).
+// getSU2Representation (WilsonLoops/Source/StandardModel.cpp:78-111,
+// Gradients/Source/StandardModel.cpp:103-136).  Quaternion links store
+// [y, c0, c1, c2, c3]; generator links store [y, w1, w2, w3] with
+// U = cos|w| + i sin|w| (w/|w|).sigma, and below the division-by-zero tolerance
+// the reference picks c1 = sin|w| without the conjugation sign.
 template <class T>
-inline void rep(const T* link, bool conj, double* U) {
+inline void rep(const Params& p, const T* link, bool conj, double* U) {
     const int cf = conj ? -1 : 1;
+    if (p.gen) {
+        double wm = 0.0;
+        for (int k = 1; k < 4; k++) wm += static_cast<double>(link[k]) * static_cast<double>(link[k]);
+        wm = std::sqrt(wm);
+        U[0] = std::cos(wm);
+        U[1] = U[2] = U[3] = 0.0;
+        if (wm > p.tol) {
+            for (int k = 1; k < 4; k++) U[k] = cf * std::sin(wm) * static_cast<double>(link[k]) / wm;
+        } else {
+            U[1] = std::sin(wm);
+        }
+        return;
+    }
     U[0] = static_cast<double>(link[1]);
     for (int k = 1; k < 4; k++) U[k] = static_cast<double>(cf * link[1 + k]);
+}
+
+// invertSU2Representation (WilsonLoops/Source/StandardModel.cpp:114-157): the
+// generator components of a unit quaternion, |w| = asin|c| (pi/2 when |c|
+// rounds above 1), reflected to pi - |w| when c0 < 0.  The reference returns
+// them as float; storage T = double keeps them in double.
+template <class T>
+inline void invert_rep(const Params& p, const double* c, T* w) {
+    double cm = 0.0;
+    for (int k = 1; k < 4; k++) cm += c[k] * c[k];
+    cm = std::sqrt(cm);
+    double ph = cm > 1 ? M_PI_2 : std::asin(cm);
+    if (c[0] < 0) ph = M_PI - ph;
+    if (cm > p.tol) {
+        for (int k = 0; k < 3; k++) w[k] = static_cast<T>(ph * c[k + 1] / cm);
+    } else {
+        w[0] = static_cast<T>(ph);
+        w[1] = w[2] = 0;
+    }
 }
 
 // SU2Product (WilsonLoops/Source/StandardModel.cpp:160-191).
@@ -155,7 +198,7 @@ inline void qmul(const double* a, const double* b, bool trace, double* out) {
 // 4x4 action per doublet (Gradients/Source/StandardModel.cpp:72-100;
 // TwoHDM.cpp:72-104 for two doublets).
 template <class T>
-inline void transform(const double* sf, int nd, const T* link, bool conj, double* out) {
+inline void transform(const Params& p, const double* sf, int nd, const T* link, bool conj, double* out) {
     const int cf = conj ? -1 : 1;
     const double y = static_cast<double>(link[0]);
     const double cy = std::cos(y);
@@ -166,7 +209,7 @@ inline void transform(const double* sf, int nd, const T* link, bool conj, double
         h[c] = cy * sf[c] + pm * sy * sf[c - pm];
     }
     double U[4];
-    rep(link, conj, U);
+    rep(p, link, conj, U);
     for (int k = 0; k < nd; k++) {
         const double* hk = h + 4 * k;
         double* o = out + 4 * k;
@@ -318,11 +361,11 @@ void site_update(const Params& p, T* S, T* V, long s, unsigned now, bool store_e
     const long N = p.N;
     const unsigned past = !now;
     const T* Sn = S + static_cast<long>(now) * N * Ns;
-    const T* Vn = V + static_cast<long>(now) * N * NV;
+    const T* Vn = V + static_cast<long>(now) * N * p.nv;
     T* sp_ = S + static_cast<long>(past) * N * Ns + s * Ns;
-    T* vp_ = V + static_cast<long>(past) * N * NV + s * NV;
+    T* vp_ = V + static_cast<long>(past) * N * p.nv + s * p.nv;
     const T* sn = Sn + s * Ns;
-    const T* vn = Vn + s * NV;
+    const T* vn = Vn + s * p.nv;
 
     long xm[3], xp[3], xpm[3][3];
     for (int a = 0; a < 3; a++) { xm[a] = shift(p, s, a, -1); xp[a] = shift(p, s, a, +1); }
@@ -345,14 +388,14 @@ void site_update(const Params& p, T* S, T* V, long s, unsigned now, bool store_e
                 double f[8];
                 const T* q = Sn + xm[a] * Ns;
                 for (int c = 0; c < Ns; c++) f[c] = static_cast<double>(q[c]);
-                transform(f, nd, Vn + xm[a] * NV + a * 5, false, tf);
+                transform(p, f, nd, Vn + xm[a] * p.nv + a * p.vd, false, tf);
             } else if (st == 1) {
                 for (int c = 0; c < Ns; c++) tf[c] = phi[c];
             } else {
                 double f[8];
                 const T* q = Sn + xp[a] * Ns;
                 for (int c = 0; c < Ns; c++) f[c] = static_cast<double>(q[c]);
-                transform(f, nd, vn + a * 5, true, tf);
+                transform(p, f, nd, vn + a * p.vd, true, tf);
             }
             for (int c = 0; c < Ns; c++) der[c] += p.lap[a][st] * tf[c];
         }
@@ -362,7 +405,7 @@ void site_update(const Params& p, T* S, T* V, long s, unsigned now, bool store_e
     double cur[12];
     for (int a = 0; a < 3; a++) {
         double tf[8], nb[8];
-        transform(phi, nd, vn + a * 5, false, tf);
+        transform(p, phi, nd, vn + a * p.vd, false, tf);
         const T* q = Sn + xp[a] * Ns;
         for (int c = 0; c < Ns; c++) nb[c] = static_cast<double>(q[c]);
         if (nd == 1) {
@@ -387,31 +430,31 @@ void site_update(const Params& p, T* S, T* V, long s, unsigned now, bool store_e
         for (int j = 0; j < 3; j++) {
             if (j == i) continue;
             const T* L_x = vn;
-            const T* L_xpi = Vn + xp[i] * NV;
-            const T* L_xpj = Vn + xp[j] * NV;
-            const T* L_xmj = Vn + xm[j] * NV;
-            const T* L_xpimj = Vn + xpm[i][j] * NV;
-            const double th_loc = -static_cast<double>(L_xpi[j * 5]) + static_cast<double>(L_xpj[i * 5])
-                                + static_cast<double>(L_x[j * 5]) - static_cast<double>(L_x[i * 5]);
-            const double th_nb = -static_cast<double>(L_xpimj[j * 5]) - static_cast<double>(L_xmj[i * 5])
-                               + static_cast<double>(L_xmj[j * 5]) + static_cast<double>(L_x[i * 5]);
+            const T* L_xpi = Vn + xp[i] * p.nv;
+            const T* L_xpj = Vn + xp[j] * p.nv;
+            const T* L_xmj = Vn + xm[j] * p.nv;
+            const T* L_xpimj = Vn + xpm[i][j] * p.nv;
+            const double th_loc = -static_cast<double>(L_xpi[j * p.vd]) + static_cast<double>(L_xpj[i * p.vd])
+                                + static_cast<double>(L_x[j * p.vd]) - static_cast<double>(L_x[i * p.vd]);
+            const double th_nb = -static_cast<double>(L_xpimj[j * p.vd]) - static_cast<double>(L_xmj[i * p.vd])
+                               + static_cast<double>(L_xmj[j * p.vd]) + static_cast<double>(L_x[i * p.vd]);
             mag[4 * i] += (std::sin(th_loc) - std::sin(th_nb)) * p.inv_sp[j];
 
             double P1[4], P2[4], Um[4];
-            rep(L_x + i * 5, true, P1);
-            rep(L_x + j * 5, false, Um);
+            rep(p, L_x + i * p.vd, true, P1);
+            rep(p, L_x + j * p.vd, false, Um);
             qmul(Um, P1, true, P1);
-            rep(L_xpj + i * 5, false, Um);
+            rep(p, L_xpj + i * p.vd, false, Um);
             qmul(Um, P1, true, P1);
-            rep(L_xpi + j * 5, true, Um);
+            rep(p, L_xpi + j * p.vd, true, Um);
             qmul(Um, P1, store_energy, P1);
 
-            rep(L_xpimj + j * 5, true, P2);
-            rep(L_xmj + i * 5, true, Um);
+            rep(p, L_xpimj + j * p.vd, true, P2);
+            rep(p, L_xmj + i * p.vd, true, Um);
             qmul(Um, P2, true, P2);
-            rep(L_xmj + j * 5, false, Um);
+            rep(p, L_xmj + j * p.vd, false, Um);
             qmul(Um, P2, true, P2);
-            rep(L_x + i * 5, false, Um);
+            rep(p, L_x + i * p.vd, false, Um);
             qmul(Um, P2, false, P2);
 
             for (int k = 1; k < 4; k++) mag[4 * i + k] += (P1[k] - P2[k]) * p.inv_sp[j];
@@ -429,13 +472,13 @@ void site_update(const Params& p, T* S, T* V, long s, unsigned now, bool store_e
     double el[12];
     float stored_el = 0.f;
     double el_e = 0.0;
-    const T* vpast = V + static_cast<long>(past) * N * NV + s * NV;
+    const T* vpast = V + static_cast<long>(past) * N * p.nv + s * p.nv;
     for (int a = 0; a < 3; a++) {
-        const double la = static_cast<double>(vn[a * 5]) - static_cast<double>(vpast[a * 5]);
+        const double la = static_cast<double>(vn[a * p.vd]) - static_cast<double>(vpast[a * p.vd]);
         el[4 * a] = std::sin(la);
         double Up[4], Um[4];
-        rep(vpast + a * 5, true, Up);
-        rep(vn + a * 5, false, Um);
+        rep(p, vpast + a * p.vd, true, Up);
+        rep(p, vn + a * p.vd, false, Um);
         qmul(Um, Up, store_energy, Up);
         for (int k = 1; k < 4; k++) el[4 * a + k] = Up[k];
         if (store_energy) {
@@ -462,7 +505,7 @@ void site_update(const Params& p, T* S, T* V, long s, unsigned now, bool store_e
                     for (int c = 0; c < Ns; c++) tf[c] = static_cast<double>(q[c]);
                     if (st == 1) {
                         double tmp[8];
-                        transform(tf, nd, vn + a * 5, false, tmp);
+                        transform(p, tf, nd, vn + a * p.vd, false, tmp);
                         for (int c = 0; c < Ns; c++) tf[c] = tmp[c];
                     }
                     for (int c = 0; c < Ns; c++)
@@ -490,7 +533,7 @@ void site_update(const Params& p, T* S, T* V, long s, unsigned now, bool store_e
             double ge = 0.0;
             for (int a = 0; a < 3; a++) {
                 double tf[8], nb[8];
-                transform(phi, nd, vn + a * 5, false, tf);
+                transform(p, phi, nd, vn + a * p.vd, false, tf);
                 const T* q = Sn + xp[a] * Ns;
                 for (int c = 0; c < Ns; c++) nb[c] = static_cast<double>(q[c]);
                 for (int c = 0; c < Ns; c++) {
@@ -524,16 +567,17 @@ void site_update(const Params& p, T* S, T* V, long s, unsigned now, bool store_e
 
     // WilsonLoops::StandardModel::evolve.
     for (int a = 0; a < 3; a++) {
-        vp_[a * 5] = static_cast<T>(std::asin(R[4 * a]) + static_cast<double>(vn[a * 5]));
+        vp_[a * p.vd] = static_cast<T>(std::asin(R[4 * a]) + static_cast<double>(vn[a * p.vd]));
         if (R[4 * a] * R[4 * a] > 1) __atomic_add_fetch(panics, 1UL, __ATOMIC_RELAXED);
         double cms = 0.0;
         for (int k = 1; k < 4; k++) cms += R[4 * a + k] * R[4 * a + k];
         if (cms > 1) __atomic_add_fetch(panics, 1UL, __ATOMIC_RELAXED);
         const double u[4] = {std::sqrt(1.0 - cms), R[4 * a + 1], R[4 * a + 2], R[4 * a + 3]};
         double Up[4];
-        rep(vn + a * 5, false, Up);
+        rep(p, vn + a * p.vd, false, Up);
         qmul(u, Up, true, Up);
-        for (int k = 0; k < 4; k++) vp_[a * 5 + 1 + k] = static_cast<T>(Up[k]);
+        if (p.gen) invert_rep(p, Up, vp_ + a * p.vd + 1);
+        else for (int k = 0; k < 4; k++) vp_[a * p.vd + 1 + k] = static_cast<T>(Up[k]);
     }
 }
 
@@ -547,8 +591,8 @@ void site_gauss(const Params& p, const T* S, const T* V, long s, unsigned now, A
     const unsigned fut = !now;
     const T* sn = S + static_cast<long>(now) * N * Ns + s * Ns;
     const T* sf = S + static_cast<long>(fut) * N * Ns + s * Ns;
-    const T* Vn = V + static_cast<long>(now) * N * NV;
-    const T* Vf = V + static_cast<long>(fut) * N * NV;
+    const T* Vn = V + static_cast<long>(now) * N * p.nv;
+    const T* Vf = V + static_cast<long>(fut) * N * p.nv;
     A grad[4] = {0, 0, 0, 0};
     if constexpr (sizeof(A) == sizeof(float)) {
         // Gradients calcConstraintContributions: float products (StandardModel.cpp:250-270).
@@ -584,21 +628,21 @@ void site_gauss(const Params& p, const T* S, const T* V, long s, unsigned now, A
     double divabs[4] = {0, 0, 0, 0};
     for (int a = 0; a < 3; a++) {
         const long xm = shift(p, s, a, -1);
-        const T* Ln = Vn + s * NV + a * 5;
-        const T* Lf = Vf + s * NV + a * 5;
-        const T* Mn = Vn + xm * NV + a * 5;
-        const T* Mf = Vf + xm * NV + a * 5;
+        const T* Ln = Vn + s * p.nv + a * p.vd;
+        const T* Lf = Vf + s * p.nv + a * p.vd;
+        const T* Mn = Vn + xm * p.nv + a * p.vd;
+        const T* Mf = Vf + xm * p.nv + a * p.vd;
         const double la = static_cast<double>(Lf[0]) - static_cast<double>(Ln[0]);
         const double na = static_cast<double>(Mf[0]) - static_cast<double>(Mn[0]);
         const double d0 = (std::sin(la) - std::sin(na)) * p.inv_sp[a];
         elc[0] = static_cast<A>(elc[0] + d0);
         divabs[0] += (std::fabs(std::sin(la)) + std::fabs(std::sin(na))) * p.inv_sp[a];
         double P1[4], P2[4], Um[4];
-        rep(Lf, false, P1);
-        rep(Ln, true, Um);
+        rep(p, Lf, false, P1);
+        rep(p, Ln, true, Um);
         qmul(Um, P1, false, P1);
-        rep(Mn, true, P2);
-        rep(Mf, false, Um);
+        rep(p, Mn, true, P2);
+        rep(p, Mf, false, Um);
         qmul(Um, P2, false, P2);
         for (int k = 1; k < 4; k++) {
             const double dk = (P1[k] - P2[k]) * p.inv_sp[a];
@@ -782,7 +826,7 @@ int oracle_evolve_f64(const gft_desc* d, double* scalar, double* vec, unsigned t
 int oracle_energy_density_f32(const gft_desc* d, const float* scalar, const float* vec, unsigned time_iter,
                               float* dens) {
     const Params p = make_params(d);
-    std::vector<float> S(scalar, scalar + 2 * p.N * p.Ns), V(vec, vec + 2 * p.N * NV);
+    std::vector<float> S(scalar, scalar + 2 * p.N * p.Ns), V(vec, vec + 2 * p.N * p.nv);
     const unsigned now = (time_iter + 1) % 2;
     unsigned long panics = 0;
     std::vector<Dens<float>> dd(p.N);

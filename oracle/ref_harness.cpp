@@ -10,9 +10,10 @@
 // all at the current slot.  Every field update, force, energy density and
 // Gauss violation is computed by the reference's own compiled code.
 //
-// usage: gft_ref_harness MODEL_CFG NS NX NY NZ DX DY DZ DT NSTEPS EFREQ GFREQ IN OUT
-//   IN : float scalar[2][N][NS], float vector[2][N][15]
-//   OUT: per step k: float scalar[2][N][NS], float vector[2][N][15];
+// usage: gft_ref_harness MODEL_CFG NS NX NY NZ DX DY DZ DT NSTEPS EFREQ GFREQ IN OUT [NV]
+//   NV : link reals per site, 15 (quaternion form, default) or 12 (generator form)
+//   IN : float scalar[2][N][NS], float vector[2][N][NV]
+//   OUT: per step k: float scalar[2][N][NS], float vector[2][N][NV];
 //        then float dens[N][5] if k % EFREQ == 0 (potential, gradient, kinetic,
 //        magnetic, electric, before the update);
 //        then float viol[N][4] if k % GFREQ == 0 (after the update).
@@ -24,8 +25,8 @@
 #include <vector>
 
 int main(int argc, char** argv) {
-    if (argc != 15) {
-        std::fprintf(stderr, "usage: %s MODEL_CFG NS NX NY NZ DX DY DZ DT NSTEPS EFREQ GFREQ IN OUT\n", argv[0]);
+    if (argc != 15 && argc != 16) {
+        std::fprintf(stderr, "usage: %s MODEL_CFG NS NX NY NZ DX DY DZ DT NSTEPS EFREQ GFREQ IN OUT [NV]\n", argv[0]);
         return 2;
     }
     const std::string model_cfg = argv[1];
@@ -40,7 +41,7 @@ int main(int argc, char** argv) {
     const unsigned nsteps = std::strtoul(argv[10], nullptr, 10);
     const unsigned efreq = std::strtoul(argv[11], nullptr, 10);
     const unsigned gfreq = std::strtoul(argv[12], nullptr, 10);
-    const unsigned Nv = 15;
+    const unsigned Nv = argc == 16 ? std::strtoul(argv[15], nullptr, 10) : 15;
     const long long N = 1LL * nx * ny * nz;
 
     Model model;

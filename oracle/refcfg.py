@@ -67,7 +67,7 @@ def set_values(root: str, lib: str, name: str, values: list) -> str:
 
 
 def write_model(root: str, model: int, g: float, gp: float, *, lam=0.13, eta_sqr=1.0, pot2=None,
-                damping=0.0, nt_damped=0, tol=1e-9) -> str:
+                damping=0.0, nt_damped=0, tol=1e-9, gen=False) -> str:
     """Model.cfg plus the potential / gradient / Wilson-loop configs; returns the
     Model.cfg path (relative to root, as the harness opens it from there).
 
@@ -75,6 +75,8 @@ def write_model(root: str, model: int, g: float, gp: float, *, lam=0.13, eta_sqr
     (Double_SO_N, TwoHDM gradient, TwoHDM loops, quaternion links).
     pot2: dict of Double_SO_N parameters (m1 m2 l1 l2 l3 m12r l4p5 l6r l7r m12i
     l4m5 l5i l6i l7i break_to_son break_to_uhalfn pot_norm pot_offset).
+    gen: the 2HDM's generator-representation flag (TwoHDM.cfg); the SM takes its
+    representation from the vector component count (write_lattice's nvd).
     """
     if model == 0:
         ids = (1, 2, 1)
@@ -90,19 +92,20 @@ def write_model(root: str, model: int, g: float, gp: float, *, lam=0.13, eta_sqr
             int(q.get("break_to_son", 1)), q["m12r"], q["l4p5"], (q["l6r"], q["l7r"]),
             int(q.get("break_to_uhalfn", 1)), q["m12i"], (q["l4m5"], q["l5i"]), (q["l6i"], q["l7i"]),
         ])
-        set_values(root, "WilsonLoops", "TwoHDM.cfg", [g, gp, 0.5, 0, tol])
+        set_values(root, "WilsonLoops", "TwoHDM.cfg", [g, gp, 0.5, int(gen), tol])
         set_values(root, "Gradients", "TwoHDM.cfg", [tol])
     set_values(root, "Core", "Model.cfg", [ids[0], ids[1], ids[2], float(damping), nt_damped, 0, 0.0])
     return os.path.join("cfg", "Core", "Config", "Model.cfg")
 
 
 def write_lattice(root: str, n, d, nt, dt, ns, *, ic_seed=42, ic_range=(-1.0, 1.0), efreq=1, gfreq=1,
-                  energy=True, gauss=True, progress=False, faces=(3, 3, 3, 3, 3, 3)):
+                  energy=True, gauss=True, progress=False, faces=(3, 3, 3, 3, 3, 3), nvd=5):
     """Lattice.cfg, RandomUniform.cfg and analyser configs for a full program run
     with a RandomUniform scalar initial state and identity links.  faces: the
-    six face boundary types (-x, +x, -y, +y, -z, +z): 1 fixed, 3 periodic."""
+    six face boundary types (-x, +x, -y, +y, -z, +z): 1 fixed, 3 periodic.
+    nvd: vector components per direction, 5 (quaternion) or 4 (generator form)."""
     set_values(root, "Core", "Lattice.cfg", [
-        tuple(n), tuple(d), nt, dt, ns, 5, 0, 3.0, (1, 0), 0, int(energy), int(gauss), tuple(faces),
+        tuple(n), tuple(d), nt, dt, ns, nvd, 0, 3.0, (1, 0), 0, int(energy), int(gauss), tuple(faces),
         int(progress), max(nt, 1),
     ])
     set_values(root, "InitialConditions", "RandomUniform.cfg", [ic_seed, tuple(ic_range), 1])

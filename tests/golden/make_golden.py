@@ -16,8 +16,15 @@ Two kinds of fixture, both produced by the reference's own compiled code:
   program_sm_fixed.npz the same SM program with fixed faces: x and z fixed
                        (faces 1 1 3 3 1 1) and y fixed (3 3 1 1 3 3), keys
                        energy_<case>, gauss_<case>.
+  harness_<model>_gen.npz, program_<model>_gen.npz
+                       the same two kinds with the SU(2) links in the generator
+                       form (4 vector components per direction, 12 per site;
+                       the 2HDM also sets TwoHDM.cfg's generator flag).  The
+                       harness state mixes small link angles with a few |w| in
+                       (1.6, 3), where c0 < 0 and the inverse conversion takes
+                       its pi - asin branch, and a few exactly zero links.
 
-    python tests/golden/make_golden.py [--fixed]   # --fixed: only the last file
+    python tests/golden/make_golden.py [--fixed | --gen]   # only those files
 """
 from __future__ import annotations
 
@@ -64,6 +71,56 @@ def harness_initial_state(model: int, seed: int = 2):
     vv[..., 0] = y
     vv[..., 1:] = q
     return s, vv.reshape(2, N, 15).astype(np.float32)
+
+
+def harness_initial_state_gen(model: int, seed: int = 4):
+    """As harness_initial_state, links in the generator form [y, w1, w2, w3]."""
+    s, _ = harness_initial_state(model, seed)
+    N = int(np.prod(HARNESS_N))
+    rng = np.random.default_rng(seed + 100)
+    w = rng.normal(size=(2, N, 3, 3)) * 0.05
+    big = rng.choice(N * 3, 6, replace=False)
+    for b in big:
+        u = rng.normal(size=3)
+        w[:, b // 3, b % 3] = u / np.linalg.norm(u) * rng.uniform(1.6, 3.0)
+    w[:, 0, 0] = 0.0
+    w[:, 7, 2] = 0.0
+    w[1] = w[0] + rng.normal(size=(N, 3, 3)) * 0.01 * (w[0] != 0)
+    vv = np.zeros((2, N, 3, 4))
+    vv[..., 0] = rng.normal(size=(2, N, 3)) * 0.05
+    vv[..., 1:] = w
+    return s, vv.reshape(2, N, 12).astype(np.float32)
+
+
+def generator_form():
+    for model, tag in ((0, "sm"), (1, "2hdm")):
+        pot2 = TWO_HDM_POTENTIAL if model == 1 else None
+        Ns = 4 if model == 0 else 8
+        s, v = harness_initial_state_gen(model)
+        with tempfile.TemporaryDirectory() as td:
+            h = O.run_harness(model, HARNESS_N, (0.5, 0.5, 0.5), 0.05, HARNESS_STEPS, HARNESS_MEAS, HARNESS_MEAS,
+                              s, v, td, pot2=pot2, gen=True)
+        out = {"scalar0": s, "vec0": v}
+        for k in HARNESS_KEEP:
+            out[f"scalar_{k}"] = h[k]["scalar"]
+            out[f"vec_{k}"] = h[k]["vec"]
+        for k in range(0, HARNESS_STEPS, HARNESS_MEAS):
+            out[f"dens_{k}"] = h[k]["dens"]
+            out[f"viol_{k}"] = h[k]["viol"]
+        np.savez_compressed(os.path.join(HERE, f"harness_{tag}_gen.npz"), **out)
+
+        with tempfile.TemporaryDirectory() as td:
+            refcfg.prepare(td)
+            refcfg.write_model(td, model, 0.65, 0.35, pot2=pot2, gen=True)
+            refcfg.write_lattice(td, PROGRAM_N, (0.5, 0.5, 0.5), PROGRAM_STEPS, 0.05, Ns, ic_seed=PROGRAM_SEED,
+                                 ic_range=PROGRAM_RANGE, efreq=PROGRAM_EFREQ, gfreq=PROGRAM_GFREQ, nvd=4)
+            subprocess.run([os.path.join(O.REF_DIR, "gft_reference")], check=True, stdout=subprocess.DEVNULL,
+                           stderr=subprocess.DEVNULL, cwd=td)
+            data = os.path.join(td, "data")
+            E = np.loadtxt(os.path.join(data, "global_quantities.dat"), ndmin=2)
+            G = np.loadtxt(os.path.join(data, "global_quantities_gauge.dat"), ndmin=2)
+        np.savez_compressed(os.path.join(HERE, f"program_{tag}_gen.npz"), energy=E, gauss=G)
+        print(tag, "generator form written")
 
 
 def main():
@@ -120,6 +177,9 @@ def fixed_faces_program():
 if __name__ == "__main__":
     if "--fixed" in sys.argv[1:]:
         fixed_faces_program()
+    elif "--gen" in sys.argv[1:]:
+        generator_form()
     else:
         main()
         fixed_faces_program()
+        generator_form()

@@ -87,14 +87,16 @@ def ref_available() -> bool:
 
 
 def run_harness(model: int, n, d, dt, nsteps, efreq, gfreq, scalar, vec, workdir, *, g=0.65, gp=0.35, lam=0.13,
-                eta_sqr=1.0, pot2=None, damping=0.0, nt_damped=0):
-    """Run the reference harness; returns a list of per-step dicts."""
+                eta_sqr=1.0, pot2=None, damping=0.0, nt_damped=0, gen=False):
+    """Run the reference harness; returns a list of per-step dicts.  gen: links
+    in the generator form (12 reals per site)."""
     import sys
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import refcfg
     refcfg.prepare(workdir)
     model_cfg = refcfg.write_model(workdir, model, g, gp, lam=lam, eta_sqr=eta_sqr, pot2=pot2, damping=damping,
-                                   nt_damped=nt_damped)
+                                   nt_damped=nt_damped, gen=gen)
+    nv = 12 if gen else NV
     Ns = 4 if model == 0 else 8
     N = n[0] * n[1] * n[2]
     fin = os.path.join(workdir, "in.bin")
@@ -103,14 +105,14 @@ def run_harness(model: int, n, d, dt, nsteps, efreq, gfreq, scalar, vec, workdir
         f.write(np.ascontiguousarray(scalar, np.float32).tobytes())
         f.write(np.ascontiguousarray(vec, np.float32).tobytes())
     subprocess.run([os.path.join(REF_DIR, "gft_ref_harness"), model_cfg, str(Ns), *map(str, n), *map(repr, d),
-                    repr(dt), str(nsteps), str(efreq), str(gfreq), fin, fout], check=True,
+                    repr(dt), str(nsteps), str(efreq), str(gfreq), fin, fout, str(nv)], check=True,
                    stdout=subprocess.DEVNULL, cwd=workdir)
     raw = np.fromfile(fout, np.float32)
     out, pos = [], 0
     for k in range(nsteps):
         rec = {}
         rec["scalar"] = raw[pos:pos + 2 * N * Ns].reshape(2, N, Ns); pos += 2 * N * Ns
-        rec["vec"] = raw[pos:pos + 2 * N * NV].reshape(2, N, NV); pos += 2 * N * NV
+        rec["vec"] = raw[pos:pos + 2 * N * nv].reshape(2, N, nv); pos += 2 * N * nv
         if efreq and k % efreq == 0:
             rec["dens"] = raw[pos:pos + N * 5].reshape(N, 5); pos += N * 5
         if gfreq and k % gfreq == 0:

@@ -28,12 +28,12 @@ needs_cfg = pytest.mark.skipif(not (os.path.isdir(TEMPLATE) and os.path.exists(G
                                reason="reference config template or gft_run not built")
 
 
-def _prepare(td, model, n, nt, ns, efreq=3, gfreq=2, faces=(3, 3, 3, 3, 3, 3)):
+def _prepare(td, model, n, nt, ns, efreq=3, gfreq=2, faces=(3, 3, 3, 3, 3, 3), gen=False, nvd=None):
     import refcfg
     refcfg.prepare(td)
-    refcfg.write_model(td, model, 0.65, 0.35, pot2=TWO_HDM_POTENTIAL if model == 1 else None)
+    refcfg.write_model(td, model, 0.65, 0.35, pot2=TWO_HDM_POTENTIAL if model == 1 else None, gen=gen)
     refcfg.write_lattice(td, n, (0.5, 0.5, 0.5), nt, 0.05, ns, ic_seed=7, ic_range=(-0.8, 0.8), efreq=efreq,
-                         gfreq=gfreq, faces=faces)
+                         gfreq=gfreq, faces=faces, nvd=nvd or (4 if gen else 5))
     return os.path.join(td, "cfg")
 
 
@@ -72,6 +72,30 @@ def test_config_reader_fixed_faces(tmp_path):
     root = _prepare(str(tmp_path / "b"), 0, (6, 5, 7), 12, 4, faces=(1, 3, 3, 3, 3, 3))
     r = subprocess.run([GFT_RUN, root, str(tmp_path / "b" / "data"), "dump_config"], capture_output=True, text=True)
     assert r.returncode != 0 and "must be assigned to both faces of an axis" in r.stderr
+
+
+@needs_cfg
+@pytest.mark.parametrize("model", [0, 1], ids=["sm", "2hdm"])
+def test_config_reader_generator_form(tmp_path, model):
+    """4 vector components per direction select the generator form of the SU(2)
+    links (SM: by the count, StandardModel.cpp:206-209; 2HDM: TwoHDM.cfg's flag
+    with the count fixing gaugeNum, TwoHDM.cpp:215-242)."""
+    ns = 4 if model == 0 else 8
+    root = _prepare(str(tmp_path / "a"), model, (6, 5, 7), 12, ns, gen=True)
+    out = subprocess.run([GFT_RUN, root, str(tmp_path / "a" / "data"), "dump_config"], capture_output=True, text=True,
+                         check=True).stdout
+    kv = {l.split()[0]: l.split()[1:] for l in out.strip().splitlines()}
+    assert kv["nv"] == ["12"] and kv["gen_rep"] == ["1"]
+    # Quaternion components with the 2HDM generator flag set: gaugeNum 5, not built here.
+    if model == 1:
+        root = _prepare(str(tmp_path / "b"), model, (6, 5, 7), 12, ns, gen=True, nvd=5)
+        r = subprocess.run([GFT_RUN, root, str(tmp_path / "b" / "data"), "dump_config"], capture_output=True,
+                           text=True)
+        assert r.returncode != 0 and "gaugeNum 5 and 7" in r.stderr
+    # A count neither form allows: the reference's message.
+    root = _prepare(str(tmp_path / "c"), model, (6, 5, 7), 12, ns, gen=model == 1, nvd=3)
+    r = subprocess.run([GFT_RUN, root, str(tmp_path / "c" / "data"), "dump_config"], capture_output=True, text=True)
+    assert r.returncode != 0 and "9 have been assigned" in r.stderr
 
 
 @needs_cfg
@@ -115,3 +139,20 @@ def test_gft_run_reproduces_reference_program_with_fixed_faces(tmp_path, case, f
     # With frozen shells the violation is physical (no current through frozen links), so it is
     # compared, not bounded: the GPU sums in FP64 what the reference sums in float.
     np.testing.assert_allclose(G, gg, rtol=1e-3, atol=1e-7)
+
+
+@needs_cfg
+@pytest.mark.gpu
+@pytest.mark.parametrize("model,tag", [(0, "sm"), (1, "2hdm")], ids=["sm", "2hdm"])
+def test_gft_run_reproduces_reference_program_generator_form(tmp_path, model, tag):
+    """The reference program with generator-form links (12 vector components)."""
+    g = np.load(os.path.join(ROOT, "tests", "golden", f"program_{tag}_gen.npz"))
+    ns = 4 if model == 0 else 8
+    root = _prepare(str(tmp_path), model, (6, 5, 7), 12, ns, gen=True)
+    data = tmp_path / "data"
+    subprocess.run([GFT_RUN, root, str(data)], check=True, capture_output=True, timeout=120)
+    E = np.loadtxt(data / "global_quantities.dat", ndmin=2)
+    G = np.loadtxt(data / "global_quantities_gauge.dat", ndmin=2)
+    assert E.shape == g["energy"].shape and G.shape == g["gauss"].shape
+    np.testing.assert_allclose(E, g["energy"], rtol=2e-5, atol=1e-5 * np.abs(g["energy"]).max())
+    assert (G <= 10 * g["gauss"] + 1e-7).all()

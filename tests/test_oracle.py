@@ -63,6 +63,100 @@ def test_oracle_reproduces_reference_program_output(model, tag, pot2):
     np.testing.assert_allclose(e64, e, rtol=2e-5, atol=1e-3 * np.abs(e).max())
 
 
+@pytest.mark.parametrize("model,tag,pot2", MODELS, ids=["sm", "2hdm"])
+def test_oracle_generator_form_bitwise_vs_reference_harness(model, tag, pot2):
+    """Links stored as generator components w (U = exp(i w.sigma)): the
+    reference converts every read to a quaternion and every updated link back
+    (WilsonLoops/Source/StandardModel.cpp:78-157, 456-514).  The fixture's state
+    has |w| up to 3 (the pi - asin branch of the inverse) and zero links (the
+    tolerance branch)."""
+    g = np.load(os.path.join(GOLD, f"harness_{tag}_gen.npz"))
+    s, v = g["scalar0"].copy(), g["vec0"].copy()
+    assert v.shape[-1] == 12
+    w = np.linalg.norm(v.reshape(2, -1, 3, 4)[..., 1:], axis=-1)
+    assert (w > np.pi / 2).any() and (w == 0).any()
+    d = make_desc(model=model, n=(4, 5, 6), pot2=pot2, gen_rep=1)
+    for k in range(6):
+        if k % 3 == 0:
+            assert np.array_equal(O.energy_density_f32(d, s, v, k), g[f"dens_{k}"]), f"energy densities, step {k}"
+        O.evolve(d, s, v, 1, time_iter=k, energy_freq=3, gauss_freq=3)
+        if k % 3 == 0:
+            assert np.array_equal(O.gauss_site_f32(d, s, v, k), g[f"viol_{k}"]), f"Gauss violations, step {k}"
+        if f"scalar_{k}" in g:
+            assert np.array_equal(s, g[f"scalar_{k}"]), f"scalar fields differ after step {k}"
+            assert np.array_equal(v, g[f"vec_{k}"]), f"link fields differ after step {k}"
+
+
+@pytest.mark.parametrize("model,tag,pot2", MODELS, ids=["sm", "2hdm"])
+def test_oracle_generator_form_reproduces_reference_program_output(model, tag, pot2):
+    """The unmodified reference program with 4 vector components per direction
+    (and TwoHDM.cfg's generator flag for the 2HDM): printed energies and Gauss
+    violations digit for digit.  Its Gauss violations differ from the quaternion
+    run's in the printed digits, so the form is exercised."""
+    g = np.load(os.path.join(GOLD, f"program_{tag}_gen.npz"))
+    n, Ns = (6, 5, 7), 4 if model == GFT_MODEL_SM else 8
+    N = int(np.prod(n))
+    s = O.random_uniform(2 * N * Ns, 7, -0.8, 0.8).reshape(2, N, Ns)
+    v = np.zeros((2, N, 12), np.float32)
+    d = make_desc(model=model, n=n, pot2=pot2, gen_rep=1)
+    ref, obs, _ = O.evolve(d, s, v, 12, energy_freq=3, gauss_freq=2)
+    fmt = np.vectorize(lambda x: float("%.6g" % x))
+    e = np.array([r[:6] for r, o in zip(ref, obs) if o.has_energy])
+    gs = np.array([r[6:] for r, o in zip(ref, obs) if o.has_gauss])
+    assert np.array_equal(fmt(e), g["energy"])
+    assert np.array_equal(fmt(gs), g["gauss"])
+    assert not np.array_equal(g["gauss"], np.load(os.path.join(GOLD, f"program_{tag}.npz"))["gauss"])
+
+
+def test_generator_form_conversions_round_trip_and_match_closed_forms():
+    """(1) Only x links non-trivial, each a function of x alone, identical slots,
+    zero scalar field: every plaquette is 1, there is no
+    current and E = 0, so the update
+    multiplies by the identity and the stored w must come back from
+    exp(i w.sigma) -> w to rounding, for |w| on both sides of pi/2 (the
+    pi - asin branch).  (2) The quaternion and generator forms of the same
+    state evolve to the same group elements (w -> cos|w| + i sin|w| w/|w|.sigma,
+    the textbook exponential map)."""
+    n = (3, 3, 3)
+    N = 27
+    rng = np.random.default_rng(5)
+    wx = rng.normal(size=(3, 3))
+    wx *= np.array([[0.4], [2.0], [2.9]]) / np.linalg.norm(wx, axis=-1, keepdims=True)
+    w = np.repeat(wx, 9, axis=0)          # site index (x * 3 + y) * 3 + z
+    assert (np.linalg.norm(w, axis=-1) > np.pi / 2).any()
+    vg = np.zeros((2, N, 3, 4))
+    vg[:, :, 0, 1:] = w
+    s = np.zeros((2, N, 4))
+    dg = make_desc(n=n, gen_rep=1)
+    sg, vgg = s.copy(), vg.reshape(2, N, 12).copy()
+    O.evolve(dg, sg, vgg, 1)
+    np.testing.assert_allclose(vgg.reshape(2, N, 3, 4)[0, :, 0, 1:], w, rtol=0, atol=1e-13)
+    assert np.array_equal(vgg.reshape(2, N, 3, 4)[0, :, 1:], vg[0, :, 1:])
+    w = rng.normal(size=(N, 3, 3))
+    w *= rng.uniform(0.1, 3.0, size=(N, 3, 1)) / np.linalg.norm(w, axis=-1, keepdims=True)
+    vg = np.zeros((2, N, 3, 4))
+    vg[..., 1:] = w
+    # Same state in the quaternion form: c0 = cos|w|, c = sin|w| w/|w|.
+    ang = np.linalg.norm(w, axis=-1, keepdims=True)
+    q = np.concatenate([np.cos(ang), np.sin(ang) * w / ang], axis=-1)
+    vq = np.zeros((2, N, 3, 5))
+    vq[..., 1:] = q
+    rng2 = np.random.default_rng(6)
+    s2 = rng2.normal(size=(2, N, 4)) * 0.3
+    s2[1] = s2[0]
+    dq = make_desc(n=n, g=0.65, gp=0.35)
+    dg = make_desc(n=n, gen_rep=1, g=0.65, gp=0.35)
+    sq_, vq_ = s2.copy(), vq.reshape(2, N, 15).copy()
+    sg_, vg_ = s2.copy(), vg.reshape(2, N, 12).copy()
+    O.evolve(dq, sq_, vq_, 3)
+    O.evolve(dg, sg_, vg_, 3)
+    np.testing.assert_allclose(sg_, sq_, rtol=0, atol=1e-12)
+    wg = vg_.reshape(2, N, 3, 4)[..., 1:]
+    a2 = np.linalg.norm(wg, axis=-1, keepdims=True)
+    qg = np.concatenate([np.cos(a2), np.sin(a2) * wg / a2], axis=-1)
+    np.testing.assert_allclose(qg, vq_.reshape(2, N, 3, 5)[..., 1:], rtol=0, atol=1e-12)
+
+
 @pytest.mark.parametrize("case,mask", [("xz", 1 | 2 | 16 | 32), ("y", 4 | 8)], ids=["xz", "y"])
 def test_oracle_reproduces_reference_program_with_fixed_faces(case, mask):
     """Fixed faces (Fixed.cpp: frozen one-site shells, left out of the
