@@ -37,14 +37,29 @@ class GftDesc(C.Structure):
         ("damping", C.c_float), ("nt_damped", C.c_uint),
         ("fixed_faces", C.c_int),
         ("gen_rep", C.c_int),
+        ("gauge_num", C.c_int),
+        ("gpp", C.c_double),
     ]
+
+
+def link_reals(desc) -> int:
+    """Link reals per site of the two-slot arrays: 3 directions x [y, SU(2) (4 as
+    a quaternion, 3 as generators), Higgs-family part (gaugeNum 5: y'; 7: a
+    second SU(2))]."""
+    base = 4 if desc.gen_rep else 5
+    gn = desc.gauge_num if desc.gauge_num in (5, 7) else 4
+    extra = {4: 0, 5: 1, 7: 3 if desc.gen_rep else 4}[gn]
+    return 3 * (base + extra)
+
+
+MAX_EQ = 7  # GFT_MAX_EQ: Gauss generators of the largest gauge group (gaugeNum 7)
 
 
 class GftObs(C.Structure):
     _fields_ = [
         ("time_iter", C.c_uint), ("has_energy", C.c_uint), ("has_gauss", C.c_uint), ("num_eq", C.c_uint),
         ("energy", C.c_double * 6),
-        ("gauss_int", C.c_double * 4), ("gauss_max", C.c_double * 4), ("gauss_scale", C.c_double * 4),
+        ("gauss_int", C.c_double * MAX_EQ), ("gauss_max", C.c_double * MAX_EQ), ("gauss_scale", C.c_double * MAX_EQ),
     ]
 
 
@@ -55,12 +70,14 @@ class GftLoops(C.Structure):
 
 def make_desc(model=GFT_MODEL_SM, form=GFT_FORM_TWO_SLOT_F64, n=(8, 8, 8), d=(0.5, 0.5, 0.5), dt=0.05,
               g=0.65, gp=0.35, lam=0.13, eta_sqr=1.0, pot2=None, damping=0.0, nt_damped=0, div_tol=1e-9,
-              fixed_faces=0, gen_rep=0):
+              fixed_faces=0, gen_rep=0, gauge_num=4, gpp=0.5):
     """A gft_desc with the SM couplings of the 32^3 / 256^3 workloads by default.
 
     pot2: Double_SO_N parameters for the 2HDM (keys as in gft_desc).
     fixed_faces: bit 2a / 2a+1 fixes the low / high face of axis a (x, y, z).
-    gen_rep: 1 stores SU(2) links as generator components (12 reals per site)."""
+    gen_rep: 1 stores SU(2) links as generator components (12 reals per site).
+    gauge_num: 4 (SU(2) x U(1)), 5 or 7 (2HDM with a Higgs-family U(1) or SU(2));
+    gpp: the Higgs-family coupling."""
     ds = GftDesc()
     ds.model, ds.form = model, form
     ds.nx, ds.ny, ds.nz = n
@@ -75,6 +92,8 @@ def make_desc(model=GFT_MODEL_SM, form=GFT_FORM_TWO_SLOT_F64, n=(8, 8, 8), d=(0.
     ds.damping, ds.nt_damped = damping, nt_damped
     ds.fixed_faces = fixed_faces
     ds.gen_rep = gen_rep
+    ds.gauge_num = gauge_num
+    ds.gpp = gpp
     return ds
 
 

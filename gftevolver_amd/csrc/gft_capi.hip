@@ -31,8 +31,14 @@ constexpr int kEventPool = 256;
 
 bool is_two_slot(int form) { return form == GFT_FORM_TWO_SLOT_F64 || form == GFT_FORM_TWO_SLOT_F32; }
 bool is_f64(int form) { return form == GFT_FORM_TWO_SLOT_F64 || form == GFT_FORM_MOMENTUM_F64; }
-// Link reals per site of the two-slot arrays: 3 x [y, c0..c3] or 3 x [y, w1..w3].
-long long link_reals(const gft_desc& d) { return d.gen_rep ? 12 : 15; }
+// Gauss generators of the gauge group: gaugeNum 4, 5 or 7.
+int gauge_num(const gft_desc& d) { return (d.gauge_num == 5 || d.gauge_num == 7) ? d.gauge_num : 4; }
+// Link reals per site of the two-slot arrays: 3 x [y, SU(2) as c0..c3 or w1..w3,
+// Higgs-family part (gaugeNum 5: y'; 7: c0'..c3' or w1'..w3')].
+long long link_reals(const gft_desc& d) {
+    const int base = d.gen_rep ? 4 : 5, g = gauge_num(d);
+    return 3 * (base + (g == 5 ? 1 : g == 7 ? base - 1 : 0));
+}
 
 }  // namespace
 
@@ -98,6 +104,9 @@ static Coef make_coef(const gft_desc& d) {
     c.inv_gp_sqr = gps > d.div_tol ? 1.0 / gps : 0.0;
     c.gsq[0] = (double)(float)gps;
     c.gsq[1] = c.gsq[2] = c.gsq[3] = (double)(float)gs;
+    const double gpps = d.gpp * d.gpp;
+    c.gsq[4] = c.gsq[5] = c.gsq[6] = (double)(float)gpps;
+    c.inv_gpp_sqr = gpps > d.div_tol ? 1.0 / gpps : 0.0;
     c.lambda = d.lambda; c.eta_sqr = d.eta_sqr;
     c.m1 = d.m1; c.m2 = d.m2; c.l1 = d.l1; c.l2 = d.l2; c.l3 = d.l3;
     c.m12r = d.m12r; c.l4p5 = d.l4p5; c.l6r = d.l6r; c.l7r = d.l7r;
@@ -157,6 +166,13 @@ int gft_create(const gft_desc* desc, gft_ctx** out) {
     }
     if (d.gen_rep != 0 && d.gen_rep != 1)
         return fail(nullptr, GFT_EINVAL, "gft_create: gen_rep is 0 (quaternion links) or 1 (generator form)");
+    if (d.gauge_num != 0 && d.gauge_num != 4 && d.gauge_num != 5 && d.gauge_num != 7)
+        return fail(nullptr, GFT_EINVAL, "gft_create: gauge_num is 4 (SU(2) x U(1)), 5 or 7 (Higgs-family U(1) or SU(2))");
+    if (gauge_num(d) != 4 && d.model != GFT_MODEL_2HDM)
+        return fail(nullptr, GFT_EINVAL, "gft_create: a Higgs-family gauge field needs two doublets (GFT_MODEL_2HDM)");
+    if (gauge_num(d) != 4 && !is_two_slot(d.form))
+        return fail(nullptr, GFT_EINVAL, "gft_create: gaugeNum 5 and 7 run in the two-slot forms (the momentum "
+                                         "forms carry the SU(2) x U(1) links only)");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
         return fail(nullptr, GFT_ENODEV, "gft_create: no HIP device");
@@ -411,7 +427,8 @@ static int collect_row(gft_ctx* ctx, bool e, bool g, gft_obs* row) {
     CK(hipStreamSynchronize(ctx->stream));
     std::memset(row, 0, sizeof(*row));
     row->time_iter = ctx->time_iter;
-    row->num_eq = 4;
+    const int G = gauge_num(ctx->desc);
+    row->num_eq = G;
     const double dv = ctx->coef.dvol;
     if (e) {
         row->has_energy = 1;
@@ -424,10 +441,10 @@ static int collect_row(gft_ctx* ctx, bool e, bool g, gft_obs* row) {
     }
     if (g) {
         row->has_gauss = 1;
-        for (int m = 0; m < 4; m++) {
+        for (int m = 0; m < G; m++) {
             row->gauss_int[m] = ctx->obs_host[8 + m] * dv;
-            row->gauss_scale[m] = ctx->obs_host[12 + m] * dv;
-            row->gauss_max[m] = ctx->obs_host[16 + m];
+            row->gauss_scale[m] = ctx->obs_host[8 + G + m] * dv;
+            row->gauss_max[m] = ctx->obs_host[8 + 2 * G + m];
         }
     }
     return GFT_OK;
@@ -435,26 +452,56 @@ static int collect_row(gft_ctx* ctx, bool e, bool g, gft_obs* row) {
 
 static long long obs_blocks(const gft_ctx* ctx) { return (ctx->N + kBlock - 1) / kBlock; }
 
-// Two-slot kernels by storage type and link width (5: quaternion, 4: generator form).
-template <class T, int VD>
+// Finishes the Gauss partials (per block: G integrated |G^a|, G scales, G
+// maxima) into obs_dev[8 ..): a row wider than the finisher is done in two
+// column slices.
+static void finish_gauss(gft_ctx* ctx, long long nblocks) {
+    const int G = gauge_num(ctx->desc), W = 3 * G;
+    if (W <= kFinWidth) {
+        finalize_partials<<<1, kFinThreads, 0, ctx->stream>>>(ctx->partial, nblocks, W, 2 * G, G, ctx->obs_dev + 8);
+        return;
+    }
+    const int s1 = std::min(2 * G, kFinWidth);
+    finalize_partials<<<1, kFinThreads, 0, ctx->stream>>>(ctx->partial, nblocks, W, s1, 0, ctx->obs_dev + 8);
+    finalize_partials<<<1, kFinThreads, 0, ctx->stream>>>(ctx->partial + s1, nblocks, W, 2 * G - s1, G,
+                                                          ctx->obs_dev + 8 + s1);
+}
+
+// Two-slot kernels by storage type, link form (quaternion or generator) and
+// gauge group (gaugeNum 4, 5, 7).
+template <class T, bool GEN, int G>
 static void twoslot_step_as(gft_ctx* ctx, long long nblocks, unsigned now, bool me, double damping_step) {
-    twoslot_step<T, VD><<<nblocks, kBlock, 0, ctx->stream>>>(ctx->coef, (T*)ctx->S, (T*)ctx->V, now, me, damping_step,
-                                                              ctx->partial, ctx->err_dev);
+    twoslot_step<T, GEN, G><<<nblocks, kBlock, 0, ctx->stream>>>(ctx->coef, (T*)ctx->S, (T*)ctx->V, now, me,
+                                                                  damping_step, ctx->partial, ctx->err_dev);
+}
+template <class T, bool GEN, int G>
+static void twoslot_gauss_as(gft_ctx* ctx, long long nblocks, unsigned now) {
+    twoslot_gauss<T, GEN, G><<<nblocks, kBlock, 0, ctx->stream>>>(ctx->coef, (const T*)ctx->S, (const T*)ctx->V, now,
+                                                                   ctx->partial);
+}
+template <class T, bool GEN>
+static void twoslot_dispatch(gft_ctx* ctx, long long nblocks, unsigned now, bool gauss, bool me, double damping_step) {
+    switch (gauge_num(ctx->desc)) {
+    case 5: gauss ? twoslot_gauss_as<T, GEN, 5>(ctx, nblocks, now)
+                  : twoslot_step_as<T, GEN, 5>(ctx, nblocks, now, me, damping_step); break;
+    case 7: gauss ? twoslot_gauss_as<T, GEN, 7>(ctx, nblocks, now)
+                  : twoslot_step_as<T, GEN, 7>(ctx, nblocks, now, me, damping_step); break;
+    default: gauss ? twoslot_gauss_as<T, GEN, 4>(ctx, nblocks, now)
+                   : twoslot_step_as<T, GEN, 4>(ctx, nblocks, now, me, damping_step); break;
+    }
+}
+static void twoslot_launch(gft_ctx* ctx, long long nblocks, unsigned now, bool gauss, bool me, double damping_step) {
+    const bool f64 = ctx->form == GFT_FORM_TWO_SLOT_F64, gen = ctx->desc.gen_rep != 0;
+    if (f64) (gen ? twoslot_dispatch<double, true> : twoslot_dispatch<double, false>)(ctx, nblocks, now, gauss, me,
+                                                                                       damping_step);
+    else (gen ? twoslot_dispatch<float, true> : twoslot_dispatch<float, false>)(ctx, nblocks, now, gauss, me,
+                                                                                 damping_step);
 }
 static void twoslot_launch_step(gft_ctx* ctx, long long nblocks, unsigned now, bool me, double damping_step) {
-    const bool f64 = ctx->form == GFT_FORM_TWO_SLOT_F64, gen = ctx->desc.gen_rep != 0;
-    if (f64) (gen ? twoslot_step_as<double, 4> : twoslot_step_as<double, 5>)(ctx, nblocks, now, me, damping_step);
-    else (gen ? twoslot_step_as<float, 4> : twoslot_step_as<float, 5>)(ctx, nblocks, now, me, damping_step);
-}
-template <class T, int VD>
-static void twoslot_gauss_as(gft_ctx* ctx, long long nblocks, unsigned now) {
-    twoslot_gauss<T, VD><<<nblocks, kBlock, 0, ctx->stream>>>(ctx->coef, (const T*)ctx->S, (const T*)ctx->V, now,
-                                                               ctx->partial);
+    twoslot_launch(ctx, nblocks, now, false, me, damping_step);
 }
 static void twoslot_launch_gauss(gft_ctx* ctx, long long nblocks, unsigned now) {
-    const bool f64 = ctx->form == GFT_FORM_TWO_SLOT_F64, gen = ctx->desc.gen_rep != 0;
-    if (f64) (gen ? twoslot_gauss_as<double, 4> : twoslot_gauss_as<double, 5>)(ctx, nblocks, now);
-    else (gen ? twoslot_gauss_as<float, 4> : twoslot_gauss_as<float, 5>)(ctx, nblocks, now);
+    twoslot_launch(ctx, nblocks, now, true, false, 0.0);
 }
 
 extern "C" int gft_evolve(gft_ctx* ctx, unsigned nsteps, unsigned energy_freq, unsigned gauss_freq, gft_obs* rows,
@@ -462,7 +509,7 @@ extern "C" int gft_evolve(gft_ctx* ctx, unsigned nsteps, unsigned energy_freq, u
     if (!ctx) return GFT_EINVAL;
     if ((energy_freq || gauss_freq) && !rows && max_rows) return fail(ctx, GFT_EINVAL, "gft_evolve: rows is NULL");
     const long long nblocks = obs_blocks(ctx);
-    int rc = ensure_partial(ctx, (size_t)nblocks * 12);
+    int rc = ensure_partial(ctx, (size_t)nblocks * 3 * GFT_MAX_EQ);
     if (rc) return rc;
     const bool two = is_two_slot(ctx->form);
     unsigned row = 0;
@@ -480,7 +527,7 @@ extern "C" int gft_evolve(gft_ctx* ctx, unsigned nsteps, unsigned energy_freq, u
             // The momentum form measures the state before the update in its own pass.
             if ((rc = momentum_observe(ctx->mom, ctx->coef, MomObs::Energy, ctx->partial, nblocks, ctx->stream, ctx->err)))
                 return rc;
-            finalize_partials<<<1, kFinThreads, 0, ctx->stream>>>(ctx->partial, nblocks, 5, 0, ctx->obs_dev);
+            finalize_partials<<<1, kFinThreads, 0, ctx->stream>>>(ctx->partial, nblocks, 5, 5, 0, ctx->obs_dev);
         }
         if ((rc = timed_begin(ctx))) return rc;
         if (two) {
@@ -490,14 +537,14 @@ extern "C" int gft_evolve(gft_ctx* ctx, unsigned nsteps, unsigned energy_freq, u
             if ((rc = momentum_step(ctx->mom, ctx->coef, damping_step, ctx->stream, ctx->err_dev, ctx->err))) return rc;
         }
         if ((rc = timed_end(ctx))) return rc;
-        if (me && two) finalize_partials<<<1, kFinThreads, 0, ctx->stream>>>(ctx->partial, nblocks, 5, 0, ctx->obs_dev);
+        if (me && two) finalize_partials<<<1, kFinThreads, 0, ctx->stream>>>(ctx->partial, nblocks, 5, 5, 0, ctx->obs_dev);
         if (mg) {
             if (two)
                 twoslot_launch_gauss(ctx, nblocks, now);
             else if ((rc = momentum_observe(ctx->mom, ctx->coef, MomObs::Gauss, ctx->partial, nblocks, ctx->stream,
                                             ctx->err)))
                 return rc;
-            finalize_partials<<<1, kFinThreads, 0, ctx->stream>>>(ctx->partial, nblocks, 8, 4, ctx->obs_dev + 8);
+            finish_gauss(ctx, nblocks);
             CK(hipGetLastError());
         }
         if ((me || mg) && row < max_rows && rows) {
@@ -550,12 +597,12 @@ extern "C" int gft_observe(gft_ctx* ctx, gft_obs* out) {
     if (rc) return rc;
     if ((rc = momentum_observe(ctx->mom, ctx->coef, MomObs::Energy, ctx->partial, nblocks, ctx->stream, ctx->err)))
         return rc;
-    finalize_partials<<<1, kFinThreads, 0, ctx->stream>>>(ctx->partial, nblocks, 5, 0, ctx->obs_dev);
+    finalize_partials<<<1, kFinThreads, 0, ctx->stream>>>(ctx->partial, nblocks, 5, 5, 0, ctx->obs_dev);
     const bool g = ctx->mom.prev_valid;
     if (g) {
         if ((rc = momentum_observe(ctx->mom, ctx->coef, MomObs::Gauss, ctx->partial, nblocks, ctx->stream, ctx->err)))
             return rc;
-        finalize_partials<<<1, kFinThreads, 0, ctx->stream>>>(ctx->partial, nblocks, 8, 4, ctx->obs_dev + 8);
+        finish_gauss(ctx, nblocks);
     }
     CK(hipGetLastError());
     return collect_row(ctx, true, g, out);
@@ -569,16 +616,16 @@ extern "C" int gft_wilson_loops(gft_ctx* ctx, gft_loops* out) {
     if (rc) return rc;
     if ((rc = momentum_observe(ctx->mom, ctx->coef, MomObs::Loops, ctx->partial, nblocks, ctx->stream, ctx->err)))
         return rc;
-    finalize_partials<<<1, kFinThreads, 0, ctx->stream>>>(ctx->partial, nblocks, 5, 0, ctx->obs_dev + 24);
+    finalize_partials<<<1, kFinThreads, 0, ctx->stream>>>(ctx->partial, nblocks, 5, 5, 0, ctx->obs_dev + 32);
     CK(hipGetLastError());
-    CK(hipMemcpyAsync(ctx->obs_host, ctx->obs_dev, 32 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    CK(hipMemcpyAsync(ctx->obs_host, ctx->obs_dev, 40 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     CK(hipStreamSynchronize(ctx->stream));
     const double n = (double)ctx->N;
-    out->w11_su2 = ctx->obs_host[24] / (3 * n);
-    out->w11_u1 = ctx->obs_host[25] / (3 * n);
-    out->w22_su2 = ctx->obs_host[26] / (3 * n);
-    out->w22_u1 = ctx->obs_host[27] / (3 * n);
-    out->phi_sq = ctx->obs_host[28] / n;
+    out->w11_su2 = ctx->obs_host[32] / (3 * n);
+    out->w11_u1 = ctx->obs_host[33] / (3 * n);
+    out->w22_su2 = ctx->obs_host[34] / (3 * n);
+    out->w22_u1 = ctx->obs_host[35] / (3 * n);
+    out->phi_sq = ctx->obs_host[36] / n;
     return GFT_OK;
 }
 
@@ -1044,12 +1091,12 @@ extern "C" int gft_init_grf(gft_ctx* ctx, const gft_grf_desc* g) {
         grf_filter<<<nb, 256, 0, st>>>(w, c.nx, c.ny, c.nz, dd.dx, dd.dy, dd.dz, g->slope, kmax);
         for (int a = 0; a < 3; a++) grf_fft_lines<<<(unsigned)lines[a], 256, 0, st>>>(w, ax[a], +1);
         grf_real_stats<<<nb, 256, 0, st>>>(w, comps + comp * N, N, ctx->partial);
-        finalize_partials<<<1, kFinThreads, 0, st>>>(ctx->partial, nb, 2, 0, ctx->obs_dev);
+        finalize_partials<<<1, kFinThreads, 0, st>>>(ctx->partial, nb, 2, 2, 0, ctx->obs_dev);
         grf_normalise<<<nb, 256, 0, st>>>(comps + comp * N, N, ctx->obs_dev, ctx->partial);
     }
     for (int k = 0; k < c.nd; k++) {
         grf_sumsq4<<<nb, 256, 0, st>>>(comps + 4 * k * N, N, ctx->partial);
-        finalize_partials<<<1, kFinThreads, 0, st>>>(ctx->partial, nb, 1, 0, ctx->obs_dev + 2 + k);
+        finalize_partials<<<1, kFinThreads, 0, st>>>(ctx->partial, nb, 1, 1, 0, ctx->obs_dev + 2 + k);
     }
     const double vev0 = g->vev[0], vev1 = g->vev[1];
     if (is_two_slot(ctx->form)) {

@@ -31,7 +31,9 @@ struct Coef {
     double inv_sp[3];            // 1/dx_i^2 (WilsonLoops StandardModel.cpp:51-53)
     double inv_sqr_dt;
     double inv_g_sqr, inv_gp_sqr;
-    double gsq[4];               // getSqrCouplings, rounded through float (StandardModel.cpp:224-230)
+    double gsq[7];               // getSqrCouplings, rounded through float (StandardModel.cpp:224-230;
+                                 // g''^2 for the Higgs-family generators 4..6, TwoHDM.cpp:258-267)
+    double inv_gpp_sqr;          // 1/g''^2 (0 below the division-by-zero tolerance)
     double lambda, eta_sqr;      // SO_N
     double m1, m2, l1, l2, l3, m12r, l4p5, l6r, l7r, m12i, l4m5, l5i, l6i, l7i;
     int break_son, break_uhalfn, pot_norm;

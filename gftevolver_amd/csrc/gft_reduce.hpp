@@ -52,7 +52,8 @@ __device__ __forceinline__ void block_reduce_store(const double* sums, const dou
 }
 
 // One block of kFinThreads threads: out[q] = Neumaier sum (q < nsum) or max
-// (q >= nsum) of partial[b * (nsum + nmax) + q] over b = 0 .. nblocks-1.  Thread
+// (q >= nsum) of partial[b * stride + q] over b = 0 .. nblocks-1 (stride >=
+// nsum + nmax: a wider row is finished in several calls on column slices).  Thread
 // t takes rows t, t + kFinThreads, ... for every q at once (independent
 // accumulator chains), then thread q combines the per-thread (sum,
 // compensation) pairs in thread order: a fixed order, so the result is
@@ -60,7 +61,8 @@ __device__ __forceinline__ void block_reduce_store(const double* sums, const dou
 constexpr int kFinThreads = 256;
 constexpr int kFinWidth = 12;
 __global__ void __launch_bounds__(kFinThreads) finalize_partials(const double* __restrict__ partial, long long nblocks,
-                                                                 int nsum, int nmax, double* __restrict__ out) {
+                                                                 int stride, int nsum, int nmax,
+                                                                 double* __restrict__ out) {
     __shared__ double s_hi[kFinWidth * kFinThreads], s_lo[kFinWidth * kFinThreads];
     const int width = nsum + nmax;
     Kahan k[kFinWidth];
@@ -68,7 +70,7 @@ __global__ void __launch_bounds__(kFinThreads) finalize_partials(const double* _
 #pragma unroll
     for (int q = 0; q < kFinWidth; q++) { k[q].init(); mx[q] = 0.0; }
     for (long long b = threadIdx.x; b < nblocks; b += kFinThreads) {
-        const double* row = partial + b * width;
+        const double* row = partial + b * stride;
 #pragma unroll
         for (int q = 0; q < kFinWidth; q++) {
             if (q < nsum) k[q].add(row[q]);

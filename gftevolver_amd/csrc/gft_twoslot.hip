@@ -49,10 +49,10 @@ __device__ __forceinline__ void su2_act_ref(const double* U, const double* h, do
 // Gradients/Source/StandardModel.cpp:103-136).  Generator form: c0 = cos|w|,
 // c = (+-) sin|w| w/|w|; at or below the division-by-zero tolerance the
 // reference sets c1 = sin|w| without the conjugation sign.
-template <int VD, class T>
+template <bool GEN, class T>
 __device__ __forceinline__ void link_rep_ref(const T* link, bool conj, double tol, double* U) {
 #pragma clang fp contract(off)
-    if constexpr (VD == 5) {
+    if constexpr (!GEN) {
         link_rep(link, conj, U);
     } else {
         double wm = 0.0;
@@ -88,9 +88,21 @@ __device__ __forceinline__ void invert_rep_ref(const double* c, double tol, T* w
     }
 }
 
+// Link layout per direction: [y, SU(2) (c0..c3 or w1..w3), Higgs-family part]
+// with the family part at OFF: y' for gaugeNum 5, a second SU(2) for 7
+// (WilsonLoops/Source/TwoHDM.cpp:215-242).
+template <bool GEN, int G>
+struct LinkLayout {
+    static constexpr int OFF = GEN ? 4 : 5;
+    static constexpr int VD = OFF + (G == 5 ? 1 : G == 7 ? OFF - 1 : 0);
+};
+
 // transform(): U(1) phase then SU(2), optionally conjugated
-// (Gradients/Source/StandardModel.cpp:72-100, TwoHDM.cpp:72-104).
-template <int VD, class T>
+// (Gradients/Source/StandardModel.cpp:72-100, TwoHDM.cpp:72-104); for
+// gaugeNum 5 then the Higgs-family phase, of opposite sign on the two
+// doublets, for 7 the Higgs-family SU(2) on (f_i, f_i+1, f_i+4, f_i+5),
+// i = 0, 2 (TwoHDM.cpp:106-141).
+template <bool GEN, int G, class T>
 __device__ __forceinline__ void transform_ref(const double* sf, int nd, const T* link, bool conj, double tol,
                                               double* out) {
 #pragma clang fp contract(off)
@@ -103,8 +115,47 @@ __device__ __forceinline__ void transform_ref(const double* sf, int nd, const T*
         h[c] = cy * sf[c] + (pm * sy) * sf[c - pm];
     }
     double U[4];
-    link_rep_ref<VD>(link, conj, tol, U);
-    for (int k = 0; k < nd; k++) su2_act_ref(U, h + 4 * k, out + 4 * k);
+    link_rep_ref<GEN>(link, conj, tol, U);
+    if constexpr (G == 4) {
+        for (int k = 0; k < nd; k++) su2_act_ref(U, h + 4 * k, out + 4 * k);
+    } else {
+        constexpr int OFF = LinkLayout<GEN, G>::OFF;
+        double sm[8];
+        for (int k = 0; k < 2; k++) su2_act_ref(U, h + 4 * k, sm + 4 * k);
+        if constexpr (G == 5) {
+            const double y2 = static_cast<double>(link[OFF]);
+            const double cy2 = cos(y2);
+            for (int c = 0; c < 8; c++) {
+                const int pm = -1 + 2 * (c % 2);
+                const int pm4 = 1 - 2 * (c / 4);
+                const int sg = conj ? -pm4 : pm4;
+                out[c] = cy2 * sm[c] + (pm * sin(sg * y2)) * sm[c - pm];
+            }
+        } else {
+            double V[4];
+            link_rep_ref<GEN>(link + OFF - 1, conj, tol, V);
+            for (int i = 0; i < 4; i += 2) {
+                out[i] = V[0] * sm[i] - V[3] * sm[i + 1] + V[2] * sm[i + 4] - V[1] * sm[i + 5];
+                out[i + 1] = V[3] * sm[i] + V[0] * sm[i + 1] + V[1] * sm[i + 4] + V[2] * sm[i + 5];
+                out[i + 4] = -V[2] * sm[i] - V[1] * sm[i + 1] + V[0] * sm[i + 4] + V[3] * sm[i + 5];
+                out[i + 5] = V[1] * sm[i] - V[2] * sm[i + 1] - V[3] * sm[i + 4] + V[0] * sm[i + 5];
+            }
+        }
+    }
+}
+
+// Higgs-family bilinears of the upper/lower pair i in {0, 2} of both doublets
+// (TwoHDM.cpp:401-424 currents, 299-319 Gauss charges), accumulated into j[4..G-1].
+template <int G>
+__device__ __forceinline__ void family_bilinears_add(const double* t, const double* n, int i, double* j) {
+#pragma clang fp contract(off)
+    if constexpr (G == 5) {
+        j[4] += 0.25 * (t[i] * n[i + 1] - t[i + 1] * n[i] - t[i + 4] * n[i + 5] + t[i + 5] * n[i + 4]);
+    } else if constexpr (G == 7) {
+        j[4] += 0.25 * (t[i] * n[i + 5] - t[i + 1] * n[i + 4] + t[i + 4] * n[i + 1] - t[i + 5] * n[i]);
+        j[5] += 0.25 * (-t[i] * n[i + 4] - t[i + 1] * n[i + 5] + t[i + 4] * n[i] + t[i + 5] * n[i + 1]);
+        j[6] += 0.25 * (t[i] * n[i + 1] - t[i + 1] * n[i] - t[i + 4] * n[i + 5] + t[i + 5] * n[i + 4]);
+    }
 }
 
 // Potential derivative in the reference's operation order (SO_N.cpp:57-75,
@@ -161,11 +212,12 @@ __device__ void potential_derivative_ref(const Coef& d, const double* f, double*
 // normalised (WilsonLoops/Source/StandardModel.cpp:374,433) and the five FP64
 // energy densities (potential, gradient, kinetic, magnetic, electric) are
 // reduced into partial[block][5].
-template <class T, int VD>
+template <class T, bool GEN, int G>
 __global__ void __launch_bounds__(256) twoslot_step(Coef c, T* __restrict__ S, T* __restrict__ V, unsigned now,
                                                     int measure, double damping_step, double* __restrict__ partial,
                                                     unsigned* __restrict__ err) {
 #pragma clang fp contract(off)
+    constexpr int VD = LinkLayout<GEN, G>::VD, OFF = LinkLayout<GEN, G>::OFF;
     __shared__ double lds[4 * 5];
     const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     double dens[5] = {0, 0, 0, 0, 0};
@@ -197,20 +249,21 @@ __global__ void __launch_bounds__(256) twoslot_step(Coef c, T* __restrict__ S, T
                     double f[8];
                     const T* q = Sn + (st == 0 ? xm[a] : xp[a]) * Ns;
                     for (int k = 0; k < Ns; k++) f[k] = static_cast<double>(q[k]);
-                    if (st == 0) transform_ref<VD>(f, nd, Vn + xm[a] * (3 * VD) + a * VD, false, c.tol, tf);
-                    else transform_ref<VD>(f, nd, vn + a * VD, true, c.tol, tf);
+                    if (st == 0) transform_ref<GEN, G>(f, nd, Vn + xm[a] * (3 * VD) + a * VD, false, c.tol, tf);
+                    else transform_ref<GEN, G>(f, nd, vn + a * VD, true, c.tol, tf);
                 }
                 for (int k = 0; k < Ns; k++) der[k] += c.lap[a][st] * tf[k];
             }
         }
 
-        double cur[12];
+        double cur[3 * G];
         for (int a = 0; a < 3; a++) {
             double tf[8], nb[8];
-            transform_ref<VD>(phi, nd, vn + a * VD, false, c.tol, tf);
+            transform_ref<GEN, G>(phi, nd, vn + a * VD, false, c.tol, tf);
             const T* q = Sn + xp[a] * Ns;
             for (int k = 0; k < Ns; k++) nb[k] = static_cast<double>(q[k]);
-            double j[4] = {0.0, 0.0, 0.0, 0.0};
+            double j[G];
+            for (int m = 0; m < G; m++) j[m] = 0.0;
             for (int k = 0; k < nd; k++) {
                 const double* t = tf + 4 * k;
                 const double* n = nb + 4 * k;
@@ -218,12 +271,13 @@ __global__ void __launch_bounds__(256) twoslot_step(Coef c, T* __restrict__ S, T
                 j[1] += 0.25 * (t[0] * n[3] - t[1] * n[2] + t[2] * n[1] - t[3] * n[0]);
                 j[2] += 0.25 * (-t[0] * n[2] - t[1] * n[3] + t[2] * n[0] + t[3] * n[1]);
                 j[3] += 0.25 * (t[0] * n[1] - t[1] * n[0] - t[2] * n[3] + t[3] * n[2]);
+                family_bilinears_add<G>(tf, nb, 2 * k, j);
             }
-            for (int m = 0; m < 4; m++) cur[4 * a + m] = j[m];
+            for (int m = 0; m < G; m++) cur[G * a + m] = j[m];
         }
 
-        double mag[12];
-        for (int k = 0; k < 12; k++) mag[k] = 0.0;
+        double mag[3 * G];
+        for (int k = 0; k < 3 * G; k++) mag[k] = 0.0;
         double mag_e = 0.0;
         for (int i = 0; i < 3; i++) {
             for (int j = 0; j < 3; j++) {
@@ -237,41 +291,81 @@ __global__ void __launch_bounds__(256) twoslot_step(Coef c, T* __restrict__ S, T
                                     + static_cast<double>(Lx[j * VD]) - static_cast<double>(Lx[i * VD]);
                 const double th_nb = -static_cast<double>(Lxpimj[j * VD]) - static_cast<double>(Lxmj[i * VD])
                                    + static_cast<double>(Lxmj[j * VD]) + static_cast<double>(Lx[i * VD]);
-                mag[4 * i] += (sin(th_loc) - sin(th_nb)) * c.inv_sp[j];
+                mag[G * i] += (sin(th_loc) - sin(th_nb)) * c.inv_sp[j];
                 double P1[4], P2[4], Um[4];
-                link_rep_ref<VD>(Lx + i * VD, true, c.tol, P1);
-                link_rep_ref<VD>(Lx + j * VD, false, c.tol, Um);
+                link_rep_ref<GEN>(Lx + i * VD, true, c.tol, P1);
+                link_rep_ref<GEN>(Lx + j * VD, false, c.tol, Um);
                 su2_product(Um, P1, true, P1);
-                link_rep_ref<VD>(Lxpj + i * VD, false, c.tol, Um);
+                link_rep_ref<GEN>(Lxpj + i * VD, false, c.tol, Um);
                 su2_product(Um, P1, true, P1);
-                link_rep_ref<VD>(Lxpi + j * VD, true, c.tol, Um);
+                link_rep_ref<GEN>(Lxpi + j * VD, true, c.tol, Um);
                 su2_product(Um, P1, measure != 0, P1);
-                link_rep_ref<VD>(Lxpimj + j * VD, true, c.tol, P2);
-                link_rep_ref<VD>(Lxmj + i * VD, true, c.tol, Um);
+                link_rep_ref<GEN>(Lxpimj + j * VD, true, c.tol, P2);
+                link_rep_ref<GEN>(Lxmj + i * VD, true, c.tol, Um);
                 su2_product(Um, P2, true, P2);
-                link_rep_ref<VD>(Lxmj + j * VD, false, c.tol, Um);
+                link_rep_ref<GEN>(Lxmj + j * VD, false, c.tol, Um);
                 su2_product(Um, P2, true, P2);
-                link_rep_ref<VD>(Lx + i * VD, false, c.tol, Um);
+                link_rep_ref<GEN>(Lx + i * VD, false, c.tol, Um);
                 su2_product(Um, P2, false, P2);
-                for (int k = 1; k < 4; k++) mag[4 * i + k] += (P1[k] - P2[k]) * c.inv_sp[j];
+                for (int k = 1; k < 4; k++) mag[G * i + k] += (P1[k] - P2[k]) * c.inv_sp[j];
                 if (measure)
                     mag_e += 2.0 * c.inv_sp[i] * c.inv_sp[j]
                            * (c.inv_g_sqr * (1.0 - P1[0]) + c.inv_gp_sqr * (1.0 - cos(th_loc)));
+                // Higgs-family loops (WilsonLoops/Source/TwoHDM.cpp:496-552).
+                if constexpr (G == 5) {
+                    const double f_loc = -static_cast<double>(Lxpi[j * VD + OFF]) + static_cast<double>(Lxpj[i * VD + OFF])
+                                       + static_cast<double>(Lx[j * VD + OFF]) - static_cast<double>(Lx[i * VD + OFF]);
+                    const double f_nb = -static_cast<double>(Lxpimj[j * VD + OFF])
+                                      - static_cast<double>(Lxmj[i * VD + OFF])
+                                      + static_cast<double>(Lxmj[j * VD + OFF]) + static_cast<double>(Lx[i * VD + OFF]);
+                    mag[G * i + 4] += (sin(f_loc) - sin(f_nb)) * c.inv_sp[j];
+                    if (measure) mag_e += 2.0 * c.inv_sp[i] * c.inv_sp[j] * c.inv_gpp_sqr * (1.0 - cos(f_loc));
+                } else if constexpr (G == 7) {
+                    const int o = OFF - 1;
+                    link_rep_ref<GEN>(Lx + i * VD + o, true, c.tol, P1);
+                    link_rep_ref<GEN>(Lx + j * VD + o, false, c.tol, Um);
+                    su2_product(Um, P1, true, P1);
+                    link_rep_ref<GEN>(Lxpj + i * VD + o, false, c.tol, Um);
+                    su2_product(Um, P1, true, P1);
+                    link_rep_ref<GEN>(Lxpi + j * VD + o, true, c.tol, Um);
+                    su2_product(Um, P1, measure != 0, P1);
+                    link_rep_ref<GEN>(Lxpimj + j * VD + o, true, c.tol, P2);
+                    link_rep_ref<GEN>(Lxmj + i * VD + o, true, c.tol, Um);
+                    su2_product(Um, P2, true, P2);
+                    link_rep_ref<GEN>(Lxmj + j * VD + o, false, c.tol, Um);
+                    su2_product(Um, P2, true, P2);
+                    link_rep_ref<GEN>(Lx + i * VD + o, false, c.tol, Um);
+                    su2_product(Um, P2, false, P2);
+                    for (int k = 1; k < 4; k++) mag[G * i + 3 + k] += (P1[k] - P2[k]) * c.inv_sp[j];
+                    if (measure) mag_e += 2.0 * c.inv_sp[i] * c.inv_sp[j] * c.inv_gpp_sqr * (1.0 - P1[0]);
+                }
             }
         }
 
-        double el[12];
+        double el[3 * G];
         double el_e = 0.0;
         const T* vpast = vp;
         for (int a = 0; a < 3; a++) {
             const double la = static_cast<double>(vn[a * VD]) - static_cast<double>(vpast[a * VD]);
-            el[4 * a] = sin(la);
+            el[G * a] = sin(la);
             double Up[4], Um[4];
-            link_rep_ref<VD>(vpast + a * VD, true, c.tol, Up);
-            link_rep_ref<VD>(vn + a * VD, false, c.tol, Um);
+            link_rep_ref<GEN>(vpast + a * VD, true, c.tol, Up);
+            link_rep_ref<GEN>(vn + a * VD, false, c.tol, Um);
             su2_product(Um, Up, measure != 0, Up);
-            for (int k = 1; k < 4; k++) el[4 * a + k] = Up[k];
+            for (int k = 1; k < 4; k++) el[G * a + k] = Up[k];
             if (measure) el_e += 4.0 * c.inv_sp[a] * (c.inv_g_sqr * (1.0 - Up[0]) + c.inv_gp_sqr * (1.0 - cos(la)));
+            // Higgs family (WilsonLoops/Source/TwoHDM.cpp:600-634).
+            if constexpr (G == 5) {
+                const double lf = static_cast<double>(vn[a * VD + OFF]) - static_cast<double>(vpast[a * VD + OFF]);
+                el[G * a + 4] = sin(lf);
+                if (measure) el_e += 4.0 * c.inv_sp[a] * c.inv_gpp_sqr * (1.0 - cos(lf));
+            } else if constexpr (G == 7) {
+                link_rep_ref<GEN>(vpast + a * VD + OFF - 1, true, c.tol, Up);
+                link_rep_ref<GEN>(vn + a * VD + OFF - 1, false, c.tol, Um);
+                su2_product(Um, Up, measure != 0, Up);
+                for (int k = 1; k < 4; k++) el[G * a + 3 + k] = Up[k];
+                if (measure) el_e += 4.0 * c.inv_sp[a] * c.inv_gpp_sqr * (1.0 - Up[0]);
+            }
         }
 
         if (measure) {
@@ -279,7 +373,7 @@ __global__ void __launch_bounds__(256) twoslot_step(Coef c, T* __restrict__ S, T
             double ge = 0.0;
             for (int a = 0; a < 3; a++) {
                 double tf[8];
-                transform_ref<VD>(phi, nd, vn + a * VD, false, c.tol, tf);
+                transform_ref<GEN, G>(phi, nd, vn + a * VD, false, c.tol, tf);
                 const T* q = Sn + xp[a] * Ns;
                 for (int k = 0; k < Ns; k++) {
                     const double dd = -c.est[a] * tf[k] + c.est[a] * static_cast<double>(q[k]);
@@ -303,22 +397,35 @@ __global__ void __launch_bounds__(256) twoslot_step(Coef c, T* __restrict__ S, T
                              - damping_step * (static_cast<double>(sn[k]) - static_cast<double>(sp[k]));
             sp[k] = static_cast<T>(2 * static_cast<double>(sn[k]) - static_cast<double>(sp[k]) + rhs);
         }
-        // WilsonLoops::StandardModel::evolve (StandardModel.cpp:456-514).
+        // WilsonLoops::StandardModel::evolve (StandardModel.cpp:456-514), TwoHDM::evolve (TwoHDM.cpp:640-755).
         unsigned bad = 0;
         for (int a = 0; a < 3; a++) {
-            double R[4];
-            for (int k = 0; k < 4; k++)
-                R[k] = (1.0 - damping_step) * el[4 * a + k] + c.dt * c.dt * (mag[4 * a + k] + c.gsq[k] * cur[4 * a + k]);
+            double R[G];
+            for (int k = 0; k < G; k++)
+                R[k] = (1.0 - damping_step) * el[G * a + k] + c.dt * c.dt * (mag[G * a + k] + c.gsq[k] * cur[G * a + k]);
             vp[a * VD] = static_cast<T>(asin(R[0]) + static_cast<double>(vn[a * VD]));
             double cms = 0.0;
             for (int k = 1; k < 4; k++) cms += R[k] * R[k];
             if (R[0] * R[0] > 1 || cms > 1) bad = 1;
             const double u[4] = {sqrt(1.0 - cms), R[1], R[2], R[3]};
             double Up[4];
-            link_rep_ref<VD>(vn + a * VD, false, c.tol, Up);
+            link_rep_ref<GEN>(vn + a * VD, false, c.tol, Up);
             su2_product(u, Up, true, Up);
-            if constexpr (VD == 4) invert_rep_ref(Up, c.tol, vp + a * VD + 1);
+            if constexpr (GEN) invert_rep_ref(Up, c.tol, vp + a * VD + 1);
             else for (int k = 0; k < 4; k++) vp[a * VD + 1 + k] = static_cast<T>(Up[k]);
+            if constexpr (G == 5) {
+                vp[a * VD + OFF] = static_cast<T>(asin(R[4]) + static_cast<double>(vn[a * VD + OFF]));
+                if (R[4] * R[4] > 1) bad = 1;
+            } else if constexpr (G == 7) {
+                double cf = 0.0;
+                for (int k = 4; k < 7; k++) cf += R[k] * R[k];
+                if (cf > 1) bad = 1;
+                const double uf[4] = {sqrt(1.0 - cf), R[4], R[5], R[6]};
+                link_rep_ref<GEN>(vn + a * VD + OFF - 1, false, c.tol, Up);
+                su2_product(uf, Up, true, Up);
+                if constexpr (GEN) invert_rep_ref(Up, c.tol, vp + a * VD + OFF);
+                else for (int k = 0; k < 4; k++) vp[a * VD + OFF + k] = static_cast<T>(Up[k]);
+            }
         }
         if (bad) atomicOr(err, 1u);
     }
@@ -327,15 +434,18 @@ __global__ void __launch_bounds__(256) twoslot_step(Coef c, T* __restrict__ S, T
 
 // Gauss law after a step (slot `now` = t, slot !now = t+dt), FP64
 // (Core/Source/Model.cpp:323-337; Gradients/Source/StandardModel.cpp:250-270;
-// WilsonLoops/Source/StandardModel.cpp:274-322).  Reduces per block:
-// sum |G^a| (4), sum (sum_i |electric terms| + g_a^2 |rho^a|) (4), max |G^a| (4).
-template <class T, int VD>
+// WilsonLoops/Source/StandardModel.cpp:274-322; Higgs family: Gradients/Source/
+// TwoHDM.cpp:299-319, WilsonLoops/Source/TwoHDM.cpp:352-386).  Reduces per
+// block: sum |G^a| (G), sum (sum_i |electric terms| + g_a^2 |rho^a|) (G), max |G^a| (G).
+template <class T, bool GEN, int G>
 __global__ void __launch_bounds__(256) twoslot_gauss(Coef c, const T* __restrict__ S, const T* __restrict__ V,
                                                      unsigned now, double* __restrict__ partial) {
 #pragma clang fp contract(off)
-    __shared__ double lds[4 * 12];
+    constexpr int VD = LinkLayout<GEN, G>::VD, OFF = LinkLayout<GEN, G>::OFF;
+    __shared__ double lds[4 * 3 * G];
     const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    double sums[8] = {0, 0, 0, 0, 0, 0, 0, 0}, maxs[4] = {0, 0, 0, 0};
+    double sums[2 * G], maxs[G];
+    for (int k = 0; k < G; k++) sums[k] = sums[G + k] = maxs[k] = 0.0;
     if (s < c.nsites && !site_frozen(c, s)) {
         const long long N = c.nsites;
         const int Ns = c.ns;
@@ -344,13 +454,20 @@ __global__ void __launch_bounds__(256) twoslot_gauss(Coef c, const T* __restrict
         const T* sf = S + (long long)fut * N * Ns + s * Ns;
         const T* Vn = V + (long long)now * N * (3 * VD);
         const T* Vf = V + (long long)fut * N * (3 * VD);
-        double rho[4] = {0, 0, 0, 0};
+        double rho[G];
+        for (int k = 0; k < G; k++) rho[k] = 0.0;
         for (int k = 0; k < c.nd; k++) {
             double t[4], n[4];
             for (int m = 0; m < 4; m++) { t[m] = static_cast<double>(sn[4 * k + m]); n[m] = static_cast<double>(sf[4 * k + m]); }
             bilinears_add(t, n, rho);
         }
-        double div[4] = {0, 0, 0, 0}, mag[4] = {0, 0, 0, 0};
+        if constexpr (G > 4) {
+            double t8[8], n8[8];
+            for (int m = 0; m < 8; m++) { t8[m] = static_cast<double>(sn[m]); n8[m] = static_cast<double>(sf[m]); }
+            for (int k = 0; k < 2; k++) family_bilinears_add<G>(t8, n8, 2 * k, rho);
+        }
+        double div[G], mag[G];
+        for (int k = 0; k < G; k++) div[k] = mag[k] = 0.0;
         for (int a = 0; a < 3; a++) {
             const long long xm = site_shift(c, s, a, -1);
             const T* Ln = Vn + s * (3 * VD) + a * VD;
@@ -362,34 +479,43 @@ __global__ void __launch_bounds__(256) twoslot_gauss(Coef c, const T* __restrict
             div[0] += (sin(la) - sin(na)) * c.inv_sp[a];
             mag[0] += (fabs(sin(la)) + fabs(sin(na))) * c.inv_sp[a];
             double P1[4], P2[4], Um[4];
-            link_rep_ref<VD>(Lf, false, c.tol, P1);
-            link_rep_ref<VD>(Ln, true, c.tol, Um);
+            link_rep_ref<GEN>(Lf, false, c.tol, P1);
+            link_rep_ref<GEN>(Ln, true, c.tol, Um);
             su2_product(Um, P1, false, P1);
-            link_rep_ref<VD>(Mn, true, c.tol, P2);
-            link_rep_ref<VD>(Mf, false, c.tol, Um);
+            link_rep_ref<GEN>(Mn, true, c.tol, P2);
+            link_rep_ref<GEN>(Mf, false, c.tol, Um);
             su2_product(Um, P2, false, P2);
             for (int k = 1; k < 4; k++) {
                 div[k] += (P1[k] - P2[k]) * c.inv_sp[a];
                 mag[k] += (fabs(P1[k]) + fabs(P2[k])) * c.inv_sp[a];
             }
+            if constexpr (G == 5) {
+                const double lf = static_cast<double>(Lf[OFF]) - static_cast<double>(Ln[OFF]);
+                const double nf = static_cast<double>(Mf[OFF]) - static_cast<double>(Mn[OFF]);
+                div[4] += (sin(lf) - sin(nf)) * c.inv_sp[a];
+                mag[4] += (fabs(sin(lf)) + fabs(sin(nf))) * c.inv_sp[a];
+            } else if constexpr (G == 7) {
+                link_rep_ref<GEN>(Lf + OFF - 1, false, c.tol, P1);
+                link_rep_ref<GEN>(Ln + OFF - 1, true, c.tol, Um);
+                su2_product(Um, P1, false, P1);
+                link_rep_ref<GEN>(Mn + OFF - 1, true, c.tol, P2);
+                link_rep_ref<GEN>(Mf + OFF - 1, false, c.tol, Um);
+                su2_product(Um, P2, false, P2);
+                for (int k = 1; k < 4; k++) {
+                    div[3 + k] += (P1[k] - P2[k]) * c.inv_sp[a];
+                    mag[3 + k] += (fabs(P1[k]) + fabs(P2[k])) * c.inv_sp[a];
+                }
+            }
         }
-        for (int k = 0; k < 4; k++) {
+        for (int k = 0; k < G; k++) {
             const double g = div[k] - c.gsq[k] * rho[k];
             sums[k] = fabs(g);
-            sums[4 + k] = mag[k] + fabs(c.gsq[k] * rho[k]);
+            sums[G + k] = mag[k] + fabs(c.gsq[k] * rho[k]);
             maxs[k] = fabs(g);
         }
     }
-    block_reduce_store<8, 4>(sums, maxs, lds, partial, blockIdx.x);
+    block_reduce_store<2 * G, G>(sums, maxs, lds, partial, blockIdx.x);
 }
 
-template __global__ void twoslot_step<float, 5>(Coef, float*, float*, unsigned, int, double, double*, unsigned*);
-template __global__ void twoslot_step<double, 5>(Coef, double*, double*, unsigned, int, double, double*, unsigned*);
-template __global__ void twoslot_step<float, 4>(Coef, float*, float*, unsigned, int, double, double*, unsigned*);
-template __global__ void twoslot_step<double, 4>(Coef, double*, double*, unsigned, int, double, double*, unsigned*);
-template __global__ void twoslot_gauss<float, 5>(Coef, const float*, const float*, unsigned, double*);
-template __global__ void twoslot_gauss<double, 5>(Coef, const double*, const double*, unsigned, double*);
-template __global__ void twoslot_gauss<float, 4>(Coef, const float*, const float*, unsigned, double*);
-template __global__ void twoslot_gauss<double, 4>(Coef, const double*, const double*, unsigned, double*);
 
 }  // namespace gft

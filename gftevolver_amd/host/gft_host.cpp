@@ -149,14 +149,15 @@ StandardModel::StandardModel(const std::string& root, unsigned ns, unsigned nv) 
                                  "multiplied by 3 internally.");
 }
 
-TwoHDM::TwoHDM(const std::string& root, unsigned ns, unsigned nv) {
+TwoHDM::TwoHDM(const std::string& root, unsigned ns, unsigned nv, bool generator) {
     ConfigReader r(cfg(root, "Gradients", "TwoHDM.cfg"));
     r.skip_lines(12);
     divisionByZeroTolerance = r.value<double>();
     if (ns != 8)
         throw std::runtime_error("GRADIENTS::TWOHDM:: 8 scalar components are required but " + std::to_string(ns)
                                  + " have been assigned.");
-    (void)nv;   // checked against the representation by WilsonLoops::TwoHDM, which is built first
+    // TwoHDM.cpp:215-242 (WilsonLoops::TwoHDM, built first, has checked the count).
+    gaugeNum = generator ? (nv == 12 ? 4 : nv == 15 ? 5 : 7) : (nv == 15 ? 4 : nv == 18 ? 5 : 7);
 }
 }  // namespace Gradients
 
@@ -200,13 +201,12 @@ TwoHDM::TwoHDM(const std::string& root, unsigned nv) {
               "27 (SM x SU(2)) vector components are required but " + std::to_string(nv) + " have been assigned.\n"
               "Note that 3 spatial dimensions are assumed so requested number of components is multiplied by 3 "
               "internally.");
-    if (nv != (usingGeneratorRepresentation ? 12u : 15u))
-        throw std::runtime_error("WILSONLOOPS::TWOHDM:: the GPU evolver supports the SM gauge group (gaugeNum 4); "
-                                 "the Higgs-family gauge fields (gaugeNum 5 and 7) are not built");
+    gaugeNum = usingGeneratorRepresentation ? (nv == 12 ? 4 : nv == 15 ? 5 : 7) : (nv == 15 ? 4 : nv == 18 ? 5 : 7);
 }
 
-float TwoHDM::getSqrCouplings(unsigned c) const {
-    return static_cast<float>(c % 4 == 0 ? gp * gp : g * g);
+float TwoHDM::getSqrCouplings(unsigned c) const {   // TwoHDM.cpp:258-267
+    const unsigned m = c % gaugeNum;
+    return static_cast<float>(m == 0 ? gp * gp : m / 4 == 0 ? g * g : gpp * gpp);
 }
 }  // namespace WilsonLoops
 
@@ -235,7 +235,9 @@ void Model::configure(const std::string& model_cfg, const std::string& root, uns
     }
     switch (gradientType) {
     case SM_GRADIENT: gradient = std::make_unique<Gradients::StandardModel>(root, ns, nv); break;
-    case TWOHDM_GRADIENT: gradient = std::make_unique<Gradients::TwoHDM>(root, ns, nv); break;
+    case TWOHDM_GRADIENT:
+        gradient = std::make_unique<Gradients::TwoHDM>(root, ns, nv, wilsonLoop->isUsingGeneratorRepresentation());
+        break;
     default:
         throw std::runtime_error("MODEL::ERROR: The chosen gradient type (" + std::to_string(gradientType) + ") is invalid.");
     }
@@ -270,6 +272,8 @@ void Model::fill(gft_desc& d) const {
     d.gp = wilsonLoop->gp;
     d.div_tol = wilsonLoop->divisionByZeroTolerance;
     d.gen_rep = wilsonLoop->isUsingGeneratorRepresentation() ? 1 : 0;
+    d.gauge_num = static_cast<int>(wilsonLoop->getNumberOfConstraintEquations());
+    if (auto* thdm = dynamic_cast<const WilsonLoops::TwoHDM*>(wilsonLoop.get())) d.gpp = thdm->gpp;
     potential->fill(d);
     d.damping = dampingFactor;
     d.nt_damped = ntDamped;
@@ -473,9 +477,14 @@ void Lattice::initFields() {
     const size_t N = static_cast<size_t>(nx) * ny * nz;
     scalarFields.assign(2 * N * numScalarFieldComponents, 0.f);
     vectorFields.assign(2 * N * numVectorFieldComponents, 0.f);
-    if (model.isUsingQuaternionRepresentation())   // identity links, Lattice.cpp:369-395
+    if (model.isUsingQuaternionRepresentation()) {   // identity links, Lattice.cpp:369-395
+        const unsigned width = numVectorFieldComponents / 3;
         for (size_t i = 0; i < 2 * N; i++)
-            for (unsigned d = 0; d < 3; d++) vectorFields[i * numVectorFieldComponents + d * 5 + 1] = 1.f;
+            for (unsigned d = 0; d < 3; d++) {
+                vectorFields[i * numVectorFieldComponents + d * width + 1] = 1.f;
+                if (width == 9) vectorFields[i * numVectorFieldComponents + d * width + 5] = 1.f;   // family SU(2)
+            }
+    }
     if (scalarInitialConditions) scalarInitialConditions->setInitialFields(scalarFields);
     if (vectorInitialConditions) vectorInitialConditions->setInitialFields(vectorFields);
 }

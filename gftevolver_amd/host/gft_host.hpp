@@ -104,10 +104,11 @@ public:
     StandardModel(const std::string& config_root, unsigned num_scalar, unsigned num_vector);
     unsigned getNumberOfConstraintEquations() const override { return 4; }
 };
-class TwoHDM : public Gradient {          // Gradients/Source/TwoHDM.cpp:200-246 (gaugeNum 4 only)
+class TwoHDM : public Gradient {          // Gradients/Source/TwoHDM.cpp:200-246
 public:
-    TwoHDM(const std::string& config_root, unsigned num_scalar, unsigned num_vector);
-    unsigned getNumberOfConstraintEquations() const override { return 4; }
+    TwoHDM(const std::string& config_root, unsigned num_scalar, unsigned num_vector, bool generator);
+    unsigned getNumberOfConstraintEquations() const override { return gaugeNum; }
+    unsigned gaugeNum = 4;
 };
 }  // namespace Gradients
 
@@ -119,13 +120,14 @@ public:
     unsigned getNumberOfConstraintEquations() const override { return 4; }
     float getSqrCouplings(unsigned c) const override;
 };
-class TwoHDM : public WilsonLoop {        // WilsonLoops/Source/TwoHDM.cpp:5-60, gaugeNum 4
+class TwoHDM : public WilsonLoop {        // WilsonLoops/Source/TwoHDM.cpp:5-60, 215-280
 public:
     TwoHDM(const std::string& config_root, unsigned num_vector);
-    unsigned getNumberOfEvolutionEquations() const override { return 12; }
-    unsigned getNumberOfConstraintEquations() const override { return 4; }
+    unsigned getNumberOfEvolutionEquations() const override { return 3 * gaugeNum; }
+    unsigned getNumberOfConstraintEquations() const override { return gaugeNum; }
     float getSqrCouplings(unsigned c) const override;
     double gpp = 0;
+    unsigned gaugeNum = 4;   // 4: SU(2) x U(1); 5, 7: with a Higgs-family U(1) or SU(2)
 };
 }  // namespace WilsonLoops
 

@@ -36,7 +36,8 @@ int main(int argc, char** argv) {
                       << "damping " << d.damping << "\nnt_damped " << d.nt_damped << "\n"
                       << "progress " << lattice.progressReport << "\n"
                       << "fixed_faces " << lattice.fixedFaces << "\n"
-                      << "gen_rep " << d.gen_rep << "\n";
+                      << "gen_rep " << d.gen_rep << "\n"
+                      << "gauge_num " << d.gauge_num << "\ngpp " << d.gpp << "\n";
             double s0 = 0;
             for (float x : lattice.scalarFields) s0 += x;
             std::cout << "scalar_sum " << s0 << "\n";

@@ -11,8 +11,8 @@ import os
 
 import numpy as np
 
-from ._abi import (GFT_FORM_MOMENTUM_F32, GFT_FORM_MOMENTUM_F64, GFT_FORM_TWO_SLOT_F32, GFT_FORM_TWO_SLOT_F64, NV,
-                   GftDesc, GftGrfDesc, GftLoops, GftMlpDesc, GftObs)
+from ._abi import (GFT_FORM_MOMENTUM_F32, GFT_FORM_MOMENTUM_F64, GFT_FORM_TWO_SLOT_F32, GFT_FORM_TWO_SLOT_F64,
+                   GftDesc, GftGrfDesc, GftLoops, GftMlpDesc, GftObs, link_reals)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libgft.so")
@@ -99,7 +99,7 @@ class Evolver:
     def __init__(self, desc: GftDesc, _handle=None):
         self.desc = desc
         self.ns = 8 if desc.model == 1 else 4
-        self.nv = 12 if desc.gen_rep else NV
+        self.nv = link_reals(desc)
         self.n = (desc.nx, desc.ny, desc.nz)
         self.nsites = desc.nx * desc.ny * desc.nz
         self._owned = _handle is None

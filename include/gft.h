@@ -92,7 +92,18 @@ typedef struct gft_desc {
      * The two-slot forms step the stored form as the reference does (convert to a
      * quaternion, multiply, convert back); the momentum forms convert at import/export. */
     int      gen_rep;
+    /* Gauge group of the 2HDM (WilsonLoops/Source/TwoHDM.cpp:215-242, gaugeNum): 0 or 4 =
+     * SU(2) x U(1); 5 adds a Higgs-family U(1) (one more phase y' per link, opposite charge
+     * on the two doublets); 7 adds a Higgs-family SU(2) acting on the doublet index.  Link
+     * reals per direction: 4 -> 5 | 4 (quaternion | generator form), 5 -> 6 | 5
+     * ([.., y']), 7 -> 9 | 7 ([.., c0'..c3'] | [.., w1'..w3']).  5 and 7 need GFT_MODEL_2HDM
+     * and a two-slot form.  gpp is the Higgs-family coupling g'' (TwoHDM.cfg).          */
+    int      gauge_num;
+    double   gpp;
 } gft_desc;
+
+/* Most Gauss-law generators of any gauge group (gaugeNum 7). */
+#define GFT_MAX_EQ 7
 
 /* Observables of one measured step.  energy[] columns follow the reference's
  * global_quantities.dat order (Analysers/Source/Energy.cpp:169-189) and are
@@ -103,11 +114,11 @@ typedef struct gft_obs {
     unsigned time_iter;      /* step index the row belongs to                                  */
     unsigned has_energy;     /* energy[] valid (measured before the update of this step)       */
     unsigned has_gauss;      /* gauss_*[] valid (measured after the update of this step)       */
-    unsigned num_eq;         /* number of Gauss generators (4)                                 */
+    unsigned num_eq;         /* number of Gauss generators: gaugeNum (4, 5 or 7)               */
     double   energy[6];      /* total, potential, gradient, kinetic, magnetic, electric        */
-    double   gauss_int[4];   /* sum_x |G^a(x)| dx dy dz, a = Y, 1, 2, 3                        */
-    double   gauss_max[4];   /* max_x |G^a(x)|                                                 */
-    double   gauss_scale[4]; /* scale of the relative violation: sum_x of the magnitudes of the
+    double   gauss_int[GFT_MAX_EQ];   /* sum_x |G^a(x)| dx dy dz, a = Y, 1, 2, 3, family..     */
+    double   gauss_max[GFT_MAX_EQ];   /* max_x |G^a(x)|                                        */
+    double   gauss_scale[GFT_MAX_EQ]; /* scale of the relative violation: sum_x of the magnitudes of the
                                 terms that cancel in G^a, sum_i (|E term at x| + |E term at x-i|)
                                 / dx_i^2 + g_a^2 |rho^a|, times dx dy dz                      */
 } gft_obs;

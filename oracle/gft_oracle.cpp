@@ -64,7 +64,7 @@ struct Params {
     double inv_sp[3];     // WilsonLoops inverse_sqr_spacings (StandardModel.cpp:51-53)
     double inv_sqr_dt;
     double g_sqr, gp_sqr, inv_g_sqr, inv_gp_sqr;
-    float gsq_f[4];       // getSqrCouplings returns float (StandardModel.cpp:224-230)
+    float gsq_f[7];       // getSqrCouplings returns float (StandardModel.cpp:224-230)
     const gft_desc* d;
     float damping = 0.f;
     unsigned nt_damped = 0;
@@ -72,6 +72,11 @@ struct Params {
     bool gen = false;     // generator representation of the SU(2) links
     int vd = 5, nv = 15;  // reals per link direction and per site: [y, c0..c3] or [y, w1..w3]
     double tol = 1e-9;    // division-by-zero tolerance of the unit-vector conversions
+    // Gauge group (WilsonLoops/Source/TwoHDM.cpp:215-242): G = 4 for the SM
+    // group, 5 adds a Higgs-family U(1) (phase y' at link[off]), 7 a
+    // Higgs-family SU(2) (link[off..] in the same form as the isospin SU(2)).
+    int G = 4, off = 0;
+    double inv_gpp_sqr = 0.0;
 };
 
 Params make_params(const gft_desc* d) {
@@ -96,13 +101,20 @@ Params make_params(const gft_desc* d) {
     p.gp_sqr = d->gp * d->gp;
     p.inv_g_sqr = p.g_sqr > d->div_tol ? 1.0 / p.g_sqr : 0.0;
     p.inv_gp_sqr = p.gp_sqr > d->div_tol ? 1.0 / p.gp_sqr : 0.0;
+    // getSqrCouplings returns float: g'^2 for generator 0, g^2 for 1..3, g''^2
+    // for the Higgs-family generators (TwoHDM.cpp:258-267).
     p.gsq_f[0] = static_cast<float>(p.gp_sqr);
     p.gsq_f[1] = p.gsq_f[2] = p.gsq_f[3] = static_cast<float>(p.g_sqr);
+    const double gpp_sqr = d->gpp * d->gpp;
+    p.gsq_f[4] = p.gsq_f[5] = p.gsq_f[6] = static_cast<float>(gpp_sqr);
+    p.inv_gpp_sqr = gpp_sqr > d->div_tol ? 1.0 / gpp_sqr : 0.0;
     p.damping = d->damping;
     p.nt_damped = d->nt_damped;
     p.fixed = d->fixed_faces;
     p.gen = d->gen_rep != 0;
-    p.vd = p.gen ? 4 : 5;
+    p.G = (d->gauge_num == 5 || d->gauge_num == 7) ? d->gauge_num : 4;
+    p.off = p.gen ? 4 : 5;
+    p.vd = p.off + (p.G == 5 ? 1 : p.G == 7 ? (p.gen ? 3 : 4) : 0);
     p.nv = 3 * p.vd;
     p.tol = d->div_tol;
     return p;
@@ -196,9 +208,42 @@ inline void qmul(const double* a, const double* b, bool trace, double* out) {
 
 // transform: U(1) phase on the pairs (0,1),(2,3),... then the SU(2) real
 // 4x4 action per doublet (Gradients/Source/StandardModel.cpp:72-100;
-// TwoHDM.cpp:72-104 for two doublets).
+// TwoHDM.cpp:72-104 for two doublets), then for G = 5 the Higgs-family phase,
+// opposite on the two doublets, or for G = 7 the Higgs-family SU(2) acting on
+// (f_i, f_i+1, f_i+4, f_i+5), i = 0, 2 (TwoHDM.cpp:106-141).
+template <class T>
+inline void transform_sm(const Params& p, const double* sf, int nd, const T* link, bool conj, double* out);
+
 template <class T>
 inline void transform(const Params& p, const double* sf, int nd, const T* link, bool conj, double* out) {
+    if (p.G == 4) {
+        transform_sm(p, sf, nd, link, conj, out);
+        return;
+    }
+    double sm[8];
+    transform_sm(p, sf, nd, link, conj, sm);
+    const int cf = conj ? -1 : 1;
+    if (p.G == 5) {
+        const double y2 = static_cast<double>(link[p.off]);
+        for (int c = 0; c < 8; c++) {
+            const int pm = -1 + 2 * (c % 2);
+            const int pm4 = 1 - 2 * (c / 4);
+            out[c] = std::cos(y2) * sm[c] + pm * std::sin(pm4 * cf * y2) * sm[c - pm];
+        }
+        return;
+    }
+    double U[4];
+    rep(p, link + p.off - 1, conj, U);
+    for (int i = 0; i < 4; i += 2) {
+        out[i] = U[0] * sm[i] - U[3] * sm[i + 1] + U[2] * sm[i + 4] - U[1] * sm[i + 5];
+        out[i + 1] = U[3] * sm[i] + U[0] * sm[i + 1] + U[1] * sm[i + 4] + U[2] * sm[i + 5];
+        out[i + 4] = -U[2] * sm[i] - U[1] * sm[i + 1] + U[0] * sm[i + 4] + U[3] * sm[i + 5];
+        out[i + 5] = U[1] * sm[i] - U[2] * sm[i + 1] - U[3] * sm[i + 4] + U[0] * sm[i + 5];
+    }
+}
+
+template <class T>
+inline void transform_sm(const Params& p, const double* sf, int nd, const T* link, bool conj, double* out) {
     const int cf = conj ? -1 : 1;
     const double y = static_cast<double>(link[0]);
     const double cy = std::cos(y);
@@ -226,6 +271,23 @@ inline void bilinears(const double* t, const double* n, double* j) {
     j[1] = 0.25 * (t[0] * n[3] - t[1] * n[2] + t[2] * n[1] - t[3] * n[0]);
     j[2] = 0.25 * (-t[0] * n[2] - t[1] * n[3] + t[2] * n[0] + t[3] * n[1]);
     j[3] = 0.25 * (t[0] * n[1] - t[1] * n[0] - t[2] * n[3] + t[3] * n[2]);
+}
+
+// Higgs-family bilinears of one upper/lower pair i in {0, 2} of both doublets
+// (TwoHDM.cpp:401-424 currents, 299-319 Gauss charges), accumulated into j[4..G-1].
+// R is the arithmetic type of the products (double for the currents, float for
+// the reference's Gauss charges, which multiply the stored floats) and A that
+// of the accumulator (the reference's contribution vectors are double for the
+// currents, float for the Gauss charges).
+template <class R, class A>
+inline void family_bilinears_add(int G, const R* t, const R* n, int i, A* j) {
+    if (G == 5) {
+        j[4] = static_cast<A>(j[4] + 0.25 * (t[i] * n[i + 1] - t[i + 1] * n[i] - t[i + 4] * n[i + 5] + t[i + 5] * n[i + 4]));
+    } else if (G == 7) {
+        j[4] = static_cast<A>(j[4] + 0.25 * (t[i] * n[i + 5] - t[i + 1] * n[i + 4] + t[i + 4] * n[i + 1] - t[i + 5] * n[i]));
+        j[5] = static_cast<A>(j[5] + 0.25 * (-t[i] * n[i + 4] - t[i + 1] * n[i + 5] + t[i + 4] * n[i] + t[i + 5] * n[i + 1]));
+        j[6] = static_cast<A>(j[6] + 0.25 * (t[i] * n[i + 1] - t[i + 1] * n[i] - t[i + 4] * n[i + 5] + t[i + 5] * n[i + 4]));
+    }
 }
 
 // Potential derivative: SO_N (SO_N.cpp:57-75) or Double_SO_N (Double_SO_N.cpp:157-254).
@@ -402,28 +464,30 @@ void site_update(const Params& p, T* S, T* V, long s, unsigned now, bool store_e
     }
 
     // Currents (Gradients calcCurrents).
-    double cur[12];
+    const int G = p.G;
+    double cur[21];
     for (int a = 0; a < 3; a++) {
         double tf[8], nb[8];
         transform(p, phi, nd, vn + a * p.vd, false, tf);
         const T* q = Sn + xp[a] * Ns;
         for (int c = 0; c < Ns; c++) nb[c] = static_cast<double>(q[c]);
         if (nd == 1) {
-            bilinears(tf, nb, cur + 4 * a);
+            bilinears(tf, nb, cur + G * a);
         } else {
-            double acc[4] = {0.0, 0.0, 0.0, 0.0};
+            double acc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
             for (int k = 0; k < 2; k++) {
                 double j[4];
                 bilinears(tf + 4 * k, nb + 4 * k, j);
                 for (int m = 0; m < 4; m++) acc[m] += j[m];
+                family_bilinears_add(G, tf, nb, 2 * k, acc);
             }
-            for (int m = 0; m < 4; m++) cur[4 * a + m] = acc[m];
+            for (int m = 0; m < G; m++) cur[G * a + m] = acc[m];
         }
     }
 
     // Magnetic (WilsonLoops calcMagneticContributions).
-    double mag[12];
-    for (int k = 0; k < 12; k++) mag[k] = 0.0;
+    double mag[21];
+    for (int k = 0; k < 21; k++) mag[k] = 0.0;
     float stored_mag = 0.f;   // mutable float member, StandardModelLoops.hpp:28
     double mag_e = 0.0;
     for (int i = 0; i < 3; i++) {
@@ -438,7 +502,7 @@ void site_update(const Params& p, T* S, T* V, long s, unsigned now, bool store_e
                                 + static_cast<double>(L_x[j * p.vd]) - static_cast<double>(L_x[i * p.vd]);
             const double th_nb = -static_cast<double>(L_xpimj[j * p.vd]) - static_cast<double>(L_xmj[i * p.vd])
                                + static_cast<double>(L_xmj[j * p.vd]) + static_cast<double>(L_x[i * p.vd]);
-            mag[4 * i] += (std::sin(th_loc) - std::sin(th_nb)) * p.inv_sp[j];
+            mag[G * i] += (std::sin(th_loc) - std::sin(th_nb)) * p.inv_sp[j];
 
             double P1[4], P2[4], Um[4];
             rep(p, L_x + i * p.vd, true, P1);
@@ -457,7 +521,7 @@ void site_update(const Params& p, T* S, T* V, long s, unsigned now, bool store_e
             rep(p, L_x + i * p.vd, false, Um);
             qmul(Um, P2, false, P2);
 
-            for (int k = 1; k < 4; k++) mag[4 * i + k] += (P1[k] - P2[k]) * p.inv_sp[j];
+            for (int k = 1; k < 4; k++) mag[G * i + k] += (P1[k] - P2[k]) * p.inv_sp[j];
 
             if (store_energy) {
                 const double e = 2.0 * p.inv_sp[i] * p.inv_sp[j]
@@ -465,26 +529,83 @@ void site_update(const Params& p, T* S, T* V, long s, unsigned now, bool store_e
                 stored_mag = static_cast<float>(stored_mag + e);
                 mag_e += e;
             }
+
+            // Higgs-family loops (TwoHDM.cpp:496-552), after the SM ones.
+            const int o = p.off;
+            if (G == 5) {
+                const double f_loc = -static_cast<double>(L_xpi[j * p.vd + o]) + static_cast<double>(L_xpj[i * p.vd + o])
+                                   + static_cast<double>(L_x[j * p.vd + o]) - static_cast<double>(L_x[i * p.vd + o]);
+                const double f_nb = -static_cast<double>(L_xpimj[j * p.vd + o]) - static_cast<double>(L_xmj[i * p.vd + o])
+                                  + static_cast<double>(L_xmj[j * p.vd + o]) + static_cast<double>(L_x[i * p.vd + o]);
+                mag[G * i + 4] += (std::sin(f_loc) - std::sin(f_nb)) * p.inv_sp[j];
+                if (store_energy) {
+                    const double e = 2.0 * p.inv_sp[i] * p.inv_sp[j] * p.inv_gpp_sqr * (1.0 - std::cos(f_loc));
+                    stored_mag = static_cast<float>(stored_mag + e);
+                    mag_e += e;
+                }
+            } else if (G == 7) {
+                rep(p, L_x + i * p.vd + o - 1, true, P1);
+                rep(p, L_x + j * p.vd + o - 1, false, Um);
+                qmul(Um, P1, true, P1);
+                rep(p, L_xpj + i * p.vd + o - 1, false, Um);
+                qmul(Um, P1, true, P1);
+                rep(p, L_xpi + j * p.vd + o - 1, true, Um);
+                qmul(Um, P1, store_energy, P1);
+                rep(p, L_xpimj + j * p.vd + o - 1, true, P2);
+                rep(p, L_xmj + i * p.vd + o - 1, true, Um);
+                qmul(Um, P2, true, P2);
+                rep(p, L_xmj + j * p.vd + o - 1, false, Um);
+                qmul(Um, P2, true, P2);
+                rep(p, L_x + i * p.vd + o - 1, false, Um);
+                qmul(Um, P2, false, P2);
+                for (int k = 1; k < 4; k++) mag[G * i + 3 + k] += (P1[k] - P2[k]) * p.inv_sp[j];
+                if (store_energy) {
+                    const double e = 2.0 * p.inv_sp[i] * p.inv_sp[j] * p.inv_gpp_sqr * (1.0 - P1[0]);
+                    stored_mag = static_cast<float>(stored_mag + e);
+                    mag_e += e;
+                }
+            }
         }
     }
 
     // Electric (WilsonLoops calcElectricContributions).
-    double el[12];
+    double el[21];
     float stored_el = 0.f;
     double el_e = 0.0;
     const T* vpast = V + static_cast<long>(past) * N * p.nv + s * p.nv;
     for (int a = 0; a < 3; a++) {
         const double la = static_cast<double>(vn[a * p.vd]) - static_cast<double>(vpast[a * p.vd]);
-        el[4 * a] = std::sin(la);
+        el[G * a] = std::sin(la);
         double Up[4], Um[4];
         rep(p, vpast + a * p.vd, true, Up);
         rep(p, vn + a * p.vd, false, Um);
         qmul(Um, Up, store_energy, Up);
-        for (int k = 1; k < 4; k++) el[4 * a + k] = Up[k];
+        for (int k = 1; k < 4; k++) el[G * a + k] = Up[k];
         if (store_energy) {
             const double e = 4.0 * p.inv_sp[a] * (p.inv_g_sqr * (1.0 - Up[0]) + p.inv_gp_sqr * (1.0 - std::cos(la)));
             stored_el = static_cast<float>(stored_el + e);
             el_e += e;
+        }
+        // Higgs family (TwoHDM.cpp:600-634).
+        const int o = p.off;
+        if (G == 5) {
+            const double lf = static_cast<double>(vn[a * p.vd + o]) - static_cast<double>(vpast[a * p.vd + o]);
+            el[G * a + 4] = std::sin(lf);
+            if (store_energy) {
+                const double e = 4.0 * p.inv_sp[a] * p.inv_gpp_sqr * (1.0 - std::cos(lf));
+                stored_el = static_cast<float>(stored_el + e);
+                el_e += e;
+            }
+        } else if (G == 7) {
+            rep(p, vpast + a * p.vd + o - 1, true, Up);
+            rep(p, vn + a * p.vd + o - 1, false, Um);
+            qmul(Um, Up, store_energy, Up);
+            for (int k = 1; k < 4; k++) el[G * a + 3 + k] = Up[k];
+            if (store_energy) {
+                const double e = 4.0 * p.inv_sp[a] * p.inv_gpp_sqr * (1.0 - Up[0]);
+                stored_el = static_cast<float>(stored_el + e);
+                el_e += e;
+            }
         }
     }
 
@@ -560,24 +681,39 @@ void site_update(const Params& p, T* S, T* V, long s, unsigned now, bool store_e
                          - damping_step * (static_cast<double>(sn[c]) - static_cast<double>(sp_[c]));
         sp_[c] = static_cast<T>(2 * static_cast<double>(sn[c]) - static_cast<double>(sp_[c]) + rhs);
     }
-    double R[12];
-    for (int c = 0; c < 12; c++)
+    double R[21];
+    for (int c = 0; c < 3 * G; c++)
         R[c] = (1.0 - damping_step) * el[c]
-             + p.dt * p.dt * (mag[c] + static_cast<double>(p.gsq_f[c % 4]) * cur[c]);
+             + p.dt * p.dt * (mag[c] + static_cast<double>(p.gsq_f[c % G]) * cur[c]);
 
-    // WilsonLoops::StandardModel::evolve.
+    // WilsonLoops::StandardModel::evolve, TwoHDM::evolve (TwoHDM.cpp:640-755).
     for (int a = 0; a < 3; a++) {
-        vp_[a * p.vd] = static_cast<T>(std::asin(R[4 * a]) + static_cast<double>(vn[a * p.vd]));
-        if (R[4 * a] * R[4 * a] > 1) __atomic_add_fetch(panics, 1UL, __ATOMIC_RELAXED);
+        const double* Ra = R + G * a;
+        vp_[a * p.vd] = static_cast<T>(std::asin(Ra[0]) + static_cast<double>(vn[a * p.vd]));
+        if (Ra[0] * Ra[0] > 1) __atomic_add_fetch(panics, 1UL, __ATOMIC_RELAXED);
         double cms = 0.0;
-        for (int k = 1; k < 4; k++) cms += R[4 * a + k] * R[4 * a + k];
+        for (int k = 1; k < 4; k++) cms += Ra[k] * Ra[k];
         if (cms > 1) __atomic_add_fetch(panics, 1UL, __ATOMIC_RELAXED);
-        const double u[4] = {std::sqrt(1.0 - cms), R[4 * a + 1], R[4 * a + 2], R[4 * a + 3]};
+        const double u[4] = {std::sqrt(1.0 - cms), Ra[1], Ra[2], Ra[3]};
         double Up[4];
         rep(p, vn + a * p.vd, false, Up);
         qmul(u, Up, true, Up);
         if (p.gen) invert_rep(p, Up, vp_ + a * p.vd + 1);
         else for (int k = 0; k < 4; k++) vp_[a * p.vd + 1 + k] = static_cast<T>(Up[k]);
+        const int o = p.off;
+        if (G == 5) {
+            vp_[a * p.vd + o] = static_cast<T>(std::asin(Ra[4]) + static_cast<double>(vn[a * p.vd + o]));
+            if (Ra[4] * Ra[4] > 1) __atomic_add_fetch(panics, 1UL, __ATOMIC_RELAXED);
+        } else if (G == 7) {
+            double cf = 0.0;
+            for (int k = 4; k < 7; k++) cf += Ra[k] * Ra[k];
+            if (cf > 1) __atomic_add_fetch(panics, 1UL, __ATOMIC_RELAXED);
+            const double uf[4] = {std::sqrt(1.0 - cf), Ra[4], Ra[5], Ra[6]};
+            rep(p, vn + a * p.vd + o - 1, false, Up);
+            qmul(uf, Up, true, Up);
+            if (p.gen) invert_rep(p, Up, vp_ + a * p.vd + o);
+            else for (int k = 0; k < 4; k++) vp_[a * p.vd + o + k] = static_cast<T>(Up[k]);
+        }
     }
 }
 
@@ -593,7 +729,8 @@ void site_gauss(const Params& p, const T* S, const T* V, long s, unsigned now, A
     const T* sf = S + static_cast<long>(fut) * N * Ns + s * Ns;
     const T* Vn = V + static_cast<long>(now) * N * p.nv;
     const T* Vf = V + static_cast<long>(fut) * N * p.nv;
-    A grad[4] = {0, 0, 0, 0};
+    const int G = p.G;
+    A grad[7] = {0, 0, 0, 0, 0, 0, 0};
     if constexpr (sizeof(A) == sizeof(float)) {
         // Gradients calcConstraintContributions: float products (StandardModel.cpp:250-270).
         for (int k = 0; k < nd; k++) {
@@ -613,19 +750,25 @@ void site_gauss(const Params& p, const T* S, const T* V, long s, unsigned now, A
                 const double e3 = 0.25 * (a0 * b1 - a1 * b0 - a2 * b3 + a3 * b2);
                 grad[0] = static_cast<float>(grad[0] + e0); grad[1] = static_cast<float>(grad[1] + e1);
                 grad[2] = static_cast<float>(grad[2] + e2); grad[3] = static_cast<float>(grad[3] + e3);
+                // Higgs-family charges (TwoHDM.cpp:299-319), float products.
+                float af[8], bf[8];
+                for (int m = 0; m < 8; m++) { af[m] = static_cast<float>(sn[m]); bf[m] = static_cast<float>(sf[m]); }
+                family_bilinears_add(G, af, bf, 2 * k, grad);
             }
         }
     } else {
+        double t8[8], n8[8];
+        for (int m = 0; m < 4 * nd; m++) { t8[m] = static_cast<double>(sn[m]); n8[m] = static_cast<double>(sf[m]); }
         for (int k = 0; k < nd; k++) {
-            double t[4], n[4], j[4];
-            for (int m = 0; m < 4; m++) { t[m] = static_cast<double>(sn[4 * k + m]); n[m] = static_cast<double>(sf[4 * k + m]); }
-            bilinears(t, n, j);
+            double j[4];
+            bilinears(t8 + 4 * k, n8 + 4 * k, j);
             for (int m = 0; m < 4; m++) grad[m] += j[m];
+            if (nd == 2) family_bilinears_add(G, t8, n8, 2 * k, grad);
         }
     }
     // WilsonLoops calcConstraintContributions (StandardModel.cpp:274-322).
-    A elc[4] = {0, 0, 0, 0};
-    double divabs[4] = {0, 0, 0, 0};
+    A elc[7] = {0, 0, 0, 0, 0, 0, 0};
+    double divabs[7] = {0, 0, 0, 0, 0, 0, 0};
     for (int a = 0; a < 3; a++) {
         const long xm = shift(p, s, a, -1);
         const T* Ln = Vn + s * p.nv + a * p.vd;
@@ -649,8 +792,28 @@ void site_gauss(const Params& p, const T* S, const T* V, long s, unsigned now, A
             elc[k] += static_cast<A>(dk);
             divabs[k] += (std::fabs(P1[k]) + std::fabs(P2[k])) * p.inv_sp[a];
         }
+        // Higgs family (TwoHDM.cpp:352-386).
+        const int o = p.off;
+        if (G == 5) {
+            const double lf = static_cast<double>(Lf[o]) - static_cast<double>(Ln[o]);
+            const double nf = static_cast<double>(Mf[o]) - static_cast<double>(Mn[o]);
+            elc[4] = static_cast<A>(elc[4] + (std::sin(lf) - std::sin(nf)) * p.inv_sp[a]);
+            divabs[4] += (std::fabs(std::sin(lf)) + std::fabs(std::sin(nf))) * p.inv_sp[a];
+        } else if (G == 7) {
+            rep(p, Lf + o - 1, false, P1);
+            rep(p, Ln + o - 1, true, Um);
+            qmul(Um, P1, false, P1);
+            rep(p, Mn + o - 1, true, P2);
+            rep(p, Mf + o - 1, false, Um);
+            qmul(Um, P2, false, P2);
+            for (int k = 1; k < 4; k++) {
+                const double dk = (P1[k] - P2[k]) * p.inv_sp[a];
+                elc[3 + k] += static_cast<A>(dk);
+                divabs[3 + k] += (std::fabs(P1[k]) + std::fabs(P2[k])) * p.inv_sp[a];
+            }
+        }
     }
-    for (int k = 0; k < 4; k++) {
+    for (int k = 0; k < G; k++) {
         if constexpr (sizeof(A) == sizeof(float)) viol[k] = elc[k] - p.gsq_f[k] * grad[k];
         else viol[k] = elc[k] - static_cast<double>(p.gsq_f[k]) * grad[k];
         // Scale of the relative violation: the magnitudes of the terms that cancel.
@@ -693,6 +856,9 @@ std::vector<long> reference_site_order(long nx, long ny, long nz) {
     return order;
 }
 
+// Columns of a reference row: 6 energies, GFT_MAX_EQ integrated |G|, GFT_MAX_EQ max |G|.
+constexpr int kRefCols = 6 + 2 * GFT_MAX_EQ;
+
 template <class T>
 int evolve(const gft_desc* d, T* S, T* V, unsigned time_iter, unsigned nsteps, unsigned energy_freq,
            unsigned gauss_freq, double* ref_rows, gft_obs* rows, unsigned max_rows, unsigned* nrows,
@@ -726,18 +892,19 @@ int evolve(const gft_desc* d, T* S, T* V, unsigned time_iter, unsigned nsteps, u
         for (long s = 0; s < p.N; s++)
             if (!frozen(p, s))
                 site_update<T, A>(p, S, V, s, now, meas_e, damping_step, meas_e ? &dens[s] : nullptr, &panics);
+        const int G = p.G;
         if (meas_g) {
-            gviol.assign(p.N * 4, A(0));
-            gscale.assign(p.N * 4, 0.0);
+            gviol.assign(p.N * G, A(0));
+            gscale.assign(p.N * G, 0.0);
 #pragma omp parallel for schedule(static)
             for (long s = 0; s < p.N; s++)
-                if (!frozen(p, s)) site_gauss<T, A>(p, S, V, s, now, &gviol[s * 4], &gscale[s * 4]);
+                if (!frozen(p, s)) site_gauss<T, A>(p, S, V, s, now, &gviol[s * G], &gscale[s * G]);
         }
         if ((meas_e || meas_g) && row < max_rows) {
             gft_obs* o = rows ? &rows[row] : nullptr;
-            double* rr = ref_rows ? &ref_rows[static_cast<size_t>(row) * 14] : nullptr;
-            if (o) { std::memset(o, 0, sizeof(*o)); o->time_iter = it; o->num_eq = 4; }
-            if (rr) for (int m = 0; m < 14; m++) rr[m] = 0.0;
+            double* rr = ref_rows ? &ref_rows[static_cast<size_t>(row) * kRefCols] : nullptr;
+            if (o) { std::memset(o, 0, sizeof(*o)); o->time_iter = it; o->num_eq = G; }
+            if (rr) for (int m = 0; m < kRefCols; m++) rr[m] = 0.0;
             if (meas_e) {
                 // Reference accumulation: float sums in the reference site order (Energy.cpp:132,156,181).
                 A acc[6] = {0, 0, 0, 0, 0, 0};
@@ -759,31 +926,31 @@ int evolve(const gft_desc* d, T* S, T* V, unsigned time_iter, unsigned nsteps, u
                 }
             }
             if (meas_g) {
-                A integ[4] = {0, 0, 0, 0}, mx[4] = {0, 0, 0, 0};
-                Neumaier ni[4], ns[4];
-                double dmx[4] = {0, 0, 0, 0};
+                A integ[7] = {0, 0, 0, 0, 0, 0, 0}, mx[7] = {0, 0, 0, 0, 0, 0, 0};
+                Neumaier ni[7], ns[7];
+                double dmx[7] = {0, 0, 0, 0, 0, 0, 0};
                 for (long s : order) {
                     if (frozen(p, s)) continue;
-                    for (int m = 0; m < 4; m++) {
-                        const A v = std::abs(gviol[s * 4 + m]);
+                    for (int m = 0; m < G; m++) {
+                        const A v = std::abs(gviol[s * G + m]);
                         integ[m] += v;
                         if (v > mx[m]) mx[m] = v;
                         ni[m].add(static_cast<double>(v));
-                        ns[m].add(gscale[s * 4 + m]);
+                        ns[m].add(gscale[s * G + m]);
                         if (static_cast<double>(v) > dmx[m]) dmx[m] = static_cast<double>(v);
                     }
                 }
                 if (rr) {
-                    for (int m = 0; m < 4; m++) {
+                    for (int m = 0; m < G; m++) {
                         // GaugeCondition.cpp:175: integrated *= dx*dy*dz in the accumulator type.
                         const A scaled = static_cast<A>(integ[m] * (p.dx * p.dy * p.dz));
                         rr[6 + m] = static_cast<double>(scaled);
-                        rr[10 + m] = static_cast<double>(mx[m]);
+                        rr[6 + GFT_MAX_EQ + m] = static_cast<double>(mx[m]);
                     }
                 }
                 if (o) {
                     o->has_gauss = 1;
-                    for (int m = 0; m < 4; m++) {
+                    for (int m = 0; m < G; m++) {
                         o->gauss_int[m] = ni[m].value() * spv;
                         o->gauss_max[m] = dmx[m];
                         o->gauss_scale[m] = ns[m].value() * spv;
@@ -803,7 +970,8 @@ int evolve(const gft_desc* d, T* S, T* V, unsigned time_iter, unsigned nsteps, u
 extern "C" {
 
 // Two-slot evolution with FP32 storage (the reference's own arithmetic).
-// ref_rows: [rows][14] = 6 energy columns, 4 integrated |G|, 4 max |G| exactly
+// ref_rows: [rows][6 + 2 * GFT_MAX_EQ] = 6 energy columns, then GFT_MAX_EQ
+// integrated |G| and GFT_MAX_EQ max |G| (the first num_eq of each used) exactly
 // as the reference accumulates them; rows: FP64 diagnostics of the same states.
 int oracle_evolve_f32(const gft_desc* d, float* scalar, float* vec, unsigned time_iter, unsigned nsteps,
                       unsigned energy_freq, unsigned gauss_freq, double* ref_rows, gft_obs* rows,
@@ -844,7 +1012,7 @@ int oracle_gauss_site_f32(const gft_desc* d, const float* scalar, const float* v
                           float* viol) {
     const Params p = make_params(d);
     const unsigned now = (time_iter + 1) % 2;
-    for (long s = 0; s < p.N; s++) site_gauss<float, float>(p, scalar, vec, s, now, viol + s * 4, nullptr);
+    for (long s = 0; s < p.N; s++) site_gauss<float, float>(p, scalar, vec, s, now, viol + s * p.G, nullptr);
     return GFT_OK;
 }
 

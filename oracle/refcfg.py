@@ -67,7 +67,7 @@ def set_values(root: str, lib: str, name: str, values: list) -> str:
 
 
 def write_model(root: str, model: int, g: float, gp: float, *, lam=0.13, eta_sqr=1.0, pot2=None,
-                damping=0.0, nt_damped=0, tol=1e-9, gen=False) -> str:
+                damping=0.0, nt_damped=0, tol=1e-9, gen=False, gpp=0.5) -> str:
     """Model.cfg plus the potential / gradient / Wilson-loop configs; returns the
     Model.cfg path (relative to root, as the harness opens it from there).
 
@@ -92,7 +92,7 @@ def write_model(root: str, model: int, g: float, gp: float, *, lam=0.13, eta_sqr
             int(q.get("break_to_son", 1)), q["m12r"], q["l4p5"], (q["l6r"], q["l7r"]),
             int(q.get("break_to_uhalfn", 1)), q["m12i"], (q["l4m5"], q["l5i"]), (q["l6i"], q["l7i"]),
         ])
-        set_values(root, "WilsonLoops", "TwoHDM.cfg", [g, gp, 0.5, int(gen), tol])
+        set_values(root, "WilsonLoops", "TwoHDM.cfg", [g, gp, gpp, int(gen), tol])
         set_values(root, "Gradients", "TwoHDM.cfg", [tol])
     set_values(root, "Core", "Model.cfg", [ids[0], ids[1], ids[2], float(damping), nt_damped, 0, 0.0])
     return os.path.join("cfg", "Core", "Config", "Model.cfg")

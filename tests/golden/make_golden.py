@@ -23,8 +23,13 @@ Two kinds of fixture, both produced by the reference's own compiled code:
                        harness state mixes small link angles with a few |w| in
                        (1.6, 3), where c0 < 0 and the inverse conversion takes
                        its pi - asin branch, and a few exactly zero links.
+  harness_2hdm_g<5|7>[_gen].npz, program_2hdm_g<5|7>[_gen].npz
+                       the 2HDM with a Higgs-family U(1) (gaugeNum 5) or SU(2)
+                       (gaugeNum 7) gauge field, quaternion and generator forms
+                       (TwoHDM.cfg's flag and the vector component count select
+                       both, TwoHDM.cpp:215-242), Higgs-family coupling 0.5.
 
-    python tests/golden/make_golden.py [--fixed | --gen]   # only those files
+    python tests/golden/make_golden.py [--fixed | --gen | --family]   # only those files
 """
 from __future__ import annotations
 
@@ -123,6 +128,71 @@ def generator_form():
         print(tag, "generator form written")
 
 
+def harness_state_family(gauge_num: int, gen: bool, seed: int = 6):
+    """2HDM harness state with a Higgs-family link part: y' small for gaugeNum 5,
+    a second near-identity SU(2) for 7; the SM part as in the other fixtures."""
+    s, _ = harness_initial_state(1, seed)
+    N = int(np.prod(HARNESS_N))
+    rng = np.random.default_rng(seed + 200)
+
+    def su2(shape):
+        if gen:
+            w = rng.normal(size=shape + (3,)) * 0.05
+            big = rng.choice(N, 3, replace=False)
+            w[:, big, 0] *= 40.0            # a few |w| above pi/2
+            return w
+        q = rng.normal(size=shape + (4,)) * 0.05
+        q[..., 0] += 1
+        return q / np.linalg.norm(q, axis=-1, keepdims=True)
+
+    parts = [rng.normal(size=(2, N, 3, 1)) * 0.05, su2((2, N, 3))]
+    if gauge_num == 5:
+        parts.append(rng.normal(size=(2, N, 3, 1)) * 0.05)
+    else:
+        parts.append(su2((2, N, 3)))
+    vv = np.concatenate(parts, axis=-1)
+    vv[1] = vv[0] + rng.normal(size=vv[0].shape) * 0.005
+    if not gen:   # keep the quaternions unit after the offset
+        vv[1, ..., 1:5] /= np.linalg.norm(vv[1, ..., 1:5], axis=-1, keepdims=True)
+        if gauge_num == 7:
+            vv[1, ..., 5:9] /= np.linalg.norm(vv[1, ..., 5:9], axis=-1, keepdims=True)
+    return s, vv.reshape(2, N, -1).astype(np.float32)
+
+
+def family_gauge():
+    pot2 = TWO_HDM_POTENTIAL
+    for gn in (5, 7):
+        for gen in (False, True):
+            tag = f"2hdm_g{gn}" + ("_gen" if gen else "")
+            s, v = harness_state_family(gn, gen)
+            with tempfile.TemporaryDirectory() as td:
+                h = O.run_harness(1, HARNESS_N, (0.5, 0.5, 0.5), 0.05, HARNESS_STEPS, HARNESS_MEAS, HARNESS_MEAS,
+                                  s, v, td, pot2=pot2, gen=gen, gauge_num=gn)
+            out = {"scalar0": s, "vec0": v}
+            for k in HARNESS_KEEP:
+                out[f"scalar_{k}"] = h[k]["scalar"]
+                out[f"vec_{k}"] = h[k]["vec"]
+            for k in range(0, HARNESS_STEPS, HARNESS_MEAS):
+                out[f"dens_{k}"] = h[k]["dens"]
+                out[f"viol_{k}"] = h[k]["viol"]
+            np.savez_compressed(os.path.join(HERE, f"harness_{tag}.npz"), **out)
+
+            nvd = v.shape[-1] // 3
+            with tempfile.TemporaryDirectory() as td:
+                refcfg.prepare(td)
+                refcfg.write_model(td, 1, 0.65, 0.35, pot2=pot2, gen=gen)
+                refcfg.write_lattice(td, PROGRAM_N, (0.5, 0.5, 0.5), PROGRAM_STEPS, 0.05, 8, ic_seed=PROGRAM_SEED,
+                                     ic_range=PROGRAM_RANGE, efreq=PROGRAM_EFREQ, gfreq=PROGRAM_GFREQ, nvd=nvd)
+                subprocess.run([os.path.join(O.REF_DIR, "gft_reference")], check=True, stdout=subprocess.DEVNULL,
+                               stderr=subprocess.DEVNULL, cwd=td)
+                data = os.path.join(td, "data")
+                E = np.loadtxt(os.path.join(data, "global_quantities.dat"), ndmin=2)
+                G = np.loadtxt(os.path.join(data, "global_quantities_gauge.dat"), ndmin=2)
+            assert G.shape[1] == 2 * gn
+            np.savez_compressed(os.path.join(HERE, f"program_{tag}.npz"), energy=E, gauss=G)
+            print(tag, "written")
+
+
 def main():
     for model, tag in ((0, "sm"), (1, "2hdm")):
         pot2 = TWO_HDM_POTENTIAL if model == 1 else None
@@ -179,7 +249,10 @@ if __name__ == "__main__":
         fixed_faces_program()
     elif "--gen" in sys.argv[1:]:
         generator_form()
+    elif "--family" in sys.argv[1:]:
+        family_gauge()
     else:
         main()
         fixed_faces_program()
         generator_form()
+        family_gauge()

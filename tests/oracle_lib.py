@@ -8,10 +8,11 @@ from __future__ import annotations
 import ctypes as C
 import os
 import subprocess
+from types import SimpleNamespace
 
 import numpy as np
 
-from gftevolver_amd._abi import NV, GftDesc, GftObs
+from gftevolver_amd._abi import MAX_EQ, NV, GftDesc, GftObs, link_reals
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
@@ -43,7 +44,9 @@ def lib():
 
 def evolve(desc: GftDesc, scalar: np.ndarray, vec: np.ndarray, nsteps: int, time_iter: int = 0,
            energy_freq: int = 0, gauss_freq: int = 0, nthreads: int = 0):
-    """Advance two-slot arrays in place.  Returns (ref_rows [n,14], obs rows list, panics)."""
+    """Advance two-slot arrays in place.  Returns (ref_rows [n, 6 + 2 G]: 6 energy
+    columns, G integrated |G^a|, G max |G^a| as the reference prints them, G =
+    gaugeNum; obs rows list; panics)."""
     dtype = scalar.dtype
     assert vec.dtype == dtype and scalar.flags.c_contiguous and vec.flags.c_contiguous
     T = C.c_float if dtype == np.float32 else C.c_double
@@ -52,7 +55,7 @@ def evolve(desc: GftDesc, scalar: np.ndarray, vec: np.ndarray, nsteps: int, time
     if energy_freq or gauss_freq:
         nmeas = sum(1 for k in range(time_iter, time_iter + nsteps)
                     if (energy_freq and k % energy_freq == 0) or (gauss_freq and k % gauss_freq == 0))
-    ref = np.zeros((max(nmeas, 1), 14), np.float64)
+    ref = np.zeros((max(nmeas, 1), 6 + 2 * MAX_EQ), np.float64)
     obs = (GftObs * max(nmeas, 1))()
     nrows = C.c_uint(0)
     panics = C.c_ulong(0)
@@ -61,6 +64,8 @@ def evolve(desc: GftDesc, scalar: np.ndarray, vec: np.ndarray, nsteps: int, time
             C.byref(nrows), nthreads, C.byref(panics))
     if rc != 0:
         raise RuntimeError(f"oracle evolve failed: {rc}")
+    G = desc.gauge_num if desc.gauge_num in (5, 7) else 4
+    ref = np.concatenate([ref[:, :6 + G], ref[:, 6 + MAX_EQ:6 + MAX_EQ + G]], axis=1)
     return ref[:nrows.value], list(obs)[:nrows.value], panics.value
 
 
@@ -87,16 +92,18 @@ def ref_available() -> bool:
 
 
 def run_harness(model: int, n, d, dt, nsteps, efreq, gfreq, scalar, vec, workdir, *, g=0.65, gp=0.35, lam=0.13,
-                eta_sqr=1.0, pot2=None, damping=0.0, nt_damped=0, gen=False):
+                eta_sqr=1.0, pot2=None, damping=0.0, nt_damped=0, gen=False, gauge_num=4, gpp=0.5):
     """Run the reference harness; returns a list of per-step dicts.  gen: links
-    in the generator form (12 reals per site)."""
+    in the generator form; gauge_num 5 or 7: the 2HDM with a Higgs-family gauge
+    field (its link width follows gen)."""
     import sys
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import refcfg
     refcfg.prepare(workdir)
     model_cfg = refcfg.write_model(workdir, model, g, gp, lam=lam, eta_sqr=eta_sqr, pot2=pot2, damping=damping,
-                                   nt_damped=nt_damped, gen=gen)
-    nv = 12 if gen else NV
+                                   nt_damped=nt_damped, gen=gen, gpp=gpp)
+    nv = link_reals(SimpleNamespace(gen_rep=int(gen), gauge_num=gauge_num))
+    neq = gauge_num
     Ns = 4 if model == 0 else 8
     N = n[0] * n[1] * n[2]
     fin = os.path.join(workdir, "in.bin")
@@ -116,7 +123,7 @@ def run_harness(model: int, n, d, dt, nsteps, efreq, gfreq, scalar, vec, workdir
         if efreq and k % efreq == 0:
             rec["dens"] = raw[pos:pos + N * 5].reshape(N, 5); pos += N * 5
         if gfreq and k % gfreq == 0:
-            rec["viol"] = raw[pos:pos + N * 4].reshape(N, 4); pos += N * 4
+            rec["viol"] = raw[pos:pos + N * neq].reshape(N, neq); pos += N * neq
         out.append(rec)
     assert pos == raw.size
     return out
@@ -136,7 +143,7 @@ def energy_density_f32(desc, scalar, vec, time_iter):
 def gauss_site_f32(desc, scalar, vec, time_iter):
     L = lib()
     N = desc.nx * desc.ny * desc.nz
-    out = np.zeros((N, 4), np.float32)
+    out = np.zeros((N, desc.gauge_num if desc.gauge_num in (5, 7) else 4), np.float32)
     P = C.POINTER(C.c_float)
     L.oracle_gauss_site_f32.argtypes = [C.POINTER(GftDesc), P, P, C.c_uint, P]
     L.oracle_gauss_site_f32(C.byref(desc), scalar.ctypes.data_as(P), vec.ctypes.data_as(P), time_iter,

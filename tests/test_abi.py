@@ -37,8 +37,10 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_struct_layouts_match_header():
-    # gft_obs: 4 unsigned then 6 + 3*4 doubles.
-    assert C.sizeof(_abi.GftObs) == 16 + 18 * 8
+    # gft_obs: 4 unsigned then 6 + 3 * GFT_MAX_EQ doubles.
+    max_eq = int(re.search(r"#define GFT_MAX_EQ (\d+)", open(HEADER).read()).group(1))
+    assert max_eq == _abi.MAX_EQ
+    assert C.sizeof(_abi.GftObs) == 16 + (6 + 3 * max_eq) * 8
     assert C.sizeof(_abi.GftLoops) == 5 * 8
     fields = [f[0] for f in _abi.GftDesc._fields_]
     txt = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
@@ -61,4 +63,19 @@ def test_create_rejects_invalid_face_masks_before_touching_a_device(faces, msg):
     ctx = C.c_void_p()
     assert lib.gft_create(C.byref(d), C.byref(ctx)) == -1
     assert not ctx.value
+    assert msg in lib.gft_last_error(None).decode()
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libgft.so not built")
+@pytest.mark.parametrize("model,form,gauge_num,msg", [
+    (_abi.GFT_MODEL_2HDM, _abi.GFT_FORM_MOMENTUM_F32, 5, "two-slot forms"),
+    (_abi.GFT_MODEL_SM, _abi.GFT_FORM_TWO_SLOT_F64, 7, "two doublets"),
+    (_abi.GFT_MODEL_2HDM, _abi.GFT_FORM_TWO_SLOT_F64, 6, "gauge_num is 4")])
+def test_create_rejects_unsupported_gauge_groups_before_touching_a_device(model, form, gauge_num, msg):
+    lib = C.CDLL(LIB)
+    lib.gft_last_error.restype = C.c_char_p
+    lib.gft_last_error.argtypes = [C.c_void_p]
+    d = _abi.make_desc(model=model, form=form, n=(8, 8, 8), pot2=_abi.TWO_HDM_POTENTIAL, gauge_num=gauge_num)
+    ctx = C.c_void_p()
+    assert lib.gft_create(C.byref(d), C.byref(ctx)) == -1
     assert msg in lib.gft_last_error(None).decode()

@@ -86,12 +86,13 @@ def test_config_reader_generator_form(tmp_path, model):
                          check=True).stdout
     kv = {l.split()[0]: l.split()[1:] for l in out.strip().splitlines()}
     assert kv["nv"] == ["12"] and kv["gen_rep"] == ["1"]
-    # Quaternion components with the 2HDM generator flag set: gaugeNum 5, not built here.
+    # 5 components per direction with the 2HDM generator flag set: gaugeNum 5.
     if model == 1:
         root = _prepare(str(tmp_path / "b"), model, (6, 5, 7), 12, ns, gen=True, nvd=5)
-        r = subprocess.run([GFT_RUN, root, str(tmp_path / "b" / "data"), "dump_config"], capture_output=True,
-                           text=True)
-        assert r.returncode != 0 and "gaugeNum 5 and 7" in r.stderr
+        out = subprocess.run([GFT_RUN, root, str(tmp_path / "b" / "data"), "dump_config"], capture_output=True,
+                             text=True, check=True).stdout
+        kv = {l.split()[0]: l.split()[1:] for l in out.strip().splitlines()}
+        assert kv["gauge_num"] == ["5"] and kv["gen_rep"] == ["1"] and float(kv["gpp"][0]) == 0.5
     # A count neither form allows: the reference's message.
     root = _prepare(str(tmp_path / "c"), model, (6, 5, 7), 12, ns, gen=model == 1, nvd=3)
     r = subprocess.run([GFT_RUN, root, str(tmp_path / "c" / "data"), "dump_config"], capture_output=True, text=True)
@@ -154,5 +155,38 @@ def test_gft_run_reproduces_reference_program_generator_form(tmp_path, model, ta
     E = np.loadtxt(data / "global_quantities.dat", ndmin=2)
     G = np.loadtxt(data / "global_quantities_gauge.dat", ndmin=2)
     assert E.shape == g["energy"].shape and G.shape == g["gauss"].shape
+    np.testing.assert_allclose(E, g["energy"], rtol=2e-5, atol=1e-5 * np.abs(g["energy"]).max())
+    assert (G <= 10 * g["gauss"] + 1e-7).all()
+
+
+@needs_cfg
+@pytest.mark.parametrize("nvd,gen,gauge_num", [(6, False, 5), (9, False, 7), (5, True, 5), (7, True, 7)],
+                         ids=["g5", "g7", "g5-gen", "g7-gen"])
+def test_config_reader_family_gauge(tmp_path, nvd, gen, gauge_num):
+    """The component count and the generator flag fix the 2HDM's gauge group
+    (TwoHDM.cpp:215-242); the Higgs-family coupling comes from TwoHDM.cfg."""
+    root = _prepare(str(tmp_path), 1, (6, 5, 7), 12, 8, gen=gen, nvd=nvd)
+    out = subprocess.run([GFT_RUN, root, str(tmp_path / "data"), "dump_config"], capture_output=True, text=True,
+                         check=True).stdout
+    kv = {l.split()[0]: l.split()[1:] for l in out.strip().splitlines()}
+    assert kv["gauge_num"] == [str(gauge_num)] and kv["nv"] == [str(3 * nvd)] and kv["gen_rep"] == [str(int(gen))]
+
+
+@needs_cfg
+@pytest.mark.gpu
+@pytest.mark.parametrize("gauge_num,gen", [(5, False), (5, True), (7, False), (7, True)],
+                         ids=["g5", "g5-gen", "g7", "g7-gen"])
+def test_gft_run_reproduces_reference_program_family_gauge(tmp_path, gauge_num, gen):
+    """The reference program's 2HDM with a Higgs-family gauge field: energies and
+    all 2 x gaugeNum Gauss columns."""
+    tag = f"2hdm_g{gauge_num}" + ("_gen" if gen else "")
+    g = np.load(os.path.join(ROOT, "tests", "golden", f"program_{tag}.npz"))
+    nvd = (4 if gen else 5) + (1 if gauge_num == 5 else (3 if gen else 4))
+    root = _prepare(str(tmp_path), 1, (6, 5, 7), 12, 8, gen=gen, nvd=nvd)
+    data = tmp_path / "data"
+    subprocess.run([GFT_RUN, root, str(data)], check=True, capture_output=True, timeout=120)
+    E = np.loadtxt(data / "global_quantities.dat", ndmin=2)
+    G = np.loadtxt(data / "global_quantities_gauge.dat", ndmin=2)
+    assert E.shape == g["energy"].shape and G.shape == g["gauss"].shape == (G.shape[0], 2 * gauge_num)
     np.testing.assert_allclose(E, g["energy"], rtol=2e-5, atol=1e-5 * np.abs(g["energy"]).max())
     assert (G <= 10 * g["gauss"] + 1e-7).all()

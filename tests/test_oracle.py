@@ -157,6 +157,67 @@ def test_generator_form_conversions_round_trip_and_match_closed_forms():
     np.testing.assert_allclose(qg, vq_.reshape(2, N, 3, 5)[..., 1:], rtol=0, atol=1e-12)
 
 
+FAMILY = [(5, False), (5, True), (7, False), (7, True)]
+FAMILY_IDS = ["g5", "g5-gen", "g7", "g7-gen"]
+
+
+def identity_links_family(N, gauge_num, gen):
+    """Zero gauge fields as the reference initialises them (Lattice.cpp:369-392):
+    c0 = 1 (and c0' = 1 for gaugeNum 7) in the quaternion form, zeros otherwise."""
+    from gftevolver_amd._abi import link_reals
+    from types import SimpleNamespace
+    nvd = link_reals(SimpleNamespace(gen_rep=int(gen), gauge_num=gauge_num)) // 3
+    v = np.zeros((2, N, 3, nvd), np.float32)
+    if not gen:
+        v[..., 1] = 1.0
+        if gauge_num == 7:
+            v[..., 5] = 1.0
+    return v.reshape(2, N, 3 * nvd)
+
+
+@pytest.mark.parametrize("gauge_num,gen", FAMILY, ids=FAMILY_IDS)
+def test_oracle_family_gauge_bitwise_vs_reference_harness(gauge_num, gen):
+    """The 2HDM with a Higgs-family U(1) (gaugeNum 5) or SU(2) (gaugeNum 7)
+    gauge field (Gradients/Source/TwoHDM.cpp:106-141, 299-319, 401-424;
+    WilsonLoops/Source/TwoHDM.cpp:352-386, 496-552, 600-634, 696-754): fields,
+    energy densities and all gaugeNum Gauss violations bitwise."""
+    tag = f"2hdm_g{gauge_num}" + ("_gen" if gen else "")
+    g = np.load(os.path.join(GOLD, f"harness_{tag}.npz"))
+    s, v = g["scalar0"].copy(), g["vec0"].copy()
+    d = make_desc(model=GFT_MODEL_2HDM, n=(4, 5, 6), pot2=TWO_HDM_POTENTIAL, gen_rep=int(gen),
+                  gauge_num=gauge_num, gpp=0.5)
+    assert g["viol_0"].shape[1] == gauge_num
+    for k in range(6):
+        if k % 3 == 0:
+            assert np.array_equal(O.energy_density_f32(d, s, v, k), g[f"dens_{k}"]), f"energy densities, step {k}"
+        O.evolve(d, s, v, 1, time_iter=k, energy_freq=3, gauss_freq=3)
+        if k % 3 == 0:
+            assert np.array_equal(O.gauss_site_f32(d, s, v, k), g[f"viol_{k}"]), f"Gauss violations, step {k}"
+        if f"scalar_{k}" in g:
+            assert np.array_equal(s, g[f"scalar_{k}"]), f"scalar fields differ after step {k}"
+            assert np.array_equal(v, g[f"vec_{k}"]), f"link fields differ after step {k}"
+
+
+@pytest.mark.parametrize("gauge_num,gen", FAMILY, ids=FAMILY_IDS)
+def test_oracle_family_gauge_reproduces_reference_program_output(gauge_num, gen):
+    """The unmodified reference program with gaugeNum 5 / 7: printed energies and
+    all 2 x gaugeNum Gauss columns digit for digit."""
+    tag = f"2hdm_g{gauge_num}" + ("_gen" if gen else "")
+    g = np.load(os.path.join(GOLD, f"program_{tag}.npz"))
+    n = (6, 5, 7)
+    N = int(np.prod(n))
+    s = O.random_uniform(2 * N * 8, 7, -0.8, 0.8).reshape(2, N, 8)
+    v = identity_links_family(N, gauge_num, gen)
+    d = make_desc(model=GFT_MODEL_2HDM, n=n, pot2=TWO_HDM_POTENTIAL, gen_rep=int(gen), gauge_num=gauge_num, gpp=0.5)
+    ref, obs, _ = O.evolve(d, s, v, 12, energy_freq=3, gauss_freq=2)
+    fmt = np.vectorize(lambda x: float("%.6g" % x))
+    e = np.array([r[:6] for r, o in zip(ref, obs) if o.has_energy])
+    gs = np.array([r[6:] for r, o in zip(ref, obs) if o.has_gauss])
+    assert gs.shape[1] == 2 * gauge_num
+    assert np.array_equal(fmt(e), g["energy"])
+    assert np.array_equal(fmt(gs), g["gauss"])
+
+
 @pytest.mark.parametrize("case,mask", [("xz", 1 | 2 | 16 | 32), ("y", 4 | 8)], ids=["xz", "y"])
 def test_oracle_reproduces_reference_program_with_fixed_faces(case, mask):
     """Fixed faces (Fixed.cpp: frozen one-site shells, left out of the
@@ -339,6 +400,6 @@ def test_fp64_oracle_gauss_law_conserved():
     s, v = L.near_vacuum_state(n, 4, seed=4, amp=0.3, mom=0.0, link_eps=0.3)
     s[0], v[0] = s[1], v[1]
     _, obs, _ = O.evolve(d, s, v, 30, energy_freq=10, gauss_freq=1)
-    gi = np.array([o.gauss_int[:] for o in obs if o.has_gauss])
-    sc = np.array([o.gauss_scale[:] for o in obs if o.has_gauss])
+    gi = np.array([o.gauss_int[:o.num_eq] for o in obs if o.has_gauss])
+    sc = np.array([o.gauss_scale[:o.num_eq] for o in obs if o.has_gauss])
     assert (gi[5:] / sc[5:]).max() < 1e-12
