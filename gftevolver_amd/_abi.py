@@ -5,6 +5,7 @@ import ctypes as C
 
 GFT_MODEL_SM = 0
 GFT_MODEL_2HDM = 1
+GFT_MODEL_GLOBAL = 2
 
 GFT_FORM_TWO_SLOT_F64 = 0
 GFT_FORM_TWO_SLOT_F32 = 1
@@ -39,13 +40,17 @@ class GftDesc(C.Structure):
         ("gen_rep", C.c_int),
         ("gauge_num", C.c_int),
         ("gpp", C.c_double),
+        ("num_scalar", C.c_int),
+        ("potential", C.c_int),
     ]
 
 
 def link_reals(desc) -> int:
     """Link reals per site of the two-slot arrays: 3 directions x [y, SU(2) (4 as
     a quaternion, 3 as generators), Higgs-family part (gaugeNum 5: y'; 7: a
-    second SU(2))]."""
+    second SU(2))]; none for the global O(N) model."""
+    if getattr(desc, "model", 0) == GFT_MODEL_GLOBAL:
+        return 0
     base = 4 if desc.gen_rep else 5
     gn = desc.gauge_num if desc.gauge_num in (5, 7) else 4
     extra = {4: 0, 5: 1, 7: 3 if desc.gen_rep else 4}[gn]
@@ -70,14 +75,15 @@ class GftLoops(C.Structure):
 
 def make_desc(model=GFT_MODEL_SM, form=GFT_FORM_TWO_SLOT_F64, n=(8, 8, 8), d=(0.5, 0.5, 0.5), dt=0.05,
               g=0.65, gp=0.35, lam=0.13, eta_sqr=1.0, pot2=None, damping=0.0, nt_damped=0, div_tol=1e-9,
-              fixed_faces=0, gen_rep=0, gauge_num=4, gpp=0.5):
+              fixed_faces=0, gen_rep=0, gauge_num=4, gpp=0.5, num_scalar=0, potential=0):
     """A gft_desc with the SM couplings of the 32^3 / 256^3 workloads by default.
 
     pot2: Double_SO_N parameters for the 2HDM (keys as in gft_desc).
     fixed_faces: bit 2a / 2a+1 fixes the low / high face of axis a (x, y, z).
     gen_rep: 1 stores SU(2) links as generator components (12 reals per site).
     gauge_num: 4 (SU(2) x U(1)), 5 or 7 (2HDM with a Higgs-family U(1) or SU(2));
-    gpp: the Higgs-family coupling."""
+    gpp: the Higgs-family coupling.
+    num_scalar, potential: the global O(N) model's N and potential (1 SO_N, 2 Double_SO_N)."""
     ds = GftDesc()
     ds.model, ds.form = model, form
     ds.nx, ds.ny, ds.nz = n
@@ -94,6 +100,8 @@ def make_desc(model=GFT_MODEL_SM, form=GFT_FORM_TWO_SLOT_F64, n=(8, 8, 8), d=(0.
     ds.gen_rep = gen_rep
     ds.gauge_num = gauge_num
     ds.gpp = gpp
+    ds.num_scalar = num_scalar
+    ds.potential = potential
     return ds
 
 

@@ -31,11 +31,15 @@ constexpr int kEventPool = 256;
 
 bool is_two_slot(int form) { return form == GFT_FORM_TWO_SLOT_F64 || form == GFT_FORM_TWO_SLOT_F32; }
 bool is_f64(int form) { return form == GFT_FORM_TWO_SLOT_F64 || form == GFT_FORM_MOMENTUM_F64; }
-// Gauss generators of the gauge group: gaugeNum 4, 5 or 7.
-int gauge_num(const gft_desc& d) { return (d.gauge_num == 5 || d.gauge_num == 7) ? d.gauge_num : 4; }
+// Gauss generators of the gauge group: gaugeNum 4, 5 or 7; none without links.
+int gauge_num(const gft_desc& d) {
+    if (d.model == GFT_MODEL_GLOBAL) return 0;
+    return (d.gauge_num == 5 || d.gauge_num == 7) ? d.gauge_num : 4;
+}
 // Link reals per site of the two-slot arrays: 3 x [y, SU(2) as c0..c3 or w1..w3,
 // Higgs-family part (gaugeNum 5: y'; 7: c0'..c3' or w1'..w3')].
 long long link_reals(const gft_desc& d) {
+    if (d.model == GFT_MODEL_GLOBAL) return 0;
     const int base = d.gen_rep ? 4 : 5, g = gauge_num(d);
     return 3 * (base + (g == 5 ? 1 : g == 7 ? base - 1 : 0));
 }
@@ -85,8 +89,10 @@ static int fail(gft_ctx* ctx, int code, const std::string& msg) {
 
 static Coef make_coef(const gft_desc& d) {
     Coef c{};
-    c.ns = d.model == GFT_MODEL_2HDM ? 8 : 4;
+    const bool global = d.model == GFT_MODEL_GLOBAL;
+    c.ns = global ? d.num_scalar : d.model == GFT_MODEL_2HDM ? 8 : 4;
     c.nd = c.ns / 4;
+    c.so_n = global ? d.potential != 2 : d.model == GFT_MODEL_SM;
     c.nx = d.nx; c.ny = d.ny; c.nz = d.nz;
     c.nsites = (long long)d.nx * d.ny * d.nz;
     c.dt = d.dt;
@@ -146,10 +152,18 @@ int gft_create(const gft_desc* desc, gft_ctx** out) {
     if (!desc || !out) return fail(nullptr, GFT_EINVAL, "gft_create: null argument");
     *out = nullptr;
     const gft_desc& d = *desc;
-    if (d.model != GFT_MODEL_SM && d.model != GFT_MODEL_2HDM)
+    if (d.model != GFT_MODEL_SM && d.model != GFT_MODEL_2HDM && d.model != GFT_MODEL_GLOBAL)
         return fail(nullptr, GFT_EINVAL, "MODEL: The chosen model (" + std::to_string(d.model) + ") is invalid.");
     if (d.form < GFT_FORM_TWO_SLOT_F64 || d.form > GFT_FORM_MOMENTUM_F32_PLAIN)
         return fail(nullptr, GFT_EINVAL, "gft_create: invalid form " + std::to_string(d.form));
+    if (d.model == GFT_MODEL_GLOBAL) {
+        if (!is_two_slot(d.form))
+            return fail(nullptr, GFT_EINVAL, "gft_create: the global O(N) model runs in the two-slot forms");
+        if (d.num_scalar < 1 || d.num_scalar > 8)
+            return fail(nullptr, GFT_EINVAL, "gft_create: the global O(N) model takes 1 to 8 scalar components");
+        if (d.potential < 0 || d.potential > 2 || (d.potential == 2 && d.num_scalar != 8))
+            return fail(nullptr, GFT_EINVAL, "gft_create: potential is 1 (SO_N) or 2 (Double_SO_N, 8 components)");
+    }
     if (d.nx < 2 || d.ny < 2 || d.nz < 2)
         return fail(nullptr, GFT_EINVAL, "LATTICE:: periodic lattices need at least 2 sites per direction");
     if ((long long)d.nx * d.ny * d.nz > (1LL << 28))
@@ -168,9 +182,10 @@ int gft_create(const gft_desc* desc, gft_ctx** out) {
         return fail(nullptr, GFT_EINVAL, "gft_create: gen_rep is 0 (quaternion links) or 1 (generator form)");
     if (d.gauge_num != 0 && d.gauge_num != 4 && d.gauge_num != 5 && d.gauge_num != 7)
         return fail(nullptr, GFT_EINVAL, "gft_create: gauge_num is 4 (SU(2) x U(1)), 5 or 7 (Higgs-family U(1) or SU(2))");
-    if (gauge_num(d) != 4 && d.model != GFT_MODEL_2HDM)
+    const bool family = gauge_num(d) == 5 || gauge_num(d) == 7;
+    if (family && d.model != GFT_MODEL_2HDM)
         return fail(nullptr, GFT_EINVAL, "gft_create: a Higgs-family gauge field needs two doublets (GFT_MODEL_2HDM)");
-    if (gauge_num(d) != 4 && !is_two_slot(d.form))
+    if (family && !is_two_slot(d.form))
         return fail(nullptr, GFT_EINVAL, "gft_create: gaugeNum 5 and 7 run in the two-slot forms (the momentum "
                                          "forms carry the SU(2) x U(1) links only)");
     int ndev = 0;
@@ -191,7 +206,7 @@ int gft_create(const gft_desc* desc, gft_ctx** out) {
     const long long N = ctx->N;
     if (is_two_slot(d.form)) {
         if (hipMalloc(&ctx->S, 2 * N * ctx->coef.ns * esz) != hipSuccess
-            || hipMalloc(&ctx->V, 2 * N * link_reals(d) * esz) != hipSuccess) {
+            || (link_reals(d) && hipMalloc(&ctx->V, 2 * N * link_reals(d) * esz) != hipSuccess)) {
             g_create_error = "gft_create: device allocation failed";
             return bail(GFT_ENOMEM);
         }
@@ -301,18 +316,18 @@ static int import_two_slot(gft_ctx* ctx, const Src* scalar, const Src* vec, unsi
     if (is_two_slot(ctx->form)) {
         if (is_f64(ctx->form) == (sizeof(Src) == 8)) {
             CK(hipMemcpyAsync(ctx->S, scalar, ns * sizeof(Src), hipMemcpyHostToDevice, ctx->stream));
-            CK(hipMemcpyAsync(ctx->V, vec, nv * sizeof(Src), hipMemcpyHostToDevice, ctx->stream));
+            if (nv) CK(hipMemcpyAsync(ctx->V, vec, nv * sizeof(Src), hipMemcpyHostToDevice, ctx->stream));
         } else if (is_f64(ctx->form)) {
             std::vector<double> s(scalar, scalar + ns), v(vec, vec + nv);
             CK(hipMemcpyAsync(ctx->S, s.data(), ns * 8, hipMemcpyHostToDevice, ctx->stream));
-            CK(hipMemcpyAsync(ctx->V, v.data(), nv * 8, hipMemcpyHostToDevice, ctx->stream));
+            if (nv) CK(hipMemcpyAsync(ctx->V, v.data(), nv * 8, hipMemcpyHostToDevice, ctx->stream));
             CK(hipStreamSynchronize(ctx->stream));
         } else {
             std::vector<float> s(ns), v(nv);
             for (size_t i = 0; i < ns; i++) s[i] = (float)scalar[i];
             for (size_t i = 0; i < nv; i++) v[i] = (float)vec[i];
             CK(hipMemcpyAsync(ctx->S, s.data(), ns * 4, hipMemcpyHostToDevice, ctx->stream));
-            CK(hipMemcpyAsync(ctx->V, v.data(), nv * 4, hipMemcpyHostToDevice, ctx->stream));
+            if (nv) CK(hipMemcpyAsync(ctx->V, v.data(), nv * 4, hipMemcpyHostToDevice, ctx->stream));
             CK(hipStreamSynchronize(ctx->stream));
         }
     } else {
@@ -341,19 +356,19 @@ static int export_two_slot(gft_ctx* ctx, Dst* scalar, Dst* vec, unsigned* time_i
     if (is_two_slot(ctx->form)) {
         if (is_f64(ctx->form) == (sizeof(Dst) == 8)) {
             CK(hipMemcpyAsync(scalar, ctx->S, ns * sizeof(Dst), hipMemcpyDeviceToHost, ctx->stream));
-            CK(hipMemcpyAsync(vec, ctx->V, nv * sizeof(Dst), hipMemcpyDeviceToHost, ctx->stream));
+            if (nv) CK(hipMemcpyAsync(vec, ctx->V, nv * sizeof(Dst), hipMemcpyDeviceToHost, ctx->stream));
             CK(hipStreamSynchronize(ctx->stream));
         } else if (is_f64(ctx->form)) {
             std::vector<double> s(ns), v(nv);
             CK(hipMemcpyAsync(s.data(), ctx->S, ns * 8, hipMemcpyDeviceToHost, ctx->stream));
-            CK(hipMemcpyAsync(v.data(), ctx->V, nv * 8, hipMemcpyDeviceToHost, ctx->stream));
+            if (nv) CK(hipMemcpyAsync(v.data(), ctx->V, nv * 8, hipMemcpyDeviceToHost, ctx->stream));
             CK(hipStreamSynchronize(ctx->stream));
             for (size_t i = 0; i < ns; i++) scalar[i] = (Dst)s[i];
             for (size_t i = 0; i < nv; i++) vec[i] = (Dst)v[i];
         } else {
             std::vector<float> s(ns), v(nv);
             CK(hipMemcpyAsync(s.data(), ctx->S, ns * 4, hipMemcpyDeviceToHost, ctx->stream));
-            CK(hipMemcpyAsync(v.data(), ctx->V, nv * 4, hipMemcpyDeviceToHost, ctx->stream));
+            if (nv) CK(hipMemcpyAsync(v.data(), ctx->V, nv * 4, hipMemcpyDeviceToHost, ctx->stream));
             CK(hipStreamSynchronize(ctx->stream));
             for (size_t i = 0; i < ns; i++) scalar[i] = (Dst)s[i];
             for (size_t i = 0; i < nv; i++) vec[i] = (Dst)v[i];
@@ -369,19 +384,19 @@ static int export_two_slot(gft_ctx* ctx, Dst* scalar, Dst* vec, unsigned* time_i
 extern "C" {
 
 int gft_import_two_slot_f32(gft_ctx* ctx, const float* s, const float* v, unsigned it) {
-    if (!ctx || !s || !v) return GFT_EINVAL;
+    if (!ctx || !s || (!v && link_reals(ctx->desc))) return GFT_EINVAL;
     return import_two_slot(ctx, s, v, it);
 }
 int gft_import_two_slot_f64(gft_ctx* ctx, const double* s, const double* v, unsigned it) {
-    if (!ctx || !s || !v) return GFT_EINVAL;
+    if (!ctx || !s || (!v && link_reals(ctx->desc))) return GFT_EINVAL;
     return import_two_slot(ctx, s, v, it);
 }
 int gft_export_two_slot_f32(gft_ctx* ctx, float* s, float* v, unsigned* it) {
-    if (!ctx || !s || !v) return GFT_EINVAL;
+    if (!ctx || !s || (!v && link_reals(ctx->desc))) return GFT_EINVAL;
     return export_two_slot(ctx, s, v, it);
 }
 int gft_export_two_slot_f64(gft_ctx* ctx, double* s, double* v, unsigned* it) {
-    if (!ctx || !s || !v) return GFT_EINVAL;
+    if (!ctx || !s || (!v && link_reals(ctx->desc))) return GFT_EINVAL;
     return export_two_slot(ctx, s, v, it);
 }
 
@@ -492,6 +507,16 @@ static void twoslot_dispatch(gft_ctx* ctx, long long nblocks, unsigned now, bool
 }
 static void twoslot_launch(gft_ctx* ctx, long long nblocks, unsigned now, bool gauss, bool me, double damping_step) {
     const bool f64 = ctx->form == GFT_FORM_TWO_SLOT_F64, gen = ctx->desc.gen_rep != 0;
+    if (ctx->desc.model == GFT_MODEL_GLOBAL) {   // no links, no Gauss law
+        if (gauss) return;
+        if (f64)
+            twoslot_step_global<double><<<nblocks, kBlock, 0, ctx->stream>>>(ctx->coef, (double*)ctx->S, now, me,
+                                                                             damping_step, ctx->partial);
+        else
+            twoslot_step_global<float><<<nblocks, kBlock, 0, ctx->stream>>>(ctx->coef, (float*)ctx->S, now, me,
+                                                                            damping_step, ctx->partial);
+        return;
+    }
     if (f64) (gen ? twoslot_dispatch<double, true> : twoslot_dispatch<double, false>)(ctx, nblocks, now, gauss, me,
                                                                                        damping_step);
     else (gen ? twoslot_dispatch<float, true> : twoslot_dispatch<float, false>)(ctx, nblocks, now, gauss, me,
@@ -520,7 +545,7 @@ extern "C" int gft_evolve(gft_ctx* ctx, unsigned nsteps, unsigned energy_freq, u
         const unsigned it = ctx->time_iter;
         const unsigned now = (it + 1) % 2;
         const bool me = energy_freq && it % energy_freq == 0;
-        const bool mg = gauss_freq && it % gauss_freq == 0;
+        const bool mg = gauss_freq && it % gauss_freq == 0 && gauge_num(ctx->desc) > 0;
         const float damp = it < ctx->desc.nt_damped ? ctx->desc.damping : 0.f;
         const double damping_step = 0.5 * (double)damp * ctx->desc.dt;
         if (me && !two) {
@@ -1058,6 +1083,8 @@ __global__ void __launch_bounds__(256) grf_assemble_two_slot(T* __restrict__ S, 
 
 extern "C" int gft_init_grf(gft_ctx* ctx, const gft_grf_desc* g) {
     if (!ctx || !g) return GFT_EINVAL;
+    if (ctx->desc.model == GFT_MODEL_GLOBAL)
+        return fail(ctx, GFT_EINVAL, "gft_init_grf: doublet fields only (not the global O(N) model)");
     const Coef& c = ctx->coef;
     FftAxis ax[3];
     if (!grf_plan(c.nz, ax[0]) || !grf_plan(c.ny, ax[1]) || !grf_plan(c.nx, ax[2]))

@@ -23,6 +23,7 @@ namespace gft {
 // All coefficients of one model, passed by value to every kernel.
 struct Coef {
     int ns, nd;                  // scalar reals per site, doublets
+    int so_n;                    // SO_N potential over the ns components (else Double_SO_N)
     int nx, ny, nz;
     long long nsites;
     double dt;
@@ -173,6 +174,12 @@ GFT_HD void potential_derivative(const Coef& c, const R* f, R* pc) {
 GFT_HD double potential_energy(const Coef& c, const double* f) {
     if (c.ns == 4) {
         double fsq = f[0] * f[0] + f[1] * f[1] + f[2] * f[2] + f[3] * f[3];
+        double dd = fsq - c.eta_sqr;
+        return 0.25 * c.lambda * dd * dd;
+    }
+    if (c.so_n) {   // SO_N over any number of components (the global O(N) model)
+        double fsq = 0.0;
+        for (int i = 0; i < c.ns; i++) fsq += f[i] * f[i];
         double dd = fsq - c.eta_sqr;
         return 0.25 * c.lambda * dd * dd;
     }

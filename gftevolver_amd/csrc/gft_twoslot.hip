@@ -162,10 +162,10 @@ __device__ __forceinline__ void family_bilinears_add(const double* t, const doub
 // Double_SO_N.cpp:157-254).
 __device__ void potential_derivative_ref(const Coef& d, const double* f, double* pc) {
 #pragma clang fp contract(off)
-    if (d.ns == 4) {
+    if (d.so_n) {
         double fsq = 0.0;
-        for (int c = 0; c < 4; c++) fsq += f[c] * f[c];
-        for (int c = 0; c < 4; c++) pc[c] = d.lambda * (fsq - d.eta_sqr) * f[c];
+        for (int c = 0; c < d.ns; c++) fsq += f[c] * f[c];
+        for (int c = 0; c < d.ns; c++) pc[c] = d.lambda * (fsq - d.eta_sqr) * f[c];
         return;
     }
     double fsq[2] = {0.0, 0.0}, dot = 0.0, anti = 0.0;
@@ -428,6 +428,63 @@ __global__ void __launch_bounds__(256) twoslot_step(Coef c, T* __restrict__ S, T
             }
         }
         if (bad) atomicOr(err, 1u);
+    }
+    if (measure) block_reduce_store<5, 0>(dens, nullptr, lds, partial, blockIdx.x);
+}
+
+// One leapfrog step of the global O(N) model (Gradients/Source/Global.cpp with
+// the null Wilson loop, WilsonLoops/Source/WilsonLoop.cpp:22-85): the plain
+// Laplacian in the reference's order (component, axis, stencil point;
+// Global.cpp:118-135), the potential and Model::evolve's scalar leapfrog
+// (Model.cpp:345-356).  measure != 0 reduces the FP64 energy densities
+// (potential, gradient, kinetic; the magnetic and electric parts are zero).
+template <class T>
+__global__ void __launch_bounds__(256) twoslot_step_global(Coef c, T* __restrict__ S, unsigned now, int measure,
+                                                           double damping_step, double* __restrict__ partial) {
+#pragma clang fp contract(off)
+    __shared__ double lds[4 * 5];
+    const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double dens[5] = {0, 0, 0, 0, 0};
+    if (s < c.nsites && !site_frozen(c, s)) {
+        const int Ns = c.ns;
+        const long long N = c.nsites;
+        const T* Sn = S + (long long)now * N * Ns;
+        T* sp = S + (long long)(!now) * N * Ns + s * Ns;
+        const T* sn = Sn + s * Ns;
+        long long nb[3][3];
+        for (int a = 0; a < 3; a++) {
+            nb[a][0] = site_shift(c, s, a, -1);
+            nb[a][1] = s;
+            nb[a][2] = site_shift(c, s, a, +1);
+        }
+        double phi[8], pot[8], der[8];
+        for (int k = 0; k < Ns; k++) phi[k] = static_cast<double>(sn[k]);
+        potential_derivative_ref(c, phi, pot);
+        for (int k = 0; k < Ns; k++) {
+            der[k] = 0.0;
+            for (int a = 0; a < 3; a++)
+                for (int st = 0; st < 3; st++) der[k] += c.lap[a][st] * static_cast<double>(Sn[nb[a][st] * Ns + k]);
+        }
+        if (measure) {
+            dens[0] = potential_energy(c, phi);
+            double ge = 0.0, ke = 0.0;
+            for (int k = 0; k < Ns; k++)
+                for (int a = 0; a < 3; a++) {
+                    const double dd = -c.est[a] * phi[k] + c.est[a] * static_cast<double>(Sn[nb[a][2] * Ns + k]);
+                    ge += dd * dd;
+                }
+            for (int k = 0; k < Ns; k++) {
+                const double dd = (static_cast<double>(sn[k]) - static_cast<double>(sp[k])) / c.dt;
+                ke += dd * dd;
+            }
+            dens[1] = 0.5 * ge;
+            dens[2] = 0.5 * ke;
+        }
+        for (int k = 0; k < Ns; k++) {
+            const double rhs = c.dt * c.dt * (der[k] - pot[k])
+                             - damping_step * (static_cast<double>(sn[k]) - static_cast<double>(sp[k]));
+            sp[k] = static_cast<T>(2 * static_cast<double>(sn[k]) - static_cast<double>(sp[k]) + rhs);
+        }
     }
     if (measure) block_reduce_store<5, 0>(dens, nullptr, lds, partial, blockIdx.x);
 }

@@ -228,12 +228,14 @@ void Model::configure(const std::string& model_cfg, const std::string& root, uns
     r.skip_lines(1);
     wilsonLoopType = r.value<int>();
     switch (wilsonLoopType) {
+    case NULL_WILSON_LOOP: wilsonLoop = std::make_unique<NullWilsonLoop>(); break;
     case SM_WILSON_LOOP: wilsonLoop = std::make_unique<WilsonLoops::StandardModel>(root, nv); break;
     case TWOHDM_WILSON_LOOP: wilsonLoop = std::make_unique<WilsonLoops::TwoHDM>(root, nv); break;
     default:
         throw std::runtime_error("MODEL: The chosen wilson loop type (" + std::to_string(wilsonLoopType) + ") is invalid.");
     }
     switch (gradientType) {
+    case GLOBAL_GRADIENT: gradient = std::make_unique<Gradients::Global>(); break;
     case SM_GRADIENT: gradient = std::make_unique<Gradients::StandardModel>(root, ns, nv); break;
     case TWOHDM_GRADIENT:
         gradient = std::make_unique<Gradients::TwoHDM>(root, ns, nv, wilsonLoop->isUsingGeneratorRepresentation());
@@ -250,9 +252,14 @@ void Model::configure(const std::string& model_cfg, const std::string& root, uns
     const bool sm = potentialType == SO_N_POTENTIAL && gradientType == SM_GRADIENT && wilsonLoopType == SM_WILSON_LOOP;
     const bool thdm = potentialType == DOUBLE_SO_N_POTENTIAL && gradientType == TWOHDM_GRADIENT
                       && wilsonLoopType == TWOHDM_WILSON_LOOP;
-    if (!sm && !thdm)
-        throw std::runtime_error("MODEL:: the GPU evolver runs the SM (potential 1, gradient 2, Wilson loop 1) or the "
-                                 "2HDM (potential 2, gradient 3, Wilson loop 2)");
+    const bool global = gradientType == GLOBAL_GRADIENT && wilsonLoopType == NULL_WILSON_LOOP
+                        && (potentialType == SO_N_POTENTIAL || (potentialType == DOUBLE_SO_N_POTENTIAL && ns == 8))
+                        && ns >= 1 && ns <= 8;
+    if (!sm && !thdm && !global)
+        throw std::runtime_error("MODEL:: the GPU evolver runs the SM (potential 1, gradient 2, Wilson loop 1), the "
+                                 "2HDM (potential 2, gradient 3, Wilson loop 2) or the global O(N) model (potential 1 "
+                                 "or 2, gradient 1, Wilson loop 0; 1 to 8 scalar components)");
+    numScalar = ns;
     getNumberOfConstraintEquations();
 }
 
@@ -264,10 +271,18 @@ unsigned Model::getNumberOfConstraintEquations() const {
     return b;
 }
 
-bool Model::isUsingQuaternionRepresentation() const { return !wilsonLoop->isUsingGeneratorRepresentation(); }
+bool Model::isUsingQuaternionRepresentation() const {   // Model.cpp:263-267
+    return wilsonLoop->getNumberOfEvolutionEquations() > 0 && !wilsonLoop->isUsingGeneratorRepresentation();
+}
 
 void Model::fill(gft_desc& d) const {
-    d.model = potentialType == DOUBLE_SO_N_POTENTIAL ? GFT_MODEL_2HDM : GFT_MODEL_SM;
+    d.model = gradientType == GLOBAL_GRADIENT          ? GFT_MODEL_GLOBAL
+              : potentialType == DOUBLE_SO_N_POTENTIAL ? GFT_MODEL_2HDM
+                                                       : GFT_MODEL_SM;
+    if (d.model == GFT_MODEL_GLOBAL) {
+        d.num_scalar = static_cast<int>(numScalar);
+        d.potential = potentialType;
+    }
     d.g = wilsonLoop->g;
     d.gp = wilsonLoop->gp;
     d.div_tol = wilsonLoop->divisionByZeroTolerance;

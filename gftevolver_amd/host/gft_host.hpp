@@ -99,6 +99,10 @@ public:
 };
 
 namespace Gradients {
+class Global : public Gradient {          // Gradients/Source/Global.cpp: plain Laplacian, no gauge fields
+public:
+    unsigned getNumberOfConstraintEquations() const override { return 0; }
+};
 class StandardModel : public Gradient {   // Gradients/Source/StandardModel.cpp:5-69, 152-170
 public:
     StandardModel(const std::string& config_root, unsigned num_scalar, unsigned num_vector);
@@ -111,6 +115,14 @@ public:
     unsigned gaugeNum = 4;
 };
 }  // namespace Gradients
+
+// WilsonLoops/Source/WilsonLoop.cpp:22-85: no gauge dynamics.
+class NullWilsonLoop : public WilsonLoop {
+public:
+    unsigned getNumberOfEvolutionEquations() const override { return 0; }
+    unsigned getNumberOfConstraintEquations() const override { return 0; }
+    float getSqrCouplings(unsigned) const override { return 0.f; }
+};
 
 namespace WilsonLoops {
 class StandardModel : public WilsonLoop { // WilsonLoops/Source/StandardModel.cpp:5-44, 195-241
@@ -143,6 +155,7 @@ public:
     bool isUsingQuaternionRepresentation() const;
     void fill(gft_desc& d) const;             // model, couplings, potential, damping
     int potentialType = NULL_POTENTIAL, gradientType = NULL_GRADIENT, wilsonLoopType = NULL_WILSON_LOOP;
+    unsigned numScalar = 0;
     float dampingFactor = 0.f;
     unsigned ntDamped = 0;
     std::unique_ptr<Potential> potential;

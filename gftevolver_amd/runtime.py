@@ -98,7 +98,7 @@ class Evolver:
 
     def __init__(self, desc: GftDesc, _handle=None):
         self.desc = desc
-        self.ns = 8 if desc.model == 1 else 4
+        self.ns = desc.num_scalar if desc.model == 2 else 8 if desc.model == 1 else 4
         self.nv = link_reals(desc)
         self.n = (desc.nx, desc.ny, desc.nz)
         self.nsites = desc.nx * desc.ny * desc.nz

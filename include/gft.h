@@ -39,6 +39,8 @@ extern "C" {
 /* ---- models (Core/Config/Model.cfg potential/gradient/wilson-loop ids) ---- */
 #define GFT_MODEL_SM    0  /* SO_N potential, Gradients::StandardModel, WilsonLoops::StandardModel; Ns=4 */
 #define GFT_MODEL_2HDM  1  /* Double_SO_N, Gradients::TwoHDM, WilsonLoops::TwoHDM with gaugeNum 4; Ns=8 */
+#define GFT_MODEL_GLOBAL 2 /* global O(N) model: SO_N (or Double_SO_N) potential, Gradients::Global,
+                              null Wilson loop; Ns = num_scalar, no links, no Gauss law; two-slot forms */
 
 /* ---- state forms ---- */
 #define GFT_FORM_TWO_SLOT_F64 0  /* reference two-slot layout, FP64 storage (FP64 GPU oracle)            */
@@ -100,6 +102,11 @@ typedef struct gft_desc {
      * and a two-slot form.  gpp is the Higgs-family coupling g'' (TwoHDM.cfg).          */
     int      gauge_num;
     double   gpp;
+    /* GFT_MODEL_GLOBAL only (Gradients/Source/Global.cpp; Potentials/Source/SO_N.cpp takes any
+     * component count): num_scalar = N (1..8); potential 0 or 1 = SO_N, 2 = Double_SO_N
+     * (needs N = 8).  The vector arrays of the two-slot calls are ignored (may be NULL).  */
+    int      num_scalar;
+    int      potential;
 } gft_desc;
 
 /* Most Gauss-law generators of any gauge group (gaugeNum 7). */

@@ -77,13 +77,19 @@ struct Params {
     // Higgs-family SU(2) (link[off..] in the same form as the isospin SU(2)).
     int G = 4, off = 0;
     double inv_gpp_sqr = 0.0;
+    // Global O(N) model (Gradients/Source/Global.cpp with the null Wilson loop):
+    // N = Ns scalar components, no links, no Gauss law; so_n selects SO_N(N)
+    // over Double_SO_N.
+    bool global = false, so_n = true;
 };
 
 Params make_params(const gft_desc* d) {
     Params p;
     p.d = d;
-    p.Ns = d->model == GFT_MODEL_2HDM ? 8 : 4;
+    p.global = d->model == GFT_MODEL_GLOBAL;
+    p.Ns = p.global ? d->num_scalar : d->model == GFT_MODEL_2HDM ? 8 : 4;
     p.nd = p.Ns / 4;
+    p.so_n = p.global ? d->potential != 2 : d->model != GFT_MODEL_2HDM;
     p.nx = d->nx; p.ny = d->ny; p.nz = d->nz; p.N = p.nx * p.ny * p.nz;
     p.dx = d->dx; p.dy = d->dy; p.dz = d->dz; p.dt = d->dt;
     const double sp[3] = {d->dx, d->dy, d->dz};
@@ -116,6 +122,7 @@ Params make_params(const gft_desc* d) {
     p.off = p.gen ? 4 : 5;
     p.vd = p.off + (p.G == 5 ? 1 : p.G == 7 ? (p.gen ? 3 : 4) : 0);
     p.nv = 3 * p.vd;
+    if (p.global) { p.G = 0; p.vd = p.nv = 0; }
     p.tol = d->div_tol;
     return p;
 }
@@ -293,10 +300,10 @@ inline void family_bilinears_add(int G, const R* t, const R* n, int i, A* j) {
 // Potential derivative: SO_N (SO_N.cpp:57-75) or Double_SO_N (Double_SO_N.cpp:157-254).
 void potential_derivative(const Params& p, const double* f, double* pc) {
     const gft_desc* d = p.d;
-    if (p.Ns == 4) {
+    if (p.so_n) {
         double fsq = 0.0;
-        for (int c = 0; c < 4; c++) fsq += f[c] * f[c];
-        for (int c = 0; c < 4; c++) pc[c] = d->lambda * (fsq - d->eta_sqr) * f[c];
+        for (int c = 0; c < p.Ns; c++) fsq += f[c] * f[c];
+        for (int c = 0; c < p.Ns; c++) pc[c] = d->lambda * (fsq - d->eta_sqr) * f[c];
         return;
     }
     const int n0 = 4, n1 = 4;
@@ -351,9 +358,9 @@ A potential_energy(const Params& p, const double* f);
 template <>
 float potential_energy<float>(const Params& p, const double* f) {
     const gft_desc* d = p.d;
-    if (p.Ns == 4) {
+    if (p.so_n) {
         double fsq = 0.0;
-        for (int c = 0; c < 4; c++) fsq += f[c] * f[c];
+        for (int c = 0; c < p.Ns; c++) fsq += f[c] * f[c];
         const float diff = static_cast<float>(fsq - d->eta_sqr);
         return static_cast<float>(0.25f * d->lambda * (diff * diff));
     }
@@ -382,9 +389,9 @@ float potential_energy<float>(const Params& p, const double* f) {
 template <>
 double potential_energy<double>(const Params& p, const double* f) {
     const gft_desc* d = p.d;
-    if (p.Ns == 4) {
+    if (p.so_n) {
         double fsq = 0.0;
-        for (int c = 0; c < 4; c++) fsq += f[c] * f[c];
+        for (int c = 0; c < p.Ns; c++) fsq += f[c] * f[c];
         const double diff = fsq - d->eta_sqr;
         return 0.25 * d->lambda * diff * diff;
     }
@@ -413,6 +420,74 @@ double potential_energy<double>(const Params& p, const double* f) {
 // (Energy.cpp:126-132): potential, gradient, kinetic, magnetic, electric.
 template <class A>
 struct Dens { A v[5]; };
+
+// One site-update of the global O(N) model: the plain Laplacian
+// (Gradients/Source/Global.cpp:118-135, component outer, axis, then stencil),
+// the potential, Model::evolve's scalar leapfrog (Model.cpp:345-356); energy
+// densities of Global.cpp:84-115 (float derivative and powf sums) with zero
+// magnetic and electric parts (the null Wilson loop, WilsonLoop.cpp:57-65).
+template <class T, class A>
+void site_update_global(const Params& p, T* S, long s, unsigned now, bool store_energy, double damping_step,
+                        Dens<A>* dens) {
+    const int Ns = p.Ns;
+    const long N = p.N;
+    const unsigned past = !now;
+    const T* Sn = S + static_cast<long>(now) * N * Ns;
+    T* sp_ = S + static_cast<long>(past) * N * Ns + s * Ns;
+    const T* sn = Sn + s * Ns;
+    long nb[3][3];
+    for (int a = 0; a < 3; a++) { nb[a][0] = shift(p, s, a, -1); nb[a][1] = s; nb[a][2] = shift(p, s, a, +1); }
+    double phi[8];
+    for (int c = 0; c < Ns; c++) phi[c] = static_cast<double>(sn[c]);
+    double pot[8];
+    potential_derivative(p, phi, pot);
+    double der[8];
+    for (int c = 0; c < Ns; c++) {
+        der[c] = 0.0;
+        for (int a = 0; a < 3; a++)
+            for (int st = 0; st < 3; st++) der[c] += p.lap[a][st] * static_cast<double>(Sn[nb[a][st] * Ns + c]);
+    }
+    if (store_energy && dens) {
+        const T* spast = S + static_cast<long>(past) * N * Ns + s * Ns;
+        if constexpr (sizeof(A) == sizeof(float)) {
+            float ge = 0.f;
+            for (int c = 0; c < Ns; c++)
+                for (int a = 0; a < 3; a++) {
+                    float dv = 0.f;
+                    for (int st = 0; st < 3; st++)
+                        dv = static_cast<float>(dv + p.est[a][st] * static_cast<float>(Sn[nb[a][st] * Ns + c]));
+                    ge += ::powf(dv, 2);
+                }
+            float ke = 0.f;
+            for (int c = 0; c < Ns; c++)
+                ke += ::powf(static_cast<float>((static_cast<float>(sn[c]) - static_cast<float>(spast[c])) / p.dt), 2);
+            dens->v[0] = potential_energy<float>(p, phi);
+            dens->v[1] = 0.5f * ge;
+            dens->v[2] = 0.5f * ke;
+            dens->v[3] = dens->v[4] = 0.f;
+        } else {
+            double ge = 0.0, ke = 0.0;
+            for (int c = 0; c < Ns; c++)
+                for (int a = 0; a < 3; a++) {
+                    const double dv = p.est[a][1] * phi[c] + p.est[a][2] * static_cast<double>(Sn[nb[a][2] * Ns + c]);
+                    ge += dv * dv;
+                }
+            for (int c = 0; c < Ns; c++) {
+                const double dd = (static_cast<double>(sn[c]) - static_cast<double>(spast[c])) / p.dt;
+                ke += dd * dd;
+            }
+            dens->v[0] = potential_energy<double>(p, phi);
+            dens->v[1] = 0.5 * ge;
+            dens->v[2] = 0.5 * ke;
+            dens->v[3] = dens->v[4] = 0.0;
+        }
+    }
+    for (int c = 0; c < Ns; c++) {
+        const double rhs = p.dt * p.dt * (der[c] - pot[c])
+                         - damping_step * (static_cast<double>(sn[c]) - static_cast<double>(sp_[c]));
+        sp_[c] = static_cast<T>(2 * static_cast<double>(sn[c]) - static_cast<double>(sp_[c]) + rhs);
+    }
+}
 
 // One site-update of the two-slot leapfrog.  Reads slot `now` everywhere and
 // slot `past` at s; writes slot `past` at s.
@@ -864,7 +939,7 @@ int evolve(const gft_desc* d, T* S, T* V, unsigned time_iter, unsigned nsteps, u
            unsigned gauss_freq, double* ref_rows, gft_obs* rows, unsigned max_rows, unsigned* nrows,
            int nthreads, unsigned long* panics_out) {
     using A = T;
-    if (!d || !S || !V) return GFT_EINVAL;
+    if (!d || !S || (!V && d->model != GFT_MODEL_GLOBAL)) return GFT_EINVAL;
     if (d->nx < 2 || d->ny < 2 || d->nz < 2) return GFT_EINVAL;
     const Params p = make_params(d);
     const std::vector<long> order = reference_site_order(p.nx, p.ny, p.nz);
@@ -889,11 +964,15 @@ int evolve(const gft_desc* d, T* S, T* V, unsigned time_iter, unsigned nsteps, u
         const double damping_step = 0.5 * static_cast<double>(cur_damp) * p.dt;
         if (meas_e) dens.assign(p.N, Dens<A>{});
 #pragma omp parallel for schedule(static)
-        for (long s = 0; s < p.N; s++)
-            if (!frozen(p, s))
+        for (long s = 0; s < p.N; s++) {
+            if (frozen(p, s)) continue;
+            if (p.global)
+                site_update_global<T, A>(p, S, s, now, meas_e, damping_step, meas_e ? &dens[s] : nullptr);
+            else
                 site_update<T, A>(p, S, V, s, now, meas_e, damping_step, meas_e ? &dens[s] : nullptr, &panics);
+        }
         const int G = p.G;
-        if (meas_g) {
+        if (meas_g && G > 0) {
             gviol.assign(p.N * G, A(0));
             gscale.assign(p.N * G, 0.0);
 #pragma omp parallel for schedule(static)
@@ -1000,7 +1079,10 @@ int oracle_energy_density_f32(const gft_desc* d, const float* scalar, const floa
     std::vector<Dens<float>> dd(p.N);
     // Each site writes only its own past slot, so updating the copy in any
     // order leaves every other site's inputs untouched.
-    for (long s = 0; s < p.N; s++) site_update<float, float>(p, S.data(), V.data(), s, now, true, 0.0, &dd[s], &panics);
+    for (long s = 0; s < p.N; s++) {
+        if (p.global) site_update_global<float, float>(p, S.data(), s, now, true, 0.0, &dd[s]);
+        else site_update<float, float>(p, S.data(), V.data(), s, now, true, 0.0, &dd[s], &panics);
+    }
     for (long s = 0; s < p.N; s++)
         for (int m = 0; m < 5; m++) dens[s * 5 + m] = dd[s].v[m];
     return GFT_OK;

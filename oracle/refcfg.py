@@ -72,19 +72,26 @@ def write_model(root: str, model: int, g: float, gp: float, *, lam=0.13, eta_sqr
     Model.cfg path (relative to root, as the harness opens it from there).
 
     model 0 = SM (SO_N, SM gradient, SM loops); 1 = 2HDM with gaugeNum 4
-    (Double_SO_N, TwoHDM gradient, TwoHDM loops, quaternion links).
+    (Double_SO_N, TwoHDM gradient, TwoHDM loops, quaternion links); 2 = global
+    O(N) model (SO_N, or Double_SO_N when pot2 is given; global gradient; null
+    Wilson loop).
     pot2: dict of Double_SO_N parameters (m1 m2 l1 l2 l3 m12r l4p5 l6r l7r m12i
     l4m5 l5i l6i l7i break_to_son break_to_uhalfn pot_norm pot_offset).
     gen: the 2HDM's generator-representation flag (TwoHDM.cfg); the SM takes its
     representation from the vector component count (write_lattice's nvd).
     """
-    if model == 0:
+    if model == 2:
+        ids = (2 if pot2 else 1, 1, 0)
+        if not pot2:
+            set_values(root, "Potentials", "SO_N.cfg", [lam, eta_sqr])
+    elif model == 0:
         ids = (1, 2, 1)
         set_values(root, "Potentials", "SO_N.cfg", [lam, eta_sqr])
         set_values(root, "WilsonLoops", "StandardModel.cfg", [g, gp, tol])
         set_values(root, "Gradients", "StandardModel.cfg", [tol])
     else:
         ids = (2, 3, 2)
+    if pot2:
         q = pot2
         set_values(root, "Potentials", "Double_SO_N.cfg", [
             4, int(q.get("pot_norm", 0)), q.get("pot_offset", 0.0),
@@ -92,6 +99,7 @@ def write_model(root: str, model: int, g: float, gp: float, *, lam=0.13, eta_sqr
             int(q.get("break_to_son", 1)), q["m12r"], q["l4p5"], (q["l6r"], q["l7r"]),
             int(q.get("break_to_uhalfn", 1)), q["m12i"], (q["l4m5"], q["l5i"]), (q["l6i"], q["l7i"]),
         ])
+    if model == 1:
         set_values(root, "WilsonLoops", "TwoHDM.cfg", [g, gp, gpp, int(gen), tol])
         set_values(root, "Gradients", "TwoHDM.cfg", [tol])
     set_values(root, "Core", "Model.cfg", [ids[0], ids[1], ids[2], float(damping), nt_damped, 0, 0.0])

@@ -29,7 +29,12 @@ Two kinds of fixture, both produced by the reference's own compiled code:
                        (TwoHDM.cfg's flag and the vector component count select
                        both, TwoHDM.cpp:215-242), Higgs-family coupling 0.5.
 
-    python tests/golden/make_golden.py [--fixed | --gen | --family]   # only those files
+  harness_global_n<N>.npz, program_global_n<N>.npz
+                       the global O(N) model (global gradient, null Wilson loop):
+                       N = 3 with SO_N, N = 8 with Double_SO_N; no Gauss law, so
+                       the program runs the energy analyser only.
+
+    python tests/golden/make_golden.py [--fixed | --gen | --family | --global]   # only those files
 """
 from __future__ import annotations
 
@@ -193,6 +198,42 @@ def family_gauge():
             print(tag, "written")
 
 
+GLOBAL_CASES = ((3, None), (8, TWO_HDM_POTENTIAL))
+
+
+def global_model():
+    for ns, pot2 in GLOBAL_CASES:
+        tag = f"global_n{ns}"
+        N = int(np.prod(HARNESS_N))
+        rng = np.random.default_rng(30 + ns)
+        s = np.zeros((2, N, ns), np.float32)
+        s[0] = (rng.normal(size=(N, ns)) * 0.3).astype(np.float32)
+        s[0][:, 0] += 1.0
+        s[1] = s[0] + (rng.normal(size=(N, ns)) * 0.01).astype(np.float32)
+        v = np.zeros((2, N, 3), np.float32)
+        with tempfile.TemporaryDirectory() as td:
+            h = O.run_harness(2, HARNESS_N, (0.5, 0.4, 0.6), 0.05, HARNESS_STEPS, HARNESS_MEAS, 0, s, v, td,
+                              pot2=pot2, ns=ns)
+        out = {"scalar0": s}
+        for k in HARNESS_KEEP:
+            out[f"scalar_{k}"] = h[k]["scalar"]
+        for k in range(0, HARNESS_STEPS, HARNESS_MEAS):
+            out[f"dens_{k}"] = h[k]["dens"]
+        np.savez_compressed(os.path.join(HERE, f"harness_{tag}.npz"), **out)
+
+        with tempfile.TemporaryDirectory() as td:
+            refcfg.prepare(td)
+            refcfg.write_model(td, 2, 0.65, 0.35, pot2=pot2)
+            refcfg.write_lattice(td, PROGRAM_N, (0.5, 0.5, 0.5), PROGRAM_STEPS, 0.05, ns, ic_seed=PROGRAM_SEED,
+                                 ic_range=PROGRAM_RANGE, efreq=PROGRAM_EFREQ, gfreq=PROGRAM_GFREQ, nvd=1,
+                                 gauss=False)
+            subprocess.run([os.path.join(O.REF_DIR, "gft_reference")], check=True, stdout=subprocess.DEVNULL,
+                           stderr=subprocess.DEVNULL, cwd=td)
+            E = np.loadtxt(os.path.join(td, "data", "global_quantities.dat"), ndmin=2)
+        np.savez_compressed(os.path.join(HERE, f"program_{tag}.npz"), energy=E)
+        print(tag, "written")
+
+
 def main():
     for model, tag in ((0, "sm"), (1, "2hdm")):
         pot2 = TWO_HDM_POTENTIAL if model == 1 else None
@@ -251,8 +292,11 @@ if __name__ == "__main__":
         generator_form()
     elif "--family" in sys.argv[1:]:
         family_gauge()
+    elif "--global" in sys.argv[1:]:
+        global_model()
     else:
         main()
         fixed_faces_program()
         generator_form()
         family_gauge()
+        global_model()

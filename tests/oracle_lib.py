@@ -48,6 +48,8 @@ def evolve(desc: GftDesc, scalar: np.ndarray, vec: np.ndarray, nsteps: int, time
     columns, G integrated |G^a|, G max |G^a| as the reference prints them, G =
     gaugeNum; obs rows list; panics)."""
     dtype = scalar.dtype
+    if vec is None:   # the global O(N) model has no links
+        vec = np.zeros(1, dtype)
     assert vec.dtype == dtype and scalar.flags.c_contiguous and vec.flags.c_contiguous
     T = C.c_float if dtype == np.float32 else C.c_double
     fn = lib().oracle_evolve_f32 if dtype == np.float32 else lib().oracle_evolve_f64
@@ -64,7 +66,7 @@ def evolve(desc: GftDesc, scalar: np.ndarray, vec: np.ndarray, nsteps: int, time
             C.byref(nrows), nthreads, C.byref(panics))
     if rc != 0:
         raise RuntimeError(f"oracle evolve failed: {rc}")
-    G = desc.gauge_num if desc.gauge_num in (5, 7) else 4
+    G = 0 if desc.model == 2 else desc.gauge_num if desc.gauge_num in (5, 7) else 4
     ref = np.concatenate([ref[:, :6 + G], ref[:, 6 + MAX_EQ:6 + MAX_EQ + G]], axis=1)
     return ref[:nrows.value], list(obs)[:nrows.value], panics.value
 
@@ -92,7 +94,7 @@ def ref_available() -> bool:
 
 
 def run_harness(model: int, n, d, dt, nsteps, efreq, gfreq, scalar, vec, workdir, *, g=0.65, gp=0.35, lam=0.13,
-                eta_sqr=1.0, pot2=None, damping=0.0, nt_damped=0, gen=False, gauge_num=4, gpp=0.5):
+                eta_sqr=1.0, pot2=None, damping=0.0, nt_damped=0, gen=False, gauge_num=4, gpp=0.5, ns=None):
     """Run the reference harness; returns a list of per-step dicts.  gen: links
     in the generator form; gauge_num 5 or 7: the 2HDM with a Higgs-family gauge
     field (its link width follows gen)."""
@@ -104,7 +106,9 @@ def run_harness(model: int, n, d, dt, nsteps, efreq, gfreq, scalar, vec, workdir
                                    nt_damped=nt_damped, gen=gen, gpp=gpp)
     nv = link_reals(SimpleNamespace(gen_rep=int(gen), gauge_num=gauge_num))
     neq = gauge_num
-    Ns = 4 if model == 0 else 8
+    if model == 2:   # global O(N) model: one unused vector component per direction, no Gauss law
+        nv, neq = 3, 0
+    Ns = ns or (4 if model == 0 else 8)
     N = n[0] * n[1] * n[2]
     fin = os.path.join(workdir, "in.bin")
     fout = os.path.join(workdir, "out.bin")

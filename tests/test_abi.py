@@ -79,3 +79,18 @@ def test_create_rejects_unsupported_gauge_groups_before_touching_a_device(model,
     ctx = C.c_void_p()
     assert lib.gft_create(C.byref(d), C.byref(ctx)) == -1
     assert msg in lib.gft_last_error(None).decode()
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libgft.so not built")
+@pytest.mark.parametrize("form,ns,potential,msg", [
+    (_abi.GFT_FORM_MOMENTUM_F32, 4, 1, "two-slot forms"),
+    (_abi.GFT_FORM_TWO_SLOT_F64, 9, 1, "1 to 8 scalar components"),
+    (_abi.GFT_FORM_TWO_SLOT_F64, 4, 2, "Double_SO_N, 8 components")])
+def test_create_rejects_unsupported_global_models_before_touching_a_device(form, ns, potential, msg):
+    lib = C.CDLL(LIB)
+    lib.gft_last_error.restype = C.c_char_p
+    lib.gft_last_error.argtypes = [C.c_void_p]
+    d = _abi.make_desc(model=_abi.GFT_MODEL_GLOBAL, form=form, n=(8, 8, 8), num_scalar=ns, potential=potential)
+    ctx = C.c_void_p()
+    assert lib.gft_create(C.byref(d), C.byref(ctx)) == -1
+    assert msg in lib.gft_last_error(None).decode()

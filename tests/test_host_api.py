@@ -190,3 +190,37 @@ def test_gft_run_reproduces_reference_program_family_gauge(tmp_path, gauge_num, 
     assert E.shape == g["energy"].shape and G.shape == g["gauss"].shape == (G.shape[0], 2 * gauge_num)
     np.testing.assert_allclose(E, g["energy"], rtol=2e-5, atol=1e-5 * np.abs(g["energy"]).max())
     assert (G <= 10 * g["gauss"] + 1e-7).all()
+
+
+@needs_cfg
+@pytest.mark.parametrize("ns,pot2", [(3, None), (8, TWO_HDM_POTENTIAL)], ids=["n3-son", "n8-double-son"])
+def test_config_reader_global_model(tmp_path, ns, pot2):
+    """Potential 1 or 2, gradient 1 (global), Wilson loop 0 (null): the global
+    O(N) model with the Lattice.cfg scalar component count."""
+    import refcfg
+    refcfg.prepare(str(tmp_path))
+    refcfg.write_model(str(tmp_path), 2, 0.65, 0.35, pot2=pot2)
+    refcfg.write_lattice(str(tmp_path), (6, 5, 7), (0.5, 0.5, 0.5), 12, 0.05, ns, ic_seed=7, ic_range=(-0.8, 0.8),
+                         efreq=3, gfreq=2, nvd=1, gauss=False)
+    out = subprocess.run([GFT_RUN, os.path.join(str(tmp_path), "cfg"), str(tmp_path / "data"), "dump_config"],
+                         capture_output=True, text=True, check=True).stdout
+    kv = {l.split()[0]: l.split()[1:] for l in out.strip().splitlines()}
+    assert kv["model"] == ["2"] and kv["ns"] == [str(ns)]
+
+
+@needs_cfg
+@pytest.mark.gpu
+@pytest.mark.parametrize("ns,pot2", [(3, None), (8, TWO_HDM_POTENTIAL)], ids=["n3-son", "n8-double-son"])
+def test_gft_run_reproduces_reference_program_global_model(tmp_path, ns, pot2):
+    import refcfg
+    g = np.load(os.path.join(ROOT, "tests", "golden", f"program_global_n{ns}.npz"))
+    refcfg.prepare(str(tmp_path))
+    refcfg.write_model(str(tmp_path), 2, 0.65, 0.35, pot2=pot2)
+    refcfg.write_lattice(str(tmp_path), (6, 5, 7), (0.5, 0.5, 0.5), 12, 0.05, ns, ic_seed=7, ic_range=(-0.8, 0.8),
+                         efreq=3, gfreq=2, nvd=1, gauss=False)
+    data = tmp_path / "data"
+    subprocess.run([GFT_RUN, os.path.join(str(tmp_path), "cfg"), str(data)], check=True, capture_output=True,
+                   timeout=120)
+    E = np.loadtxt(data / "global_quantities.dat", ndmin=2)
+    assert E.shape == g["energy"].shape
+    np.testing.assert_allclose(E, g["energy"], rtol=2e-5, atol=1e-5 * np.abs(g["energy"]).max())

@@ -218,6 +218,71 @@ def test_oracle_family_gauge_reproduces_reference_program_output(gauge_num, gen)
     assert np.array_equal(fmt(gs), g["gauss"])
 
 
+GLOBAL = [(3, None), (8, TWO_HDM_POTENTIAL)]
+
+
+def global_desc(n, ns, pot2, d=(0.5, 0.5, 0.5), form=0):
+    from gftevolver_amd._abi import GFT_MODEL_GLOBAL
+    return make_desc(model=GFT_MODEL_GLOBAL, form=form, n=n, d=d, pot2=pot2, num_scalar=ns,
+                     potential=2 if pot2 else 1)
+
+
+@pytest.mark.parametrize("ns,pot2", GLOBAL, ids=["n3-son", "n8-double-son"])
+def test_oracle_global_model_bitwise_vs_reference_harness(ns, pot2):
+    """The global O(N) model (Gradients/Source/Global.cpp with the null Wilson
+    loop; SO_N with 3 components, Double_SO_N with 8) on unequal spacings:
+    fields after steps 1, 3, 6 and the energy densities bitwise."""
+    g = np.load(os.path.join(GOLD, f"harness_global_n{ns}.npz"))
+    s = g["scalar0"].copy()
+    d = global_desc((4, 5, 6), ns, pot2, d=(0.5, 0.4, 0.6))
+    for k in range(6):
+        if k % 3 == 0:
+            assert np.array_equal(O.energy_density_f32(d, s, np.zeros(1, np.float32), k), g[f"dens_{k}"]), k
+        O.evolve(d, s, None, 1, time_iter=k)
+        if f"scalar_{k}" in g:
+            assert np.array_equal(s, g[f"scalar_{k}"]), f"scalar fields differ after step {k}"
+
+
+@pytest.mark.parametrize("ns,pot2", GLOBAL, ids=["n3-son", "n8-double-son"])
+def test_oracle_global_model_reproduces_reference_program_output(ns, pot2):
+    g = np.load(os.path.join(GOLD, f"program_global_n{ns}.npz"))
+    n = (6, 5, 7)
+    N = int(np.prod(n))
+    s = O.random_uniform(2 * N * ns, 7, -0.8, 0.8).reshape(2, N, ns)
+    ref, obs, _ = O.evolve(global_desc(n, ns, pot2), s, None, 12, energy_freq=3)
+    fmt = np.vectorize(lambda x: float("%.6g" % x))
+    assert ref.shape[1] == 6
+    assert np.array_equal(fmt(ref), g["energy"])
+
+
+def test_fp64_global_model_free_field_dispersion_and_energy():
+    """Closed forms: with lambda = 0 the global model is N free fields; a plane
+    wave cos(k.x - w t) with the leapfrog dispersion
+    4 sin^2(w dt/2)/dt^2 = sum_a 4 sin^2(k_a dx_a/2)/dx_a^2 stays a plane wave, and
+    the FP64 energy is conserved to the leapfrog's O(dt^2) oscillation."""
+    n = (8, 6, 10)
+    dxs = (0.5, 0.4, 0.6)
+    dt = 0.05
+    d = global_desc(n, 2, None, d=dxs)
+    d.lambda_ = 0.0
+    d.dt = dt
+    k = [2 * np.pi * m / (ni * dx) for m, ni, dx in zip((1, 2, 1), n, dxs)]
+    om = 2 / dt * np.arcsin(dt / 2 * np.sqrt(sum(4 / dx ** 2 * np.sin(ki * dx / 2) ** 2 for ki, dx in zip(k, dxs))))
+    x = np.stack(np.meshgrid(*[np.arange(ni) * dx for ni, dx in zip(n, dxs)], indexing="ij"), -1).reshape(-1, 3)
+    ph = x @ np.array(k)
+    s = np.zeros((2, x.shape[0], 2))
+    s[1, :, 0] = np.cos(ph)              # time 0 (current slot at time_iter 0)
+    s[0, :, 0] = np.cos(ph + om * dt)    # time -dt
+    s[1, :, 1] = 0.5 * np.sin(ph)
+    s[0, :, 1] = 0.5 * np.sin(ph + om * dt)
+    _, obs, _ = O.evolve(d, s, None, 20, energy_freq=5)
+    cur = (20 + 1) % 2
+    np.testing.assert_allclose(s[cur, :, 0], np.cos(ph - om * 20 * dt), atol=1e-10)
+    np.testing.assert_allclose(s[cur, :, 1], 0.5 * np.sin(ph - om * 20 * dt), atol=1e-10)
+    e = np.array([o.energy[0] for o in obs])
+    assert e.min() > 0 and np.ptp(e) / e.mean() < 1e-2
+
+
 @pytest.mark.parametrize("case,mask", [("xz", 1 | 2 | 16 | 32), ("y", 4 | 8)], ids=["xz", "y"])
 def test_oracle_reproduces_reference_program_with_fixed_faces(case, mask):
     """Fixed faces (Fixed.cpp: frozen one-site shells, left out of the
