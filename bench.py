@@ -89,28 +89,39 @@ def measured_leg(rt, desc, init, launch, steps=1000, every=100, warmup=10):
             "energy_and_gauss_every": every, "measurements": len(rows)}
 
 
-def cpu_baseline(nsteps=60):
-    """The reference program (oracle/_ref/gft_reference, built from the
-    reference's own sources) on a bounded sample: SM 32^3, `nsteps` steps,
-    analysers off, one core; its own 'Evolution finished in' timer."""
+def reference_program_rate(model=0, n=32, nsteps=60, ns=4, nvd=5, pot2=None, gen=False):
+    """Site-updates/s of the reference program (oracle/_ref/gft_reference, built
+    from the reference's own sources) on a bounded sample, analysers off, one
+    core, by its own 'Evolution finished in' timer; None if it is not built."""
     exe = os.path.join(ROOT, "oracle", "_ref", "gft_reference")
+    if not os.path.exists(exe):
+        return None
+    import tempfile
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import refcfg
+    with tempfile.TemporaryDirectory() as td:
+        refcfg.prepare(td)
+        refcfg.write_model(td, model, 0.65, 0.35, lam=0.13, eta_sqr=1.0, pot2=pot2, gen=gen)
+        refcfg.write_lattice(td, (n, n, n), (0.5, 0.5, 0.5), nsteps, 0.05, ns, energy=False, gauss=False,
+                             progress=True, ic_range=(-0.1, 0.1), nvd=nvd)
+        r = subprocess.run([exe], capture_output=True, text=True, timeout=600, cwd=td)
+    m = re.findall(r"Evolution finished in ([0-9.eE+-]+) seconds", r.stdout)
+    if r.returncode != 0 or not m:
+        return None
+    sec = float(m[-1])
+    return n ** 3 * nsteps / sec, sec
+
+
+def cpu_baseline(nsteps=60):
+    """The reference program on a bounded sample: SM 32^3, `nsteps` steps,
+    analysers off, one core (the CPU restatement if the program is not built)."""
     n = 32
-    if os.path.exists(exe):
-        import tempfile
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import refcfg
-        with tempfile.TemporaryDirectory() as td:
-            refcfg.prepare(td)
-            refcfg.write_model(td, 0, 0.65, 0.35, lam=0.13, eta_sqr=1.0)
-            refcfg.write_lattice(td, (n, n, n), (0.5, 0.5, 0.5), nsteps, 0.05, 4, energy=False, gauss=False,
-                                 progress=True, ic_range=(-0.1, 0.1))
-            r = subprocess.run([exe], capture_output=True, text=True, timeout=600, cwd=td)
-        m = re.findall(r"Evolution finished in ([0-9.eE+-]+) seconds", r.stdout)
-        if r.returncode == 0 and m:
-            sec = float(m[-1])
-            return {"value": n ** 3 * nsteps / sec, "unit": "site-updates/s", "cores": 1, "kind": "reference",
-                    "sample": f"reference program built from its sources, SM {n}^3, {nsteps} steps, analysers off, "
-                              f"single-threaded on one host core ({sec:.2f} s)"}
+    ref = reference_program_rate(0, n, nsteps)
+    if ref:
+        rate, sec = ref
+        return {"value": rate, "unit": "site-updates/s", "cores": 1, "kind": "reference",
+                "sample": f"reference program built from its sources, SM {n}^3, {nsteps} steps, analysers off, "
+                          f"single-threaded on one host core ({sec:.2f} s)"}
     # Fallback: the FP64 CPU restatement on one thread.
     import numpy as np
     from gftevolver_amd._abi import make_desc
@@ -123,6 +134,88 @@ def cpu_baseline(nsteps=60):
     sec = time.perf_counter() - t0
     return {"value": n ** 3 * nsteps / sec, "unit": "site-updates/s", "cores": 1, "kind": "port",
             "sample": f"CPU restatement (FP64 two-slot), SM {n}^3, {nsteps} steps, one thread ({sec:.2f} s)"}
+
+
+def options_leg(rt, with_cpu):
+    """The reference's other model options on the GPU, each beside the
+    reference program on the host (one core, 32^3 sample): fixed faces in the
+    fast path, the generator form (boundary conversion and the two-slot
+    kernel), the 2HDM with Higgs-family U(1) / SU(2) gauge fields and the
+    global O(4) model in the two-slot forms."""
+    import numpy as np
+    from gftevolver_amd._abi import (GFT_FORM_MOMENTUM_F32, GFT_FORM_TWO_SLOT_F32, GFT_MODEL_2HDM,
+                                     GFT_MODEL_GLOBAL, TWO_HDM_POTENTIAL, make_desc)
+    out = {}
+
+    def timed(desc, steps, init=None, state=None):
+        with rt.Evolver(desc) as ev:
+            if init:
+                init(ev)
+            else:
+                ev.import_two_slot(*state, 0)
+            ev.evolve(2)
+            ev.sync()
+            t0 = time.perf_counter()
+            ev.evolve(steps)
+            ev.sync()
+            wall = time.perf_counter() - t0
+            kms = ev.last_step_ms()[0]
+        N = desc.nx * desc.ny * desc.nz
+        return {"value": N * steps / wall, "unit": "site-updates/s", "kernel_ms": kms, "steps": steps,
+                "lattice": [desc.nx, desc.ny, desc.nz]}
+
+    def grf(ev):
+        ev.init_grf(seed=20261203, rms=0.1, slope=-1.0)
+
+    n256 = (256,) * 3
+    out["fixed_faces_xz"] = timed(make_desc(form=GFT_FORM_MOMENTUM_F32, n=n256, fixed_faces=1 | 2 | 16 | 32), 200,
+                                  grf)
+    # Generator form: the fast path converts at the boundary; the two-slot kernel steps it as the reference.
+    n128 = (128,) * 3
+    N = 128 ** 3
+    rng = np.random.default_rng(1)
+    s = np.zeros((2, N, 4), np.float32)
+    s[:] = (rng.normal(size=(N, 4)) * 0.1).astype(np.float32)
+    s[:, :, 2] += 1.0
+    vg = np.zeros((2, N, 12), np.float32)
+    dg = make_desc(form=GFT_FORM_MOMENTUM_F32, n=n128, gen_rep=1)
+    with rt.Evolver(dg) as ev:
+        t0 = time.perf_counter()
+        ev.import_two_slot(s, vg, 0)
+        t1 = time.perf_counter()
+        ev.export_two_slot(np.float32)
+        t2 = time.perf_counter()
+    out["generator_form_boundary"] = {"lattice": list(n128), "import_s": t1 - t0, "export_s": t2 - t1}
+    out["generator_form_two_slot_f32"] = timed(make_desc(form=GFT_FORM_TWO_SLOT_F32, n=n128, gen_rep=1), 50,
+                                               state=(s, vg))
+    vq = np.zeros((2, N, 15), np.float32)
+    vq[..., 1::5] = 1.0
+    out["quaternion_two_slot_f32"] = timed(make_desc(form=GFT_FORM_TWO_SLOT_F32, n=n128), 50, state=(s, vq))
+    s8 = np.concatenate([s, s * 0.75], axis=-1)
+    for g, nvd in ((4, 5), (5, 6), (7, 9)):
+        v = np.zeros((2, N, 3, nvd), np.float32)
+        v[..., 1] = 1.0
+        if g == 7:
+            v[..., 5] = 1.0
+        d = make_desc(model=GFT_MODEL_2HDM, form=GFT_FORM_TWO_SLOT_F32, n=n128, pot2=TWO_HDM_POTENTIAL,
+                      gauge_num=g, gpp=0.5)
+        r = timed(d, 50, state=(s8, v.reshape(2, N, 3 * nvd)))
+        if with_cpu:
+            ref = reference_program_rate(1, 32, 10, 8, nvd, TWO_HDM_POTENTIAL)
+            r["cpu_reference"] = ref[0] if ref else None
+        out[f"2hdm_gauge{g}_two_slot_f32"] = r
+    dgl = make_desc(model=GFT_MODEL_GLOBAL, form=GFT_FORM_TWO_SLOT_F32, n=n256, num_scalar=4, potential=1)
+    Ng = 256 ** 3
+    sg = (np.random.default_rng(2).normal(size=(2, Ng, 4)) * 0.1 + np.array([1.0, 0, 0, 0])).astype(np.float32)
+    sg[0] = sg[1]
+    r = timed(dgl, 100, state=(sg, np.zeros(0, np.float32)))
+    if with_cpu:
+        ref = reference_program_rate(2, 32, 60, 4, 1)
+        r["cpu_reference"] = ref[0] if ref else None
+    out["global_o4_two_slot_f32"] = r
+    for k, v in out.items():
+        log(f"option {k}: {v}")
+    return out
 
 
 def ensemble_leg(rt, steps):
@@ -172,6 +265,8 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=2000,
                     help="steps of the 64-member ensemble workload (0 skips it)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-options", action="store_true",
+                    help="skip the other model options (fixed faces, generator form, gaugeNum 5/7, O(N))")
     ap.add_argument("--only-f32", action="store_true")
     ap.add_argument("--launch", type=str, default="", help="bz,by,order,min_waves[,stage] (skips the autotuner)")
     ap.add_argument("--no-tune", action="store_true")
@@ -263,6 +358,8 @@ def main():
 
     if not args.only_f32 and args.ensemble_steps:
         extra["ensemble"] = ensemble_leg(rt, args.ensemble_steps)
+    if not args.only_f32 and not args.no_options:
+        extra["options"] = options_leg(rt, not args.no_cpu)
 
     cpu = None if args.no_cpu else cpu_baseline()
     traffic = read_traffic()
