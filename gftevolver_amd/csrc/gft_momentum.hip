@@ -319,6 +319,15 @@ __device__ __forceinline__ void transport(const Qt<R>& u, R c, R s, bool conj, c
 
 template <class V4, class R>
 __device__ __forceinline__ void unpack(const V4& v, R* o) { o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w; }
+template <class V4, class R>
+__device__ __forceinline__ V4 pack(const R* i) {
+    V4 v;
+    v.x = i[0];
+    v.y = i[1];
+    v.z = i[2];
+    v.w = i[3];
+    return v;
+}
 
 template <class R>
 struct StepArgs {
@@ -491,9 +500,12 @@ __device__ __forceinline__ void load_own(const StepArgs<R>& a, unsigned s, OwnSt
 // PRE: own state already loaded into o (else loaded here, next to its use).
 // U1: no damping (keep = 1), with the kick's keep products dropped (exact).
 template <class R, int ND, bool CMP, bool PRE = false, bool U1 = false>
+// frozen: a site of a fixed face's shell -- every lane still stores, but a
+// frozen lane stores the values it read, so every store instruction writes
+// whole cache lines (a divergent copy branch would write partial lines twice).
 __device__ __forceinline__ void kick_drift_t(const StepArgs<R>& a, unsigned s, const Qt<R>* U, const R* yv,
                                              const R* phi, const R* F, const R (*J)[4], const R (*M)[4],
-                                             const OwnState<R, ND>* o) {
+                                             const OwnState<R, ND>* o, bool frozen) {
     using V4 = typename V4T<R>::type;
 #pragma unroll
     for (int k = 0; k < ND; k++) {
@@ -519,9 +531,10 @@ __device__ __forceinline__ void kick_drift_t(const StepArgs<R>& a, unsigned s, c
                 fh[m] = cmp_add(phi[4 * k + m], res_val(L.x, m, phi[4 * k + m]), ph[m], res_from_q(qp[m], ph[m]), &fr[m]);
             pn = make_float4(ph[0], ph[1], ph[2], ph[3]);
             fn = make_float4(fh[0], fh[1], fh[2], fh[3]);
-            st_nt(make_uint2(res_pack4(res_q(fr[0], fh[0]), res_q(fr[1], fh[1]), res_q(fr[2], fh[2]), res_q(fr[3], fh[3])),
-                             res_pack4(qp[0], qp[1], qp[2], qp[3])),
-                  ptr_at(a.lo_s[k], s));
+            uint2 Ln = make_uint2(res_pack4(res_q(fr[0], fh[0]), res_q(fr[1], fh[1]), res_q(fr[2], fh[2]), res_q(fr[3], fh[3])),
+                                  res_pack4(qp[0], qp[1], qp[2], qp[3]));
+            if (frozen) Ln = L;
+            st_nt(Ln, ptr_at(a.lo_s[k], s));
         } else {
             // Explicit FMAs fix which product is fused (exact keep * p for keep = 1).
             const R kp[4] = {U1 ? p[0] : a.keep * p[0], U1 ? p[1] : a.keep * p[1], U1 ? p[2] : a.keep * p[2],
@@ -534,6 +547,10 @@ __device__ __forceinline__ void kick_drift_t(const StepArgs<R>& a, unsigned s, c
             fn.y = phi[4 * k + 1] + pn.y;
             fn.z = phi[4 * k + 2] + pn.z;
             fn.w = phi[4 * k + 3] + pn.w;
+        }
+        if (frozen) {
+            pn = pack<V4, R>(p);
+            fn = pack<V4, R>(phi + 4 * k);
         }
         st_nt(pn, ptr_at(a.P[k], s));
         st_nt(fn, ptr_at(a.phi_out[k], s));
@@ -555,7 +572,7 @@ __device__ __forceinline__ void kick_drift_t(const StepArgs<R>& a, unsigned s, c
                 if constexpr (U1) eh[m] = cmp_update1(e[m], res_val(L, m, e[m]), a.dt2 * (M[i][m] + a.gsq[m] * J[i][m]), &er[m]);
                 else eh[m] = cmp_update(e[m], res_val(L, m, e[m]), a.keep, a.dt2 * (M[i][m] + a.gsq[m] * J[i][m]), &er[m]);
             en = make_float4(eh[0], eh[1], eh[2], eh[3]);
-            st_nt(res_pack4(res_q(er[0], eh[0]), res_q(er[1], eh[1]), res_q(er[2], eh[2]), res_q(er[3], eh[3])),
+            st_nt(frozen ? L : res_pack4(res_q(er[0], eh[0]), res_q(er[1], eh[1]), res_q(er[2], eh[2]), res_q(er[3], eh[3])),
                   ptr_at(a.lo_E[i], s));
         } else {
             en.x = fma(a.dt2, fma(a.gsq[0], J[i][0], M[i][0]), U1 ? e[0] : a.keep * e[0]);
@@ -564,7 +581,7 @@ __device__ __forceinline__ void kick_drift_t(const StepArgs<R>& a, unsigned s, c
             en.w = fma(a.dt2, fma(a.gsq[3], J[i][3], M[i][3]), U1 ? e[3] : a.keep * e[3]);
         }
         const R e2 = fma(en.w, en.w, fma(en.z, en.z, en.y * en.y));
-        if (!(e2 < R(1)) || !(fabs(en.x) < R(1))) bad = 1;
+        if (!frozen && (!(e2 < R(1)) || !(fabs(en.x) < R(1)))) bad = 1;
         const R yn = wrap_pi(yv[i] + asin_small(en.x));
         const Qt<R> u{sqrt(R(1) - e2), en.y, en.z, en.w};
         Qt<R> qn = qmul(u, U[i]);
@@ -574,8 +591,15 @@ __device__ __forceinline__ void kick_drift_t(const StepArgs<R>& a, unsigned s, c
         qo.y = qn.x * inv;
         qo.z = qn.y * inv;
         qo.w = qn.z * inv;
+        if (frozen) {
+            en = pack<V4, R>(e);
+            qo.x = U[i].w;
+            qo.y = U[i].x;
+            qo.z = U[i].y;
+            qo.w = U[i].z;
+        }
         st_nt(en, ptr_at(a.E[i], s));
-        st_nt(yn, ptr_at(a.y_out[i], s));
+        st_nt(frozen ? yv[i] : yn, ptr_at(a.y_out[i], s));
         st_nt(qo, ptr_at(a.q_out[i], s));
     }
     if (bad) atomicOr(a.err, 1u);
@@ -585,9 +609,9 @@ __device__ __forceinline__ void kick_drift_t(const StepArgs<R>& a, unsigned s, c
 template <class R, int ND, bool CMP, bool PRE = false>
 __device__ __forceinline__ void kick_drift(const StepArgs<R>& a, unsigned s, const Qt<R>* U, const R* yv,
                                            const R* phi, const R* F, const R (*J)[4], const R (*M)[4],
-                                           const OwnState<R, ND>* o = nullptr) {
-    if (a.keep == R(1)) kick_drift_t<R, ND, CMP, PRE, true>(a, s, U, yv, phi, F, J, M, o);
-    else kick_drift_t<R, ND, CMP, PRE, false>(a, s, U, yv, phi, F, J, M, o);
+                                           const OwnState<R, ND>* o = nullptr, bool frozen = false) {
+    if (a.keep == R(1)) kick_drift_t<R, ND, CMP, PRE, true>(a, s, U, yv, phi, F, J, M, o, frozen);
+    else kick_drift_t<R, ND, CMP, PRE, false>(a, s, U, yv, phi, F, J, M, o, frozen);
 }
 
 // One site-update at (x, y, z): forces, kick and drift (shared by the
@@ -602,16 +626,8 @@ __device__ __forceinline__ void mom_site(const StepArgs<R>& a, const Coef& c, in
     // become a scalar base plus one 32-bit vector offset, with no 64-bit math.
     const int sy = a.nz, sx = a.ny * a.nz;
     const unsigned s = static_cast<unsigned>(x * sx + y * sy + z);
-    if (site_frozen(c, x, y, z)) {   // shell of a fixed face: carried into the next buffer unchanged
-#pragma unroll
-        for (int k = 0; k < ND; k++) st_nt(at(a.phi[k], s), ptr_at(a.phi_out[k], s));
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-            st_nt(at(a.y[i], s), ptr_at(a.y_out[i], s));
-            st_nt(at(a.q[i], s), ptr_at(a.q_out[i], s));
-        }
-        return;
-    }
+    // Shell of a fixed face: computed like any site, then carried over unchanged.
+    const bool frozen = site_frozen(c, x, y, z);
     // Offsets to the +e_a and -e_a neighbours with periodic wrap.
     int dp[3], dm[3];
     dp[0] = (x == a.nx - 1) ? -(a.nx - 1) * sx : sx;
@@ -779,7 +795,7 @@ __device__ __forceinline__ void mom_site(const StepArgs<R>& a, const Coef& c, in
         }
     }
 
-    kick_drift<R, ND, CMP>(a, s, U, yv, phi, F, J, M);
+    kick_drift<R, ND, CMP>(a, s, U, yv, phi, F, J, M, nullptr, frozen);
 }
 
 // ---- shared plaquettes and covariant differences (FP32 SM, staged) ----------
@@ -1021,17 +1037,10 @@ __device__ __forceinline__ void mom_block_shared(const StepArgs<float>& a, const
     }
     __syncthreads();
     if (!valid) return;
-    if (site_frozen(c, x, y, z)) {
-        // Shell of a fixed face: its plaquettes and differences served the
-        // neighbours above; the site itself is carried over unchanged.
-        st_nt(make_float4(phi[0], phi[1], phi[2], phi[3]), ptr_at(a.phi_out[0], s));
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-            st_nt(yv[i], ptr_at(a.y_out[i], s));
-            st_nt(make_float4(U[i].w, U[i].x, U[i].y, U[i].z), ptr_at(a.q_out[i], s));
-        }
-        return;
-    }
+    // Shell of a fixed face: its plaquettes and differences served the
+    // neighbours above; the site is computed like any other, then carried over
+    // unchanged by the kick's stores.
+    const bool frozen = site_frozen(c, x, y, z);
 
     // ---- phase B: the scalar force's i = 1, 2 terms ----
     {
@@ -1065,7 +1074,7 @@ __device__ __forceinline__ void mom_block_shared(const StepArgs<float>& a, const
         M[1][2] -= a.inv_sp[2] * p12b.y;
         M[1][3] -= a.inv_sp[2] * p12b.z;
     }
-    kick_drift<R, 1, CMP, true>(a, s, U, yv, phi, F, J, M, &own);
+    kick_drift<R, 1, CMP, true>(a, s, U, yv, phi, F, J, M, &own, frozen);
 }
 
 // MINW: minimum waves per SIMD the register allocation must allow (0: the
