@@ -3,7 +3,7 @@
 
 One step = one leapfrog update of every site of the lattice (all fields).
 The timed region runs exactly --steps steps (default 10^4) of the FP32-bulk
-momentum-form kernel (FP32 arithmetic, phi/P/E stored as FP32 + 8-bit
+momentum-form kernel (FP32 arithmetic, phi/P/E stored as FP32 + 4-bit
 residual) on data already resident in HBM, bracketed by stream
 synchronisations; `value` is site-updates per second over that wall-clock
 interval, and the Gauss-law violation is measured right after it.  Beside it:
@@ -30,7 +30,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "site-updates/s + HBM GB/s, FP32-mixed vs FP64 at Gauss err<=1e-6, 256^3 SM"
 BYTES_F32 = 280   # 35 reals read + 35 written per site-update, 4 B each (algorithmic)
-STORED_F32C = 320  # what the compensated form moves: + 8-bit residuals of phi, P, E (20 x 1 B each way)
+STORED_F32C = 300  # what the compensated form moves: + 4-bit residuals of phi, P, E (20 x 0.5 B each way)
 BYTES_F64 = 560
 HBM_PEAK_GBS = 8000.0
 
@@ -378,7 +378,7 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"SM {args.n}^3 periodic, FP32-bulk momentum-form leapfrog (FP32 arithmetic, phi/P/E "
-                        f"stored FP32 + 8-bit residual); GRF Higgs IC "
+                        f"stored FP32 + 4-bit residual); GRF Higgs IC "
                         f"(P(k)~1/k, k<=pi/(2dx), rms 0.1v), links identity, E=0; dx=0.5 dt=0.05 g=0.65 "
                         f"g'=0.35 lambda=0.13 v=1",
             "lattice": list(n),

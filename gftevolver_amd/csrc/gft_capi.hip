@@ -707,8 +707,8 @@ __global__ void __launch_bounds__(256) gft_stream_kernel(StepArgs<float> a) {
     const unsigned s = static_cast<unsigned>((tile.y * a.ny + ys) * a.nz + zs);
     float4 q[3], f = at(a.phi[0], s), p = ld_nt(ptr_at(a.P[0], s)), e[3];
     float y[3];
-    uint2 ls = make_uint2(0, 0);
-    unsigned le[3] = {0u, 0u, 0u};
+    unsigned ls = 0u;
+    unsigned short le[3] = {0, 0, 0};
 #pragma unroll
     for (int k = 0; k < 3; k++) {
         q[k] = at(a.q[k], s);
@@ -722,13 +722,13 @@ __global__ void __launch_bounds__(256) gft_stream_kernel(StepArgs<float> a) {
     f.x += t;
     st_nt(p, ptr_at(a.P[0], s));
     st_nt(f, ptr_at(a.phi_out[0], s));
-    if (CMP) st_nt(make_uint2(ls.x + 1u, ls.y), ptr_at(a.lo_s[0], s));
+    if (CMP) st_nt(ls + 1u, ptr_at(a.lo_s[0], s));
 #pragma unroll
     for (int k = 0; k < 3; k++) {
         e[k].x += t;
         q[k].x += t;
         st_nt(e[k], ptr_at(a.E[k], s));
-        if (CMP) st_nt(le[k] + 1u, ptr_at(a.lo_E[k], s));
+        if (CMP) st_nt(static_cast<unsigned short>(le[k] + 1u), ptr_at(a.lo_E[k], s));
         st_nt(y[k] + t, ptr_at(a.y_out[k], s));
         st_nt(q[k], ptr_at(a.q_out[k], s));
     }
@@ -1014,10 +1014,10 @@ __global__ void __launch_bounds__(256) grf_sumsq4(const double* __restrict__ f, 
     block_reduce_store<1, 0>(v, nullptr, lds, partial, blockIdx.x);
 }
 
-// 8-bit residual of v next to its FP32 hi, as the host import encodes it.
+// 4-bit residual of v next to its FP32 hi, as the host import encodes it.
 __device__ inline unsigned grf_res_q(double v, float hi) {
-    const double q = rint(ldexp(v - (double)hi, 157 - res_exp(hi)));
-    return static_cast<unsigned>(static_cast<int>(fmin(127.0, fmax(-127.0, q)))) & 0xffu;
+    const double q = rint(ldexp(v - (double)hi, 153 - res_exp(hi)));
+    return static_cast<unsigned>(static_cast<int>(fmin(7.0, fmax(-8.0, q)))) & 0xfu;
 }
 
 // phi_k = (0, 0, vev_k, 0) + scale_k * comps, with scale_k = rms / sqrt(<sum_c f_c^2>)
@@ -1041,8 +1041,8 @@ __global__ void __launch_bounds__(256) grf_assemble_mom(MomFields<R> f, const do
         f.P[k][s] = z;
         if (f.lo_s[k]) {
             unsigned w = 0;
-            for (int m = 0; m < 4; m++) w |= grf_res_q(v[m], (float)((&hi.x)[m])) << (8 * m);
-            f.lo_s[k][s] = make_uint2(w, 0u);
+            for (int m = 0; m < 4; m++) w |= grf_res_q(v[m], (float)((&hi.x)[m])) << (4 * m);
+            f.lo_s[k][s] = w;   // P = 0: no P residuals
         }
     }
     for (int i = 0; i < 3; i++) {
@@ -1052,7 +1052,7 @@ __global__ void __launch_bounds__(256) grf_assemble_mom(MomFields<R> f, const do
         f.E[i][s] = e;
         f.q[0][i][s] = q;
         f.y[0][i][s] = R(0);
-        if (f.lo_E[i]) f.lo_E[i][s] = 0u;
+        if (f.lo_E[i]) f.lo_E[i][s] = 0;
     }
 }
 

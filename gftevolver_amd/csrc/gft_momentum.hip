@@ -52,17 +52,18 @@ struct MomFields {
     R* y[2][3];      // [buffer][direction]
     V4* q[2][3];     // [buffer][direction]
     V4* E[3];        // [direction]
-    // Compensated FP32 storage only: 8-bit residuals of (phi, P) per doublet
-    // (x: phi_lo[4], y: P_lo[4], one byte each) and of E per direction; null otherwise.
-    uint2* lo_s[2];      // [doublet]
-    unsigned* lo_E[3];   // [direction]
+    // Compensated FP32 storage only: 4-bit residuals of (phi, P) per doublet
+    // (bits 4m..4m+3: phi_lo[m], 16 + 4m..: P_lo[m]) and of E per direction
+    // (bits 4m..4m+3: E_lo[m]); null otherwise.
+    unsigned* lo_s[2];         // [doublet]
+    unsigned short* lo_E[3];   // [direction]
 };
 
 struct MomentumState {
     void* base = nullptr;
     size_t bytes = 0;
     bool f64 = false;
-    bool cmp = false;        // FP32 storage with 8-bit residuals of phi, P and E
+    bool cmp = false;        // FP32 storage with 4-bit residuals of phi, P and E
     int cur = 0;             // buffer holding (phi, U) at the current time
     int prev_valid = 0;      // buffer !cur holds the state one step earlier
     MomFields<float> ff{};
@@ -165,6 +166,8 @@ __device__ __forceinline__ void st_nt(float v, float* p) { __builtin_nontemporal
 __device__ __forceinline__ unsigned ld_nt(const unsigned* p) { return __builtin_nontemporal_load(p); }
 __device__ __forceinline__ void st_nt(unsigned v, unsigned* p) { __builtin_nontemporal_store(v, p); }
 __device__ __forceinline__ void st_nt(double v, double* p) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ unsigned short ld_nt(const unsigned short* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ void st_nt(unsigned short v, unsigned short* p) { __builtin_nontemporal_store(v, p); }
 
 template <class R>
 __device__ __forceinline__ R wrap_pi(R y) {
@@ -236,28 +239,33 @@ __device__ __forceinline__ void qrot(const Qt<R>& q, const R* v, R* o) {
 }
 
 // ---- compensated FP32 storage ------------------------------------------------
-// A stored value is hi + lo: hi the FP32 value the stencil reads, lo an 8-bit
+// A stored value is hi + lo: hi the FP32 value the stencil reads, lo a 4-bit
 // residual kept at the owning site only, in units of hi's ulp:
-//   lo = q ulp(hi) / 2^7,  ulp(hi) = 2^(e - 150) for hi's biased exponent e,
-// with q an integer in [-127, 127] (|lo| <= ulp(hi) / 2 gives |q| <= 64 for a
-// normal hi).  Updates are error-free transformations (TwoSum, exact products
-// by FMA), so the only rounding per step is that of the residual to q, at most
-// 2^-8 ulp(hi): the state keeps about 31 significant bits for one extra byte
-// per value, while the kernel's arithmetic and neighbour traffic stay FP32.
+//   lo = q ulp(hi) / 2^3,  ulp(hi) = 2^(e - 150) for hi's biased exponent e,
+// with q an integer in [-8, 7]; |lo| <= ulp(hi) / 2 gives |q| <= 4, so the
+// rounding of the residual to q (nearest even) is never clamped and carries no
+// bias.  (Units of ulp/2^4 would need q = 8 for residuals near a tie; clamping
+// those to 7 biases every update the same way and the Gauss-law violation then
+// grows linearly.)  Updates are error-free transformations (TwoSum, exact
+// products by FMA), so the only rounding per step is that of the residual, at
+// most 2^-4 ulp(hi): the state keeps about 28 significant bits for half a byte
+// per value -- below the random walk of the FP32 force arithmetic in the Gauss
+// law -- while the kernel's arithmetic and neighbour traffic stay FP32.
 __device__ __forceinline__ int res_exp(float hi) { return static_cast<int>((__float_as_uint(hi) >> 23) & 0xffu); }
-// q of residual r next to hi (round to nearest even, clamped).
+// q of residual r next to hi (round to nearest even; the clamp is a guard).
 __device__ __forceinline__ int res_q(float r, float hi) {
-    const int q = static_cast<int>(rintf(ldexpf(r, 157 - res_exp(hi))));
-    return min(max(q, -127), 127);
+    const int q = static_cast<int>(rintf(ldexpf(r, 153 - res_exp(hi))));
+    return min(max(q, -8), 7);
 }
-__device__ __forceinline__ float res_from_q(int q, float hi) { return ldexpf(static_cast<float>(q), res_exp(hi) - 157); }
-// Four q in one word (byte m = value m) and back.
+__device__ __forceinline__ float res_from_q(int q, float hi) { return ldexpf(static_cast<float>(q), res_exp(hi) - 153); }
+// Four q in 16 bits (nibble m = value m) and back; nibbles 4..7 of a 32-bit
+// word are a second group of four (the P residuals of a doublet word).
 __device__ __forceinline__ unsigned res_pack4(int q0, int q1, int q2, int q3) {
-    return (static_cast<unsigned>(q0) & 0xffu) | ((static_cast<unsigned>(q1) & 0xffu) << 8) |
-           ((static_cast<unsigned>(q2) & 0xffu) << 16) | (static_cast<unsigned>(q3) << 24);
+    return (static_cast<unsigned>(q0) & 0xfu) | ((static_cast<unsigned>(q1) & 0xfu) << 4) |
+           ((static_cast<unsigned>(q2) & 0xfu) << 8) | ((static_cast<unsigned>(q3) & 0xfu) << 12);
 }
-__device__ __forceinline__ int res_byte(unsigned w, int m) { return static_cast<int>(w << (24 - 8 * m)) >> 24; }
-__device__ __forceinline__ float res_val(unsigned w, int m, float hi) { return res_from_q(res_byte(w, m), hi); }
+__device__ __forceinline__ int res_nib(unsigned w, int m) { return static_cast<int>(w << (28 - 4 * m)) >> 28; }
+__device__ __forceinline__ float res_val(unsigned w, int m, float hi) { return res_from_q(res_nib(w, m), hi); }
 
 // (h, l) <- keep (h + l) + delta; returns the new hi, *r the exact residual.
 __device__ __forceinline__ float cmp_update(float h, float l, float keep, float delta, float* r) {
@@ -340,8 +348,8 @@ struct StepArgs {
     const V4* q[3];
     V4* q_out[3];
     V4* E[3];
-    uint2* lo_s[2];          // compensated storage: residuals of (phi, P) and E
-    unsigned* lo_E[3];
+    unsigned* lo_s[2];       // compensated storage: residuals of (phi, P) and E
+    unsigned short* lo_E[3];
     int nx, ny, nz, nbz, nby;
     long long nblocks;
     int bz, by, order, bz_shift;   // bz = 1 << bz_shift
@@ -479,9 +487,9 @@ __device__ __forceinline__ void plane_fill(const StepArgs<float>& a, const Plane
 template <class R, int ND>
 struct OwnState {
     typename V4T<R>::type P[ND];
-    uint2 ls[ND];
+    unsigned ls[ND];
     typename V4T<R>::type E[3];
-    unsigned lE[3];
+    unsigned short lE[3];
 };
 template <class R, int ND, bool CMP>
 __device__ __forceinline__ void load_own(const StepArgs<R>& a, unsigned s, OwnState<R, ND>& o) {
@@ -514,25 +522,25 @@ __device__ __forceinline__ void kick_drift_t(const StepArgs<R>& a, unsigned s, c
         else unpack(ld_nt(ptr_at(a.P[k], s)), p);
         V4 pn, fn;
         if constexpr (CMP) {
-            uint2 L;
+            unsigned L;   // nibbles 0..3: phi residuals, 4..7: P residuals
             if constexpr (PRE) L = o->ls[k];
             else L = ld_nt(ptr_at(a.lo_s[k], s));
             float ph[4], pr[4], fh[4], fr[4];
             int qp[4];
 #pragma unroll
             for (int m = 0; m < 4; m++) {
-                if constexpr (U1) ph[m] = cmp_update1(p[m], res_val(L.y, m, p[m]), a.dt2 * F[4 * k + m], &pr[m]);
-                else ph[m] = cmp_update(p[m], res_val(L.y, m, p[m]), a.keep, a.dt2 * F[4 * k + m], &pr[m]);
+                if constexpr (U1) ph[m] = cmp_update1(p[m], res_val(L, 4 + m, p[m]), a.dt2 * F[4 * k + m], &pr[m]);
+                else ph[m] = cmp_update(p[m], res_val(L, 4 + m, p[m]), a.keep, a.dt2 * F[4 * k + m], &pr[m]);
                 qp[m] = res_q(pr[m], ph[m]);
             }
             // phi(t+dt) = phi(t) + P(t+dt) with P as stored (its residual rounded).
 #pragma unroll
             for (int m = 0; m < 4; m++)
-                fh[m] = cmp_add(phi[4 * k + m], res_val(L.x, m, phi[4 * k + m]), ph[m], res_from_q(qp[m], ph[m]), &fr[m]);
+                fh[m] = cmp_add(phi[4 * k + m], res_val(L, m, phi[4 * k + m]), ph[m], res_from_q(qp[m], ph[m]), &fr[m]);
             pn = make_float4(ph[0], ph[1], ph[2], ph[3]);
             fn = make_float4(fh[0], fh[1], fh[2], fh[3]);
-            uint2 Ln = make_uint2(res_pack4(res_q(fr[0], fh[0]), res_q(fr[1], fh[1]), res_q(fr[2], fh[2]), res_q(fr[3], fh[3])),
-                                  res_pack4(qp[0], qp[1], qp[2], qp[3]));
+            unsigned Ln = res_pack4(res_q(fr[0], fh[0]), res_q(fr[1], fh[1]), res_q(fr[2], fh[2]), res_q(fr[3], fh[3]))
+                        | (res_pack4(qp[0], qp[1], qp[2], qp[3]) << 16);
             if (frozen) Ln = L;
             st_nt(Ln, ptr_at(a.lo_s[k], s));
         } else {
@@ -563,7 +571,7 @@ __device__ __forceinline__ void kick_drift_t(const StepArgs<R>& a, unsigned s, c
         else unpack(ld_nt(ptr_at(a.E[i], s)), e);
         V4 en;
         if constexpr (CMP) {
-            unsigned L;
+            unsigned short L;
             if constexpr (PRE) L = o->lE[i];
             else L = ld_nt(ptr_at(a.lo_E[i], s));
             float eh[4], er[4];
@@ -572,7 +580,9 @@ __device__ __forceinline__ void kick_drift_t(const StepArgs<R>& a, unsigned s, c
                 if constexpr (U1) eh[m] = cmp_update1(e[m], res_val(L, m, e[m]), a.dt2 * (M[i][m] + a.gsq[m] * J[i][m]), &er[m]);
                 else eh[m] = cmp_update(e[m], res_val(L, m, e[m]), a.keep, a.dt2 * (M[i][m] + a.gsq[m] * J[i][m]), &er[m]);
             en = make_float4(eh[0], eh[1], eh[2], eh[3]);
-            st_nt(frozen ? L : res_pack4(res_q(er[0], eh[0]), res_q(er[1], eh[1]), res_q(er[2], eh[2]), res_q(er[3], eh[3])),
+            st_nt(frozen ? L
+                         : static_cast<unsigned short>(res_pack4(res_q(er[0], eh[0]), res_q(er[1], eh[1]), res_q(er[2], eh[2]),
+                                                                 res_q(er[3], eh[3]))),
                   ptr_at(a.lo_E[i], s));
         } else {
             en.x = fma(a.dt2, fma(a.gsq[0], J[i][0], M[i][0]), U1 ? e[0] : a.keep * e[0]);
@@ -1165,8 +1175,8 @@ struct ObsArgs {
     const V4* q[3];
     const V4* q_old[3];     // links one step earlier (Gauss only)
     const V4* E[3];
-    const uint2* lo_s[2];      // residuals of compensated storage (null otherwise)
-    const unsigned* lo_E[3];
+    const unsigned* lo_s[2];         // residuals of compensated storage (null otherwise)
+    const unsigned short* lo_E[3];
     int nx, ny, nz;
 };
 
@@ -1183,30 +1193,31 @@ __device__ __forceinline__ void load_d(const V4& v, double* o) {
     o[0] = (double)v.x; o[1] = (double)v.y; o[2] = (double)v.z; o[3] = (double)v.w;
 }
 
-// Full-precision values of the compensated FP32 storage (hi + 8-bit residual;
-// exact in FP64), or the stored values when there are no residuals.
+// Full-precision values of the compensated FP32 storage (hi + 4-bit residual;
+// exact in FP64), or the stored values when there are no residuals.  Nibbles
+// m0 .. m0 + 3 of w belong to the four values.
 template <class V4>
-__device__ __forceinline__ void obs_add_res(const V4& hi, unsigned w, double* o) {
+__device__ __forceinline__ void obs_add_res(const V4& hi, unsigned w, int m0, double* o) {
     const float h[4] = {(float)hi.x, (float)hi.y, (float)hi.z, (float)hi.w};
-    for (int m = 0; m < 4; m++) o[m] += ldexp((double)res_byte(w, m), res_exp(h[m]) - 157);
+    for (int m = 0; m < 4; m++) o[m] += ldexp((double)res_nib(w, m0 + m), res_exp(h[m]) - 153);
 }
 template <class R>
 __device__ __forceinline__ void obs_phi(const ObsArgs<R>& a, int k, long long s, double* o) {
     const auto hi = a.phi[k][s];
     load_d(hi, o);
-    if (a.lo_s[k]) obs_add_res(hi, a.lo_s[k][s].x, o);
+    if (a.lo_s[k]) obs_add_res(hi, a.lo_s[k][s], 0, o);
 }
 template <class R>
 __device__ __forceinline__ void obs_P(const ObsArgs<R>& a, int k, long long s, double* o) {
     const auto hi = a.P[k][s];
     load_d(hi, o);
-    if (a.lo_s[k]) obs_add_res(hi, a.lo_s[k][s].y, o);
+    if (a.lo_s[k]) obs_add_res(hi, a.lo_s[k][s], 4, o);
 }
 template <class R>
 __device__ __forceinline__ void obs_E(const ObsArgs<R>& a, int i, long long s, double* o) {
     const auto hi = a.E[i][s];
     load_d(hi, o);
-    if (a.lo_E[i]) obs_add_res(hi, a.lo_E[i][s], o);
+    if (a.lo_E[i]) obs_add_res(hi, a.lo_E[i][s], 0, o);
 }
 
 // 1 - sqrt(1 - x) without cancellation.
@@ -1423,8 +1434,11 @@ inline size_t carve(MomFields<R>* f, char* base, long long N, bool cmp) {
     for (int b = 0; b < 2; b++)
         for (int i = 0; i < 3; i++) { char* p = take(N, sizeof(V4)); if (f) f->q[b][i] = reinterpret_cast<V4*>(p); }
     for (int i = 0; i < 3; i++) { char* p = take(N, sizeof(V4)); if (f) f->E[i] = reinterpret_cast<V4*>(p); }
-    for (int k = 0; k < 2; k++) { char* p = cmp ? take(N, sizeof(uint2)) : nullptr; if (f) f->lo_s[k] = reinterpret_cast<uint2*>(p); }
-    for (int i = 0; i < 3; i++) { char* p = cmp ? take(N, sizeof(unsigned)) : nullptr; if (f) f->lo_E[i] = reinterpret_cast<unsigned*>(p); }
+    for (int k = 0; k < 2; k++) { char* p = cmp ? take(N, sizeof(unsigned)) : nullptr; if (f) f->lo_s[k] = reinterpret_cast<unsigned*>(p); }
+    for (int i = 0; i < 3; i++) {
+        char* p = cmp ? take(N, sizeof(unsigned short)) : nullptr;
+        if (f) f->lo_E[i] = reinterpret_cast<unsigned short*>(p);
+    }
     return off;
 }
 
@@ -1507,25 +1521,27 @@ inline double wrap_host(double y) {
     return y;
 }
 
-// 8-bit residuals on the host, as the kernels encode them: q = r 2^(157 - e)
-// rounded to nearest even and clamped to [-127, 127], e the biased exponent of
-// the FP32 hi; value q 2^(e - 157).
+// 4-bit residuals on the host, as the kernels encode them: q = r 2^(153 - e)
+// rounded to nearest even (|q| <= 4; clamped to [-8, 7] as a guard), e the
+// biased exponent of the FP32 hi; value q 2^(e - 153); nibble m of a word holds
+// value m.
 inline int res_exp_host(float hi) {
     unsigned u;
     std::memcpy(&u, &hi, 4);
     return static_cast<int>((u >> 23) & 0xffu);
 }
 inline unsigned res_q_host(double r, float hi) {
-    const double q = std::nearbyint(std::ldexp(r, 157 - res_exp_host(hi)));
-    return static_cast<unsigned>(static_cast<int>(std::min(127.0, std::max(-127.0, q)))) & 0xffu;
+    const double q = std::nearbyint(std::ldexp(r, 153 - res_exp_host(hi)));
+    return static_cast<unsigned>(static_cast<int>(std::min(7.0, std::max(-8.0, q)))) & 0xfu;
 }
 inline double res_val_host(unsigned w, int m, float hi) {
-    const int q = static_cast<int>(static_cast<signed char>((w >> (8 * m)) & 0xffu));
-    return std::ldexp(static_cast<double>(q), res_exp_host(hi) - 157);
+    int q = static_cast<int>((w >> (4 * m)) & 0xfu);
+    if (q >= 8) q -= 16;
+    return std::ldexp(static_cast<double>(q), res_exp_host(hi) - 153);
 }
 
 // Two-slot (reference layout) -> momentum form, on the host in FP64.  With
-// compensated storage each value is split into its FP32 hi and 8-bit residual.
+// compensated storage each value is split into its FP32 hi and 4-bit residual.
 template <class R, class Src>
 int momentum_import_t(MomentumState& m, MomFields<R>& f, const Coef& c, const Src* S, const Src* V,
                       unsigned time_iter, hipStream_t st, std::string& err) {
@@ -1534,8 +1550,8 @@ int momentum_import_t(MomentumState& m, MomFields<R>& f, const Coef& c, const Sr
     const unsigned now = (time_iter + 1) % 2, past = !now;
     std::vector<R> phi(4 * N * 2, R(0)), P(4 * N * 2, R(0)), y(3 * N), q(12 * N), E(12 * N);
     const bool cmp = f.lo_s[0] != nullptr;
-    std::vector<uint2> lo_s(cmp ? 2 * N : 0, make_uint2(0, 0));
-    std::vector<unsigned> lo_E(cmp ? 3 * N : 0, 0u);
+    std::vector<unsigned> lo_s(cmp ? 2 * N : 0, 0u);
+    std::vector<unsigned short> lo_E(cmp ? 3 * N : 0, 0);
     for (long long s = 0; s < N; s++) {
         for (int k = 0; k < nd; k++) {
             unsigned lf = 0, lp = 0;
@@ -1544,10 +1560,10 @@ int momentum_import_t(MomentumState& m, MomFields<R>& f, const Coef& c, const Sr
                 const double b = (double)S[(past * N + s) * Ns + 4 * k + m4];
                 phi[(k * N + s) * 4 + m4] = (R)a;
                 P[(k * N + s) * 4 + m4] = (R)(a - b);
-                lf |= res_q_host(a - (double)(float)a, (float)a) << (8 * m4);
-                lp |= res_q_host((a - b) - (double)(float)(a - b), (float)(a - b)) << (8 * m4);
+                lf |= res_q_host(a - (double)(float)a, (float)a) << (4 * m4);
+                lp |= res_q_host((a - b) - (double)(float)(a - b), (float)(a - b)) << (4 * m4);
             }
-            if (cmp) lo_s[k * N + s] = make_uint2(lf, lp);
+            if (cmp) lo_s[k * N + s] = lf | (lp << 16);
         }
         for (int i = 0; i < 3; i++) {
             const Src* ln = V + (now * N + s) * 15 + 5 * i;
@@ -1562,9 +1578,9 @@ int momentum_import_t(MomentumState& m, MomFields<R>& f, const Coef& c, const Sr
             unsigned le = 0;
             for (int k = 0; k < 4; k++) {
                 E[(i * N + s) * 4 + k] = (R)ev[k];
-                le |= res_q_host(ev[k] - (double)(float)ev[k], (float)ev[k]) << (8 * k);
+                le |= res_q_host(ev[k] - (double)(float)ev[k], (float)ev[k]) << (4 * k);
             }
-            if (cmp) lo_E[i * N + s] = le;
+            if (cmp) lo_E[i * N + s] = static_cast<unsigned short>(le);
         }
     }
     m.cur = 0;
@@ -1581,10 +1597,10 @@ int momentum_import_t(MomentumState& m, MomFields<R>& f, const Coef& c, const Sr
         ok &= cp(f.y[0][i], &y[i * N], N * sizeof(R));
         ok &= cp(f.q[0][i], &q[i * N * 4], N * 4 * sizeof(R));
         ok &= cp(f.E[i], &E[i * N * 4], N * 4 * sizeof(R));
-        if (cmp) ok &= cp(f.lo_E[i], &lo_E[i * N], N * sizeof(unsigned));
+        if (cmp) ok &= cp(f.lo_E[i], &lo_E[i * N], N * sizeof(unsigned short));
     }
     if (cmp)
-        for (int k = 0; k < nd; k++) ok &= cp(f.lo_s[k], &lo_s[k * N], N * sizeof(uint2));
+        for (int k = 0; k < nd; k++) ok &= cp(f.lo_s[k], &lo_s[k * N], N * sizeof(unsigned));
     ok &= hipStreamSynchronize(st) == hipSuccess;
     if (!ok) { err = "momentum import: copy failed"; return GFT_EHIP; }
     return GFT_OK;
@@ -1600,8 +1616,8 @@ int momentum_export_t(MomentumState& m, MomFields<R>& f, const Coef& c, Dst* S, 
     const unsigned now = (time_iter + 1) % 2, past = !now;
     std::vector<R> phi(4 * N * 2), P(4 * N * 2), y(3 * N), q(12 * N), E(12 * N);
     const bool cmp = f.lo_s[0] != nullptr;
-    std::vector<uint2> lo_s(cmp ? 2 * N : 0);
-    std::vector<unsigned> lo_E(cmp ? 3 * N : 0);
+    std::vector<unsigned> lo_s(cmp ? 2 * N : 0);
+    std::vector<unsigned short> lo_E(cmp ? 3 * N : 0);
     auto cp = [&](void* dst, const void* src, size_t bytes) {
         return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st) == hipSuccess;
     };
@@ -1614,10 +1630,10 @@ int momentum_export_t(MomentumState& m, MomFields<R>& f, const Coef& c, Dst* S, 
         ok &= cp(&y[i * N], f.y[m.cur][i], N * sizeof(R));
         ok &= cp(&q[i * N * 4], f.q[m.cur][i], N * 4 * sizeof(R));
         ok &= cp(&E[i * N * 4], f.E[i], N * 4 * sizeof(R));
-        if (cmp) ok &= cp(&lo_E[i * N], f.lo_E[i], N * sizeof(unsigned));
+        if (cmp) ok &= cp(&lo_E[i * N], f.lo_E[i], N * sizeof(unsigned short));
     }
     if (cmp)
-        for (int k = 0; k < nd; k++) ok &= cp(&lo_s[k * N], f.lo_s[k], N * sizeof(uint2));
+        for (int k = 0; k < nd; k++) ok &= cp(&lo_s[k * N], f.lo_s[k], N * sizeof(unsigned));
     ok &= hipStreamSynchronize(st) == hipSuccess;
     if (!ok) { err = "momentum export: copy failed"; return GFT_EHIP; }
     for (long long s = 0; s < N; s++) {
@@ -1626,9 +1642,9 @@ int momentum_export_t(MomentumState& m, MomFields<R>& f, const Coef& c, Dst* S, 
                 double a = (double)phi[(k * N + s) * 4 + m4];
                 double pp = (double)P[(k * N + s) * 4 + m4];
                 if (cmp) {
-                    const uint2 L = lo_s[k * N + s];
-                    a += res_val_host(L.x, m4, (float)phi[(k * N + s) * 4 + m4]);
-                    pp += res_val_host(L.y, m4, (float)P[(k * N + s) * 4 + m4]);
+                    const unsigned L = lo_s[k * N + s];
+                    a += res_val_host(L, m4, (float)phi[(k * N + s) * 4 + m4]);
+                    pp += res_val_host(L, 4 + m4, (float)P[(k * N + s) * 4 + m4]);
                 }
                 S[(now * N + s) * Ns + 4 * k + m4] = (Dst)a;
                 S[(past * N + s) * Ns + 4 * k + m4] = (Dst)(a - pp);
