@@ -28,12 +28,16 @@
 // kernels compiled in the same translation unit keep their FMAs.
 namespace gft {
 
+// Periodic neighbour of site s along axis a by delta in {-1, +1}.  32-bit
+// index math: the host limits a lattice to 2^28 sites.
 __device__ __forceinline__ long long site_shift(const Coef& c, long long s, int a, int delta) {
-    long long z = s % c.nz, y = (s / c.nz) % c.ny, x = s / ((long long)c.nz * c.ny);
-    if (a == 0) x = (x + delta + c.nx) % c.nx;
-    else if (a == 1) y = (y + delta + c.ny) % c.ny;
-    else z = (z + delta + c.nz) % c.nz;
-    return (x * c.ny + y) * c.nz + z;
+    const unsigned u = static_cast<unsigned>(s), nz = c.nz, ny = c.ny, nx = c.nx;
+    const unsigned t = u / nz;
+    unsigned z = u - t * nz, x = t / ny, y = t - x * ny;
+    if (a == 0) x = delta > 0 ? (x + 1 == nx ? 0 : x + 1) : (x == 0 ? nx - 1 : x - 1);
+    else if (a == 1) y = delta > 0 ? (y + 1 == ny ? 0 : y + 1) : (y == 0 ? ny - 1 : y - 1);
+    else z = delta > 0 ? (z + 1 == nz ? 0 : z + 1) : (z == 0 ? nz - 1 : z - 1);
+    return static_cast<long long>((x * ny + y) * nz + z);
 }
 
 // SU(2) action on one doublet, contraction off (StandardModel.cpp:94-97).
