@@ -426,6 +426,81 @@ Lattice::~Lattice() {
     if (ctx) gft_destroy(ctx);
 }
 
+void Lattice::initBoundaryCondition(const std::vector<int>& faces) {   // Lattice.cpp:217-355
+    if (faces.size() != 6) throw std::runtime_error("LATTICE:: Exactly six face boundary conditions must be assigned.");
+    fixedFaces = 0;
+    for (int f = 0; f < 6; f++) {
+        if (faces[f] != FIXED && faces[f] != PERIODIC)
+            throw std::runtime_error("LATTICE:: Face boundary condition (" + std::to_string(f) + ") has unsupported type "
+                                     + std::to_string(faces[f]) + ".");
+        if (faces[f] == FIXED) fixedFaces |= 1 << f;
+    }
+    for (int a = 0; a < 3; a++)
+        if ((faces[2 * a] == PERIODIC) != (faces[2 * a + 1] == PERIODIC))
+            throw std::runtime_error("LATTICE:: Periodic boundary conditions must be assigned to both faces of an axis.");
+    // Regions by how many axes they lie on the shell of (faces, then edges, then
+    // corners); within that by the set of those axes in lexicographic order (x
+    // before y before z), then by the sign pattern, low shell first.
+    boundaryConditions.clear();
+    std::vector<std::vector<int>> vectors;
+    const int axis_sets[7][3] = {{0, -1, -1}, {1, -1, -1}, {2, -1, -1}, {0, 1, -1}, {0, 2, -1}, {1, 2, -1}, {0, 1, 2}};
+    for (const auto& set : axis_sets) {
+        const int k = set[2] >= 0 ? 3 : set[1] >= 0 ? 2 : 1;
+        for (int signs = 0; signs < (1 << k); signs++) {
+            std::vector<int> bv(3, 0);
+            for (int j = 0; j < k; j++) bv[set[j]] = (signs >> (k - 1 - j)) & 1 ? 1 : -1;
+            vectors.push_back(bv);
+        }
+    }
+    for (const auto& bv : vectors) {
+        bool fixed = false;   // a region on any fixed face is fixed
+        for (int a = 0; a < 3; a++)
+            if (bv[a] != 0 && faces[2 * a + (bv[a] > 0 ? 1 : 0)] == FIXED) fixed = true;
+        std::unique_ptr<BoundaryCondition> bc;
+        if (fixed) bc = std::make_unique<Fixed>(nx, ny, nz, bv);
+        else bc = std::make_unique<Periodic>(nx, ny, nz, bv);
+        if (!bc->determineResponsibilities(stencilSize)) boundaryConditions.push_back(std::move(bc));
+    }
+}
+
+std::vector<std::vector<unsigned>> Lattice::determineResponsibilities(const unsigned& stencil_size) const {
+    // Lattice.cpp:417-438: [s, n - s) per axis, empty when n < s.
+    const unsigned n[3] = {nx, ny, nz};
+    std::vector<std::vector<unsigned>> limits(3, std::vector<unsigned>(2, stencil_size));
+    for (int a = 0; a < 3; a++) limits[a][1] = n[a] < stencil_size ? 0 : n[a] - stencil_size;
+    return limits;
+}
+
+BoundaryCondition::BoundaryCondition(unsigned nx_, unsigned ny_, unsigned nz_, std::vector<int> bound_vector)
+    : boundVector(std::move(bound_vector)), loopLimits(3, std::vector<unsigned>(2, 0)), nx(nx_), ny(ny_), nz(nz_) {
+    if (boundVector.size() != 3) throw std::runtime_error("BOUNDARYCONDITION::ERROR: bound vector needs 3 components.");
+}
+
+bool BoundaryCondition::determineResponsibilities(const unsigned& s) {
+    const unsigned n[3] = {nx, ny, nz};
+    bool empty = false;
+    for (int a = 0; a < 3; a++) {
+        // An axis too short for two shells keeps only its low shell.
+        const unsigned high = n[a] < 2 * s ? s : n[a] - s;
+        switch (boundVector[a]) {
+        case -1: loopLimits[a] = {0, s}; break;
+        case 0: loopLimits[a] = {s, high}; break;
+        case 1: loopLimits[a] = {high, n[a]}; break;
+        default:
+            throw std::runtime_error("BOUNDARYCONDITION::ERROR: Invalid boundVector component: "
+                                     + std::to_string(boundVector[a]));
+        }
+        if (loopLimits[a][0] >= loopLimits[a][1]) empty = true;
+    }
+    return empty;
+}
+
+unsigned long long BoundaryCondition::numSites() const {
+    unsigned long long c = 1;
+    for (int a = 0; a < 3; a++) c *= loopLimits[a][1] > loopLimits[a][0] ? loopLimits[a][1] - loopLimits[a][0] : 0;
+    return c;
+}
+
 void Lattice::configure(const std::string& path) {
     ConfigReader r(path);   // Core/Source/Lattice.cpp:33-170
     if (!r.is_open()) throw std::runtime_error("LATTICE:: cannot open " + path);
@@ -462,18 +537,9 @@ void Lattice::configure(const std::string& path) {
     const bool want_energy = r.value<bool>();
     const bool want_gauss = r.value<bool>();
     r.skip_lines(7);
-    int faces[6] = {0, 0, 0, 0, 0, 0};
-    r.values(faces, 6);
-    fixedFaces = 0;
-    for (int f = 0; f < 6; f++) {   // Lattice.cpp:217-260
-        if (faces[f] != FIXED && faces[f] != PERIODIC)
-            throw std::runtime_error("LATTICE:: Face boundary condition (" + std::to_string(f) + ") has unsupported type "
-                                     + std::to_string(faces[f]) + ".");
-        if (faces[f] == FIXED) fixedFaces |= 1 << f;
-    }
-    for (int a = 0; a < 3; a++)
-        if ((faces[2 * a] == PERIODIC) != (faces[2 * a + 1] == PERIODIC))
-            throw std::runtime_error("LATTICE:: Periodic boundary conditions must be assigned to both faces of an axis.");
+    std::vector<int> faces(6, 0);
+    r.values(faces.data(), 6);
+    initBoundaryCondition(faces);
     r.skip_lines(1);
     progressReport = r.value<bool>();
     reportFrequency = r.value<unsigned>();

@@ -247,6 +247,42 @@ private:
     unsigned ns, nv;
 };
 
+// ---- Boundary conditions -------------------------------------------------------------
+// BoundaryConditions/Header/BoundaryCondition.hpp:5-80, Fixed.hpp, Periodic.hpp.  The
+// lattice's boundary regions (6 faces, 12 edges, 8 corners), each named by its bound
+// vector: per axis -1 (the low shell), 0 (between the shells) or +1 (the high shell).
+// A region is Fixed when any face it lies on is fixed, else Periodic
+// (Lattice.cpp:292-310).  The GPU step evolves the interior and every region in one
+// launch: neighbour indices wrap periodically, and sites of Fixed regions keep their
+// stored values (gft_desc::fixed_faces), so these classes carry the regions' extents
+// and types rather than a per-region evolve loop.
+class BoundaryCondition {
+public:
+    BoundaryCondition(unsigned nx, unsigned ny, unsigned nz, std::vector<int> bound_vector);
+    virtual ~BoundaryCondition() = default;
+    // Sets loopLimits for a stencil of the given half-width; true when the region
+    // holds no site (BoundaryCondition.cpp:21-70).
+    bool determineResponsibilities(const unsigned& stencil_size);
+    virtual int type() const = 0;   // FIXED or PERIODIC
+    unsigned long long numSites() const;
+    const std::vector<int> boundVector;
+    std::vector<std::vector<unsigned>> loopLimits;   // [axis][begin, end)
+protected:
+    unsigned nx, ny, nz;
+};
+
+class Periodic : public BoundaryCondition {   // BoundaryConditions/Source/Periodic.cpp
+public:
+    using BoundaryCondition::BoundaryCondition;
+    int type() const override { return PERIODIC; }
+};
+
+class Fixed : public BoundaryCondition {      // BoundaryConditions/Source/Fixed.cpp: sites left unchanged
+public:
+    using BoundaryCondition::BoundaryCondition;
+    int type() const override { return FIXED; }
+};
+
 // ---- Lattice -------------------------------------------------------------------------
 class Lattice {
 public:
@@ -269,9 +305,15 @@ public:
     int fixedFaces = 0;          // fixed faces, gft_desc::fixed_faces bits
     unsigned reportFrequency = 1;
     double lastEvolveSeconds = 0;
+    unsigned stencilSize = 1;    // half-width of the nearest-neighbour stencils
+    // Non-empty boundary regions in the reference's order (Lattice.cpp:262-355, 706-716).
+    std::vector<std::unique_ptr<BoundaryCondition>> boundaryConditions;
+    // Loop limits [axis][begin, end) of the sites no boundary region holds.
+    std::vector<std::vector<unsigned>> determineResponsibilities(const unsigned& stencil_size) const;
 
 private:
     void configure(const std::string& path);
+    void initBoundaryCondition(const std::vector<int>& face_boundary_condition_types);
     void initFields();
     std::string root, dataDir;
     int form;

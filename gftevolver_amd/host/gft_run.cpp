@@ -38,6 +38,18 @@ int main(int argc, char** argv) {
                       << "fixed_faces " << lattice.fixedFaces << "\n"
                       << "gen_rep " << d.gen_rep << "\n"
                       << "gauge_num " << d.gauge_num << "\ngpp " << d.gpp << "\n";
+            // Boundary regions: bound vector, type, loop limits; then the interior's limits.
+            for (const auto& bc : lattice.boundaryConditions) {
+                std::cout << "region";
+                for (int a = 0; a < 3; a++) std::cout << " " << bc->boundVector[a];
+                std::cout << " " << bc->type();
+                for (int a = 0; a < 3; a++) std::cout << " " << bc->loopLimits[a][0] << " " << bc->loopLimits[a][1];
+                std::cout << "\n";
+            }
+            const auto inner = lattice.determineResponsibilities(lattice.stencilSize);
+            std::cout << "interior";
+            for (int a = 0; a < 3; a++) std::cout << " " << inner[a][0] << " " << inner[a][1];
+            std::cout << "\n";
             double s0 = 0;
             for (float x : lattice.scalarFields) s0 += x;
             std::cout << "scalar_sum " << s0 << "\n";

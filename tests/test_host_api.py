@@ -75,6 +75,50 @@ def test_config_reader_fixed_faces(tmp_path):
 
 
 @needs_cfg
+@pytest.mark.parametrize("n,faces", [((6, 5, 7), (3, 3, 3, 3, 3, 3)), ((6, 5, 7), (1, 1, 3, 3, 1, 1)),
+                                     ((2, 3, 5), (1, 1, 1, 1, 1, 1)), ((4, 2, 3), (3, 3, 1, 1, 3, 3))],
+                         ids=["periodic", "fixed-xz", "thin-fixed", "thin-mixed"])
+def test_boundary_regions_match_reference_order_and_frozen_shell(tmp_path, n, faces):
+    """Lattice::boundaryConditions: the non-empty regions, listed in order and
+    followed by the interior, visit every site once in the order of the oracle's
+    restatement of the reference loops (Lattice.cpp:262-355, 706-716,
+    BoundaryCondition.cpp:21-70); the Fixed regions hold exactly the sites the
+    GPU kernels keep frozen (a site on the shell of any fixed face)."""
+    root = _prepare(str(tmp_path), 0, n, 12, 4, faces=faces)
+    out = subprocess.run([GFT_RUN, root, str(tmp_path / "data"), "dump_config"], capture_output=True, text=True,
+                         check=True).stdout
+    nx, ny, nz = n
+    order, fixed = [], set()
+    lines = [l.split() for l in out.splitlines()]
+    regions = [[int(v) for v in l[1:]] for l in lines if l[0] == "region"]
+    inner = [[int(v) for v in l[1:]] for l in lines if l[0] == "interior"][0]
+
+    def sites(lim):
+        return [(x * ny + y) * nz + z for x in range(lim[0], lim[1]) for y in range(lim[2], lim[3])
+                for z in range(lim[4], lim[5])]
+
+    for r in regions:
+        assert r[3] in (1, 3)
+        s = sites(r[4:])
+        assert s, "empty regions are dropped"
+        order += s
+        if r[3] == 1:
+            fixed.update(s)
+            assert any(r[a] != 0 and faces[2 * a + (r[a] > 0)] == 1 for a in range(3))
+    order += sites(inner)
+    assert np.array_equal(np.array(order), O.site_order(*n))
+    idx = np.arange(nx * ny * nz)
+    x, y, z = idx // (ny * nz), (idx // nz) % ny, idx % nz
+    frozen = np.zeros(idx.size, bool)
+    for f, (c, v) in enumerate(((x, 0), (x, nx - 1), (y, 0), (y, ny - 1), (z, 0), (z, nz - 1))):
+        if faces[f] == 1:
+            frozen |= c == v
+    assert fixed == set(idx[frozen].tolist())
+    kv = {l[0]: l[1:] for l in lines}
+    assert int(kv["fixed_faces"][0]) == sum(1 << f for f in range(6) if faces[f] == 1)
+
+
+@needs_cfg
 @pytest.mark.parametrize("model", [0, 1], ids=["sm", "2hdm"])
 def test_config_reader_generator_form(tmp_path, model):
     """4 vector components per direction select the generator form of the SU(2)
