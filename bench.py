@@ -112,7 +112,7 @@ def reference_program_rate(model=0, n=32, nsteps=60, ns=4, nvd=5, pot2=None, gen
     return n ** 3 * nsteps / sec, sec
 
 
-def cpu_baseline(nsteps=60):
+def cpu_baseline(nsteps=150):
     """The reference program on a bounded sample: SM 32^3, `nsteps` steps,
     analysers off, one core (the CPU restatement if the program is not built)."""
     n = 32
