@@ -218,7 +218,7 @@ def options_leg(rt, with_cpu):
     return out
 
 
-def ensemble_leg(rt, steps):
+def ensemble_leg(rt, steps, tiles=None):
     """The 64-member ensemble workload: aggregate FP32 throughput of the
     persistent graph-replayed launch, and its ensemble-mean observables against
     the same members run one by one in FP64."""
@@ -227,7 +227,7 @@ def ensemble_leg(rt, steps):
     specs = E.members()
     sites = sum(int(np.prod(n)) for n, _, _ in specs)
     every = max(steps // 4, 1)
-    o32, sec, kms, tiles = E.run_fp32(rt, specs, steps, every, log=log)
+    o32, sec, kms, tiles = E.run_fp32(rt, specs, steps, every, log=log, tiles=tiles)
     value = sites * steps / sec
     log(f"ensemble: {len(specs)} members, {sites} sites, {steps} steps in {sec:.3f} s = {value:.3e} site-updates/s "
         f"({kms:.3f} ms per step by HIP events)")
@@ -264,6 +264,8 @@ def main():
     ap.add_argument("--twoslot-steps", type=int, default=20)
     ap.add_argument("--ensemble-steps", type=int, default=2000,
                     help="steps of the 64-member ensemble workload (0 skips it)")
+    ap.add_argument("--ensemble-tiles", type=str, default="",
+                    help="bz x by per member size 32,48,64,96, e.g. 32x8,16x16,32x8,32x8 (skips the ensemble tuner)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-options", action="store_true",
                     help="skip the other model options (fixed faces, generator form, gaugeNum 5/7, O(N))")
@@ -357,7 +359,11 @@ def main():
         })
 
     if not args.only_f32 and args.ensemble_steps:
-        extra["ensemble"] = ensemble_leg(rt, args.ensemble_steps)
+        tiles = None
+        if args.ensemble_tiles:
+            shapes = [tuple(int(v) for v in t.split("x")) for t in args.ensemble_tiles.split(",")]
+            tiles = dict(zip((32, 48, 64, 96), shapes))
+        extra["ensemble"] = ensemble_leg(rt, args.ensemble_steps, tiles)
     if not args.only_f32 and not args.no_options:
         extra["options"] = options_leg(rt, not args.no_cpu)
 

@@ -75,16 +75,27 @@ def tune_tiles(ens, groups_idx, tiles=None, steps=20, passes=2):
     return best
 
 
-def run_fp32(rt, specs, steps, measure_every, tune=True, use_graph=True, log=None):
+def run_fp32(rt, specs, steps, measure_every, tune=True, use_graph=True, log=None, tiles=None):
     """Advances the whole ensemble `steps` steps from t = 0.  Returns
     (observables [n_meas][n_members][5], timed seconds, avg kernel ms per step,
-    tiles).  Timing excludes tuning and measurements: only the evolve calls."""
+    tiles).  Timing excludes tuning and measurements: only the evolve calls.
+    `tiles` ({size: (bz, by)}) fixes the tile of every member of each size and
+    skips the tuner."""
     descs = [make_desc(form=GFT_FORM_MOMENTUM_F32, n=n) for n, _, _ in specs]
     sizes = sorted({n for n, _, _ in specs})
     groups_idx = [[i for i, s in enumerate(specs) if s[0] == n] for n in sizes]
     with rt.Ensemble(descs) as ens:
-        tiles = None
-        if tune:
+        if tiles is not None:
+            # Keys are member shapes (nx, ny, nz) or, for cubic members, the side.
+            given = {n: tiles.get(n, tiles.get(n[0]) if len(set(n)) == 1 else None) for n in sizes}
+            missing = [n for n, t in given.items() if t is None]
+            if missing:
+                raise ValueError(f"no tile given for member sizes {missing}")
+            for g, n in enumerate(sizes):
+                for i in groups_idx[g]:
+                    ens.set_launch(i, *given[n])
+            tiles = {g: tuple(given[n]) for g, n in enumerate(sizes)}
+        elif tune:
             # Tune on a throw-away copy of the state, then start afresh.
             for m, (_, rms, seed) in zip(ens.members, specs):
                 init_member(m, rms, seed)
