@@ -885,6 +885,7 @@ __device__ __forceinline__ void mom_block_shared(const StepArgs<float>& a, const
     OwnState<R, 1> own;
     if (valid) {
         load_own<R, 1, CMP>(a, s, own);   // issued early: its latency overlaps phase A
+        __builtin_amdgcn_s_setprio(0);    // back to the default (mom_step raised it for the loads)
 #pragma unroll
         for (int i = 0; i < 3; i++) {
             U[i] = toq<float4, R>(pl.q[i][r0]);
@@ -1105,6 +1106,12 @@ __global__ void __launch_bounds__(256, MINW > 0 ? MINW : 1) mom_step(StepArgs<R>
         extern __shared__ float4 gft_plane_lds[];
         char* lds = reinterpret_cast<char*>(gft_plane_lds);
         PlaneLds pl = plane_layout<ND>(lds, a.bz, a.by);
+        // Stage 2: the wave issues at raised priority until its own-state loads
+        // are out (mom_block_shared drops it), so a wave starting a tile gets
+        // its memory requests in ahead of the waves that are computing (1.5%
+        // faster at 256^3; the ensemble kernel measured 0.5% slower with it and
+        // keeps the default).  Priority changes issue order only, not results.
+        if constexpr (PL == 2) __builtin_amdgcn_s_setprio(3);
         plane_fill<ND>(a, pl, t.y, t.z * a.by, t.w * a.bz, a.bz, a.by);
         pl.r0 = (ty + 1) * pl.w + tz + 1;
         if constexpr (PL == 2) {
