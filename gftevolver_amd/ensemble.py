@@ -75,6 +75,16 @@ def tune_tiles(ens, groups_idx, tiles=None, steps=20, passes=2):
     return best
 
 
+def fixed_tiles(sizes, tiles):
+    """{member shape: (bz, by)} for every shape in `sizes` from `tiles`, whose
+    keys are member shapes (nx, ny, nz) or, for cubic members, the side."""
+    given = {n: tiles.get(n, tiles.get(n[0]) if len(set(n)) == 1 else None) for n in sizes}
+    missing = [n for n, t in given.items() if t is None]
+    if missing:
+        raise ValueError(f"no tile given for member sizes {missing}")
+    return {n: tuple(t) for n, t in given.items()}
+
+
 def run_fp32(rt, specs, steps, measure_every, tune=True, use_graph=True, log=None, tiles=None):
     """Advances the whole ensemble `steps` steps from t = 0.  Returns
     (observables [n_meas][n_members][5], timed seconds, avg kernel ms per step,
@@ -84,13 +94,9 @@ def run_fp32(rt, specs, steps, measure_every, tune=True, use_graph=True, log=Non
     descs = [make_desc(form=GFT_FORM_MOMENTUM_F32, n=n) for n, _, _ in specs]
     sizes = sorted({n for n, _, _ in specs})
     groups_idx = [[i for i, s in enumerate(specs) if s[0] == n] for n in sizes]
+    given = fixed_tiles(sizes, tiles) if tiles is not None else None
     with rt.Ensemble(descs) as ens:
-        if tiles is not None:
-            # Keys are member shapes (nx, ny, nz) or, for cubic members, the side.
-            given = {n: tiles.get(n, tiles.get(n[0]) if len(set(n)) == 1 else None) for n in sizes}
-            missing = [n for n, t in given.items() if t is None]
-            if missing:
-                raise ValueError(f"no tile given for member sizes {missing}")
+        if given is not None:
             for g, n in enumerate(sizes):
                 for i in groups_idx[g]:
                     ens.set_launch(i, *given[n])

@@ -75,3 +75,14 @@ def test_gaussian_state_statistics():
     d = s[1] - np.array([0.0, 0.0, 1.0, 0.0])
     assert np.abs(d.mean(axis=0)).max() < 0.05 * 4 / np.sqrt(32 ** 3)
     assert np.abs(d.std(axis=0) - 0.05).max() < 0.05 * 4 / np.sqrt(2 * 32 ** 3)
+
+
+def test_ensemble_fixed_tiles_lookup():
+    """Fixed ensemble tiles are looked up by member shape or by a cubic
+    member's side, and a size without a tile is refused before any launch."""
+    from gftevolver_amd.ensemble import fixed_tiles
+    sizes = [(32, 32, 32), (48, 48, 48), (16, 24, 32)]
+    got = fixed_tiles(sizes, {32: (32, 8), 48: [16, 16], (16, 24, 32): (16, 16)})
+    assert got == {(32, 32, 32): (32, 8), (48, 48, 48): (16, 16), (16, 24, 32): (16, 16)}
+    with pytest.raises(ValueError, match="no tile given"):
+        fixed_tiles(sizes, {32: (32, 8), 48: (16, 16), 16: (16, 16)})   # a side does not name a non-cubic shape
