@@ -6,7 +6,9 @@ The timed region runs exactly --steps steps (default 10^4) of the FP32-bulk
 momentum-form kernel (FP32 arithmetic, phi/P/E stored as FP32 + 4-bit
 residual) on data already resident in HBM, bracketed by stream
 synchronisations; `value` is site-updates per second over that wall-clock
-interval, and the Gauss-law violation is measured right after it.  Beside it:
+interval.  The Gauss-law violation of the metric's accuracy condition is
+measured on a separate run of 10^4 steps (--gauss-steps) from the same initial
+state, whatever --steps is.  Beside it:
 the same kernel with plain FP32 storage, the FP64 momentum form (the plain
 FP64 GPU evolver) and the FP64 two-slot reference-layout kernel on the same
 state, the 64-member ensemble workload, the step kernel's roofline position
@@ -69,6 +71,26 @@ def run_form(rt, desc, init, steps, warmup, launch=None, tune=False, occupancy=F
         rows = ev.evolve(1, gauss_freq=1)
         e1 = ev.observe().energy[0] if desc.form >= 2 else None
     return {"wall_s": wall, "kernel_ms": avg_ms, "gauss_rel": gauss_rel(rows[0]), "e0": e0, "e1": e1, **info}
+
+
+def gauss_leg(rt, desc, init, launch, steps):
+    """The headline's accuracy condition on its own run length: the same form
+    and launch from the same initial state, `steps` steps (10^4 by default,
+    whatever --steps times), then the relative Gauss-law violation and the
+    energy drift.  Untimed; the step kernel's HIP-event average is reported."""
+    with rt.Evolver(desc) as ev:
+        init(ev)
+        if launch:
+            ev.set_launch(*launch)
+        e0 = ev.observe().energy[0]
+        t0 = time.perf_counter()
+        ev.evolve(steps - 1)
+        rows = ev.evolve(1, gauss_freq=1)
+        wall = time.perf_counter() - t0
+        kms = ev.last_step_ms()[0]
+        e1 = ev.observe().energy[0]
+    return {"steps": steps, "gauss_rel": gauss_rel(rows[0]), "energy_drift_rel": (e1 - e0) / abs(e0),
+            "wall_s": wall, "kernel_ms_last_call": kms}
 
 
 def measured_leg(rt, desc, init, launch, steps=1000, every=100, warmup=10):
@@ -257,6 +279,8 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10000)
+    ap.add_argument("--gauss-steps", type=int, default=10000,
+                    help="run length of the Gauss-law leg (the headline's accuracy condition); 0 skips it")
     ap.add_argument("--plain-steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=256)
@@ -312,6 +336,12 @@ def main():
     log(f"FP32 momentum: {r32['wall_s'] / args.steps * 1e3:.3f} ms/step wall, kernel {r32['kernel_ms']:.3f} ms, "
         f"{value:.3e} site-updates/s, {achieved:.0f} GB/s algorithmic, Gauss rel {r32['gauss_rel']:.2e} "
         f"after {args.steps + args.warmup + 1} steps")
+
+    gl = None
+    if args.gauss_steps:
+        gl = gauss_leg(rt, d32, init, r32.get("launch"), args.gauss_steps)
+        log(f"Gauss leg: {gl['steps']} steps from t = 0, relative Gauss violation {gl['gauss_rel']:.2e}, "
+            f"energy drift {gl['energy_drift_rel']:.2e} ({gl['wall_s']:.1f} s)")
 
     stream_ms = None
     if args.calibrate_copy or not args.only_f32:
@@ -398,9 +428,13 @@ def main():
                      "stream_floor_ms": stream_ms,
                      "stream_floor_GBps": STORED_F32C * N / (stream_ms * 1e-3) / 1e9 if stream_ms else None},
         "cpu_baseline": cpu,
-        "gauss_rel_violation": r32["gauss_rel"],
-        "gauss_measured_after_steps": args.steps + args.warmup + 1,
-        "energy_drift_rel": (r32["e1"] - r32["e0"]) / abs(r32["e0"]),
+        # The accuracy condition of the metric, on the 10^4-step run of the Gauss
+        # leg (the timed run's own end state when the leg is skipped).
+        "gauss_rel_violation": gl["gauss_rel"] if gl else r32["gauss_rel"],
+        "gauss_measured_after_steps": gl["steps"] if gl else args.steps + args.warmup + 1,
+        "energy_drift_rel": gl["energy_drift_rel"] if gl else (r32["e1"] - r32["e0"]) / abs(r32["e0"]),
+        "timed_run_end": {"gauss_rel": r32["gauss_rel"], "after_steps": args.steps + args.warmup + 1,
+                          "energy_drift_rel": (r32["e1"] - r32["e0"]) / abs(r32["e0"])},
         **extra,
     }
     print(json.dumps(out), flush=True)

@@ -126,9 +126,12 @@ static Coef make_coef(const gft_desc& d) {
 }
 
 static int ensure_partial(gft_ctx* ctx, size_t doubles) {
-    if (ctx->partial_cap >= doubles) return GFT_OK;
+    if (ctx->partial && ctx->partial_cap >= doubles) return GFT_OK;
     if (ctx->partial) hipFree(ctx->partial);
+    // The capacity goes with the buffer: a failed growth leaves no buffer and
+    // no capacity, so the next call allocates again instead of writing through null.
     ctx->partial = nullptr;
+    ctx->partial_cap = 0;
     CK(hipMalloc(&ctx->partial, doubles * sizeof(double)));
     ctx->partial_cap = doubles;
     return GFT_OK;

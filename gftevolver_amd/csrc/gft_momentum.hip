@@ -883,9 +883,11 @@ __device__ __forceinline__ void mom_block_shared(const StepArgs<float>& a, const
     R cy[3], sy_[3], yv[3], phi[4], F[4], J[3][4], M[3][4], dpv[3][4], dsh[3][4];
     Plaq<R> P[3];   // at x: (0,1), (0,2), (1,2)
     OwnState<R, 1> own;
+    if (valid) load_own<R, 1, CMP>(a, s, own);   // issued early: its latency overlaps phase A
+    // Back to the default priority (mom_step raised it for the loads), in every
+    // wave: a wave with no valid lane must not keep the raised priority.
+    __builtin_amdgcn_s_setprio(0);
     if (valid) {
-        load_own<R, 1, CMP>(a, s, own);   // issued early: its latency overlaps phase A
-        __builtin_amdgcn_s_setprio(0);    // back to the default (mom_step raised it for the loads)
 #pragma unroll
         for (int i = 0; i < 3; i++) {
             U[i] = toq<float4, R>(pl.q[i][r0]);

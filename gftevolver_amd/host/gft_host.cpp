@@ -574,9 +574,17 @@ void Lattice::initialAnalysis() {
     if (outputFields && outputFields->outputInitial) outputFields->outputFields(outputFields->initialAnalysisPath, 0);
 }
 
+int Lattice::resolvedForm() const {
+    if (form != GFT_FORM_AUTO) return form;
+    gft_desc d{};
+    model.fill(d);
+    const bool family = d.gauge_num == 5 || d.gauge_num == 7;
+    return (d.model == GFT_MODEL_GLOBAL || family) ? GFT_FORM_TWO_SLOT_F32 : GFT_FORM_MOMENTUM_F32;
+}
+
 void Lattice::evolve() {
     gft_desc d{};
-    d.form = form;
+    d.form = resolvedForm();
     d.nx = nx; d.ny = ny; d.nz = nz;
     d.dx = dx; d.dy = dy; d.dz = dz; d.dt = dt;
     d.fixed_faces = fixedFaces;

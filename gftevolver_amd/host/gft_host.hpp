@@ -25,6 +25,11 @@
 
 namespace gft {
 
+// Lattice's default form: the fast path (GFT_FORM_MOMENTUM_F32) for the models it
+// carries (SM, and the 2HDM with SU(2) x U(1) links), the reference-order FP32
+// two-slot form for the others (global O(N) model, 2HDM with gaugeNum 5 or 7).
+constexpr int GFT_FORM_AUTO = -1;
+
 // Positional reader of the reference's .cfg format: descriptions end with ':'
 // and values follow on the same line (e.g. Core/Source/Lattice.cpp:47-143).
 class ConfigReader {
@@ -289,8 +294,11 @@ public:
     // config_root holds <Lib>/Config/*.cfg as in the reference tree; data_dir
     // receives the outputs.  form selects the GPU state form (GFT_FORM_*):
     // GFT_FORM_TWO_SLOT_F32 keeps the reference's float storage and arithmetic
-    // order; GFT_FORM_MOMENTUM_F32 is the fast path.
-    Lattice(const std::string& config_root, const std::string& data_dir, int form = GFT_FORM_TWO_SLOT_F32);
+    // order; GFT_FORM_MOMENTUM_F32 is the fast path; GFT_FORM_AUTO (the default)
+    // takes the fast path wherever the configured model allows it.
+    Lattice(const std::string& config_root, const std::string& data_dir, int form = GFT_FORM_AUTO);
+    // The form the GPU context runs: the requested one, GFT_FORM_AUTO resolved.
+    int resolvedForm() const;
     ~Lattice();
     void initialAnalysis();
     void evolve();

@@ -4,9 +4,11 @@
 //
 // CONFIG_ROOT holds the reference's config tree (<Lib>/Config/*.cfg, default
 // "."); DATA_DIR receives global_quantities*.dat and field dumps (default
-// CONFIG_ROOT/Data); FORM is two_slot_f32 (default: the reference's float
-// storage and operation order), two_slot_f64, momentum_f32, momentum_f32_plain
-// or momentum_f64;
+// CONFIG_ROOT/Data); FORM is auto (default: momentum_f32, the fast path, for
+// the SM and the 2HDM with SU(2) x U(1) links; two_slot_f32 for the global
+// O(N) model and gaugeNum 5 / 7), two_slot_f32 (the reference's float storage
+// and operation order), two_slot_f64, momentum_f32, momentum_f32_plain or
+// momentum_f64;
 // FORM = dump_config parses the configuration, prints it and exits (no GPU).
 #include <filesystem>
 #include <iostream>
@@ -18,8 +20,8 @@ int main(int argc, char** argv) {
     try {
         const std::string root = argc > 1 ? argv[1] : ".";
         const std::string data = argc > 2 ? argv[2] : root + "/Data";
-        const std::string form_name = argc > 3 ? argv[3] : "two_slot_f32";
-        int form = GFT_FORM_TWO_SLOT_F32;
+        const std::string form_name = argc > 3 ? argv[3] : "auto";
+        int form = gft::GFT_FORM_AUTO;
         if (form_name == "dump_config") {
             gft::Lattice lattice(root, data, form);
             gft_desc d{};
@@ -37,7 +39,8 @@ int main(int argc, char** argv) {
                       << "progress " << lattice.progressReport << "\n"
                       << "fixed_faces " << lattice.fixedFaces << "\n"
                       << "gen_rep " << d.gen_rep << "\n"
-                      << "gauge_num " << d.gauge_num << "\ngpp " << d.gpp << "\n";
+                      << "gauge_num " << d.gauge_num << "\ngpp " << d.gpp << "\n"
+                      << "form " << lattice.resolvedForm() << "\n";
             // Boundary regions: bound vector, type, loop limits; then the interior's limits.
             for (const auto& bc : lattice.boundaryConditions) {
                 std::cout << "region";
@@ -55,11 +58,12 @@ int main(int argc, char** argv) {
             std::cout << "scalar_sum " << s0 << "\n";
             return EXIT_SUCCESS;
         }
-        if (form_name == "two_slot_f64") form = GFT_FORM_TWO_SLOT_F64;
+        if (form_name == "two_slot_f32") form = GFT_FORM_TWO_SLOT_F32;
+        else if (form_name == "two_slot_f64") form = GFT_FORM_TWO_SLOT_F64;
         else if (form_name == "momentum_f32") form = GFT_FORM_MOMENTUM_F32;
         else if (form_name == "momentum_f64") form = GFT_FORM_MOMENTUM_F64;
         else if (form_name == "momentum_f32_plain") form = GFT_FORM_MOMENTUM_F32_PLAIN;
-        else if (form_name != "two_slot_f32") throw std::runtime_error("unknown form " + form_name);
+        else if (form_name != "auto") throw std::runtime_error("unknown form " + form_name);
         const std::filesystem::path dd(data);
         if (std::filesystem::exists(dd) && !std::filesystem::is_empty(dd))
             std::cerr << "WARNING: The output directory " << dd

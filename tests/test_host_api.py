@@ -48,6 +48,8 @@ def test_config_reader_matches_values_written(tmp_path, model):
     assert kv["n"] == ["6", "5", "7"] and float(kv["dt"][0]) == 0.05 and kv["nt"] == ["12"]
     assert kv["ns"] == [str(ns)] and kv["nv"] == ["15"] and kv["model"] == [str(model)]
     assert float(kv["g"][0]) == 0.65 and float(kv["gp"][0]) == 0.35
+    # The default form is the fast path (GFT_FORM_MOMENTUM_F32 = 2) for both models.
+    assert kv["form"] == ["2"]
     if model == 0:
         assert float(kv["lambda"][0]) == 0.13 and float(kv["eta_sqr"][0]) == 1.0
     else:
@@ -160,10 +162,32 @@ def test_gft_run_reproduces_reference_program_output(tmp_path, model, tag):
     ns = 4 if model == 0 else 8
     root = _prepare(str(tmp_path), model, (6, 5, 7), 12, ns)
     data = tmp_path / "data"
+    subprocess.run([GFT_RUN, root, str(data), "two_slot_f32"], check=True, capture_output=True, timeout=120)
+    E = np.loadtxt(data / "global_quantities.dat", ndmin=2)
+    G = np.loadtxt(data / "global_quantities_gauge.dat", ndmin=2)
+    assert E.shape == g["energy"].shape and G.shape == g["gauss"].shape
+    np.testing.assert_allclose(E, g["energy"], rtol=2e-5, atol=1e-5 * np.abs(g["energy"]).max())
+    assert (G <= 10 * g["gauss"] + 1e-7).all()
+
+
+@needs_cfg
+@pytest.mark.gpu
+@pytest.mark.parametrize("model,tag", [(0, "sm"), (1, "2hdm")], ids=["sm", "2hdm"])
+def test_gft_run_default_fast_path_reproduces_reference_program(tmp_path, model, tag):
+    """gft_run with no form argument runs the FP32 momentum form (the fast path)
+    for the SM and the 2HDM: it reorders the reference's arithmetic, so the
+    energies agree to the same 2e-5 as the float reference's own accumulation
+    error and the Gauss violation stays at the reference's level."""
+    g = np.load(os.path.join(ROOT, "tests", "golden", f"program_{tag}.npz"))
+    ns = 4 if model == 0 else 8
+    root = _prepare(str(tmp_path), model, (6, 5, 7), 12, ns)
+    data = tmp_path / "data"
     subprocess.run([GFT_RUN, root, str(data)], check=True, capture_output=True, timeout=120)
     E = np.loadtxt(data / "global_quantities.dat", ndmin=2)
     G = np.loadtxt(data / "global_quantities_gauge.dat", ndmin=2)
     assert E.shape == g["energy"].shape and G.shape == g["gauss"].shape
+    err = np.abs(E - g["energy"]).max() / np.abs(g["energy"]).max()
+    print(f"{tag}: default form, energies vs the reference program: max rel err {err:.2e}")
     np.testing.assert_allclose(E, g["energy"], rtol=2e-5, atol=1e-5 * np.abs(g["energy"]).max())
     assert (G <= 10 * g["gauss"] + 1e-7).all()
 
@@ -175,7 +199,7 @@ def test_gft_run_reproduces_reference_program_with_fixed_faces(tmp_path, case, f
     g = np.load(os.path.join(ROOT, "tests", "golden", "program_sm_fixed.npz"))
     root = _prepare(str(tmp_path), 0, (6, 5, 7), 12, 4, faces=faces)
     data = tmp_path / "data"
-    subprocess.run([GFT_RUN, root, str(data)], check=True, capture_output=True, timeout=120)
+    subprocess.run([GFT_RUN, root, str(data), "two_slot_f32"], check=True, capture_output=True, timeout=120)
     E = np.loadtxt(data / "global_quantities.dat", ndmin=2)
     G = np.loadtxt(data / "global_quantities_gauge.dat", ndmin=2)
     ge, gg = g[f"energy_{case}"], g[f"gauss_{case}"]
@@ -195,7 +219,7 @@ def test_gft_run_reproduces_reference_program_generator_form(tmp_path, model, ta
     ns = 4 if model == 0 else 8
     root = _prepare(str(tmp_path), model, (6, 5, 7), 12, ns, gen=True)
     data = tmp_path / "data"
-    subprocess.run([GFT_RUN, root, str(data)], check=True, capture_output=True, timeout=120)
+    subprocess.run([GFT_RUN, root, str(data), "two_slot_f32"], check=True, capture_output=True, timeout=120)
     E = np.loadtxt(data / "global_quantities.dat", ndmin=2)
     G = np.loadtxt(data / "global_quantities_gauge.dat", ndmin=2)
     assert E.shape == g["energy"].shape and G.shape == g["gauss"].shape
@@ -214,6 +238,8 @@ def test_config_reader_family_gauge(tmp_path, nvd, gen, gauge_num):
                          check=True).stdout
     kv = {l.split()[0]: l.split()[1:] for l in out.strip().splitlines()}
     assert kv["gauge_num"] == [str(gauge_num)] and kv["nv"] == [str(3 * nvd)] and kv["gen_rep"] == [str(int(gen))]
+    # Family gauge fields run in the FP32 two-slot form (GFT_FORM_TWO_SLOT_F32 = 1) by default.
+    assert kv["form"] == ["1"]
 
 
 @needs_cfg
@@ -228,7 +254,7 @@ def test_gft_run_reproduces_reference_program_family_gauge(tmp_path, gauge_num, 
     nvd = (4 if gen else 5) + (1 if gauge_num == 5 else (3 if gen else 4))
     root = _prepare(str(tmp_path), 1, (6, 5, 7), 12, 8, gen=gen, nvd=nvd)
     data = tmp_path / "data"
-    subprocess.run([GFT_RUN, root, str(data)], check=True, capture_output=True, timeout=120)
+    subprocess.run([GFT_RUN, root, str(data), "two_slot_f32"], check=True, capture_output=True, timeout=120)
     E = np.loadtxt(data / "global_quantities.dat", ndmin=2)
     G = np.loadtxt(data / "global_quantities_gauge.dat", ndmin=2)
     assert E.shape == g["energy"].shape and G.shape == g["gauss"].shape == (G.shape[0], 2 * gauge_num)
@@ -249,7 +275,7 @@ def test_config_reader_global_model(tmp_path, ns, pot2):
     out = subprocess.run([GFT_RUN, os.path.join(str(tmp_path), "cfg"), str(tmp_path / "data"), "dump_config"],
                          capture_output=True, text=True, check=True).stdout
     kv = {l.split()[0]: l.split()[1:] for l in out.strip().splitlines()}
-    assert kv["model"] == ["2"] and kv["ns"] == [str(ns)]
+    assert kv["model"] == ["2"] and kv["ns"] == [str(ns)] and kv["form"] == ["1"]
 
 
 @needs_cfg
