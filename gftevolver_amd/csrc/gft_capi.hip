@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -85,6 +86,36 @@ struct gft_ctx {
 static int fail(gft_ctx* ctx, int code, const std::string& msg) {
     if (ctx) ctx->err = msg; else g_create_error = msg;
     return code;
+}
+
+// ---- allocation-free scratch ---------------------------------------------------
+// Reduction partials of gft_observe_noalloc: a file-scope device array that the
+// code object carries, one copy per device, held until the process ends.  Host
+// code never uses the symbol itself (in host code it is only a stand-in):
+// gft_create resolves the device address once per device with that device
+// current (scratch_resolve) and every launch takes that pointer.  Every
+// gft_observe_noalloc call on one device shares the array: calls that overlap on
+// one device (other streams or threads) race on it.
+constexpr size_t kScratchDoubles = size_t(1) << 20;   // 8 MiB
+constexpr int kScratchMaxDevices = 64;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kScratchMaxDevices] = {};
+
+// Device address of `scratch` on device dev (the current device), resolved on
+// the first call for that device.
+static int scratch_resolve(int dev, std::string& msg) {
+    if (dev < 0 || dev >= kScratchMaxDevices) { msg = "scratch: device index out of range"; return GFT_EINVAL; }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[dev]) return GFT_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        msg = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+        return GFT_EHIP;
+    }
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+    return GFT_OK;
 }
 
 static Coef make_coef(const gft_desc& d) {
@@ -201,6 +232,10 @@ int gft_create(const gft_desc* desc, gft_ctx** out) {
     ctx->N = ctx->coef.nsites;
     hipGetDevice(&ctx->device);
     auto bail = [&](int rc) { gft_destroy(ctx); return rc; };
+    {
+        std::string msg;
+        if (int rc = scratch_resolve(ctx->device, msg)) { g_create_error = msg; return bail(rc); }
+    }
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         g_create_error = "hipStreamCreate failed";
         return bail(GFT_EHIP);
@@ -473,15 +508,15 @@ static long long obs_blocks(const gft_ctx* ctx) { return (ctx->N + kBlock - 1) /
 // Finishes the Gauss partials (per block: G integrated |G^a|, G scales, G
 // maxima) into obs_dev[8 ..): a row wider than the finisher is done in two
 // column slices.
-static void finish_gauss(gft_ctx* ctx, long long nblocks) {
+static void finish_gauss(gft_ctx* ctx, const double* partial, long long nblocks) {
     const int G = gauge_num(ctx->desc), W = 3 * G;
     if (W <= kFinWidth) {
-        finalize_partials<<<1, kFinThreads, 0, ctx->stream>>>(ctx->partial, nblocks, W, 2 * G, G, ctx->obs_dev + 8);
+        finalize_partials<<<1, kFinThreads, 0, ctx->stream>>>(partial, nblocks, W, 2 * G, G, ctx->obs_dev + 8);
         return;
     }
     const int s1 = std::min(2 * G, kFinWidth);
-    finalize_partials<<<1, kFinThreads, 0, ctx->stream>>>(ctx->partial, nblocks, W, s1, 0, ctx->obs_dev + 8);
-    finalize_partials<<<1, kFinThreads, 0, ctx->stream>>>(ctx->partial + s1, nblocks, W, 2 * G - s1, G,
+    finalize_partials<<<1, kFinThreads, 0, ctx->stream>>>(partial, nblocks, W, s1, 0, ctx->obs_dev + 8);
+    finalize_partials<<<1, kFinThreads, 0, ctx->stream>>>(partial + s1, nblocks, W, 2 * G - s1, G,
                                                           ctx->obs_dev + 8 + s1);
 }
 
@@ -572,7 +607,7 @@ extern "C" int gft_evolve(gft_ctx* ctx, unsigned nsteps, unsigned energy_freq, u
             else if ((rc = momentum_observe(ctx->mom, ctx->coef, MomObs::Gauss, ctx->partial, nblocks, ctx->stream,
                                             ctx->err)))
                 return rc;
-            finish_gauss(ctx, nblocks);
+            finish_gauss(ctx, ctx->partial, nblocks);
             CK(hipGetLastError());
         }
         if ((me || mg) && row < max_rows && rows) {
@@ -616,6 +651,23 @@ extern "C" int gft_last_step_ms(gft_ctx* ctx, double* avg_ms, double* total_ms, 
     return GFT_OK;
 }
 
+// Energy and (after a step) Gauss-law reductions of the current state with the
+// per-block partials in `partial` (nblocks x 12 doubles at least).
+static int observe_into(gft_ctx* ctx, double* partial, long long nblocks, gft_obs* out) {
+    int rc;
+    if ((rc = momentum_observe(ctx->mom, ctx->coef, MomObs::Energy, partial, nblocks, ctx->stream, ctx->err)))
+        return rc;
+    finalize_partials<<<1, kFinThreads, 0, ctx->stream>>>(partial, nblocks, 5, 5, 0, ctx->obs_dev);
+    const bool g = ctx->mom.prev_valid;
+    if (g) {
+        if ((rc = momentum_observe(ctx->mom, ctx->coef, MomObs::Gauss, partial, nblocks, ctx->stream, ctx->err)))
+            return rc;
+        finish_gauss(ctx, partial, nblocks);
+    }
+    CK(hipGetLastError());
+    return collect_row(ctx, true, g, out);
+}
+
 extern "C" int gft_observe(gft_ctx* ctx, gft_obs* out) {
     if (!ctx || !out) return GFT_EINVAL;
     if (is_two_slot(ctx->form))
@@ -623,17 +675,21 @@ extern "C" int gft_observe(gft_ctx* ctx, gft_obs* out) {
     const long long nblocks = obs_blocks(ctx);
     int rc = ensure_partial(ctx, (size_t)nblocks * 12);
     if (rc) return rc;
-    if ((rc = momentum_observe(ctx->mom, ctx->coef, MomObs::Energy, ctx->partial, nblocks, ctx->stream, ctx->err)))
-        return rc;
-    finalize_partials<<<1, kFinThreads, 0, ctx->stream>>>(ctx->partial, nblocks, 5, 5, 0, ctx->obs_dev);
-    const bool g = ctx->mom.prev_valid;
-    if (g) {
-        if ((rc = momentum_observe(ctx->mom, ctx->coef, MomObs::Gauss, ctx->partial, nblocks, ctx->stream, ctx->err)))
-            return rc;
-        finish_gauss(ctx, nblocks);
-    }
-    CK(hipGetLastError());
-    return collect_row(ctx, true, g, out);
+    return observe_into(ctx, ctx->partial, nblocks, out);
+}
+
+extern "C" int gft_observe_noalloc(gft_ctx* ctx, gft_obs* out) {
+    if (!ctx || !out) return GFT_EINVAL;
+    if (is_two_slot(ctx->form))
+        return fail(ctx, GFT_EINVAL, "gft_observe_noalloc: the two-slot forms measure inside gft_evolve");
+    const long long nblocks = obs_blocks(ctx);
+    if ((size_t)nblocks * 12 > kScratchDoubles)
+        return fail(ctx, GFT_EINVAL, "gft_observe_noalloc: the lattice has " + std::to_string(ctx->N) +
+                                         " sites; the scratch array holds the partials of at most " +
+                                         std::to_string(kScratchDoubles / 12 * kBlock) + " sites");
+    double* partial = (ctx->device >= 0 && ctx->device < kScratchMaxDevices) ? g_scratch_ptr[ctx->device] : nullptr;
+    if (!partial) return fail(ctx, GFT_EHIP, "gft_observe_noalloc: scratch address of the device not resolved");
+    return observe_into(ctx, partial, nblocks, out);
 }
 
 extern "C" int gft_wilson_loops(gft_ctx* ctx, gft_loops* out) {

@@ -34,6 +34,7 @@ for _n, _t in (("f32", C.c_float), ("f64", C.c_double)):
     getattr(_L, f"gft_export_two_slot_{_n}").argtypes = [C.c_void_p, _P(_t), _P(_t), _P(C.c_uint)]
 _L.gft_evolve.argtypes = [C.c_void_p, C.c_uint, C.c_uint, C.c_uint, _P(GftObs), C.c_uint, _P(C.c_uint)]
 _L.gft_observe.argtypes = [C.c_void_p, _P(GftObs)]
+_L.gft_observe_noalloc.argtypes = [C.c_void_p, _P(GftObs)]
 _L.gft_wilson_loops.argtypes = [C.c_void_p, _P(GftLoops)]
 _L.gft_sync.argtypes = [C.c_void_p]
 _L.gft_set_launch.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
@@ -169,6 +170,12 @@ class Evolver:
     def observe(self) -> GftObs:
         o = GftObs()
         self._check(_L.gft_observe(self._h, C.byref(o)))
+        return o
+
+    def observe_noalloc(self) -> GftObs:
+        """gft_observe_noalloc: the same observables without device allocation."""
+        o = GftObs()
+        self._check(_L.gft_observe_noalloc(self._h, C.byref(o)))
         return o
 
     def wilson_loops(self) -> GftLoops:

@@ -161,6 +161,19 @@ int gft_evolve(gft_ctx* ctx, unsigned nsteps, unsigned energy_freq, unsigned gau
  * the current time; two-slot forms: the reference's mixed-time energy). */
 int gft_observe(gft_ctx* ctx, gft_obs* out);
 
+/* gft_observe without any device allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The observables are those
+ * of gft_observe, bitwise: the same kernels, with their per-block reduction
+ * partials in a library-static device array of GFT_NOALLOC_SCRATCH_BYTES per
+ * device (held until the process ends) instead of a context buffer.  The
+ * momentum forms only; lattices of at most GFT_NOALLOC_MAX_SITES sites (more:
+ * GFT_EINVAL).  Every call on one device shares that device's array, so calls
+ * that overlap on one device (contexts on other streams or threads) race. */
+#define GFT_NOALLOC_SCRATCH_BYTES (8u << 20)
+#define GFT_NOALLOC_MAX_SITES 22369536LL   /* (8 MiB / (12 doubles per 256-site block)) x 256 */
+int gft_observe_noalloc(gft_ctx* ctx, gft_obs* out);
+
 /* Extra lattice observables (FP64): mean 1/2 tr of SU(2) Wilson loops and mean
  * cos of U(1) Wilson loops of size 1x1 and 2x2 over the three spatial planes,
  * and <|phi|^2>. */
