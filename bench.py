@@ -32,7 +32,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "site-updates/s + HBM GB/s, FP32-mixed vs FP64 at Gauss err<=1e-6, 256^3 SM"
 BYTES_F32 = 280   # 35 reals read + 35 written per site-update, 4 B each (algorithmic)
-STORED_F32C = 300  # what the compensated form moves: + 4-bit residuals of phi, P, E (20 x 0.5 B each way)
+STORED_F32C = 296  # what the compensated form moves: + one 64-bit word of 4-bit residuals of P and E each way
 BYTES_F64 = 560
 HBM_PEAK_GBS = 8000.0
 

@@ -755,7 +755,7 @@ extern "C" int gft_bench_copy(size_t bytes, int reps, double* avg_ms) {
 
 // Streams the arrays of one FP32 SM momentum-form step, own site only, with
 // the step kernel's access kinds (cached loads of q, y, phi; streaming loads
-// of P, E and residuals; streaming stores), its tiles and block order, and
+// of P, E and the residual word; streaming stores), its tiles and block order, and
 // trivial arithmetic: the HBM floor of the step's data set without its stencil.
 template <bool CMP>
 __global__ void __launch_bounds__(256) gft_stream_kernel(StepArgs<float> a) {
@@ -766,28 +766,25 @@ __global__ void __launch_bounds__(256) gft_stream_kernel(StepArgs<float> a) {
     const unsigned s = static_cast<unsigned>((tile.y * a.ny + ys) * a.nz + zs);
     float4 q[3], f = at(a.phi[0], s), p = ld_nt(ptr_at(a.P[0], s)), e[3];
     float y[3];
-    unsigned ls = 0u;
-    unsigned short le[3] = {0, 0, 0};
+    uint2 lw = make_uint2(0u, 0u);
 #pragma unroll
     for (int k = 0; k < 3; k++) {
         q[k] = at(a.q[k], s);
         y[k] = at(a.y[k], s);
         e[k] = ld_nt(ptr_at(a.E[k], s));
-        if (CMP) le[k] = ld_nt(ptr_at(a.lo_E[k], s));
     }
-    if (CMP) ls = ld_nt(ptr_at(a.lo_s[0], s));
+    if (CMP) lw = ld_nt(ptr_at(a.lo, s));
     const float t = f.x + p.x + y[0] + y[1] + y[2];
     p.x += t;
     f.x += t;
     st_nt(p, ptr_at(a.P[0], s));
     st_nt(f, ptr_at(a.phi_out[0], s));
-    if (CMP) st_nt(ls + 1u, ptr_at(a.lo_s[0], s));
+    if (CMP) st_nt(make_uint2(lw.x + 1u, lw.y), ptr_at(a.lo, s));
 #pragma unroll
     for (int k = 0; k < 3; k++) {
         e[k].x += t;
         q[k].x += t;
         st_nt(e[k], ptr_at(a.E[k], s));
-        if (CMP) st_nt(static_cast<unsigned short>(le[k] + 1u), ptr_at(a.lo_E[k], s));
         st_nt(y[k] + t, ptr_at(a.y_out[k], s));
         st_nt(q[k], ptr_at(a.q_out[k], s));
     }
@@ -838,7 +835,7 @@ struct gft_ensemble {
     int stage = 0;           // staging mode of the ensemble kernel
     unsigned* err_dev = nullptr;
     bool dirty = true;
-    std::vector<int4> launch_key;   // (bz, by, stage) of every member at the last prepare
+    std::vector<int4> launch_key;   // (bz, by, stage, state epoch) of every member at the last prepare
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int graph_parity = -1;
@@ -869,12 +866,13 @@ static void ens_drop_graph(gft_ensemble* ens) {
 
 // Uploads the per-member step arguments (both parities) and the work list.
 static int ens_prepare(gft_ensemble* ens) {
-    // A member's launch shape may also change through its own context (gft_set_launch).
+    // A member's launch shape may also change through its own context (gft_set_launch),
+    // and its reference doublets (part of the step arguments) with every import.
     std::vector<int4> key;
-    for (auto c : ens->members) key.push_back(make_int4(c->mom.bz, c->mom.by, c->mom.stage, 0));
+    for (auto c : ens->members) key.push_back(make_int4(c->mom.bz, c->mom.by, c->mom.stage, (int)c->mom.epoch));
     const bool same = key.size() == ens->launch_key.size() &&
                       std::equal(key.begin(), key.end(), ens->launch_key.begin(), [](const int4& p, const int4& q) {
-                          return p.x == q.x && p.y == q.y && p.z == q.z;
+                          return p.x == q.x && p.y == q.y && p.z == q.z && p.w == q.w;
                       });
     if (!same) ens->dirty = true;
     if (!ens->dirty) return GFT_OK;
@@ -1073,36 +1071,28 @@ __global__ void __launch_bounds__(256) grf_sumsq4(const double* __restrict__ f, 
     block_reduce_store<1, 0>(v, nullptr, lds, partial, blockIdx.x);
 }
 
-// 4-bit residual of v next to its FP32 hi, as the host import encodes it.
-__device__ inline unsigned grf_res_q(double v, float hi) {
-    const double q = rint(ldexp(v - (double)hi, 153 - res_exp(hi)));
-    return static_cast<unsigned>(static_cast<int>(fmin(7.0, fmax(-8.0, q)))) & 0xfu;
-}
-
 // phi_k = (0, 0, vev_k, 0) + scale_k * comps, with scale_k = rms / sqrt(<sum_c f_c^2>)
-// from sq[k] (sum over sites); P = 0, E = 0, links identity.
+// from sq[k] (sum over sites), stored as phi_k - ref_k (ref_k = (0, 0, vevr_k, 0),
+// the vev rounded to FP32 for the FP32 forms, zero for FP64); P = 0, E = 0,
+// links identity.
 template <class R>
 __global__ void __launch_bounds__(256) grf_assemble_mom(MomFields<R> f, const double* __restrict__ comps, long long n,
-                                                        int nd, double rms, double vev0, double vev1,
-                                                        const double* __restrict__ sq) {
+                                                        int nd, double rms, double vev0, double vev1, double vevr0,
+                                                        double vevr1, const double* __restrict__ sq) {
     using V4 = typename V4T<R>::type;
     const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     for (int k = 0; k < nd; k++) {
         const double scale = rms / sqrt(sq[k] / (double)n);
         double v[4];
-        for (int m = 0; m < 4; m++) v[m] = (m == 2 ? (k ? vev1 : vev0) : 0.0) + scale * comps[(4 * k + m) * n + s];
+        for (int m = 0; m < 4; m++)
+            v[m] = (m == 2 ? (k ? vev1 - vevr1 : vev0 - vevr0) : 0.0) + scale * comps[(4 * k + m) * n + s];
         V4 hi;
         hi.x = (R)v[0]; hi.y = (R)v[1]; hi.z = (R)v[2]; hi.w = (R)v[3];
         f.phi[0][k][s] = hi;
         V4 z;
         z.x = z.y = z.z = z.w = R(0);
         f.P[k][s] = z;
-        if (f.lo_s[k]) {
-            unsigned w = 0;
-            for (int m = 0; m < 4; m++) w |= grf_res_q(v[m], (float)((&hi.x)[m])) << (4 * m);
-            f.lo_s[k][s] = w;   // P = 0: no P residuals
-        }
     }
     for (int i = 0; i < 3; i++) {
         V4 e, q;
@@ -1111,8 +1101,9 @@ __global__ void __launch_bounds__(256) grf_assemble_mom(MomFields<R> f, const do
         f.E[i][s] = e;
         f.q[0][i][s] = q;
         f.y[0][i][s] = R(0);
-        if (f.lo_E[i]) f.lo_E[i][s] = 0;
     }
+    if (f.lo) f.lo[s] = make_uint2(0u, 0u);   // P = 0, E = 0: no residuals
+    if (f.lo_P2) f.lo_P2[s] = 0;
 }
 
 template <class T>
@@ -1197,10 +1188,16 @@ extern "C" int gft_init_grf(gft_ctx* ctx, const gft_grf_desc* g) {
     } else {
         ctx->mom.cur = 0;
         ctx->mom.prev_valid = 0;
+        const double vr0 = ctx->mom.f64 ? 0.0 : (double)(float)vev0, vr1 = ctx->mom.f64 ? 0.0 : (double)(float)vev1;
+        for (int k = 0; k < 2; k++)
+            for (int m4 = 0; m4 < 4; m4++) ctx->mom.ref[k][m4] = (m4 == 2 && k < c.nd) ? (k ? vr1 : vr0) : 0.0;
+        ctx->mom.epoch++;
         if (ctx->mom.f64)
-            grf_assemble_mom<double><<<nb, 256, 0, st>>>(ctx->mom.fd, comps, N, c.nd, g->rms, vev0, vev1, ctx->obs_dev + 2);
+            grf_assemble_mom<double><<<nb, 256, 0, st>>>(ctx->mom.fd, comps, N, c.nd, g->rms, vev0, vev1, vr0, vr1,
+                                                         ctx->obs_dev + 2);
         else
-            grf_assemble_mom<float><<<nb, 256, 0, st>>>(ctx->mom.ff, comps, N, c.nd, g->rms, vev0, vev1, ctx->obs_dev + 2);
+            grf_assemble_mom<float><<<nb, 256, 0, st>>>(ctx->mom.ff, comps, N, c.nd, g->rms, vev0, vev1, vr0, vr1,
+                                                        ctx->obs_dev + 2);
     }
     const bool ok = hipGetLastError() == hipSuccess && hipStreamSynchronize(st) == hipSuccess;
     cleanup();

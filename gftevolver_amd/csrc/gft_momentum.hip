@@ -2,7 +2,8 @@
 // reference leapfrog, one fused kernel per step.
 //
 // State per site (SM; the 2HDM doubles phi and P):
-//   phi  float4            scalar doublet at t                 (ping-pong)
+//   phi  float4            scalar doublet at t, stored as its difference
+//                          from a constant reference doublet   (ping-pong)
 //   P    float4            phi(t) - phi(t-dt)                  (in place)
 //   y[3] float             U(1) link phases at t, wrapped      (ping-pong)
 //   q[3] float4            SU(2) link quaternions at t         (ping-pong)
@@ -52,18 +53,27 @@ struct MomFields {
     R* y[2][3];      // [buffer][direction]
     V4* q[2][3];     // [buffer][direction]
     V4* E[3];        // [direction]
-    // Compensated FP32 storage only: 4-bit residuals of (phi, P) per doublet
-    // (bits 4m..4m+3: phi_lo[m], 16 + 4m..: P_lo[m]) and of E per direction
-    // (bits 4m..4m+3: E_lo[m]); null otherwise.
-    unsigned* lo_s[2];         // [doublet]
-    unsigned short* lo_E[3];   // [direction]
+    // Compensated FP32 storage only (null otherwise): the 4-bit residuals of P
+    // and E of a site in one 64-bit word, nibble m of a 16-bit group = value m:
+    //   lo.x bits 0..15: P of doublet 0, bits 16..31: E[0];
+    //   lo.y bits 0..15: E[1],           bits 16..31: E[2];
+    // and those of the second doublet's P (2HDM) in lo_P2.
+    uint2* lo;
+    unsigned short* lo_P2;
 };
 
 struct MomentumState {
     void* base = nullptr;
     size_t bytes = 0;
     bool f64 = false;
-    bool cmp = false;        // FP32 storage with 4-bit residuals of phi, P and E
+    bool cmp = false;        // FP32 storage with 4-bit residuals of P and E
+    // Reference doublets: phi is stored as phi - ref (FP32 forms: the mean of the
+    // imported phi rounded to FP32, or the vacuum (0, 0, v, 0) of a generated
+    // state; FP64 form: zero), so its FP32 rounding is relative to the
+    // fluctuation, not to the vacuum value.
+    double ref[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+    unsigned epoch = 0;      // counts state replacements (import, generation): kernel arguments
+                             // that carry ref are rebuilt when it moves
     int cur = 0;             // buffer holding (phi, U) at the current time
     int prev_valid = 0;      // buffer !cur holds the state one step earlier
     MomFields<float> ff{};
@@ -266,6 +276,10 @@ __device__ __forceinline__ unsigned res_pack4(int q0, int q1, int q2, int q3) {
 }
 __device__ __forceinline__ int res_nib(unsigned w, int m) { return static_cast<int>(w << (28 - 4 * m)) >> 28; }
 __device__ __forceinline__ float res_val(unsigned w, int m, float hi) { return res_from_q(res_nib(w, m), hi); }
+// The 16-bit group of the residual word of a site: g = 0 P (doublet 0), 1..3 E[g - 1].
+__device__ __forceinline__ unsigned res_group(const uint2& w, int g) {
+    return ((g < 2 ? w.x : w.y) >> (16 * (g & 1))) & 0xffffu;
+}
 
 // (h, l) <- keep (h + l) + delta; returns the new hi, *r the exact residual.
 __device__ __forceinline__ float cmp_update(float h, float l, float keep, float delta, float* r) {
@@ -348,8 +362,9 @@ struct StepArgs {
     const V4* q[3];
     V4* q_out[3];
     V4* E[3];
-    unsigned* lo_s[2];       // compensated storage: residuals of (phi, P) and E
-    unsigned short* lo_E[3];
+    uint2* lo;               // compensated storage: residuals of P and E (see MomFields)
+    unsigned short* lo_P2;
+    V4 ref[2];               // reference doublets: phi = stored + ref
     int nx, ny, nz, nbz, nby;
     long long nblocks;
     int bz, by, order, bz_shift;   // bz = 1 << bz_shift
@@ -487,34 +502,55 @@ __device__ __forceinline__ void plane_fill(const StepArgs<float>& a, const Plane
 template <class R, int ND>
 struct OwnState {
     typename V4T<R>::type P[ND];
-    unsigned ls[ND];
     typename V4T<R>::type E[3];
-    unsigned short lE[3];
+    uint2 lw;              // residual word: P of doublet 0 and E
+    unsigned short lp2;    // residuals of the second doublet's P
 };
 template <class R, int ND, bool CMP>
 __device__ __forceinline__ void load_own(const StepArgs<R>& a, unsigned s, OwnState<R, ND>& o) {
 #pragma unroll
-    for (int k = 0; k < ND; k++) {
-        o.P[k] = ld_nt(ptr_at(a.P[k], s));
-        if constexpr (CMP) o.ls[k] = ld_nt(ptr_at(a.lo_s[k], s));
-    }
+    for (int k = 0; k < ND; k++) o.P[k] = ld_nt(ptr_at(a.P[k], s));
 #pragma unroll
-    for (int i = 0; i < 3; i++) {
-        o.E[i] = ld_nt(ptr_at(a.E[i], s));
-        if constexpr (CMP) o.lE[i] = ld_nt(ptr_at(a.lo_E[i], s));
+    for (int i = 0; i < 3; i++) o.E[i] = ld_nt(ptr_at(a.E[i], s));
+    if constexpr (CMP) {
+        o.lw = ld_nt(ptr_at(a.lo, s));
+        if constexpr (ND == 2) o.lp2 = ld_nt(ptr_at(a.lo_P2, s));
     }
+}
+
+// Full doublet = stored value + reference (one rounding per component, the same
+// expression wherever a doublet is read, so every staging mode agrees bitwise).
+template <class V4, class R>
+__device__ __forceinline__ void add_ref(const V4& ref, const R* raw, R* o) {
+    o[0] = raw[0] + ref.x;
+    o[1] = raw[1] + ref.y;
+    o[2] = raw[2] + ref.z;
+    o[3] = raw[3] + ref.w;
 }
 
 // PRE: own state already loaded into o (else loaded here, next to its use).
 // U1: no damping (keep = 1), with the kick's keep products dropped (exact).
+// praw: the site's stored doublet(s) (phi - ref), which the drift advances.
 template <class R, int ND, bool CMP, bool PRE = false, bool U1 = false>
 // frozen: a site of a fixed face's shell -- every lane still stores, but a
 // frozen lane stores the values it read, so every store instruction writes
 // whole cache lines (a divergent copy branch would write partial lines twice).
 __device__ __forceinline__ void kick_drift_t(const StepArgs<R>& a, unsigned s, const Qt<R>* U, const R* yv,
-                                             const R* phi, const R* F, const R (*J)[4], const R (*M)[4],
+                                             const R* praw, const R* F, const R (*J)[4], const R (*M)[4],
                                              const OwnState<R, ND>* o, bool frozen) {
     using V4 = typename V4T<R>::type;
+    uint2 lw = make_uint2(0u, 0u);
+    unsigned lp2 = 0u;
+    if constexpr (CMP) {
+        if constexpr (PRE) {
+            lw = o->lw;
+            if constexpr (ND == 2) lp2 = o->lp2;
+        } else {
+            lw = ld_nt(ptr_at(a.lo, s));
+            if constexpr (ND == 2) lp2 = ld_nt(ptr_at(a.lo_P2, s));
+        }
+    }
+    unsigned nwx = 0u, nwy = 0u, nlp2 = 0u;   // the new residual groups
 #pragma unroll
     for (int k = 0; k < ND; k++) {
         R p[4];
@@ -522,27 +558,19 @@ __device__ __forceinline__ void kick_drift_t(const StepArgs<R>& a, unsigned s, c
         else unpack(ld_nt(ptr_at(a.P[k], s)), p);
         V4 pn, fn;
         if constexpr (CMP) {
-            unsigned L;   // nibbles 0..3: phi residuals, 4..7: P residuals
-            if constexpr (PRE) L = o->ls[k];
-            else L = ld_nt(ptr_at(a.lo_s[k], s));
-            float ph[4], pr[4], fh[4], fr[4];
+            const unsigned L = k == 0 ? res_group(lw, 0) : lp2;
+            float ph[4], pr[4];
             int qp[4];
 #pragma unroll
             for (int m = 0; m < 4; m++) {
-                if constexpr (U1) ph[m] = cmp_update1(p[m], res_val(L, 4 + m, p[m]), a.dt2 * F[4 * k + m], &pr[m]);
-                else ph[m] = cmp_update(p[m], res_val(L, 4 + m, p[m]), a.keep, a.dt2 * F[4 * k + m], &pr[m]);
+                if constexpr (U1) ph[m] = cmp_update1(p[m], res_val(L, m, p[m]), a.dt2 * F[4 * k + m], &pr[m]);
+                else ph[m] = cmp_update(p[m], res_val(L, m, p[m]), a.keep, a.dt2 * F[4 * k + m], &pr[m]);
                 qp[m] = res_q(pr[m], ph[m]);
             }
-            // phi(t+dt) = phi(t) + P(t+dt) with P as stored (its residual rounded).
-#pragma unroll
-            for (int m = 0; m < 4; m++)
-                fh[m] = cmp_add(phi[4 * k + m], res_val(L, m, phi[4 * k + m]), ph[m], res_from_q(qp[m], ph[m]), &fr[m]);
             pn = make_float4(ph[0], ph[1], ph[2], ph[3]);
-            fn = make_float4(fh[0], fh[1], fh[2], fh[3]);
-            unsigned Ln = res_pack4(res_q(fr[0], fh[0]), res_q(fr[1], fh[1]), res_q(fr[2], fh[2]), res_q(fr[3], fh[3]))
-                        | (res_pack4(qp[0], qp[1], qp[2], qp[3]) << 16);
-            if (frozen) Ln = L;
-            st_nt(Ln, ptr_at(a.lo_s[k], s));
+            const unsigned Ln = frozen ? L : res_pack4(qp[0], qp[1], qp[2], qp[3]);
+            if (k == 0) nwx = Ln;
+            else nlp2 = Ln;
         } else {
             // Explicit FMAs fix which product is fused (exact keep * p for keep = 1).
             const R kp[4] = {U1 ? p[0] : a.keep * p[0], U1 ? p[1] : a.keep * p[1], U1 ? p[2] : a.keep * p[2],
@@ -551,14 +579,17 @@ __device__ __forceinline__ void kick_drift_t(const StepArgs<R>& a, unsigned s, c
             pn.y = fma(a.dt2, F[4 * k + 1], kp[1]);
             pn.z = fma(a.dt2, F[4 * k + 2], kp[2]);
             pn.w = fma(a.dt2, F[4 * k + 3], kp[3]);
-            fn.x = phi[4 * k + 0] + pn.x;
-            fn.y = phi[4 * k + 1] + pn.y;
-            fn.z = phi[4 * k + 2] + pn.z;
-            fn.w = phi[4 * k + 3] + pn.w;
         }
+        // Drift of the stored doublet: (phi - ref)(t+dt) = (phi - ref)(t) + P(t+dt).
+        // The rounding is relative to the fluctuation phi - ref; P's residual lies
+        // far below its ulp.
+        fn.x = praw[4 * k + 0] + pn.x;
+        fn.y = praw[4 * k + 1] + pn.y;
+        fn.z = praw[4 * k + 2] + pn.z;
+        fn.w = praw[4 * k + 3] + pn.w;
         if (frozen) {
             pn = pack<V4, R>(p);
-            fn = pack<V4, R>(phi + 4 * k);
+            fn = pack<V4, R>(praw + 4 * k);
         }
         st_nt(pn, ptr_at(a.P[k], s));
         st_nt(fn, ptr_at(a.phi_out[k], s));
@@ -571,19 +602,18 @@ __device__ __forceinline__ void kick_drift_t(const StepArgs<R>& a, unsigned s, c
         else unpack(ld_nt(ptr_at(a.E[i], s)), e);
         V4 en;
         if constexpr (CMP) {
-            unsigned short L;
-            if constexpr (PRE) L = o->lE[i];
-            else L = ld_nt(ptr_at(a.lo_E[i], s));
+            const unsigned L = res_group(lw, 1 + i);
             float eh[4], er[4];
 #pragma unroll
             for (int m = 0; m < 4; m++)
                 if constexpr (U1) eh[m] = cmp_update1(e[m], res_val(L, m, e[m]), a.dt2 * (M[i][m] + a.gsq[m] * J[i][m]), &er[m]);
                 else eh[m] = cmp_update(e[m], res_val(L, m, e[m]), a.keep, a.dt2 * (M[i][m] + a.gsq[m] * J[i][m]), &er[m]);
             en = make_float4(eh[0], eh[1], eh[2], eh[3]);
-            st_nt(frozen ? L
-                         : static_cast<unsigned short>(res_pack4(res_q(er[0], eh[0]), res_q(er[1], eh[1]), res_q(er[2], eh[2]),
-                                                                 res_q(er[3], eh[3]))),
-                  ptr_at(a.lo_E[i], s));
+            const unsigned Ln =
+                frozen ? L : res_pack4(res_q(er[0], eh[0]), res_q(er[1], eh[1]), res_q(er[2], eh[2]), res_q(er[3], eh[3]));
+            if (i == 0) nwx |= Ln << 16;
+            else if (i == 1) nwy = Ln;
+            else nwy |= Ln << 16;
         } else {
             en.x = fma(a.dt2, fma(a.gsq[0], J[i][0], M[i][0]), U1 ? e[0] : a.keep * e[0]);
             en.y = fma(a.dt2, fma(a.gsq[1], J[i][1], M[i][1]), U1 ? e[1] : a.keep * e[1]);
@@ -612,16 +642,20 @@ __device__ __forceinline__ void kick_drift_t(const StepArgs<R>& a, unsigned s, c
         st_nt(frozen ? yv[i] : yn, ptr_at(a.y_out[i], s));
         st_nt(qo, ptr_at(a.q_out[i], s));
     }
+    if constexpr (CMP) {
+        st_nt(make_uint2(nwx, nwy), ptr_at(a.lo, s));
+        if constexpr (ND == 2) st_nt(static_cast<unsigned short>(nlp2), ptr_at(a.lo_P2, s));
+    }
     if (bad) atomicOr(a.err, 1u);
 }
 
 // The kick with a uniform branch on damping: keep = 1 takes the shorter path.
 template <class R, int ND, bool CMP, bool PRE = false>
 __device__ __forceinline__ void kick_drift(const StepArgs<R>& a, unsigned s, const Qt<R>* U, const R* yv,
-                                           const R* phi, const R* F, const R (*J)[4], const R (*M)[4],
+                                           const R* praw, const R* F, const R (*J)[4], const R (*M)[4],
                                            const OwnState<R, ND>* o = nullptr, bool frozen = false) {
-    if (a.keep == R(1)) kick_drift_t<R, ND, CMP, PRE, true>(a, s, U, yv, phi, F, J, M, o, frozen);
-    else kick_drift_t<R, ND, CMP, PRE, false>(a, s, U, yv, phi, F, J, M, o, frozen);
+    if (a.keep == R(1)) kick_drift_t<R, ND, CMP, PRE, true>(a, s, U, yv, praw, F, J, M, o, frozen);
+    else kick_drift_t<R, ND, CMP, PRE, false>(a, s, U, yv, praw, F, J, M, o, frozen);
 }
 
 // One site-update at (x, y, z): forces, kick and drift (shared by the
@@ -673,11 +707,17 @@ __device__ __forceinline__ void mom_site(const StepArgs<R>& a, const Coef& c, in
         }
         return at(a.y[f], gsite(i, si, j, sj));
     };
-    auto PHI = [&](int k, int i, int si, R* o) {
+    // Stored doublet k at x + si e_i, and the full doublet (stored + reference).
+    auto PHIRAW = [&](int k, int i, int si, R* o) {
         if constexpr (PL) {
             if (in_plane(i, si, 0, 0)) { unpack(pl.phi[k][slot(i, si, 0, 0)], o); return; }
         }
         unpack(at(a.phi[k], gsite(i, si, 0, 0)), o);
+    };
+    auto PHI = [&](int k, int i, int si, R* o) {
+        R raw[4];
+        PHIRAW(k, i, si, raw);
+        add_ref(a.ref[k], raw, o);
     };
 
     // Own links and their phases.
@@ -690,9 +730,12 @@ __device__ __forceinline__ void mom_site(const StepArgs<R>& a, const Coef& c, in
         sincos_fast(yv[i], &sy_[i], &cy[i]);
     }
 
-    R phi[4 * ND];
+    R phi[4 * ND], praw[4 * ND];
 #pragma unroll
-    for (int k = 0; k < ND; k++) PHI(k, 0, 0, phi + 4 * k);
+    for (int k = 0; k < ND; k++) {
+        PHIRAW(k, 0, 0, praw + 4 * k);
+        add_ref(a.ref[k], praw + 4 * k, phi + 4 * k);
+    }
 
     // Scalar force and currents.
     R F[4 * ND];
@@ -805,7 +848,7 @@ __device__ __forceinline__ void mom_site(const StepArgs<R>& a, const Coef& c, in
         }
     }
 
-    kick_drift<R, ND, CMP>(a, s, U, yv, phi, F, J, M, nullptr, frozen);
+    kick_drift<R, ND, CMP>(a, s, U, yv, praw, F, J, M, nullptr, frozen);
 }
 
 // ---- shared plaquettes and covariant differences (FP32 SM, staged) ----------
@@ -880,7 +923,7 @@ __device__ __forceinline__ void mom_block_shared(const StepArgs<float>& a, const
     const int r0 = pl.r0;
     const unsigned sm0 = static_cast<unsigned>(xm * sx + y * a.nz + z);   // x - e_0
     Qt<R> U[3];
-    R cy[3], sy_[3], yv[3], phi[4], F[4], J[3][4], M[3][4], dpv[3][4], dsh[3][4];
+    R cy[3], sy_[3], yv[3], phi[4], praw[4], F[4], J[3][4], M[3][4], dpv[3][4], dsh[3][4];
     Plaq<R> P[3];   // at x: (0,1), (0,2), (1,2)
     OwnState<R, 1> own;
     if (valid) load_own<R, 1, CMP>(a, s, own);   // issued early: its latency overlaps phase A
@@ -894,7 +937,8 @@ __device__ __forceinline__ void mom_block_shared(const StepArgs<float>& a, const
             yv[i] = pl.y[i][r0];
             sincos_fast(yv[i], &sy_[i], &cy[i]);
         }
-        unpack(pl.phi[0][r0], phi);
+        unpack(pl.phi[0][r0], praw);
+        add_ref(a.ref[0], praw, phi);
         potential_derivative<R>(c, phi, F);
 #pragma unroll
         for (int k = 0; k < 4; k++) F[k] = -F[k];
@@ -903,9 +947,10 @@ __device__ __forceinline__ void mom_block_shared(const StepArgs<float>& a, const
         const int step[3] = {0, W, 1};
 #pragma unroll
         for (int i = 0; i < 3; i++) {
-            R fp[4], tx[4];
-            if (i == 0) unpack(at(a.phi[0], sp0), fp);
-            else unpack(pl.phi[0][r0 + step[i]], fp);
+            R fr[4], fp[4], tx[4];
+            if (i == 0) unpack(at(a.phi[0], sp0), fr);
+            else unpack(pl.phi[0][r0 + step[i]], fr);
+            add_ref(a.ref[0], fr, fp);
             transport(U[i], cy[i], sy_[i], false, phi, tx);
             J[i][0] = J[i][1] = J[i][2] = J[i][3] = R(0);
 #pragma unroll
@@ -916,9 +961,10 @@ __device__ __forceinline__ void mom_block_shared(const StepArgs<float>& a, const
         // d_x(x - e_x) from the plane x - 1; the scalar force's i = 0 term.
         {
             const Qt<R> Um = toq<float4, R>(at(a.q[0], sm0));
-            R cm, smn, fm[4], tm[4], dmv0[4];
+            R cm, smn, fr[4], fm[4], tm[4], dmv0[4];
             sincos_fast(at(a.y[0], sm0), &smn, &cm);
-            unpack(at(a.phi[0], sm0), fm);
+            unpack(at(a.phi[0], sm0), fr);
+            add_ref(a.ref[0], fr, fm);
             transport(Um, cm, smn, false, fm, tm);
 #pragma unroll
             for (int m = 0; m < 4; m++) dmv0[m] = phi[m] - tm[m];
@@ -998,10 +1044,12 @@ __device__ __forceinline__ void mom_block_shared(const StepArgs<float>& a, const
         {
             const Qt<R> Ur = rdir == 1 ? Ub[1] : Ub[2];
             const R yr = rdir == 1 ? yb[1] : yb[2];
-            R cr, sr, fb[4], fn[4], tx[4], d[4];
+            R cr, sr, rawb[4], rawn[4], fb[4], fn[4], tx[4], d[4];
             sincos_fast(yr, &sr, &cr);
-            unpack(pl.phi[0][rb], fb);
-            unpack(pl.phi[0][rbr], fn);
+            unpack(pl.phi[0][rb], rawb);
+            unpack(pl.phi[0][rbr], rawn);
+            add_ref(a.ref[0], rawb, fb);
+            add_ref(a.ref[0], rawn, fn);
             transport(Ur, cr, sr, false, fb, tx);
 #pragma unroll
             for (int m = 0; m < 4; m++) d[m] = fn[m] - tx[m];
@@ -1087,7 +1135,7 @@ __device__ __forceinline__ void mom_block_shared(const StepArgs<float>& a, const
         M[1][2] -= a.inv_sp[2] * p12b.y;
         M[1][3] -= a.inv_sp[2] * p12b.z;
     }
-    kick_drift<R, 1, CMP, true>(a, s, U, yv, phi, F, J, M, &own, frozen);
+    kick_drift<R, 1, CMP, true>(a, s, U, yv, praw, F, J, M, &own, frozen);
 }
 
 // MINW: minimum waves per SIMD the register allocation must allow (0: the
@@ -1184,8 +1232,9 @@ struct ObsArgs {
     const V4* q[3];
     const V4* q_old[3];     // links one step earlier (Gauss only)
     const V4* E[3];
-    const unsigned* lo_s[2];         // residuals of compensated storage (null otherwise)
-    const unsigned short* lo_E[3];
+    const uint2* lo;                 // residuals of compensated storage (null otherwise)
+    const unsigned short* lo_P2;
+    double ref[2][4];                // reference doublets: phi = stored + ref (exact in FP64)
     int nx, ny, nz;
 };
 
@@ -1203,30 +1252,34 @@ __device__ __forceinline__ void load_d(const V4& v, double* o) {
 }
 
 // Full-precision values of the compensated FP32 storage (hi + 4-bit residual;
-// exact in FP64), or the stored values when there are no residuals.  Nibbles
-// m0 .. m0 + 3 of w belong to the four values.
+// exact in FP64), or the stored values when there are no residuals.  Nibble m
+// of the 16-bit group w belongs to value m.
 template <class V4>
-__device__ __forceinline__ void obs_add_res(const V4& hi, unsigned w, int m0, double* o) {
+__device__ __forceinline__ void obs_add_res(const V4& hi, unsigned w, double* o) {
     const float h[4] = {(float)hi.x, (float)hi.y, (float)hi.z, (float)hi.w};
-    for (int m = 0; m < 4; m++) o[m] += ldexp((double)res_nib(w, m0 + m), res_exp(h[m]) - 153);
+    for (int m = 0; m < 4; m++) o[m] += ldexp((double)res_nib(w, m), res_exp(h[m]) - 153);
+}
+// The doublet k at site s of buffer `f` (current or previous): stored + ref.
+template <class R, class V4>
+__device__ __forceinline__ void obs_phi_of(const ObsArgs<R>& a, const V4* f, int k, long long s, double* o) {
+    load_d(f[s], o);
+    for (int m = 0; m < 4; m++) o[m] += a.ref[k][m];
 }
 template <class R>
 __device__ __forceinline__ void obs_phi(const ObsArgs<R>& a, int k, long long s, double* o) {
-    const auto hi = a.phi[k][s];
-    load_d(hi, o);
-    if (a.lo_s[k]) obs_add_res(hi, a.lo_s[k][s], 0, o);
+    obs_phi_of(a, a.phi[k], k, s, o);
 }
 template <class R>
 __device__ __forceinline__ void obs_P(const ObsArgs<R>& a, int k, long long s, double* o) {
     const auto hi = a.P[k][s];
     load_d(hi, o);
-    if (a.lo_s[k]) obs_add_res(hi, a.lo_s[k][s], 4, o);
+    if (a.lo) obs_add_res(hi, k == 0 ? res_group(a.lo[s], 0) : (unsigned)a.lo_P2[s], o);
 }
 template <class R>
 __device__ __forceinline__ void obs_E(const ObsArgs<R>& a, int i, long long s, double* o) {
     const auto hi = a.E[i][s];
     load_d(hi, o);
-    if (a.lo_E[i]) obs_add_res(hi, a.lo_E[i][s], 0, o);
+    if (a.lo) obs_add_res(hi, res_group(a.lo[s], 1 + i), o);
 }
 
 // 1 - sqrt(1 - x) without cancellation.
@@ -1326,13 +1379,7 @@ __global__ void __launch_bounds__(256) mom_gauss(ObsArgs<R> a, Coef c, double* _
         for (int k = 0; k < ND; k++) {
             double t[4], n[4];
             obs_P(a, k, s, n);
-            if (a.lo_s[k]) {
-                // phi(t) = phi(t+dt) - P: exact to the residuals' rounding.
-                obs_phi(a, k, s, t);
-                for (int m = 0; m < 4; m++) t[m] -= n[m];
-            } else {
-                load_d(a.phi_old[k][s], t);
-            }
+            obs_phi_of(a, a.phi_old[k], k, s, t);   // phi(t) as stored in the previous buffer
             bilinears_add(t, n, rho);
         }
         double div[4] = {0, 0, 0, 0}, mag[4] = {0, 0, 0, 0};
@@ -1443,11 +1490,8 @@ inline size_t carve(MomFields<R>* f, char* base, long long N, bool cmp) {
     for (int b = 0; b < 2; b++)
         for (int i = 0; i < 3; i++) { char* p = take(N, sizeof(V4)); if (f) f->q[b][i] = reinterpret_cast<V4*>(p); }
     for (int i = 0; i < 3; i++) { char* p = take(N, sizeof(V4)); if (f) f->E[i] = reinterpret_cast<V4*>(p); }
-    for (int k = 0; k < 2; k++) { char* p = cmp ? take(N, sizeof(unsigned)) : nullptr; if (f) f->lo_s[k] = reinterpret_cast<unsigned*>(p); }
-    for (int i = 0; i < 3; i++) {
-        char* p = cmp ? take(N, sizeof(unsigned short)) : nullptr;
-        if (f) f->lo_E[i] = reinterpret_cast<unsigned short*>(p);
-    }
+    { char* p = cmp ? take(N, sizeof(uint2)) : nullptr; if (f) f->lo = reinterpret_cast<uint2*>(p); }
+    { char* p = cmp ? take(N, sizeof(unsigned short)) : nullptr; if (f) f->lo_P2 = reinterpret_cast<unsigned short*>(p); }
     return off;
 }
 
@@ -1549,8 +1593,10 @@ inline double res_val_host(unsigned w, int m, float hi) {
     return std::ldexp(static_cast<double>(q), res_exp_host(hi) - 153);
 }
 
-// Two-slot (reference layout) -> momentum form, on the host in FP64.  With
-// compensated storage each value is split into its FP32 hi and 4-bit residual.
+// Two-slot (reference layout) -> momentum form, on the host in FP64.  The FP32
+// forms store each doublet as its difference from a reference doublet (the
+// mean of the current slot, rounded to FP32); with compensated storage P and E
+// are split into their FP32 hi and 4-bit residuals.
 template <class R, class Src>
 int momentum_import_t(MomentumState& m, MomFields<R>& f, const Coef& c, const Src* S, const Src* V,
                       unsigned time_iter, hipStream_t st, std::string& err) {
@@ -1558,21 +1604,33 @@ int momentum_import_t(MomentumState& m, MomFields<R>& f, const Coef& c, const Sr
     const int Ns = c.ns, nd = c.nd;
     const unsigned now = (time_iter + 1) % 2, past = !now;
     std::vector<R> phi(4 * N * 2, R(0)), P(4 * N * 2, R(0)), y(3 * N), q(12 * N), E(12 * N);
-    const bool cmp = f.lo_s[0] != nullptr;
-    std::vector<unsigned> lo_s(cmp ? 2 * N : 0, 0u);
-    std::vector<unsigned short> lo_E(cmp ? 3 * N : 0, 0);
+    const bool cmp = f.lo != nullptr;
+    std::vector<uint2> lo(cmp ? N : 0, make_uint2(0u, 0u));
+    std::vector<unsigned short> lo_P2(cmp ? N : 0, 0);
+    for (int k = 0; k < 2; k++)
+        for (int m4 = 0; m4 < 4; m4++) m.ref[k][m4] = 0.0;
+    if (sizeof(R) == 4) {
+        for (int k = 0; k < nd; k++)
+            for (int m4 = 0; m4 < 4; m4++) {
+                double sum = 0.0;
+                for (long long s = 0; s < N; s++) sum += (double)S[(now * N + s) * Ns + 4 * k + m4];
+                m.ref[k][m4] = (double)(float)(sum / (double)N);
+            }
+    }
     for (long long s = 0; s < N; s++) {
         for (int k = 0; k < nd; k++) {
-            unsigned lf = 0, lp = 0;
+            unsigned lp = 0;
             for (int m4 = 0; m4 < 4; m4++) {
                 const double a = (double)S[(now * N + s) * Ns + 4 * k + m4];
                 const double b = (double)S[(past * N + s) * Ns + 4 * k + m4];
-                phi[(k * N + s) * 4 + m4] = (R)a;
+                phi[(k * N + s) * 4 + m4] = (R)(a - m.ref[k][m4]);
                 P[(k * N + s) * 4 + m4] = (R)(a - b);
-                lf |= res_q_host(a - (double)(float)a, (float)a) << (4 * m4);
                 lp |= res_q_host((a - b) - (double)(float)(a - b), (float)(a - b)) << (4 * m4);
             }
-            if (cmp) lo_s[k * N + s] = lf | (lp << 16);
+            if (cmp) {
+                if (k == 0) lo[s].x = lp;
+                else lo_P2[s] = static_cast<unsigned short>(lp);
+            }
         }
         for (int i = 0; i < 3; i++) {
             const Src* ln = V + (now * N + s) * 15 + 5 * i;
@@ -1589,11 +1647,16 @@ int momentum_import_t(MomentumState& m, MomFields<R>& f, const Coef& c, const Sr
                 E[(i * N + s) * 4 + k] = (R)ev[k];
                 le |= res_q_host(ev[k] - (double)(float)ev[k], (float)ev[k]) << (4 * k);
             }
-            if (cmp) lo_E[i * N + s] = static_cast<unsigned short>(le);
+            if (cmp) {
+                if (i == 0) lo[s].x |= le << 16;
+                else if (i == 1) lo[s].y = le;
+                else lo[s].y |= le << 16;
+            }
         }
     }
     m.cur = 0;
     m.prev_valid = 0;
+    m.epoch++;
     auto cp = [&](void* dst, const void* src, size_t bytes) {
         return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st) == hipSuccess;
     };
@@ -1606,10 +1669,11 @@ int momentum_import_t(MomentumState& m, MomFields<R>& f, const Coef& c, const Sr
         ok &= cp(f.y[0][i], &y[i * N], N * sizeof(R));
         ok &= cp(f.q[0][i], &q[i * N * 4], N * 4 * sizeof(R));
         ok &= cp(f.E[i], &E[i * N * 4], N * 4 * sizeof(R));
-        if (cmp) ok &= cp(f.lo_E[i], &lo_E[i * N], N * sizeof(unsigned short));
     }
-    if (cmp)
-        for (int k = 0; k < nd; k++) ok &= cp(f.lo_s[k], &lo_s[k * N], N * sizeof(unsigned));
+    if (cmp) {
+        ok &= cp(f.lo, lo.data(), N * sizeof(uint2));
+        ok &= cp(f.lo_P2, lo_P2.data(), N * sizeof(unsigned short));
+    }
     ok &= hipStreamSynchronize(st) == hipSuccess;
     if (!ok) { err = "momentum import: copy failed"; return GFT_EHIP; }
     return GFT_OK;
@@ -1624,9 +1688,9 @@ int momentum_export_t(MomentumState& m, MomFields<R>& f, const Coef& c, Dst* S, 
     const int Ns = c.ns, nd = c.nd;
     const unsigned now = (time_iter + 1) % 2, past = !now;
     std::vector<R> phi(4 * N * 2), P(4 * N * 2), y(3 * N), q(12 * N), E(12 * N);
-    const bool cmp = f.lo_s[0] != nullptr;
-    std::vector<unsigned> lo_s(cmp ? 2 * N : 0);
-    std::vector<unsigned short> lo_E(cmp ? 3 * N : 0);
+    const bool cmp = f.lo != nullptr;
+    std::vector<uint2> lo(cmp ? N : 0);
+    std::vector<unsigned short> lo_P2(cmp ? N : 0);
     auto cp = [&](void* dst, const void* src, size_t bytes) {
         return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st) == hipSuccess;
     };
@@ -1639,22 +1703,23 @@ int momentum_export_t(MomentumState& m, MomFields<R>& f, const Coef& c, Dst* S, 
         ok &= cp(&y[i * N], f.y[m.cur][i], N * sizeof(R));
         ok &= cp(&q[i * N * 4], f.q[m.cur][i], N * 4 * sizeof(R));
         ok &= cp(&E[i * N * 4], f.E[i], N * 4 * sizeof(R));
-        if (cmp) ok &= cp(&lo_E[i * N], f.lo_E[i], N * sizeof(unsigned short));
     }
-    if (cmp)
-        for (int k = 0; k < nd; k++) ok &= cp(&lo_s[k * N], f.lo_s[k], N * sizeof(unsigned));
+    if (cmp) {
+        ok &= cp(lo.data(), f.lo, N * sizeof(uint2));
+        ok &= cp(lo_P2.data(), f.lo_P2, N * sizeof(unsigned short));
+    }
     ok &= hipStreamSynchronize(st) == hipSuccess;
     if (!ok) { err = "momentum export: copy failed"; return GFT_EHIP; }
+    auto group = [&](long long s, int g) -> unsigned {
+        const unsigned w = g < 2 ? lo[s].x : lo[s].y;
+        return (w >> (16 * (g & 1))) & 0xffffu;
+    };
     for (long long s = 0; s < N; s++) {
         for (int k = 0; k < nd; k++)
             for (int m4 = 0; m4 < 4; m4++) {
-                double a = (double)phi[(k * N + s) * 4 + m4];
+                const double a = (double)phi[(k * N + s) * 4 + m4] + m.ref[k][m4];
                 double pp = (double)P[(k * N + s) * 4 + m4];
-                if (cmp) {
-                    const unsigned L = lo_s[k * N + s];
-                    a += res_val_host(L, m4, (float)phi[(k * N + s) * 4 + m4]);
-                    pp += res_val_host(L, 4 + m4, (float)P[(k * N + s) * 4 + m4]);
-                }
+                if (cmp) pp += res_val_host(k == 0 ? group(s, 0) : lo_P2[s], m4, (float)P[(k * N + s) * 4 + m4]);
                 S[(now * N + s) * Ns + 4 * k + m4] = (Dst)a;
                 S[(past * N + s) * Ns + 4 * k + m4] = (Dst)(a - pp);
             }
@@ -1665,7 +1730,7 @@ int momentum_export_t(MomentumState& m, MomFields<R>& f, const Coef& c, Dst* S, 
             double e[4] = {(double)E[(i * N + s) * 4], (double)E[(i * N + s) * 4 + 1],
                            (double)E[(i * N + s) * 4 + 2], (double)E[(i * N + s) * 4 + 3]};
             if (cmp)
-                for (int k = 0; k < 4; k++) e[k] += res_val_host(lo_E[i * N + s], k, (float)E[(i * N + s) * 4 + k]);
+                for (int k = 0; k < 4; k++) e[k] += res_val_host(group(s, 1 + i), k, (float)E[(i * N + s) * 4 + k]);
             const double un[4] = {(double)q[(i * N + s) * 4], (double)q[(i * N + s) * 4 + 1],
                                   (double)q[(i * N + s) * 4 + 2], (double)q[(i * N + s) * 4 + 3]};
             const double ud[4] = {std::sqrt(std::max(0.0, 1.0 - e[1] * e[1] - e[2] * e[2] - e[3] * e[3])), -e[1], -e[2],
@@ -1712,11 +1777,16 @@ StepArgs<R> make_step_args(MomentumState& m, MomFields<R>& f, const Coef& c, dou
         a.q[i] = f.q[cur][i];
         a.q_out[i] = f.q[nxt][i];
         a.E[i] = f.E[i];
-        a.lo_E[i] = f.lo_E[i];
         a.inv_sp[i] = (R)c.inv_sp[i];
     }
-    a.lo_s[0] = f.lo_s[0];
-    a.lo_s[1] = f.lo_s[1];
+    a.lo = f.lo;
+    a.lo_P2 = f.lo_P2;
+    for (int k = 0; k < 2; k++) {
+        a.ref[k].x = (R)m.ref[k][0];
+        a.ref[k].y = (R)m.ref[k][1];
+        a.ref[k].z = (R)m.ref[k][2];
+        a.ref[k].w = (R)m.ref[k][3];
+    }
     a.nx = c.nx; a.ny = c.ny; a.nz = c.nz;
     a.bz = m.bz; a.by = m.by;
     a.nbz = (c.nz + m.bz - 1) / m.bz;
@@ -1809,10 +1879,11 @@ ObsArgs<R> make_obs_args(const MomentumState& m, const MomFields<R>& f, const Co
         a.q[i] = f.q[cur][i];
         a.q_old[i] = f.q[old][i];
         a.E[i] = f.E[i];
-        a.lo_E[i] = f.lo_E[i];
     }
-    a.lo_s[0] = f.lo_s[0];
-    a.lo_s[1] = f.lo_s[1];
+    a.lo = f.lo;
+    a.lo_P2 = f.lo_P2;
+    for (int k = 0; k < 2; k++)
+        for (int m4 = 0; m4 < 4; m4++) a.ref[k][m4] = m.ref[k][m4];
     a.nx = c.nx; a.ny = c.ny; a.nz = c.nz;
     return a;
 }

@@ -46,10 +46,12 @@ extern "C" {
 #define GFT_FORM_TWO_SLOT_F64 0  /* reference two-slot layout, FP64 storage (FP64 GPU oracle)            */
 #define GFT_FORM_TWO_SLOT_F32 1  /* reference two-slot layout, FP32 storage (the reference's own storage) */
 #define GFT_FORM_MOMENTUM_F32 2  /* structure-of-arrays kick/drift form, FP32 bulk (the fast path): FP32   */
-                                 /* arithmetic and neighbour traffic, phi/P/E kept as FP32 + 4-bit       */
-                                 /* residual in units of the FP32 value's ulp (about 28 bits)            */
+                                 /* arithmetic and neighbour traffic; phi stored in FP32 as its          */
+                                 /* difference from a constant reference doublet (the mean at import),   */
+                                 /* P and E as FP32 + 4-bit residual in units of the FP32 value's ulp    */
+                                 /* (about 28 bits)                                                      */
 #define GFT_FORM_MOMENTUM_F64 3  /* structure-of-arrays kick/drift form, FP64 (plain FP64 GPU evolver)    */
-#define GFT_FORM_MOMENTUM_F32_PLAIN 4 /* as MOMENTUM_F32 with plain FP32 storage (no residuals): 7% fewer  */
+#define GFT_FORM_MOMENTUM_F32_PLAIN 4 /* as MOMENTUM_F32 with plain FP32 P and E (no residuals): 5% fewer  */
                                  /* bytes, Gauss-law drift of FP32 rounding (~1e-6 after 1e4 steps)       */
 
 /* ---- error codes ---- */
