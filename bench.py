@@ -418,7 +418,7 @@ def main():
                         f"(P(k)~1/k, k<=pi/(2dx), rms 0.1v), links identity, E=0; dx=0.5 dt=0.05 g=0.65 "
                         f"g'=0.35 lambda=0.13 v=1",
             "lattice": list(n),
-            "launch": r32.get("launch", [64, 4, 2, 3, 2]),
+            "launch": r32.get("launch", [64, 4, 2, 4, 2]),
         },
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,

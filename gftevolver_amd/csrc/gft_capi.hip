@@ -921,10 +921,10 @@ static int ens_prepare(gft_ensemble* ens) {
     EK(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
     ens->lds = lds;
     ens->stage = stage;
-    // Three 256-thread blocks per CU (the kernel's register occupancy) or as
-    // many as the CU's LDS holds, a multiple of 8.
-    long long per_cu = 3;
-    if (ens->lds) per_cu = std::max<long long>(1, std::min<long long>(3, (long long)lds_cu / (long long)ens->lds));
+    // Four 256-thread blocks per CU (the kernel's register occupancy, 4 waves per
+    // SIMD) or as many as the CU's LDS holds, a multiple of 8.
+    long long per_cu = 4;
+    if (ens->lds) per_cu = std::max<long long>(1, std::min<long long>(4, (long long)lds_cu / (long long)ens->lds));
     long long grid = std::min<long long>(ens->nwork, per_cu * cus);
     grid = std::max<long long>(8, (grid / 8) * 8);
     ens->grid = (int)grid;
@@ -1009,9 +1009,9 @@ int gft_ensemble_evolve(gft_ensemble* ens, unsigned nsteps, int use_graph) {
     if (rc) return rc;
     const dim3 grid(ens->grid), block(256);
     auto launch = [&](int p) {
-        if (ens->stage == 2) ens_step<3, 2><<<grid, block, ens->lds, ens->stream>>>(ens->args[p], ens->work, ens->nwork, ens->blocks_per_xcd);
-        else if (ens->stage == 1) ens_step<3, 1><<<grid, block, ens->lds, ens->stream>>>(ens->args[p], ens->work, ens->nwork, ens->blocks_per_xcd);
-        else ens_step<3, 0><<<grid, block, 0, ens->stream>>>(ens->args[p], ens->work, ens->nwork, ens->blocks_per_xcd);
+        if (ens->stage == 2) ens_step<4, 2><<<grid, block, ens->lds, ens->stream>>>(ens->args[p], ens->work, ens->nwork, ens->blocks_per_xcd);
+        else if (ens->stage == 1) ens_step<4, 1><<<grid, block, ens->lds, ens->stream>>>(ens->args[p], ens->work, ens->nwork, ens->blocks_per_xcd);
+        else ens_step<4, 0><<<grid, block, 0, ens->stream>>>(ens->args[p], ens->work, ens->nwork, ens->blocks_per_xcd);
     };
     if (use_graph && nsteps >= 2 && ens->graph_parity != parity) {
         ens_drop_graph(ens);

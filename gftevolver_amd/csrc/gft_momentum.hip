@@ -80,7 +80,7 @@ struct MomentumState {
     MomFields<double> fd{};
     int bz = 64, by = 4;     // tile shape of the step kernel
     int order = 2;           // block order (see decode_block)
-    int min_waves = 3;       // occupancy target of the FP32 SM step kernel (0 = unconstrained, 3, 4)
+    int min_waves = 4;       // occupancy target of the FP32 SM step kernel (0 = unconstrained, 3, 4)
     int stage = 2;           // FP32 step kernels: 0 global loads, 1 LDS plane tile, 2 + shared plaquettes
     int4* tiles = nullptr;   // device tile table of the current launch shape
     int tiles_key[3] = {0, 0, -1};
@@ -88,23 +88,43 @@ struct MomentumState {
 
 // ---- float special functions on the small ranges the evolution produces ----
 // Taylor polynomials (coefficients 1/n!) after a quadrant reduction with a
-// three-part pi/2; |error| < 1e-9 absolute for |x| <= 16.
-__device__ __forceinline__ void sincos_fast(float x, float* s, float* c) {
-    const float q = rintf(x * 0.636619772367581343f);
-    float r = fmaf(-q, 1.57079637050628662109375f, x);
-    r = fmaf(-q, -4.371138828673793e-8f, r);
-    r = fmaf(-q, -1.7151245100058819e-15f, r);
-    const float r2 = r * r;
+// three-part pi/2; |error| < 1e-9 absolute for |x| <= 16.  Arguments below
+// 0.78 < pi/4 -- every plaquette angle and the phases of smooth fields -- have
+// quadrant 0, where the reduction returns x itself: they skip it (a branch per
+// lane, taken by whole waves in practice) and get bitwise the same result.
+__device__ __forceinline__ float sin_poly(float r, float r2) {
     float sp = fmaf(r2, 2.75573192239858907e-6f, -1.98412698412698413e-4f);
     sp = fmaf(r2, sp, 8.33333333333333333e-3f);
     sp = fmaf(r2, sp, -1.66666666666666667e-1f);
-    const float sr = fmaf(r * r2, sp, r);
+    return fmaf(r * r2, sp, r);
+}
+__device__ __forceinline__ float cos_poly(float r2) {
     float cp = fmaf(r2, -2.75573192239858907e-7f, 2.48015873015873016e-5f);
     cp = fmaf(r2, cp, -1.38888888888888889e-3f);
     cp = fmaf(r2, cp, 4.16666666666666667e-2f);
     cp = fmaf(r2, cp, -0.5f);
-    const float cr = fmaf(r2, cp, 1.0f);
-    const int k = static_cast<int>(q) & 3;
+    return fmaf(r2, cp, 1.0f);
+}
+// Quadrant reduction: r = x - q pi/2 (three-part pi/2), k = q mod 4.
+__device__ __forceinline__ float quadrant_reduce(float x, int* k) {
+    const float q = rintf(x * 0.636619772367581343f);
+    float r = fmaf(-q, 1.57079637050628662109375f, x);
+    r = fmaf(-q, -4.371138828673793e-8f, r);
+    r = fmaf(-q, -1.7151245100058819e-15f, r);
+    *k = static_cast<int>(q) & 3;
+    return r;
+}
+__device__ __forceinline__ void sincos_fast(float x, float* s, float* c) {
+    if (fabsf(x) < 0.78f) {
+        const float r2 = x * x;
+        *s = sin_poly(x, r2);
+        *c = cos_poly(r2);
+        return;
+    }
+    int k;
+    const float r = quadrant_reduce(x, &k);
+    const float r2 = r * r;
+    const float sr = sin_poly(r, r2), cr = cos_poly(r2);
     const float ss = (k & 1) ? cr : sr;
     const float cc = (k & 1) ? sr : cr;
     *s = (k & 2) ? -ss : ss;
@@ -112,9 +132,12 @@ __device__ __forceinline__ void sincos_fast(float x, float* s, float* c) {
 }
 
 __device__ __forceinline__ float sin_fast(float x) {
-    float s, c;
-    sincos_fast(x, &s, &c);
-    return s;
+    if (fabsf(x) < 0.78f) return sin_poly(x, x * x);
+    int k;
+    const float r = quadrant_reduce(x, &k);
+    const float r2 = r * r;
+    const float ss = (k & 1) ? cos_poly(r2) : sin_poly(r, r2);
+    return (k & 2) ? -ss : ss;
 }
 
 // asin by its Maclaurin series for |x| <= 1/8 (terms to x^11, error < 1e-13
@@ -262,10 +285,11 @@ __device__ __forceinline__ void qrot(const Qt<R>& q, const R* v, R* o) {
 // per value -- below the random walk of the FP32 force arithmetic in the Gauss
 // law -- while the kernel's arithmetic and neighbour traffic stay FP32.
 __device__ __forceinline__ int res_exp(float hi) { return static_cast<int>((__float_as_uint(hi) >> 23) & 0xffu); }
-// q of residual r next to hi (round to nearest even; the clamp is a guard).
+// q of residual r next to hi (round to nearest even).  r is the exact error of
+// the FP32 sum hi, so |r| <= ulp(hi) / 2 and |q| <= 4: no clamp is needed (and
+// res_pack4 masks every q to its own nibble).
 __device__ __forceinline__ int res_q(float r, float hi) {
-    const int q = static_cast<int>(rintf(ldexpf(r, 153 - res_exp(hi))));
-    return min(max(q, -8), 7);
+    return static_cast<int>(rintf(ldexpf(r, 153 - res_exp(hi))));
 }
 __device__ __forceinline__ float res_from_q(int q, float hi) { return ldexpf(static_cast<float>(q), res_exp(hi) - 153); }
 // Four q in 16 bits (nibble m = value m) and back; nibbles 4..7 of a 32-bit
@@ -528,16 +552,28 @@ __device__ __forceinline__ void add_ref(const V4& ref, const R* raw, R* o) {
     o[3] = raw[3] + ref.w;
 }
 
+// The gauge source enters the link force as M_i + g_a^2 J_i.  Every step
+// kernel adds it at the same point of the magnetic sum -- after the plaquettes
+// based at x and at x - e_0, before those based at x - e_1 and x - e_2 -- so the
+// currents need no registers past that point and all staging modes round alike.
+template <class R>
+__device__ __forceinline__ void fold_current(const StepArgs<R>& a, const R (*J)[4], R (*M)[4]) {
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int m = 0; m < 4; m++) M[i][m] = fma(a.gsq[m], J[i][m], M[i][m]);
+}
 // PRE: own state already loaded into o (else loaded here, next to its use).
 // U1: no damping (keep = 1), with the kick's keep products dropped (exact).
 // praw: the site's stored doublet(s) (phi - ref), which the drift advances.
+// M: the link forces with the gauge source already folded in (fold_current).
 template <class R, int ND, bool CMP, bool PRE = false, bool U1 = false>
 // frozen: a site of a fixed face's shell -- every lane still stores, but a
 // frozen lane stores the values it read, so every store instruction writes
 // whole cache lines (a divergent copy branch would write partial lines twice).
 __device__ __forceinline__ void kick_drift_t(const StepArgs<R>& a, unsigned s, const Qt<R>* U, const R* yv,
-                                             const R* praw, const R* F, const R (*J)[4], const R (*M)[4],
-                                             const OwnState<R, ND>* o, bool frozen) {
+                                             const R* praw, const R* F, const R (*M)[4], const OwnState<R, ND>* o,
+                                             bool frozen) {
     using V4 = typename V4T<R>::type;
     uint2 lw = make_uint2(0u, 0u);
     unsigned lp2 = 0u;
@@ -606,8 +642,8 @@ __device__ __forceinline__ void kick_drift_t(const StepArgs<R>& a, unsigned s, c
             float eh[4], er[4];
 #pragma unroll
             for (int m = 0; m < 4; m++)
-                if constexpr (U1) eh[m] = cmp_update1(e[m], res_val(L, m, e[m]), a.dt2 * (M[i][m] + a.gsq[m] * J[i][m]), &er[m]);
-                else eh[m] = cmp_update(e[m], res_val(L, m, e[m]), a.keep, a.dt2 * (M[i][m] + a.gsq[m] * J[i][m]), &er[m]);
+                if constexpr (U1) eh[m] = cmp_update1(e[m], res_val(L, m, e[m]), a.dt2 * M[i][m], &er[m]);
+                else eh[m] = cmp_update(e[m], res_val(L, m, e[m]), a.keep, a.dt2 * M[i][m], &er[m]);
             en = make_float4(eh[0], eh[1], eh[2], eh[3]);
             const unsigned Ln =
                 frozen ? L : res_pack4(res_q(er[0], eh[0]), res_q(er[1], eh[1]), res_q(er[2], eh[2]), res_q(er[3], eh[3]));
@@ -615,10 +651,10 @@ __device__ __forceinline__ void kick_drift_t(const StepArgs<R>& a, unsigned s, c
             else if (i == 1) nwy = Ln;
             else nwy |= Ln << 16;
         } else {
-            en.x = fma(a.dt2, fma(a.gsq[0], J[i][0], M[i][0]), U1 ? e[0] : a.keep * e[0]);
-            en.y = fma(a.dt2, fma(a.gsq[1], J[i][1], M[i][1]), U1 ? e[1] : a.keep * e[1]);
-            en.z = fma(a.dt2, fma(a.gsq[2], J[i][2], M[i][2]), U1 ? e[2] : a.keep * e[2]);
-            en.w = fma(a.dt2, fma(a.gsq[3], J[i][3], M[i][3]), U1 ? e[3] : a.keep * e[3]);
+            en.x = fma(a.dt2, M[i][0], U1 ? e[0] : a.keep * e[0]);
+            en.y = fma(a.dt2, M[i][1], U1 ? e[1] : a.keep * e[1]);
+            en.z = fma(a.dt2, M[i][2], U1 ? e[2] : a.keep * e[2]);
+            en.w = fma(a.dt2, M[i][3], U1 ? e[3] : a.keep * e[3]);
         }
         const R e2 = fma(en.w, en.w, fma(en.z, en.z, en.y * en.y));
         if (!frozen && (!(e2 < R(1)) || !(fabs(en.x) < R(1)))) bad = 1;
@@ -652,10 +688,10 @@ __device__ __forceinline__ void kick_drift_t(const StepArgs<R>& a, unsigned s, c
 // The kick with a uniform branch on damping: keep = 1 takes the shorter path.
 template <class R, int ND, bool CMP, bool PRE = false>
 __device__ __forceinline__ void kick_drift(const StepArgs<R>& a, unsigned s, const Qt<R>* U, const R* yv,
-                                           const R* praw, const R* F, const R (*J)[4], const R (*M)[4],
+                                           const R* praw, const R* F, const R (*M)[4],
                                            const OwnState<R, ND>* o = nullptr, bool frozen = false) {
-    if (a.keep == R(1)) kick_drift_t<R, ND, CMP, PRE, true>(a, s, U, yv, praw, F, J, M, o, frozen);
-    else kick_drift_t<R, ND, CMP, PRE, false>(a, s, U, yv, praw, F, J, M, o, frozen);
+    if (a.keep == R(1)) kick_drift_t<R, ND, CMP, PRE, true>(a, s, U, yv, praw, F, M, o, frozen);
+    else kick_drift_t<R, ND, CMP, PRE, false>(a, s, U, yv, praw, F, M, o, frozen);
 }
 
 // One site-update at (x, y, z): forces, kick and drift (shared by the
@@ -846,9 +882,10 @@ __device__ __forceinline__ void mom_site(const StepArgs<R>& a, const Coef& c, in
                 for (int m = 0; m < 3; m++) M[i][1 + m] += a.inv_sp[j] * w[m];
             }
         }
+        if (j == 0) fold_current(a, J, M);
     }
 
-    kick_drift<R, ND, CMP>(a, s, U, yv, praw, F, J, M, nullptr, frozen);
+    kick_drift<R, ND, CMP>(a, s, U, yv, praw, F, M, nullptr, frozen);
 }
 
 // ---- shared plaquettes and covariant differences (FP32 SM, staged) ----------
@@ -909,7 +946,10 @@ __device__ __forceinline__ float4 pack_wu(const Plaq<float>& p) { return make_fl
 // Whole block, after plane_fill: one site-update per valid thread (tile
 // coordinates ty, tz; site (x, y0 + ty, z0 + tz)) with the in-plane plaquettes
 // and differences shared.  Needs bz + by <= 256 (one ring item per thread).
-template <bool CMP>
+// EARLY: the site's own P, E and residuals are loaded at the start of phase A,
+// where their latency overlaps the phase (at the cost of registers held across
+// the exchange barriers); otherwise after the exchange, next to their use.
+template <bool CMP, bool EARLY = true>
 __device__ __forceinline__ void mom_block_shared(const StepArgs<float>& a, const Coef& c, int x, int y0, int z0, int ty,
                                                  int tz, bool valid, const PlaneLds& pl, char* lds) {
     using R = float;
@@ -926,7 +966,7 @@ __device__ __forceinline__ void mom_block_shared(const StepArgs<float>& a, const
     R cy[3], sy_[3], yv[3], phi[4], praw[4], F[4], J[3][4], M[3][4], dpv[3][4], dsh[3][4];
     Plaq<R> P[3];   // at x: (0,1), (0,2), (1,2)
     OwnState<R, 1> own;
-    if (valid) load_own<R, 1, CMP>(a, s, own);   // issued early: its latency overlaps phase A
+    if (EARLY && valid) load_own<R, 1, CMP>(a, s, own);   // issued early: its latency overlaps phase A
     // Back to the default priority (mom_step raised it for the loads), in every
     // wave: a wave with no valid lane must not keep the raised priority.
     __builtin_amdgcn_s_setprio(0);
@@ -1018,6 +1058,7 @@ __device__ __forceinline__ void mom_block_shared(const StepArgs<float>& a, const
                 for (int m = 0; m < 3; m++) M[i][1 + m] += a.inv_sp[0] * pq.wu[m];
             }
         }
+        fold_current(a, J, M);
     }
 
     // ---- phase A: ring item (y-ring site for k < bz, z-ring site after) ----
@@ -1098,6 +1139,7 @@ __device__ __forceinline__ void mom_block_shared(const StepArgs<float>& a, const
     }
     __syncthreads();
     if (!valid) return;
+    if constexpr (!EARLY) load_own<R, 1, CMP>(a, s, own);
     // Shell of a fixed face: its plaquettes and differences served the
     // neighbours above; the site is computed like any other, then carried over
     // unchanged by the kick's stores.
@@ -1135,7 +1177,7 @@ __device__ __forceinline__ void mom_block_shared(const StepArgs<float>& a, const
         M[1][2] -= a.inv_sp[2] * p12b.y;
         M[1][3] -= a.inv_sp[2] * p12b.z;
     }
-    kick_drift<R, 1, CMP, true>(a, s, U, yv, praw, F, J, M, &own, frozen);
+    kick_drift<R, 1, CMP, true>(a, s, U, yv, praw, F, M, &own, frozen);
 }
 
 // MINW: minimum waves per SIMD the register allocation must allow (0: the
@@ -1165,7 +1207,9 @@ __global__ void __launch_bounds__(256, MINW > 0 ? MINW : 1) mom_step(StepArgs<R>
         plane_fill<ND>(a, pl, t.y, t.z * a.by, t.w * a.bz, a.bz, a.by);
         pl.r0 = (ty + 1) * pl.w + tz + 1;
         if constexpr (PL == 2) {
-            mom_block_shared<CMP>(a, c, t.y, t.z * a.by, t.w * a.bz, ty, tz, valid, pl, lds);
+            // At 4 waves per SIMD the registers do not hold the own state across the
+            // exchange barriers: it is loaded after them.
+            mom_block_shared<CMP, MINW != 4>(a, c, t.y, t.z * a.by, t.w * a.bz, ty, tz, valid, pl, lds);
         } else {
             if (!valid) return;
             mom_site<R, ND, CMP, true>(a, c, t.y, y, z, pl);
@@ -1182,7 +1226,9 @@ struct EnsMember {
     Coef c;
 };
 
-// One step of every member of an ensemble in one persistent launch.  The work
+// One step of every member of an ensemble in one persistent launch (MINW waves
+// per SIMD: at 4 the own state is loaded after the exchange barriers, as in
+// mom_step).  The work
 // list holds (member, x, y-tile, z-tile) per 256-site tile, members one after
 // another; XCD k (hardware blocks b with b % 8 == k) takes the k-th eighth of
 // the list and its blocks stride through it, so each XCD works on a compact
@@ -1212,7 +1258,8 @@ __global__ void __launch_bounds__(256, MINW) ens_step(const EnsMember* __restric
             PlaneLds pl = plane_layout<1>(lds, a.bz, a.by);
             plane_fill<1>(a, pl, t.y, t.z * a.by, t.w * a.bz, a.bz, a.by);
             pl.r0 = (ty + 1) * pl.w + tz + 1;
-            if constexpr (PL == 2) mom_block_shared<true>(a, c, t.y, t.z * a.by, t.w * a.bz, ty, tz, valid, pl, lds);
+            if constexpr (PL == 2)
+                mom_block_shared<true, MINW != 4>(a, c, t.y, t.z * a.by, t.w * a.bz, ty, tz, valid, pl, lds);
             else if (valid) mom_site<float, 1, true, true>(a, c, t.y, y, z, pl);
         } else {
             if (valid) mom_site<float, 1, true>(a, c, t.y, y, z);

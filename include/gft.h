@@ -190,7 +190,7 @@ int gft_wilson_loops(gft_ctx* ctx, gft_loops* out);
  * x fastest in (y,z) column tiles; 1: the same dealt to the XCDs in
  * contiguous ranges; 2, the default: each XCD sweeps the x-planes of its own
  * y-band); and the occupancy target of the FP32 SM kernel (min_waves 3 or 4
- * waves per SIMD, the default being 3; 0 leaves the register allocation
+ * waves per SIMD, the default being 4; 0 leaves the register allocation
  * unconstrained); and `stage` for the FP32 forms (0: every neighbour read from
  * global memory; 1: each block gathers q, y and phi of its tile and a one-site
  * ring in y and z into LDS, so that only the neighbours in the planes x +- 1

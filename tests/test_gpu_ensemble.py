@@ -45,7 +45,7 @@ def _run_alone(rt, d, s, v, steps, tile, use_graph):
 def _run_single_kernel(rt, d, s, v, steps, tile):
     with rt.Evolver(d) as ev:
         ev.import_two_slot(s.astype(np.float32), v.astype(np.float32), 0)
-        ev.set_launch(tile[0], tile[1], order=0, min_waves=3)
+        ev.set_launch(tile[0], tile[1], order=0, min_waves=4)
         ev.evolve(steps)
         return ev.export_two_slot(np.float32)
 
@@ -91,7 +91,7 @@ def test_ensemble_staging_modes_bitwise_equal(rt):
         with rt.Ensemble(descs) as ens:
             for i, (m, (s, v)) in enumerate(zip(ens.members, states)):
                 m.import_two_slot(s.astype(np.float32), v.astype(np.float32), 0)
-                m.set_launch(tiles[i][0], tiles[i][1], 2, 3, stage)
+                m.set_launch(tiles[i][0], tiles[i][1], 2, 4, stage)
             ens.evolve(9)
             out[stage] = [m.export_two_slot(np.float32) for m in ens.members]
     for stage in (1, 0):
