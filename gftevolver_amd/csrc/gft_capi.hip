@@ -66,6 +66,13 @@ struct gft_ctx {
     unsigned* err_dev = nullptr;
     unsigned time_iter = 0;
     bool imported = false;
+    // device-side snapshot (gft_snapshot / gft_rollback): a copy of the state
+    // arrays and the host-side fields that go with them
+    void* snap = nullptr;
+    bool snap_valid = false;
+    unsigned snap_time_iter = 0;
+    int snap_cur = 0, snap_prev_valid = 0;
+    double snap_ref[2][4] = {};
     std::string err;
     // step-kernel timing
     std::vector<hipEvent_t> ev_start, ev_stop;
@@ -278,6 +285,7 @@ void gft_destroy(gft_ctx* ctx) {
     if (ctx->S) hipFree(ctx->S);
     if (ctx->V) hipFree(ctx->V);
     momentum_free(ctx->mom);
+    if (ctx->snap) hipFree(ctx->snap);
     if (ctx->partial) hipFree(ctx->partial);
     if (ctx->obs_dev) hipFree(ctx->obs_dev);
     if (ctx->obs_host) hipHostFree(ctx->obs_host);
@@ -821,6 +829,187 @@ extern "C" int gft_bench_streams(gft_ctx* ctx, int reps, double* avg_ms) {
     ctx->imported = false;   // the state now holds the kernel's scratch values
     return rc;
 }
+
+// ---- checkpoint, restore, snapshot and rollback ---------------------------------
+// The state of a context is a set of device arrays (two-slot forms: scalar and
+// vector; momentum forms: both ping-pong buffers, P, E and the residual words,
+// one allocation) plus host-side fields: the step counter, which ping-pong
+// buffer is current, whether the previous step's buffer is valid (Gauss law) and
+// the reference doublets.  A checkpoint holds all of it, so a restored context
+// steps on bitwise as the original would have.
+
+namespace {
+
+constexpr unsigned long long kCkptMagic = 0x31544b5043544647ull;   // "GFTCPKT1"
+
+// Fixed-size header in front of the raw state bytes.
+struct CkptHeader {
+    unsigned long long magic;
+    int form, model, nx, ny, nz, ns, gauge_num, gen_rep, fixed_faces;
+    int cur, prev_valid;
+    unsigned time_iter;
+    double ref[2][4];
+    unsigned long long payload;   // bytes of state after the header
+};
+
+// The device arrays of the state: up to two (pointer, bytes) ranges.
+int state_ranges(const gft_ctx* ctx, void* ptr[2], size_t bytes[2]) {
+    const long long N = ctx->N;
+    const size_t esz = is_f64(ctx->form) ? 8 : 4;
+    if (is_two_slot(ctx->form)) {
+        ptr[0] = ctx->S;
+        bytes[0] = (size_t)(2 * N * ctx->coef.ns) * esz;
+        ptr[1] = ctx->V;
+        bytes[1] = ctx->V ? (size_t)(2 * N * link_reals(ctx->desc)) * esz : 0;
+    } else {
+        ptr[0] = ctx->mom.base;
+        bytes[0] = ctx->mom.bytes;
+        ptr[1] = nullptr;
+        bytes[1] = 0;
+    }
+    return GFT_OK;
+}
+
+size_t state_bytes(const gft_ctx* ctx) {
+    void* p[2];
+    size_t b[2];
+    state_ranges(ctx, p, b);
+    return b[0] + b[1];
+}
+
+CkptHeader make_header(const gft_ctx* ctx) {
+    CkptHeader h{};
+    h.magic = kCkptMagic;
+    h.form = ctx->form;
+    h.model = ctx->desc.model;
+    h.nx = ctx->desc.nx; h.ny = ctx->desc.ny; h.nz = ctx->desc.nz;
+    h.ns = ctx->coef.ns;
+    h.gauge_num = gauge_num(ctx->desc);
+    h.gen_rep = ctx->desc.gen_rep;
+    h.fixed_faces = ctx->desc.fixed_faces;
+    h.cur = ctx->mom.cur;
+    h.prev_valid = ctx->mom.prev_valid;
+    h.time_iter = ctx->time_iter;
+    for (int k = 0; k < 2; k++)
+        for (int m = 0; m < 4; m++) h.ref[k][m] = ctx->mom.ref[k][m];
+    h.payload = state_bytes(ctx);
+    return h;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gft_checkpoint_size(const gft_ctx* ctx, size_t* bytes) {
+    if (!ctx || !bytes) return GFT_EINVAL;
+    *bytes = sizeof(CkptHeader) + state_bytes(ctx);
+    return GFT_OK;
+}
+
+int gft_checkpoint(gft_ctx* ctx, void* buf, size_t bytes) {
+    if (!ctx || !buf) return GFT_EINVAL;
+    if (!ctx->imported) return fail(ctx, GFT_EINVAL, "gft_checkpoint: the context holds no state (import or generate one)");
+    const CkptHeader h = make_header(ctx);
+    if (bytes < sizeof(CkptHeader) + h.payload)
+        return fail(ctx, GFT_EINVAL, "gft_checkpoint: buffer smaller than gft_checkpoint_size");
+    std::memcpy(buf, &h, sizeof(h));
+    void* p[2];
+    size_t b[2];
+    state_ranges(ctx, p, b);
+    char* dst = static_cast<char*>(buf) + sizeof(h);
+    for (int r = 0; r < 2; r++) {
+        if (!b[r]) continue;
+        CK(hipMemcpyAsync(dst, p[r], b[r], hipMemcpyDeviceToHost, ctx->stream));
+        dst += b[r];
+    }
+    CK(hipStreamSynchronize(ctx->stream));
+    return GFT_OK;
+}
+
+int gft_restore(gft_ctx* ctx, const void* buf, size_t bytes) {
+    if (!ctx || !buf) return GFT_EINVAL;
+    if (bytes < sizeof(CkptHeader)) return fail(ctx, GFT_EINVAL, "gft_restore: buffer too short for a checkpoint");
+    CkptHeader h;
+    std::memcpy(&h, buf, sizeof(h));
+    const CkptHeader mine = make_header(ctx);
+    if (h.magic != kCkptMagic) return fail(ctx, GFT_EINVAL, "gft_restore: not a gft checkpoint");
+    if (h.form != mine.form || h.model != mine.model || h.nx != mine.nx || h.ny != mine.ny || h.nz != mine.nz ||
+        h.ns != mine.ns || h.gauge_num != mine.gauge_num || h.gen_rep != mine.gen_rep ||
+        h.fixed_faces != mine.fixed_faces || h.payload != mine.payload)
+        return fail(ctx, GFT_EINVAL, "gft_restore: the checkpoint was taken from a context of another form, model, "
+                                     "lattice or boundary");
+    if (bytes < sizeof(CkptHeader) + h.payload) return fail(ctx, GFT_EINVAL, "gft_restore: truncated checkpoint");
+    void* p[2];
+    size_t b[2];
+    state_ranges(ctx, p, b);
+    const char* src = static_cast<const char*>(buf) + sizeof(h);
+    for (int r = 0; r < 2; r++) {
+        if (!b[r]) continue;
+        CK(hipMemcpyAsync(p[r], src, b[r], hipMemcpyHostToDevice, ctx->stream));
+        src += b[r];
+    }
+    CK(hipStreamSynchronize(ctx->stream));
+    ctx->time_iter = h.time_iter;
+    ctx->mom.cur = h.cur;
+    ctx->mom.prev_valid = h.prev_valid;
+    for (int k = 0; k < 2; k++)
+        for (int m = 0; m < 4; m++) ctx->mom.ref[k][m] = h.ref[k][m];
+    ctx->mom.epoch++;
+    ctx->imported = true;
+    return GFT_OK;
+}
+
+int gft_snapshot(gft_ctx* ctx) {
+    if (!ctx) return GFT_EINVAL;
+    if (!ctx->imported) return fail(ctx, GFT_EINVAL, "gft_snapshot: the context holds no state (import or generate one)");
+    void* p[2];
+    size_t b[2];
+    state_ranges(ctx, p, b);
+    if (!ctx->snap && hipMalloc(&ctx->snap, b[0] + b[1]) != hipSuccess) {
+        ctx->snap = nullptr;
+        return fail(ctx, GFT_ENOMEM, "gft_snapshot: device allocation failed");
+    }
+    char* dst = static_cast<char*>(ctx->snap);
+    for (int r = 0; r < 2; r++) {
+        if (!b[r]) continue;
+        CK(hipMemcpyAsync(dst, p[r], b[r], hipMemcpyDeviceToDevice, ctx->stream));
+        dst += b[r];
+    }
+    CK(hipStreamSynchronize(ctx->stream));
+    ctx->snap_valid = true;
+    ctx->snap_time_iter = ctx->time_iter;
+    ctx->snap_cur = ctx->mom.cur;
+    ctx->snap_prev_valid = ctx->mom.prev_valid;
+    for (int k = 0; k < 2; k++)
+        for (int m = 0; m < 4; m++) ctx->snap_ref[k][m] = ctx->mom.ref[k][m];
+    return GFT_OK;
+}
+
+int gft_rollback(gft_ctx* ctx, unsigned* time_iter) {
+    if (!ctx) return GFT_EINVAL;
+    if (!ctx->snap_valid) return fail(ctx, GFT_EINVAL, "gft_rollback: no snapshot (gft_snapshot first)");
+    void* p[2];
+    size_t b[2];
+    state_ranges(ctx, p, b);
+    const char* src = static_cast<const char*>(ctx->snap);
+    for (int r = 0; r < 2; r++) {
+        if (!b[r]) continue;
+        CK(hipMemcpyAsync(p[r], src, b[r], hipMemcpyDeviceToDevice, ctx->stream));
+        src += b[r];
+    }
+    CK(hipStreamSynchronize(ctx->stream));
+    ctx->time_iter = ctx->snap_time_iter;
+    ctx->mom.cur = ctx->snap_cur;
+    ctx->mom.prev_valid = ctx->snap_prev_valid;
+    for (int k = 0; k < 2; k++)
+        for (int m = 0; m < 4; m++) ctx->mom.ref[k][m] = ctx->snap_ref[k][m];
+    ctx->mom.epoch++;
+    ctx->imported = true;
+    if (time_iter) *time_iter = ctx->time_iter;
+    return GFT_OK;
+}
+
+}  // extern "C"
 
 // ---- batched ensembles -----------------------------------------------------------
 

@@ -574,6 +574,38 @@ void Lattice::initialAnalysis() {
     if (outputFields && outputFields->outputInitial) outputFields->outputFields(outputFields->initialAnalysisPath, 0);
 }
 
+void Lattice::saveCheckpoint(const std::string& path) {
+    if (!ctx) throw std::runtime_error("checkpoint: the lattice has not been evolved on the GPU yet");
+    size_t bytes = 0;
+    if (gft_checkpoint_size(ctx, &bytes) != GFT_OK) throw std::runtime_error(gft_last_error(ctx));
+    std::vector<char> buf(bytes);
+    if (gft_checkpoint(ctx, buf.data(), bytes) != GFT_OK) throw std::runtime_error(gft_last_error(ctx));
+    std::ofstream f(path, std::ios::binary);
+    f.write(buf.data(), (std::streamsize)bytes);
+    if (!f) throw std::runtime_error("checkpoint: cannot write " + path);
+}
+
+void Lattice::loadCheckpoint(const std::string& path) {
+    gft_desc d{};
+    d.form = resolvedForm();
+    d.nx = nx; d.ny = ny; d.nz = nz;
+    d.dx = dx; d.dy = dy; d.dz = dz; d.dt = dt;
+    d.fixed_faces = fixedFaces;
+    model.fill(d);
+    if (!ctx && gft_create(&d, &ctx) != GFT_OK) throw std::runtime_error(gft_last_error(nullptr));
+    std::ifstream f(path, std::ios::binary | std::ios::ate);
+    if (!f) throw std::runtime_error("checkpoint: cannot read " + path);
+    const std::streamsize bytes = f.tellg();
+    f.seekg(0);
+    std::vector<char> buf((size_t)bytes);
+    f.read(buf.data(), bytes);
+    if (gft_restore(ctx, buf.data(), (size_t)bytes) != GFT_OK) throw std::runtime_error(gft_last_error(ctx));
+    unsigned it = 0;
+    if (gft_export_two_slot_f32(ctx, scalarFields.data(), vectorFields.data(), &it) != GFT_OK)
+        throw std::runtime_error(gft_last_error(ctx));
+    timeIter = it;
+}
+
 int Lattice::resolvedForm() const {
     if (form != GFT_FORM_AUTO) return form;
     gft_desc d{};
@@ -609,8 +641,26 @@ void Lattice::evolve() {
     for (unsigned done = 0; done < nt;) {
         const unsigned n = std::min(chunk - (timeIter % chunk), nt - done);
         unsigned nrows = 0;
-        if (gft_evolve(ctx, n, efreq, gfreq, rows.data(), n, &nrows) != GFT_OK)
-            throw std::runtime_error(gft_last_error(ctx));
+        // A snapshot on the device before every chunk: a link update out of the
+        // asin/sqrt domain (the reference's "Panic in evolve",
+        // WilsonLoops/Source/StandardModel.cpp:474-475, after which it goes on with
+        // NaN) rolls the lattice back to the chunk's start, so the caller holds the
+        // last valid state when the error reaches it.
+        if (rollback && gft_snapshot(ctx) != GFT_OK) throw std::runtime_error(gft_last_error(ctx));
+        const int rc = gft_evolve(ctx, n, efreq, gfreq, rows.data(), n, &nrows);
+        if (rc == GFT_EDOMAIN && rollback) {
+            const std::string msg = gft_last_error(ctx);
+            unsigned back = 0;
+            if (gft_rollback(ctx, &back) != GFT_OK) throw std::runtime_error(gft_last_error(ctx));
+            unsigned it = 0;
+            if (gft_export_two_slot_f32(ctx, scalarFields.data(), vectorFields.data(), &it) != GFT_OK)
+                throw std::runtime_error(gft_last_error(ctx));
+            timeIter = back;
+            throw std::runtime_error(msg + " within steps " + std::to_string(back) + ".." +
+                                     std::to_string(back + n - 1) + "; the lattice is rolled back to step " +
+                                     std::to_string(back));
+        }
+        if (rc != GFT_OK) throw std::runtime_error(gft_last_error(ctx));
         for (unsigned k = 0; k < nrows; k++)
             for (auto& a : analysers) a->timestepAnalysis(rows[k]);
         done += n;

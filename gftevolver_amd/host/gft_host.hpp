@@ -299,6 +299,14 @@ public:
     Lattice(const std::string& config_root, const std::string& data_dir, int form = GFT_FORM_AUTO);
     // The form the GPU context runs: the requested one, GFT_FORM_AUTO resolved.
     int resolvedForm() const;
+    // Binary checkpoint of the complete GPU state (gft_checkpoint): evolving a
+    // lattice restored from it continues bitwise as the saved one would.  The
+    // reference has no reader for its text field dumps (OutputFields.cpp:63-133).
+    void saveCheckpoint(const std::string& path);
+    void loadCheckpoint(const std::string& path);
+    // Roll back to the start of the chunk on a link-update domain error before
+    // reporting it (gft_snapshot / gft_rollback; one device copy of the state).
+    bool rollback = true;
     ~Lattice();
     void initialAnalysis();
     void evolve();

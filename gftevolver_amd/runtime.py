@@ -51,6 +51,11 @@ _L.gft_ensemble_set_launch.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
 _L.gft_ensemble_evolve.argtypes = [C.c_void_p, C.c_uint, C.c_int]
 _L.gft_ensemble_last_step_ms.argtypes = [C.c_void_p, _P(C.c_double)]
 _L.gft_init_grf.argtypes = [C.c_void_p, _P(GftGrfDesc)]
+_L.gft_checkpoint_size.argtypes = [C.c_void_p, _P(C.c_size_t)]
+_L.gft_checkpoint.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
+_L.gft_restore.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
+_L.gft_snapshot.argtypes = [C.c_void_p]
+_L.gft_rollback.argtypes = [C.c_void_p, _P(C.c_uint)]
 _F = _P(C.c_float)
 _L.gft_mlp_create.argtypes = [_P(GftMlpDesc), _P(C.c_void_p)]
 _L.gft_mlp_destroy.argtypes = [C.c_void_p]
@@ -166,6 +171,25 @@ class Evolver:
         nrows = C.c_uint(0)
         self._check(_L.gft_evolve(self._h, nsteps, energy_freq, gauss_freq, rows, max_rows, C.byref(nrows)))
         return list(rows)[:nrows.value]
+
+    def checkpoint(self) -> bytes:
+        """The complete state (gft_checkpoint) as bytes."""
+        n = C.c_size_t()
+        self._check(_L.gft_checkpoint_size(self._h, C.byref(n)))
+        buf = C.create_string_buffer(n.value)
+        self._check(_L.gft_checkpoint(self._h, buf, n.value))
+        return buf.raw
+
+    def restore(self, data: bytes):
+        self._check(_L.gft_restore(self._h, C.c_char_p(data), len(data)))
+
+    def snapshot(self):
+        self._check(_L.gft_snapshot(self._h))
+
+    def rollback(self) -> int:
+        it = C.c_uint()
+        self._check(_L.gft_rollback(self._h, C.byref(it)))
+        return it.value
 
     def observe(self) -> GftObs:
         o = GftObs()

@@ -148,6 +148,28 @@ int gft_import_two_slot_f64(gft_ctx* ctx, const double* scalar, const double* ve
 int gft_export_two_slot_f32(gft_ctx* ctx, float* scalar, float* vec, unsigned* time_iter);
 int gft_export_two_slot_f64(gft_ctx* ctx, double* scalar, double* vec, unsigned* time_iter);
 
+/* Checkpoints (the reference dumps fields as 6-digit text with no reader,
+ * Analysers/Source/OutputFields.cpp:63-133; binary resume is this library's
+ * own).  A checkpoint holds every device array of the state -- for the momentum
+ * forms both ping-pong buffers, P, E and the compensated storage's residual
+ * words -- with the step counter, the current buffer and the reference
+ * doublets, in a header-tagged host buffer of gft_checkpoint_size bytes.
+ * gft_restore accepts only a checkpoint of a context with the same form, model,
+ * lattice, gauge group, link representation and fixed faces, and a restored
+ * context evolves on bitwise as the checkpointed one would have. */
+int gft_checkpoint_size(const gft_ctx* ctx, size_t* bytes);
+int gft_checkpoint(gft_ctx* ctx, void* buf, size_t bytes);
+int gft_restore(gft_ctx* ctx, const void* buf, size_t bytes);
+
+/* Device-side snapshot for error recovery: gft_snapshot copies the state on
+ * the device (allocating one state-sized buffer on its first call);
+ * gft_rollback restores the last snapshot and its step counter (*time_iter,
+ * may be NULL).  After GFT_EDOMAIN -- where the reference prints "Panic in
+ * evolve" and carries on with NaN (WilsonLoops/Source/StandardModel.cpp:474-475)
+ * -- rollback returns the lattice to its last valid state. */
+int gft_snapshot(gft_ctx* ctx);
+int gft_rollback(gft_ctx* ctx, unsigned* time_iter);
+
 /* Evolution.  Replaces Lattice::evolve() (Core/Source/Lattice.cpp:755-876)
  * for nsteps steps.  energy_freq / gauss_freq reproduce the Energy and
  * GaugeCondition cadence (measured on step 0 of the context and then every
