@@ -119,6 +119,10 @@ class GftMlpDesc(C.Structure):
                 ("n_out", C.c_int), ("max_batch", C.c_int)]
 
 
+class GftGaussDesc(C.Structure):
+    _fields_ = [("seed", C.c_ulonglong), ("sigma", C.c_double), ("vev", C.c_double * 2)]
+
+
 class GftGrfDesc(C.Structure):
     _fields_ = [("seed", C.c_ulonglong), ("rms", C.c_double), ("slope", C.c_double), ("kmax", C.c_double),
                 ("vev", C.c_double * 2)]

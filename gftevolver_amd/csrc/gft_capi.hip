@@ -1272,7 +1272,7 @@ __global__ void __launch_bounds__(256) grf_assemble_mom(MomFields<R> f, const do
     const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     for (int k = 0; k < nd; k++) {
-        const double scale = rms / sqrt(sq[k] / (double)n);
+        const double scale = sq ? rms / sqrt(sq[k] / (double)n) : rms;   // no sq: comps are scaled by rms as they are
         double v[4];
         for (int m = 0; m < 4; m++)
             v[m] = (m == 2 ? (k ? vev1 - vevr1 : vev0 - vevr0) : 0.0) + scale * comps[(4 * k + m) * n + s];
@@ -1304,7 +1304,7 @@ __global__ void __launch_bounds__(256) grf_assemble_two_slot(T* __restrict__ S, 
     if (s >= n) return;
     const int ns = 4 * nd;
     for (int k = 0; k < nd; k++) {
-        const double scale = rms / sqrt(sq[k] / (double)n);
+        const double scale = sq ? rms / sqrt(sq[k] / (double)n) : rms;
         for (int m = 0; m < 4; m++) {
             const T v = (T)((m == 2 ? (k ? vev1 : vev0) : 0.0) + scale * comps[(4 * k + m) * n + s]);
             S[s * ns + 4 * k + m] = v;
@@ -1391,6 +1391,65 @@ extern "C" int gft_init_grf(gft_ctx* ctx, const gft_grf_desc* g) {
     const bool ok = hipGetLastError() == hipSuccess && hipStreamSynchronize(st) == hipSuccess;
     cleanup();
     if (!ok) return fail(ctx, GFT_EHIP, "gft_init_grf: kernel failure");
+    ctx->time_iter = 0;
+    ctx->imported = true;
+    return GFT_OK;
+}
+
+namespace gft {
+
+// comps[c * n + s] = standard normal of (site s, component c): the GRF's white
+// noise, unfiltered.
+__global__ void __launch_bounds__(256) gauss_comps(double* __restrict__ comps, long long n, int nc,
+                                                   unsigned long long seed) {
+    const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    for (int c = 0; c < nc; c++) comps[c * n + s] = grf_normal(seed, (unsigned)s, (unsigned)c);
+}
+
+}  // namespace gft
+
+extern "C" int gft_init_gaussian(gft_ctx* ctx, const gft_gauss_desc* g) {
+    if (!ctx || !g) return GFT_EINVAL;
+    if (ctx->desc.model == GFT_MODEL_GLOBAL)
+        return fail(ctx, GFT_EINVAL, "gft_init_gaussian: doublet fields only (not the global O(N) model)");
+    if (!(g->sigma >= 0.0)) return fail(ctx, GFT_EINVAL, "gft_init_gaussian: sigma must be non-negative");
+    const Coef& c = ctx->coef;
+    const long long N = ctx->N;
+    const int nc = 4 * c.nd;
+    const long long nb = (N + 255) / 256;
+    double* comps = nullptr;
+    if (hipMalloc(&comps, nc * N * sizeof(double)) != hipSuccess)
+        return fail(ctx, GFT_ENOMEM, "gft_init_gaussian: work array");
+    hipStream_t st = ctx->stream;
+    gauss_comps<<<nb, 256, 0, st>>>(comps, N, nc, g->seed);
+    const double vev0 = g->vev[0], vev1 = g->vev[1];
+    if (is_two_slot(ctx->form)) {
+        if (is_f64(ctx->form))
+            grf_assemble_two_slot<double><<<nb, 256, 0, st>>>((double*)ctx->S, (double*)ctx->V, comps, N, c.nd,
+                                                                g->sigma, vev0, vev1, nullptr,
+                                                                (int)link_reals(ctx->desc) / 3);
+        else
+            grf_assemble_two_slot<float><<<nb, 256, 0, st>>>((float*)ctx->S, (float*)ctx->V, comps, N, c.nd,
+                                                               g->sigma, vev0, vev1, nullptr,
+                                                               (int)link_reals(ctx->desc) / 3);
+    } else {
+        ctx->mom.cur = 0;
+        ctx->mom.prev_valid = 0;
+        const double vr0 = ctx->mom.f64 ? 0.0 : (double)(float)vev0, vr1 = ctx->mom.f64 ? 0.0 : (double)(float)vev1;
+        for (int k = 0; k < 2; k++)
+            for (int m4 = 0; m4 < 4; m4++) ctx->mom.ref[k][m4] = (m4 == 2 && k < c.nd) ? (k ? vr1 : vr0) : 0.0;
+        ctx->mom.epoch++;
+        if (ctx->mom.f64)
+            grf_assemble_mom<double><<<nb, 256, 0, st>>>(ctx->mom.fd, comps, N, c.nd, g->sigma, vev0, vev1, vr0, vr1,
+                                                         nullptr);
+        else
+            grf_assemble_mom<float><<<nb, 256, 0, st>>>(ctx->mom.ff, comps, N, c.nd, g->sigma, vev0, vev1, vr0, vr1,
+                                                        nullptr);
+    }
+    const bool ok = hipGetLastError() == hipSuccess && hipStreamSynchronize(st) == hipSuccess;
+    (void)hipFree(comps);
+    if (!ok) return fail(ctx, GFT_EHIP, "gft_init_gaussian: kernel failure");
     ctx->time_iter = 0;
     ctx->imported = true;
     return GFT_OK;

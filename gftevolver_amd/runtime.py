@@ -12,7 +12,7 @@ import os
 import numpy as np
 
 from ._abi import (GFT_FORM_MOMENTUM_F32, GFT_FORM_MOMENTUM_F64, GFT_FORM_TWO_SLOT_F32, GFT_FORM_TWO_SLOT_F64,
-                   GftDesc, GftGrfDesc, GftLoops, GftMlpDesc, GftObs, link_reals)
+                   GftDesc, GftGaussDesc, GftGrfDesc, GftLoops, GftMlpDesc, GftObs, link_reals)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libgft.so")
@@ -51,6 +51,7 @@ _L.gft_ensemble_set_launch.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
 _L.gft_ensemble_evolve.argtypes = [C.c_void_p, C.c_uint, C.c_int]
 _L.gft_ensemble_last_step_ms.argtypes = [C.c_void_p, _P(C.c_double)]
 _L.gft_init_grf.argtypes = [C.c_void_p, _P(GftGrfDesc)]
+_L.gft_init_gaussian.argtypes = [C.c_void_p, _P(GftGaussDesc)]
 _L.gft_checkpoint_size.argtypes = [C.c_void_p, _P(C.c_size_t)]
 _L.gft_checkpoint.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
 _L.gft_restore.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
@@ -154,6 +155,12 @@ class Evolver:
         both slots equal, identity links, E = 0."""
         g = GftGrfDesc(seed, rms, slope, kmax, (C.c_double * 2)(*vevs))
         self._check(_L.gft_init_grf(self._h, C.byref(g)))
+
+    def init_gaussian(self, seed: int, sigma: float = 0.05, vevs=(1.0, 1.0)):
+        """iid Gaussian state generated on the GPU (gft_init_gaussian): vev plus
+        sigma per real component, time 0, both slots equal, identity links, E = 0."""
+        g = GftGaussDesc(seed, sigma, (C.c_double * 2)(*vevs))
+        self._check(_L.gft_init_gaussian(self._h, C.byref(g)))
 
     def export_two_slot(self, dtype=np.float64):
         s = np.zeros((2, self.nsites, self.ns), dtype)

@@ -314,6 +314,20 @@ typedef struct gft_grf_desc {
 } gft_grf_desc;
 int gft_init_grf(gft_ctx* ctx, const gft_grf_desc* grf);
 
+/* Gaussian initial state generated on the GPU (the 32^3 workload's; no reference
+ * counterpart, as for gft_init_grf): each real component c of each doublet is
+ * (0, 0, vev[k], 0)[c] + sigma * (standard normal of (site, component)) from
+ * the same Philox4x32-10 stream as gft_init_grf's white noise, unfiltered.
+ * Time 0, both time slots equal (zero scalar momentum), identity links, E = 0.
+ * The reference's own RandomUniform initial condition (std::mt19937, a
+ * sequential stream, RandomUniform.cpp:60-77) stays on the host (gft::RandomUniform). */
+typedef struct gft_gauss_desc {
+    unsigned long long seed;
+    double sigma;      /* standard deviation of every real component */
+    double vev[2];     /* vev of doublet 0 and 1 (component 2)        */
+} gft_gauss_desc;
+int gft_init_gaussian(gft_ctx* ctx, const gft_gauss_desc* g);
+
 /* Wait for all queued work on the context's stream. */
 int gft_sync(gft_ctx* ctx);
 

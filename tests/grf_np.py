@@ -48,6 +48,20 @@ def component(n, d, seed, comp, slope, kmax):
     return (f / np.sqrt((f ** 2).mean())).ravel()
 
 
+def gaussian_state(n, seed, sigma=0.05, vevs=(1.0,), ndoublets=1):
+    """Two-slot arrays (FP64) of the state gft_init_gaussian builds: the vev
+    plus sigma times the white noise of (site, component), unfiltered."""
+    N = int(np.prod(n))
+    s = np.zeros((2, N, 4 * ndoublets))
+    for k in range(ndoublets):
+        for m in range(4):
+            s[1, :, 4 * k + m] = sigma * normal(seed, N, 4 * k + m) + (vevs[k] if m == 2 else 0.0)
+    s[0] = s[1]
+    v = np.zeros((2, N, 15))
+    v[:, :, 1::5] = 1.0
+    return s, v
+
+
 def grf_state(n, d, seed, rms=0.1, slope=-1.0, kmax=None, vevs=(1.0,), ndoublets=1):
     """Two-slot arrays (FP64) of the state gft_init_grf builds."""
     kmax = np.pi / (2 * d[0]) if not kmax or kmax <= 0 else kmax

@@ -20,7 +20,8 @@ def rt():
     return runtime
 
 
-# FP32 + 4-bit residual: at most ulp(hi)/16 <= 2^-27 |hi|, 1.5e-8 for |phi| < 2.
+# FP32 momentum form: phi stored as its FP32 difference from the vev, at most
+# ulp(|phi - vev|) / 2 <= 1.5e-8 for |phi - vev| < 0.5.
 @pytest.mark.parametrize("form,tol", [(GFT_FORM_MOMENTUM_F64, 1e-12), (GFT_FORM_TWO_SLOT_F64, 1e-12),
                                       (GFT_FORM_MOMENTUM_F32, 1.5e-8), (GFT_FORM_TWO_SLOT_F32, 1e-7)],
                          ids=["mom64", "two64", "mom32", "two32"])
@@ -68,3 +69,42 @@ def test_grf_256_cube_statistics(rt):
         w = ev.wilson_loops()
     # <|phi|^2> = v^2 + rms^2 (zero-mean fluctuations), to the rounding of the sums.
     assert abs(w.phi_sq - 1.01) < 1e-9 and w.w11_su2 == 1.0
+
+
+# gft_init_gaussian: the same white noise as the GRF, unfiltered, sigma per
+# component around the vev (the 32^3 workload's initial state).
+@pytest.mark.parametrize("form,tol", [(GFT_FORM_MOMENTUM_F64, 1e-14), (GFT_FORM_TWO_SLOT_F64, 1e-14),
+                                      (GFT_FORM_MOMENTUM_F32, 1e-8), (GFT_FORM_TWO_SLOT_F32, 7e-8)],
+                         ids=["mom64", "two64", "mom32", "two32"])
+def test_gaussian_matches_restatement(rt, form, tol):
+    n = (32, 32, 32)
+    desc = make_desc(form=form, n=n)
+    with rt.Evolver(desc) as ev:
+        ev.init_gaussian(seed=20261203, sigma=0.05, vevs=(1.0, 1.0))
+        gs, gv, it = ev.export_two_slot(np.float64)
+        if form in (GFT_FORM_MOMENTUM_F64, GFT_FORM_MOMENTUM_F32):
+            o = ev.observe()
+            assert o.energy[3] == 0.0 and o.energy[5] == 0.0      # P = 0, E = 0
+    rs, rv = G.gaussian_state(n, 20261203, sigma=0.05)
+    err = np.abs(gs - rs).max()
+    print(f"form {form}: max |GPU - CPU| = {err:.2e}")
+    assert it == 0 and err <= tol
+    assert np.array_equal(gv, rv) and np.array_equal(gs[0], gs[1])
+    # iid N(vev, sigma^2) per component: sample mean and standard deviation of
+    # 32768 values within 4 standard errors, no correlation between components.
+    x = gs[1] - np.array([0.0, 0.0, 1.0, 0.0])
+    se = 0.05 / np.sqrt(x.shape[0])
+    assert np.all(np.abs(x.mean(axis=0)) < 4 * se)
+    assert np.all(np.abs(x.std(axis=0) - 0.05) < 4 * 0.05 / np.sqrt(2 * x.shape[0]))
+    c = np.corrcoef(x.T)
+    assert np.abs(c - np.eye(4)).max() < 4 / np.sqrt(x.shape[0])
+
+
+def test_gaussian_two_doublets(rt):
+    n = (16, 12, 18)
+    desc = make_desc(model=GFT_MODEL_2HDM, form=GFT_FORM_TWO_SLOT_F64, n=n, pot2=TWO_HDM_POTENTIAL)
+    with rt.Evolver(desc) as ev:
+        ev.init_gaussian(seed=7, sigma=0.1, vevs=(0.8, 0.6))
+        gs, _, _ = ev.export_two_slot(np.float64)
+    rs, _ = G.gaussian_state(n, 7, sigma=0.1, vevs=(0.8, 0.6), ndoublets=2)
+    assert np.abs(gs - rs).max() <= 1e-14
