@@ -107,13 +107,15 @@ def write_model(root: str, model: int, g: float, gp: float, *, lam=0.13, eta_sqr
 
 
 def write_lattice(root: str, n, d, nt, dt, ns, *, ic_seed=42, ic_range=(-1.0, 1.0), efreq=1, gfreq=1,
-                  energy=True, gauss=True, progress=False, faces=(3, 3, 3, 3, 3, 3), nvd=5):
+                  energy=True, gauss=True, progress=False, faces=(3, 3, 3, 3, 3, 3), nvd=5, output_fields=False):
     """Lattice.cfg, RandomUniform.cfg and analyser configs for a full program run
     with a RandomUniform scalar initial state and identity links.  faces: the
     six face boundary types (-x, +x, -y, +y, -z, +z): 1 fixed, 3 periodic.
-    nvd: vector components per direction, 5 (quaternion) or 4 (generator form)."""
+    nvd: vector components per direction, 5 (quaternion) or 4 (generator form).
+    output_fields: switch the OutputFields analyser on (its own options in
+    OutputFields.cfg)."""
     set_values(root, "Core", "Lattice.cfg", [
-        tuple(n), tuple(d), nt, dt, ns, nvd, 0, 3.0, (1, 0), 0, int(energy), int(gauss), tuple(faces),
+        tuple(n), tuple(d), nt, dt, ns, nvd, 0, 3.0, (1, 0), int(output_fields), int(energy), int(gauss), tuple(faces),
         int(progress), max(nt, 1),
     ])
     set_values(root, "InitialConditions", "RandomUniform.cfg", [ic_seed, tuple(ic_range), 1])

@@ -29,12 +29,16 @@ Two kinds of fixture, both produced by the reference's own compiled code:
                        (TwoHDM.cfg's flag and the vector component count select
                        both, TwoHDM.cpp:215-242), Higgs-family coupling 0.5.
 
+  program_sm_dumps.npz the SM program with the OutputFields analyser on: the
+                       text of every field dump it writes (initial, continual every
+                       5 steps, final), with both time slots ("both_*") and with
+                       the current slot only ("cur_*"), as printed (6 digits).
   harness_global_n<N>.npz, program_global_n<N>.npz
                        the global O(N) model (global gradient, null Wilson loop):
                        N = 3 with SO_N, N = 8 with Double_SO_N; no Gauss law, so
                        the program runs the energy analyser only.
 
-    python tests/golden/make_golden.py [--fixed | --gen | --family | --global]   # only those files
+    python tests/golden/make_golden.py [--fixed | --gen | --family | --global | --dumps]   # only those files
 """
 from __future__ import annotations
 
@@ -285,8 +289,38 @@ def fixed_faces_program():
     print("sm_fixed written")
 
 
+DUMP_CASES = {"both": 1, "cur": 0}
+
+
+def field_dumps():
+    """The reference program's OutputFields text (OutputFields.cpp:63-133, 157-202):
+    initial, every 5 completed steps and final, both slots or the current one."""
+    out = {}
+    for case, both in DUMP_CASES.items():
+        with tempfile.TemporaryDirectory() as td:
+            refcfg.prepare(td)
+            refcfg.write_model(td, 0, 0.65, 0.35)
+            refcfg.write_lattice(td, PROGRAM_N, (0.5, 0.5, 0.5), PROGRAM_STEPS, 0.05, 4, ic_seed=PROGRAM_SEED,
+                                 ic_range=PROGRAM_RANGE, efreq=PROGRAM_EFREQ, gfreq=PROGRAM_GFREQ, output_fields=True)
+            refcfg.set_values(td, "Analysers", "OutputFields.cfg",
+                              [both, 1, 1, 1, 5, "initial_fields.dat", "Continual/fields", "final_fields.dat"])
+            subprocess.run([os.path.join(O.REF_DIR, "gft_reference")], check=True, stdout=subprocess.DEVNULL,
+                           stderr=subprocess.DEVNULL, cwd=td)
+            data = os.path.join(td, "data")
+            names = ["initial_fields.dat", "final_fields.dat"] + sorted(
+                os.path.join("Continual", f) for f in os.listdir(os.path.join(data, "Continual")) if f.startswith("fields"))
+            out[f"{case}_names"] = np.array(names)
+            for i, nm in enumerate(names):
+                with open(os.path.join(data, nm)) as f:
+                    out[f"{case}_{i}"] = np.array(f.read())
+    np.savez_compressed(os.path.join(HERE, "program_sm_dumps.npz"), **out)
+    print("sm_dumps written")
+
+
 if __name__ == "__main__":
-    if "--fixed" in sys.argv[1:]:
+    if "--dumps" in sys.argv[1:]:
+        field_dumps()
+    elif "--fixed" in sys.argv[1:]:
         fixed_faces_program()
     elif "--gen" in sys.argv[1:]:
         generator_form()
@@ -300,3 +334,4 @@ if __name__ == "__main__":
         generator_form()
         family_gauge()
         global_model()
+        field_dumps()

@@ -294,3 +294,43 @@ def test_gft_run_reproduces_reference_program_global_model(tmp_path, ns, pot2):
     E = np.loadtxt(data / "global_quantities.dat", ndmin=2)
     assert E.shape == g["energy"].shape
     np.testing.assert_allclose(E, g["energy"], rtol=2e-5, atol=1e-5 * np.abs(g["energy"]).max())
+
+
+def _dump_values(text):
+    rows = [np.array(line.split(), dtype=np.float64) for line in str(text).splitlines()]
+    assert len({r.size for r in rows}) == 1
+    return np.stack(rows)
+
+
+@needs_cfg
+@pytest.mark.gpu
+@pytest.mark.parametrize("form", ["two_slot_f32", None], ids=["two_slot_f32", "default"])
+@pytest.mark.parametrize("case,both", [("both", 1), ("cur", 0)], ids=["both_slots", "current_slot"])
+def test_gft_run_field_dumps_match_reference_program(tmp_path, form, case, both):
+    """The OutputFields analyser (OutputFields.cpp:63-133, 157-202) with every
+    dump on: the initial fields, continual dumps every 5 completed steps and the
+    final fields, both time slots or the current one (slot (t + 1) % 2 after t
+    steps).  gft_run writes the same files, the same lines and the same
+    components per line as the reference program; the values, printed to 6
+    significant digits by both, agree to 1e-5 (the last printed digit plus the
+    difference of the FP32 arithmetic orders over 12 steps)."""
+    import refcfg
+    g = np.load(os.path.join(ROOT, "tests", "golden", "program_sm_dumps.npz"))
+    root = _prepare(str(tmp_path), 0, (6, 5, 7), 12, 4)
+    refcfg.write_lattice(str(tmp_path), (6, 5, 7), (0.5, 0.5, 0.5), 12, 0.05, 4, ic_seed=7, ic_range=(-0.8, 0.8),
+                         efreq=3, gfreq=2, output_fields=True)
+    refcfg.set_values(str(tmp_path), "Analysers", "OutputFields.cfg",
+                      [both, 1, 1, 1, 5, "initial_fields.dat", "Continual/fields", "final_fields.dat"])
+    data = tmp_path / "data"
+    subprocess.run([GFT_RUN, root, str(data)] + ([form] if form else []), check=True, capture_output=True,
+                   timeout=120)
+    names = [str(x) for x in g[f"{case}_names"]]
+    written = sorted(os.path.join("Continual", f) for f in os.listdir(data / "Continual"))
+    assert written == sorted(n for n in names if n.startswith("Continual"))
+    for i, nm in enumerate(names):
+        ref = _dump_values(g[f"{case}_{i}"])
+        got = _dump_values((data / nm).read_text())
+        assert got.shape == ref.shape == (6 * 5 * 7 * (2 if both else 1), 4 + 15), (nm, got.shape, ref.shape)
+        err = np.abs(got - ref).max()
+        print(f"{form or 'default'} {case} {nm}: max abs diff {err:.2e}")
+        np.testing.assert_allclose(got, ref, rtol=1e-5, atol=1e-5, err_msg=nm)
