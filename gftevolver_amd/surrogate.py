@@ -13,10 +13,19 @@ GELU, 4 heads, BF16 on the MFMA cores, trained on the GPU) maps the scaled
 inputs to standardised outputs; each head is trained on its own Poisson(1)
 bootstrap weights, so the heads agree where the data pin the map and spread
 where they do not.  A sweep is screened by the network; a point whose head
-spread exceeds `threshold` (2%) of an observable's range is rerun by the
-evolver, which also measures the error of the points it accepts.
+spread is too large is rerun by the evolver, which also measures the error of
+the points it accepts.
 
-    python -m gftevolver_amd.surrogate [--train 4096] [--sweep 512] [--L 64] [--steps 2000]
+The spread cut is calibrated, not guessed (split conformal prediction): on a
+held-out set of evolver runs (independent of the training runs and of the
+sweep) the ratio error / spread is ranked, and q is its ceil((n+1)(1-alpha))-th
+smallest value.  A point is accepted when q * spread <= eps.  For a sweep
+point exchangeable with the calibration points, P(error > q * spread) <= alpha,
+so at most a fraction alpha of the sweep is accepted with an error above eps:
+the stated bound of an accepted point is eps (5% of the observable's range) at
+confidence 1 - alpha (95%).  The evolver judges it on the whole sweep.
+
+    python -m gftevolver_amd.surrogate [--train 4096] [--cal 512] [--sweep 512] [--L 64] [--steps 2000]
 """
 from __future__ import annotations
 
@@ -155,26 +164,49 @@ def error(pred, truth, rng_):
     return (np.abs(pred - truth) / rng_).max(axis=(-2, -1))
 
 
-def screen(sur, points, threshold=0.02):
+def calibrate(sur, points, Y, eps=0.05, alpha=0.05):
+    """Split-conformal spread cut from held-out evolver runs (points, Y):
+    returns (threshold on the spread, q, calibration errors and spreads).  q is the
+    ceil((n+1)(1-alpha))-th smallest error/spread ratio; accepting a point
+    when its spread <= eps / q bounds P(accepted and error > eps) by alpha."""
+    pred, _, spread = sur.predict(points)
+    err = error(pred, Y, sur.range)
+    ratio = np.sort(err / np.maximum(spread, 1e-12))
+    n = len(ratio)
+    k = int(np.ceil((n + 1) * (1.0 - alpha)))
+    q = float(ratio[min(k, n) - 1])
+    return eps / q, q, err, spread
+
+
+def screen(sur, points, threshold):
     """Surrogate pass over a sweep: (prediction, spread, accepted mask, seconds)."""
     t0 = time.perf_counter()
     mean, _, spread = sur.predict(points)
     return mean, spread, spread <= threshold, time.perf_counter() - t0
 
 
-def gain_study(rt, n_train=4096, n_sweep=512, L=64, steps=2000, epochs=1500, threshold=0.02, log=print):
-    """Train on n_train evolver runs, screen an independent n_sweep-point
-    Latin-hypercube sweep, and compare with evolving every sweep point."""
+def gain_study(rt, n_train=4096, n_sweep=512, L=64, steps=2000, epochs=1500, n_cal=512, eps=0.05, alpha=0.05,
+               log=print):
+    """Train on n_train evolver runs, calibrate the spread cut on n_cal
+    held-out runs, screen an independent n_sweep-point Latin-hypercube sweep,
+    and compare with evolving every sweep point (which also judges the error
+    of every accepted point)."""
     P = latin_hypercube(n_train, seed=1)
     log(f"training data: {n_train} runs of SM {L}^3, {steps} steps")
     Y, te, tw = run_points(rt, P, L, steps, log=log)
+    C = latin_hypercube(n_cal, seed=3)
+    log(f"calibration data: {n_cal} runs")
+    Yc, _, _ = run_points(rt, C, L, steps, log=log)
     sur = Surrogate(rt)
     try:
         t0 = time.perf_counter()
         hist = sur.fit(P, Y, epochs=epochs)
         t_fit = time.perf_counter() - t0
         log(f"trained in {t_fit:.1f} s; loss {hist[0]:.3e} -> {hist[-1]:.3e}")
+        threshold, q, cal_err, cal_spread = calibrate(sur, C, Yc, eps, alpha)
+        log(f"calibrated spread cut {threshold:.4f} (q = {q:.3f}; eps {eps}, alpha {alpha})")
         S = latin_hypercube(n_sweep, seed=2)
+        log(f"sweep: {n_sweep} points, all evolved to judge the screen")
         truth, se, sw = run_points(rt, S, L, steps, log=log)
         pred, spread, ok, t_pred = screen(sur, S, threshold)
         train_pred, _, _ = sur.predict(P[:n_sweep])
@@ -185,7 +217,11 @@ def gain_study(rt, n_train=4096, n_sweep=512, L=64, steps=2000, epochs=1500, thr
     per_point = sw / n_sweep
     screened_cost = t_pred + (~ok).sum() * per_point
     res = {
-        "train_runs": n_train, "sweep_points": n_sweep, "lattice": L, "steps": steps, "threshold": threshold,
+        "train_runs": n_train, "calibration_runs": n_cal, "sweep_points": n_sweep, "lattice": L, "steps": steps,
+        "eps": eps, "alpha": alpha, "threshold": threshold, "conformal_q": q,
+        "sweep_accepted_over_eps": int((ok & (err > eps)).sum()),
+        "accepted_over_eps_fraction": float((err[ok] > eps).mean()) if ok.any() else None,
+        "calibration_err_mean": float(cal_err.mean()),
         "evolver_seconds_per_point": per_point, "surrogate_seconds_per_sweep": t_pred, "train_seconds": t_fit,
         "data_seconds": tw, "accepted": int(ok.sum()), "rerun": int((~ok).sum()),
         "gain_cost": (n_sweep * per_point) / screened_cost,
@@ -198,6 +234,10 @@ def gain_study(rt, n_train=4096, n_sweep=512, L=64, steps=2000, epochs=1500, thr
         "spread_median": float(np.median(spread)),
         "screened_sweep_err_max": float(np.where(ok, err, 0.0).max()),
         "loss_first_last": [float(hist[0]), float(hist[-1])],
+        # per point: the head spread and the evolver-judged error of every sweep
+        # point, and the calibration set's sorted error/spread ratios
+        "sweep_spread": spread.tolist(), "sweep_err": err.tolist(),
+        "calibration_ratio_sorted": np.sort(cal_err / np.maximum(cal_spread, 1e-12)).tolist(),
     }
     return res
 
@@ -205,14 +245,18 @@ def gain_study(rt, n_train=4096, n_sweep=512, L=64, steps=2000, epochs=1500, thr
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--train", type=int, default=4096)
+    ap.add_argument("--cal", type=int, default=512)
     ap.add_argument("--sweep", type=int, default=512)
+    ap.add_argument("--eps", type=float, default=0.05)
+    ap.add_argument("--alpha", type=float, default=0.05)
     ap.add_argument("--L", type=int, default=64)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--epochs", type=int, default=1500)
     ap.add_argument("--out", type=str, default="")
     a = ap.parse_args(argv)
     from . import runtime as rt
-    res = gain_study(rt, a.train, a.sweep, a.L, a.steps, a.epochs, log=lambda *x: print(*x, flush=True))
+    res = gain_study(rt, a.train, a.sweep, a.L, a.steps, a.epochs, a.cal, a.eps, a.alpha,
+                     log=lambda *x: print(*x, flush=True))
     txt = json.dumps(res)
     print(txt, flush=True)
     if a.out:
