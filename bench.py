@@ -158,6 +158,43 @@ def cpu_baseline(nsteps=150):
             "sample": f"CPU restatement (FP64 two-slot), SM {n}^3, {nsteps} steps, one thread ({sec:.2f} s)"}
 
 
+def host_cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_restatement_all_cores(n=128, nsteps=40):
+    """The CPU restatement in the reference's own FP32 arithmetic (two-slot
+    arrays, the reference's site order within each step) with its site loop
+    spread over every host thread this process may use (OpenMP, static
+    schedule): SM n^3, iid Gaussian state, `nsteps` steps, analysers off.
+    Threads = the CPU affinity set, capped by OMP_NUM_THREADS when set."""
+    import numpy as np
+    from gftevolver_amd._abi import make_desc
+    from gftevolver_amd.ic import gaussian_state
+    from tests import oracle_lib as O
+    threads = len(os.sched_getaffinity(0))
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        threads = min(threads, int(os.environ["OMP_NUM_THREADS"]))
+    d = make_desc(n=(n, n, n))
+    s, v = gaussian_state((n, n, n))
+    s, v = s.astype(np.float32), v.astype(np.float32)
+    O.evolve(d, s, v, 1, nthreads=threads)       # first touch of the pages, thread start-up
+    t0 = time.perf_counter()
+    O.evolve(d, s, v, nsteps, time_iter=1, nthreads=threads)
+    sec = time.perf_counter() - t0
+    return {"value": n ** 3 * nsteps / sec, "unit": "site-updates/s", "cores": threads, "kind": "port",
+            "nproc": os.cpu_count(), "cpu_model": host_cpu_model(),
+            "sample": f"CPU restatement (FP32 two-slot, reference arithmetic), SM {n}^3, {nsteps} steps, "
+                      f"OpenMP over {threads} threads of nproc={os.cpu_count()} ({sec:.2f} s)"}
+
+
 def options_leg(rt, with_cpu):
     """The reference's other model options on the GPU, each beside the
     reference program on the host (one core, 32^3 sample): fixed faces in the
@@ -397,7 +434,12 @@ def main():
     if not args.only_f32 and not args.no_options:
         extra["options"] = options_leg(rt, not args.no_cpu)
 
-    cpu = None if args.no_cpu else cpu_baseline()
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline()
+        cpu["all_cores"] = cpu_restatement_all_cores()
+        print(f"cpu restatement over all cores: {cpu['all_cores']['value']:.3e} site-updates/s "
+              f"({cpu['all_cores']['cores']} threads)", file=sys.stderr, flush=True)
     traffic = read_traffic()
     out = {
         "metric": METRIC,
