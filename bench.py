@@ -279,7 +279,7 @@ def options_leg(rt, with_cpu):
 
 def ensemble_leg(rt, steps, tiles=None):
     """The 64-member ensemble workload: aggregate FP32 throughput of the
-    persistent graph-replayed launch, and its ensemble-mean observables against
+    graph-replayed launch per step, and its ensemble-mean observables against
     the same members run one by one in FP64."""
     import numpy as np
     from gftevolver_amd import ensemble as E

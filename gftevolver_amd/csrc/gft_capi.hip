@@ -640,7 +640,7 @@ extern "C" int gft_set_launch(gft_ctx* ctx, int bz, int by, int order, int min_w
     if (stage < 0 || stage > 2) return fail(ctx, GFT_EINVAL, "gft_set_launch: stage must be 0, 1 or 2");
     if (min_waves != 0 && min_waves != 3 && min_waves != 4)
         return fail(ctx, GFT_EINVAL, "gft_set_launch: min_waves must be 0, 3 or 4");
-    if (order < 0 || order > 2) return fail(ctx, GFT_EINVAL, "gft_set_launch: order must be 0, 1 or 2");
+    if (order < 0 || order > 4) return fail(ctx, GFT_EINVAL, "gft_set_launch: order must be 0 .. 4");
     if (bz < 1 || by < 1 || bz * by != 256 || (bz & (bz - 1)) || bz < 16)
         return fail(ctx, GFT_EINVAL, "gft_set_launch: need bz*by = 256 with bz a power of two >= 16");
     ctx->mom.bz = bz;
@@ -1019,7 +1019,7 @@ struct gft_ensemble {
     EnsMember* args[2] = {nullptr, nullptr};   // step arguments with buffer 0 / 1 current
     int4* work = nullptr;
     long long nwork = 0;
-    int grid = 0, blocks_per_xcd = 0;
+    long long grid = 0;      // blocks of the ensemble kernel: one per tile, 8 x the longest XCD range
     size_t lds = 0;          // dynamic LDS per block of the staged kernel (0: unstaged)
     int stage = 0;           // staging mode of the ensemble kernel
     unsigned* err_dev = nullptr;
@@ -1104,20 +1104,10 @@ static int ens_prepare(gft_ensemble* ens) {
     EK(hipMemcpyAsync(ens->work, work.data(), work.size() * sizeof(int4), hipMemcpyHostToDevice, ens->stream));
     EK(hipStreamSynchronize(ens->stream));
     ens->nwork = (long long)work.size();
-    int cus = 0, lds_cu = 0, dev = 0;
-    EK(hipGetDevice(&dev));
-    EK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    EK(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
     ens->lds = lds;
     ens->stage = stage;
-    // Four 256-thread blocks per CU (the kernel's register occupancy, 4 waves per
-    // SIMD) or as many as the CU's LDS holds, a multiple of 8.
-    long long per_cu = 4;
-    if (ens->lds) per_cu = std::max<long long>(1, std::min<long long>(4, (long long)lds_cu / (long long)ens->lds));
-    long long grid = std::min<long long>(ens->nwork, per_cu * cus);
-    grid = std::max<long long>(8, (grid / 8) * 8);
-    ens->grid = (int)grid;
-    ens->blocks_per_xcd = (int)(grid / 8);
+    // XCD k owns work items [k n / 8, (k + 1) n / 8), at most ceil(n / 8) of them.
+    ens->grid = 8 * ((ens->nwork + 7) / 8);
     ens->dirty = false;
     return GFT_OK;
 }
@@ -1196,11 +1186,11 @@ int gft_ensemble_evolve(gft_ensemble* ens, unsigned nsteps, int use_graph) {
     }
     int rc = ens_prepare(ens);
     if (rc) return rc;
-    const dim3 grid(ens->grid), block(256);
+    const dim3 grid((unsigned)ens->grid), block(256);
     auto launch = [&](int p) {
-        if (ens->stage == 2) ens_step<4, 2><<<grid, block, ens->lds, ens->stream>>>(ens->args[p], ens->work, ens->nwork, ens->blocks_per_xcd);
-        else if (ens->stage == 1) ens_step<4, 1><<<grid, block, ens->lds, ens->stream>>>(ens->args[p], ens->work, ens->nwork, ens->blocks_per_xcd);
-        else ens_step<4, 0><<<grid, block, 0, ens->stream>>>(ens->args[p], ens->work, ens->nwork, ens->blocks_per_xcd);
+        if (ens->stage == 2) ens_step<4, 2><<<grid, block, ens->lds, ens->stream>>>(ens->args[p], ens->work, ens->nwork);
+        else if (ens->stage == 1) ens_step<4, 1><<<grid, block, ens->lds, ens->stream>>>(ens->args[p], ens->work, ens->nwork);
+        else ens_step<4, 0><<<grid, block, 0, ens->stream>>>(ens->args[p], ens->work, ens->nwork);
     };
     if (use_graph && nsteps >= 2 && ens->graph_parity != parity) {
         ens_drop_graph(ens);

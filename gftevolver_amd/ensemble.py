@@ -5,7 +5,7 @@ The ensemble workload: 64 independent SM lattices of uneven sizes
 Gaussian random field around the vev with P(k) ~ 1/k up to k_max = pi/(2 dx),
 its own seed and an rms amplitude drawn uniformly from [0.05, 0.2]; couplings
 and spacings of the 256^3 workload.  All members advance together in the
-persistent ensemble kernel (gft_ensemble_*), replayed from a two-step
+ensemble kernel (gft_ensemble_*), replayed from a two-step
 hipGraph, with per-member tile shapes picked by the autotuner.
 
 Observables per member, all FP64 reductions on the GPU:

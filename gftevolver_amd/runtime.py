@@ -235,7 +235,7 @@ class Evolver:
 
 class Ensemble:
     """Independent FP32 momentum-form lattices advanced together by one
-    persistent kernel per step (gft_ensemble_* in include/gft.h).  members[i]
+    kernel launch per step (gft_ensemble_* in include/gft.h).  members[i]
     is an Evolver view of member i for import, export and observables; it
     stays valid while the ensemble is open."""
 

@@ -8,7 +8,7 @@ points x 4 observables of an SM lattice evolved from that initial state
 plaquette 1/2 tr), measured at steps round(k * steps / 63), k = 0..63.
 
 Training data come from the batched evolver (gft_ensemble: 64 lattices per
-persistent launch, FP32-bulk).  The network (gft_mlp: 4 hidden layers of 256,
+launch, FP32-bulk).  The network (gft_mlp: 4 hidden layers of 256,
 GELU, 4 heads, BF16 on the MFMA cores, trained on the GPU) maps the scaled
 inputs to standardised outputs; each head is trained on its own Poisson(1)
 bootstrap weights, so the heads agree where the data pin the map and spread

@@ -12,7 +12,7 @@ tried only with occupancy=True.  A candidate is (bz, by, order, min_waves, stage
 from __future__ import annotations
 
 TILES = ((256, 1), (128, 2), (64, 4), (32, 8), (16, 16))
-ORDERS = (2, 1)
+ORDERS = (3, 2, 1)
 
 
 def candidates(occupancy: bool = False):

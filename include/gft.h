@@ -210,8 +210,9 @@ int gft_wilson_loops(gft_ctx* ctx, gft_loops* out);
 /* Launch shape of the momentum-form step kernel: tiles of bz x by sites
  * (bz along z, by along y, bz*by = 256); the block order (0: hardware order,
  * x fastest in (y,z) column tiles; 1: the same dealt to the XCDs in
- * contiguous ranges; 2, the default: each XCD sweeps the x-planes of its own
- * y-band); and the occupancy target of the FP32 SM kernel (min_waves 3 or 4
+ * contiguous ranges; 2: each XCD sweeps the x-planes of its own y-band; 3,
+ * the default, and 4: as 2 with two or four narrower y-bands per XCD, swept
+ * one after the other); and the occupancy target of the FP32 SM kernel (min_waves 3 or 4
  * waves per SIMD, the default being 4; 0 leaves the register allocation
  * unconstrained); and `stage` for the FP32 forms (0: every neighbour read from
  * global memory; 1: each block gathers q, y and phi of its tile and a one-site
@@ -244,7 +245,7 @@ int gft_bench_streams(gft_ctx* ctx, int reps, double* avg_ms);
 
 /* Batched ensembles (no reference counterpart: the reference evolves one
  * lattice per process).  Independent FP32 momentum-form SM lattices of any
- * sizes advance together: one persistent kernel launch per step over a work
+ * sizes advance together: one kernel launch per step over a work
  * list of 256-site tiles of all members, optionally replayed from a hipGraph
  * that holds two steps.  Each member is an ordinary context (gft_ensemble_member)
  * for import, export, gft_observe and gft_wilson_loops; its fields are bitwise

@@ -1,5 +1,5 @@
 """GPU checks of the batched ensemble (gft_ensemble_*): independent FP32
-momentum-form lattices of different sizes advanced by one persistent kernel
+momentum-form lattices of different sizes advanced by one kernel launch
 per step, with and without a captured two-step hipGraph.
 
 Each member must be bitwise the same lattice evolved alone by the same kernel
@@ -116,7 +116,7 @@ def test_ensemble_rejects_mixed_steps(rt):
 
 def test_ensemble_workload_observables_match_fp64_sequential(rt):
     """The 64-member ensemble workload (16 x 32^3, 24 x 48^3, 16 x 64^3,
-    8 x 96^3), 2000 FP32 steps in one persistent graph-replayed launch per
+    8 x 96^3), 2000 FP32 steps in one graph-replayed launch per
     step, against the same members run one by one in FP64: the ensemble means
     of <|phi|^2>/v^2, the electric and magnetic energy fractions and the 1x1
     SU(2) and U(1) plaquettes agree to 1e-4 at every measurement (every 500

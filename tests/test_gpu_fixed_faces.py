@@ -177,7 +177,8 @@ def test_fixed_faces_launch_shapes_bitwise_identical(rt, case):
     d = make_desc(form=GFT_FORM_MOMENTUM_F32, n=n, fixed_faces=faces)
     s, v, _ = _state(n, faces, seed=6)
     cfgs = ((64, 4, 2, 4, 2), (64, 4, 2, 4, 1), (64, 4, 2, 4, 0), (64, 4, 0, 4, 2), (256, 1, 2, 4, 2),
-            (128, 2, 1, 4, 2), (32, 8, 2, 4, 2), (32, 8, 2, 4, 1), (16, 16, 2, 4, 2), (16, 16, 0, 4, 0))
+            (128, 2, 1, 4, 2), (32, 8, 2, 4, 2), (32, 8, 2, 4, 1), (16, 16, 2, 4, 2), (16, 16, 0, 4, 0),
+            (64, 4, 3, 4, 2), (32, 8, 4, 4, 1))
     out = {}
     for c in cfgs:
         with rt.Evolver(d) as ev:

@@ -124,7 +124,8 @@ def test_momentum_launch_shapes_bitwise_identical(rt, form, model):
     cfgs = ((64, 4, 2, 4, 2), (64, 4, 2, 4, 1), (64, 4, 2, 4, 0), (64, 4, 0, 4, 2), (64, 4, 1, 4, 1),
             (256, 1, 2, 4, 2), (256, 1, 2, 4, 0), (128, 2, 2, 4, 2), (128, 2, 2, 4, 1), (32, 8, 1, 4, 2),
             (32, 8, 1, 4, 0), (16, 16, 0, 4, 2), (16, 16, 2, 4, 1), (16, 16, 2, 4, 0), (64, 4, 2, 0, 2),
-            (64, 4, 2, 0, 1), (64, 4, 2, 0, 0), (64, 4, 2, 3, 2), (64, 4, 2, 3, 1), (64, 4, 2, 3, 0))
+            (64, 4, 2, 0, 1), (64, 4, 2, 0, 0), (64, 4, 2, 3, 2), (64, 4, 2, 3, 1), (64, 4, 2, 3, 0),
+            (64, 4, 3, 4, 2), (32, 8, 4, 4, 2), (16, 16, 3, 4, 1), (128, 2, 4, 4, 0))
     out = {}
     for c in cfgs:
         with rt.Evolver(d) as ev:
