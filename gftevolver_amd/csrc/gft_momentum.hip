@@ -286,10 +286,13 @@ __device__ __forceinline__ void qrot(const Qt<R>& q, const R* v, R* o) {
 // law -- while the kernel's arithmetic and neighbour traffic stay FP32.
 __device__ __forceinline__ int res_exp(float hi) { return static_cast<int>((__float_as_uint(hi) >> 23) & 0xffu); }
 // q of residual r next to hi (round to nearest even).  r is the exact error of
-// the FP32 sum hi, so |r| <= ulp(hi) / 2 and |q| <= 4: no clamp is needed (and
-// res_pack4 masks every q to its own nibble).
+// the FP32 sum hi, so |r| <= ulp(hi) / 2 and |q| <= 4: no clamp is needed.
+// The returned word carries q in its low nibble (two's complement), which is
+// all res_pack4 keeps: adding 1.5 * 2^23 to x = r / (ulp(hi) / 2^3) rounds x to
+// the nearest integer, ties to even (as rintf), into the last bits of the sum,
+// whose bit patterns are consecutive integers around 0x4B400000.
 __device__ __forceinline__ int res_q(float r, float hi) {
-    return static_cast<int>(rintf(ldexpf(r, 153 - res_exp(hi))));
+    return static_cast<int>(__float_as_uint(ldexpf(r, 153 - res_exp(hi)) + 12582912.0f));
 }
 __device__ __forceinline__ float res_from_q(int q, float hi) { return ldexpf(static_cast<float>(q), res_exp(hi) - 153); }
 // Four q in 16 bits (nibble m = value m) and back; nibbles 4..7 of a 32-bit
@@ -572,14 +575,17 @@ __device__ __forceinline__ void fold_current(const StepArgs<R>& a, const R (*J)[
 // U1: no damping (keep = 1), with the kick's keep products dropped (exact).
 // praw: the site's stored doublet(s) (phi - ref), which the drift advances.
 // M: the link forces with the gauge source already folded in (fold_current).
-template <class R, int ND, bool CMP, bool PRE = false, bool U1 = false>
+template <class R, int ND, bool CMP, bool PRE = false, bool U1 = false, bool FZ = true>
 // frozen: a site of a fixed face's shell -- every lane still stores, but a
 // frozen lane stores the values it read, so every store instruction writes
 // whole cache lines (a divergent copy branch would write partial lines twice).
+// FZ false (a lattice without fixed faces): no lane is frozen, and the
+// selects between computed and read values are not generated.
 __device__ __forceinline__ void kick_drift_t(const StepArgs<R>& a, unsigned s, const Qt<R>* U, const R* yv,
                                              const R* praw, const R* F, const R (*M)[4], const OwnState<R, ND>* o,
                                              bool frozen) {
     using V4 = typename V4T<R>::type;
+    if constexpr (!FZ) frozen = false;
     uint2 lw = make_uint2(0u, 0u);
     unsigned lp2 = 0u;
     if constexpr (CMP) {
@@ -690,13 +696,19 @@ __device__ __forceinline__ void kick_drift_t(const StepArgs<R>& a, unsigned s, c
     if (bad) atomicOr(a.err, 1u);
 }
 
-// The kick with a uniform branch on damping: keep = 1 takes the shorter path.
+// The kick with uniform branches on damping (keep = 1 takes the shorter path)
+// and on fixed faces (a periodic lattice has no frozen lane).
 template <class R, int ND, bool CMP, bool PRE = false>
 __device__ __forceinline__ void kick_drift(const StepArgs<R>& a, unsigned s, const Qt<R>* U, const R* yv,
-                                           const R* praw, const R* F, const R (*M)[4],
-                                           const OwnState<R, ND>* o = nullptr, bool frozen = false) {
-    if (a.keep == R(1)) kick_drift_t<R, ND, CMP, PRE, true>(a, s, U, yv, praw, F, M, o, frozen);
-    else kick_drift_t<R, ND, CMP, PRE, false>(a, s, U, yv, praw, F, M, o, frozen);
+                                           const R* praw, const R* F, const R (*M)[4], const OwnState<R, ND>* o,
+                                           bool frozen, bool any_fixed) {
+    if (!any_fixed) {
+        if (a.keep == R(1)) kick_drift_t<R, ND, CMP, PRE, true, false>(a, s, U, yv, praw, F, M, o, false);
+        else kick_drift_t<R, ND, CMP, PRE, false, false>(a, s, U, yv, praw, F, M, o, false);
+    } else {
+        if (a.keep == R(1)) kick_drift_t<R, ND, CMP, PRE, true, true>(a, s, U, yv, praw, F, M, o, frozen);
+        else kick_drift_t<R, ND, CMP, PRE, false, true>(a, s, U, yv, praw, F, M, o, frozen);
+    }
 }
 
 // One site-update at (x, y, z): forces, kick and drift (shared by the
@@ -890,7 +902,7 @@ __device__ __forceinline__ void mom_site(const StepArgs<R>& a, const Coef& c, in
         if (j == 0) fold_current(a, J, M);
     }
 
-    kick_drift<R, ND, CMP>(a, s, U, yv, praw, F, M, nullptr, frozen);
+    kick_drift<R, ND, CMP>(a, s, U, yv, praw, F, M, nullptr, frozen, c.fixed != 0);
 }
 
 // ---- shared plaquettes and covariant differences (FP32 SM, staged) ----------
@@ -1182,7 +1194,7 @@ __device__ __forceinline__ void mom_block_shared(const StepArgs<float>& a, const
         M[1][2] -= a.inv_sp[2] * p12b.y;
         M[1][3] -= a.inv_sp[2] * p12b.z;
     }
-    kick_drift<R, 1, CMP, true>(a, s, U, yv, praw, F, M, &own, frozen);
+    kick_drift<R, 1, CMP, true>(a, s, U, yv, praw, F, M, &own, frozen, c.fixed != 0);
 }
 
 // MINW: minimum waves per SIMD the register allocation must allow (0: the
