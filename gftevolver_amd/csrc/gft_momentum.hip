@@ -157,6 +157,19 @@ __device__ __forceinline__ double sin_fast(double x) { return sin(x); }
 __device__ __forceinline__ double asin_small(double x) { return asin(x); }
 __device__ __forceinline__ float rsqrt_r(float x) { return __frsqrt_rn(x); }
 __device__ __forceinline__ double rsqrt_r(double x) { return 1.0 / sqrt(x); }
+// Correctly rounded sqrt of x in [2^-24, 1] (1 - |E|^2 for a float |E|^2 < 1):
+// the hardware estimate, then the neighbour whose square brackets x -- the
+// library's correction step, without its rescaling of inputs below 2^-96 and
+// its special-value fixup, which no input in this range needs (0 and 1 also
+// come out exact).  Bitwise sqrtf for every such input.
+__device__ __forceinline__ float sqrt_unit(float x) {
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float sm = __uint_as_float(__float_as_uint(s) - 1u), sp = __uint_as_float(__float_as_uint(s) + 1u);
+    const float vm = __builtin_fmaf(-sm, s, x), vp = __builtin_fmaf(-sp, s, x);
+    const float r = vm <= 0.f ? sm : s;
+    return vp > 0.f ? sp : r;
+}
+__device__ __forceinline__ double sqrt_unit(double x) { return sqrt(x); }
 
 // Streaming (non-temporal) access to own-site data that no neighbour reads.
 typedef float f4n __attribute__((ext_vector_type(4)));
@@ -670,7 +683,7 @@ __device__ __forceinline__ void kick_drift_t(const StepArgs<R>& a, unsigned s, c
         const R e2 = fma(en.w, en.w, fma(en.z, en.z, en.y * en.y));
         if (!frozen && (!(e2 < R(1)) || !(fabs(en.x) < R(1)))) bad = 1;
         const R yn = wrap_pi(yv[i] + asin_small(en.x));
-        const Qt<R> u{sqrt(R(1) - e2), en.y, en.z, en.w};
+        const Qt<R> u{sqrt_unit(R(1) - e2), en.y, en.z, en.w};
         Qt<R> qn = qmul(u, U[i]);
         const R inv = rsqrt_r(fma(qn.z, qn.z, fma(qn.y, qn.y, fma(qn.x, qn.x, qn.w * qn.w))));
         V4 qo;
