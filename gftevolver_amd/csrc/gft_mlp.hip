@@ -6,10 +6,14 @@
 //   bootstrap weights, so their spread measures the network's uncertainty).
 //
 // Every matrix product is one kernel, mlp_gemm: a wave computes a 16x16 tile
-// of C = A B with v_mfma_f32_16x16x16_bf16 (A, B in BF16, FP32 accumulate),
-// A and B addressed through (row, column) strides so the transposed products
-// of the backward pass need no copies.  The shapes are small (batch x 256 x
-// 256), so the kernel keeps operands in registers without LDS staging.
+// of C = A B with gfx950's v_mfma_f32_16x16x32_bf16 (A, B in BF16, FP32
+// accumulate; a 16-deep remainder of K with v_mfma_f32_16x16x16_bf16), A and B
+// addressed through (row, column) strides so the transposed products of the
+// backward pass need no copies.  Where an operand is contiguous along K (the
+// forward products and the products with the weights) a lane reads its 8 K
+// values as one 16-byte load; the transposed operands of the weight gradients
+// are gathered.  The shapes are small (batch x 256 x 256), so the kernel keeps
+// operands in registers without LDS staging.
 // Master weights, gradients and Adam moments are FP32; the BF16 copies the
 // GEMMs read are refreshed after each update.  All reductions (bias
 // gradients, loss) run in a fixed order, so training is deterministic.
@@ -18,6 +22,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <random>
 #include <string>
@@ -26,6 +31,8 @@
 namespace gft {
 
 typedef short mlp_bf16x4 __attribute__((ext_vector_type(4)));
+typedef short mlp_bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 mlp_bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float mlp_f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ unsigned short mlp_f2bf(float f) {   // round to nearest even
@@ -59,9 +66,27 @@ struct MlpGemm {
     long long ldh;
 };
 
-// One wave per 16x16 output tile, four waves per block.  Lane l supplies
-// A(row l%16, k0 + 4(l/16) + j) and B(k0 + 4(l/16) + j, col l%16), j = 0..3,
-// and receives D(4(l/16) + j, l%16).
+// The 8 K values k0 + 8 q .. k0 + 8 q + 7 of a row (A) or column (B) of an
+// operand whose K stride is s: one 16-byte load when VEC (s = 1, 16-byte
+// aligned), else gathered.
+template <bool VEC>
+__device__ __forceinline__ mlp_bf16x8 mlp_load8(const unsigned short* p, long long s, int k) {
+    if constexpr (VEC) {
+        return *reinterpret_cast<const mlp_bf16x8*>(p + k);
+    } else {
+        mlp_bf16x8 v;
+#pragma unroll
+        for (int j = 0; j < 8; j++) v[j] = static_cast<short>(p[(long long)(k + j) * s]);
+        return v;
+    }
+}
+
+// One wave per 16x16 output tile, four waves per block.  For each 32-deep slice
+// of K, lane l supplies A(row l%16, k0 + 8(l/16) + j) and B(k0 + 8(l/16) + j,
+// col l%16), j = 0..7 (any assignment of the slice's K values to (l/16, j) that
+// A and B share gives the same products); a 16-deep remainder uses 4 values
+// per lane.  Lane l receives D(4(l/16) + j, l%16), j = 0..3.
+template <bool VA, bool VB>
 __global__ void __launch_bounds__(256) mlp_gemm(MlpGemm g) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int tiles_n = g.N >> 4;
@@ -72,11 +97,18 @@ __global__ void __launch_bounds__(256) mlp_gemm(MlpGemm g) {
     mlp_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
     const unsigned short* Ar = g.A + (long long)(tm * 16 + r) * g.sa_m;
     const unsigned short* Bc = g.B + (long long)(tn * 16 + r) * g.sb_n;
-    for (int k0 = 0; k0 < g.K; k0 += 16) {
+    const int K32 = g.K & ~31;
+    for (int k0 = 0; k0 < K32; k0 += 32) {
+        const mlp_bf16x8 a = mlp_load8<VA>(Ar, g.sa_k, k0 + 8 * q);
+        const mlp_bf16x8 b = mlp_load8<VB>(Bc, g.sb_k, k0 + 8 * q);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(mlp_bf16x8_t, a),
+                                                      __builtin_bit_cast(mlp_bf16x8_t, b), acc, 0, 0, 0);
+    }
+    if (K32 < g.K) {   // K is a multiple of 16: one 16-deep remainder
         mlp_bf16x4 a, b;
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-            const long long k = k0 + 4 * q + j;
+            const long long k = K32 + 4 * q + j;
             a[j] = static_cast<short>(Ar[k * g.sa_k]);
             b[j] = static_cast<short>(Bc[k * g.sb_k]);
         }
@@ -254,7 +286,17 @@ void mlp_launch_gemm(gft_mlp* m, const unsigned short* A, long long sa_m, long l
                      const float* bias, unsigned short* H, long long ldh) {
     gft::MlpGemm g{A, sa_m, sa_k, B, sb_k, sb_n, M, N, K, mode, C, ldc, bias, H, ldh};
     const long long tiles = (long long)(M / 16) * (N / 16);
-    gft::mlp_gemm<<<(unsigned)((tiles + 3) / 4), 256, 0, m->st>>>(g);
+    // An operand takes the 16-byte loads when it is contiguous along K and every
+    // row's (column's) start is 16-byte aligned.
+    auto vec = [](const unsigned short* p, long long s_k, long long s_other) {
+        return s_k == 1 && s_other % 8 == 0 && reinterpret_cast<uintptr_t>(p) % 16 == 0;
+    };
+    const bool va = vec(A, sa_k, sa_m), vb = vec(B, sb_k, sb_n);
+    const dim3 grid((unsigned)((tiles + 3) / 4)), block(256);
+    if (va && vb) gft::mlp_gemm<true, true><<<grid, block, 0, m->st>>>(g);
+    else if (va) gft::mlp_gemm<true, false><<<grid, block, 0, m->st>>>(g);
+    else if (vb) gft::mlp_gemm<false, true><<<grid, block, 0, m->st>>>(g);
+    else gft::mlp_gemm<false, false><<<grid, block, 0, m->st>>>(g);
 }
 
 // Forward pass of a batch already gathered into xb (batch a multiple of 16).
