@@ -214,7 +214,7 @@ class Evolver:
         self._check(_L.gft_wilson_loops(self._h, C.byref(o)))
         return o
 
-    def set_launch(self, bz: int, by: int, order: int = 2, min_waves: int = 4, stage: int = 2):
+    def set_launch(self, bz: int, by: int, order: int = 3, min_waves: int = 4, stage: int = 2):
         self._check(_L.gft_set_launch(self._h, bz, by, int(order), min_waves, int(stage)))
 
     def sync(self):
