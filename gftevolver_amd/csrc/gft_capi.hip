@@ -77,6 +77,7 @@ struct gft_ctx {
     // step-kernel timing
     std::vector<hipEvent_t> ev_start, ev_stop;
     int ev_used = 0;
+    bool seg_open = false;   // a timing segment's start event is queued
     double step_ms_total = 0.0;
     unsigned step_count = 0;
 };
@@ -450,7 +451,12 @@ int gft_export_two_slot_f64(gft_ctx* ctx, double* s, double* v, unsigned* it) {
 
 // ---- evolution ----------------------------------------------------------------
 
+// Step timing: one pair of HIP events brackets each run of consecutive step
+// launches (a segment), closed before any other kernel is queued; no event
+// sits between two steps of a segment, so the steps follow each other as they
+// would untimed.  avg = sum of segment times / steps.
 static int timed_begin(gft_ctx* ctx) {
+    if (ctx->seg_open) return GFT_OK;
     if (ctx->ev_used == kEventPool) {
         CK(hipStreamSynchronize(ctx->stream));
         for (int i = 0; i < kEventPool; i++) {
@@ -461,13 +467,15 @@ static int timed_begin(gft_ctx* ctx) {
         ctx->ev_used = 0;
     }
     CK(hipEventRecord(ctx->ev_start[ctx->ev_used], ctx->stream));
+    ctx->seg_open = true;
     return GFT_OK;
 }
 
 static int timed_end(gft_ctx* ctx) {
+    if (!ctx->seg_open) return GFT_OK;
     CK(hipEventRecord(ctx->ev_stop[ctx->ev_used], ctx->stream));
     ctx->ev_used++;
-    ctx->step_count++;
+    ctx->seg_open = false;
     return GFT_OK;
 }
 
@@ -587,6 +595,7 @@ extern "C" int gft_evolve(gft_ctx* ctx, unsigned nsteps, unsigned energy_freq, u
     ctx->step_ms_total = 0.0;
     ctx->step_count = 0;
     ctx->ev_used = 0;
+    ctx->seg_open = false;
     for (unsigned k = 0; k < nsteps; k++) {
         const unsigned it = ctx->time_iter;
         const unsigned now = (it + 1) % 2;
@@ -596,6 +605,7 @@ extern "C" int gft_evolve(gft_ctx* ctx, unsigned nsteps, unsigned energy_freq, u
         const double damping_step = 0.5 * (double)damp * ctx->desc.dt;
         if (me && !two) {
             // The momentum form measures the state before the update in its own pass.
+            if ((rc = timed_end(ctx))) return rc;
             if ((rc = momentum_observe(ctx->mom, ctx->coef, MomObs::Energy, ctx->partial, nblocks, ctx->stream, ctx->err)))
                 return rc;
             finalize_partials<<<1, kFinThreads, 0, ctx->stream>>>(ctx->partial, nblocks, 5, 5, 0, ctx->obs_dev);
@@ -607,7 +617,9 @@ extern "C" int gft_evolve(gft_ctx* ctx, unsigned nsteps, unsigned energy_freq, u
         } else {
             if ((rc = momentum_step(ctx->mom, ctx->coef, damping_step, ctx->stream, ctx->err_dev, ctx->err))) return rc;
         }
-        if ((rc = timed_end(ctx))) return rc;
+        ctx->step_count++;
+        if ((me && two) || mg)
+            if ((rc = timed_end(ctx))) return rc;
         if (me && two) finalize_partials<<<1, kFinThreads, 0, ctx->stream>>>(ctx->partial, nblocks, 5, 5, 0, ctx->obs_dev);
         if (mg) {
             if (two)
@@ -624,6 +636,7 @@ extern "C" int gft_evolve(gft_ctx* ctx, unsigned nsteps, unsigned energy_freq, u
         }
         ctx->time_iter++;
     }
+    if ((rc = timed_end(ctx))) return rc;
     if ((rc = flush_timing(ctx))) return rc;
     if (nrows) *nrows = row;
     unsigned bad = 0;

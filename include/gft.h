@@ -332,9 +332,11 @@ int gft_init_gaussian(gft_ctx* ctx, const gft_gauss_desc* g);
 /* Wait for all queued work on the context's stream. */
 int gft_sync(gft_ctx* ctx);
 
-/* Timing: average duration (ms) of the step kernel over the last gft_evolve
- * call, from HIP events recorded on the context's own stream around every
- * step launch.  Returns GFT_EINVAL when no step ran. */
+/* Timing: average duration (ms) of a step over the last gft_evolve call, from
+ * HIP events on the context's own stream that bracket each run of consecutive
+ * step launches (no event between two steps, so the steps run as they would
+ * untimed; the average is the step kernel's duration plus the gaps between
+ * consecutive steps).  Returns GFT_EINVAL when no step ran. */
 int gft_last_step_ms(gft_ctx* ctx, double* avg_ms, double* total_ms, unsigned* nsteps);
 
 #ifdef __cplusplus
