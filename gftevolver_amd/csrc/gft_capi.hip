@@ -559,16 +559,31 @@ static void twoslot_dispatch(gft_ctx* ctx, long long nblocks, unsigned now, bool
                    : twoslot_step_as<T, GEN, 4>(ctx, nblocks, now, me, damping_step); break;
     }
 }
+template <class T, int NS>
+static void global_step_as(gft_ctx* ctx, long long nblocks, unsigned now, bool me, double damping_step) {
+    // nblocks logical blocks over 8 XCD ranges (the kernel's block map).
+    twoslot_step_global<T, NS><<<8 * ((nblocks + 7) / 8), kBlock, 0, ctx->stream>>>(
+        ctx->coef, (T*)ctx->S, now, me, damping_step, ctx->partial, static_cast<unsigned>(nblocks));
+}
+// The global model's kernel for its component count (1..8, checked by gft_create).
+template <class T>
+static void global_step(gft_ctx* ctx, long long nblocks, unsigned now, bool me, double damping_step) {
+    switch (ctx->coef.ns) {
+    case 1: global_step_as<T, 1>(ctx, nblocks, now, me, damping_step); break;
+    case 2: global_step_as<T, 2>(ctx, nblocks, now, me, damping_step); break;
+    case 3: global_step_as<T, 3>(ctx, nblocks, now, me, damping_step); break;
+    case 4: global_step_as<T, 4>(ctx, nblocks, now, me, damping_step); break;
+    case 5: global_step_as<T, 5>(ctx, nblocks, now, me, damping_step); break;
+    case 6: global_step_as<T, 6>(ctx, nblocks, now, me, damping_step); break;
+    case 7: global_step_as<T, 7>(ctx, nblocks, now, me, damping_step); break;
+    default: global_step_as<T, 8>(ctx, nblocks, now, me, damping_step); break;
+    }
+}
 static void twoslot_launch(gft_ctx* ctx, long long nblocks, unsigned now, bool gauss, bool me, double damping_step) {
     const bool f64 = ctx->form == GFT_FORM_TWO_SLOT_F64, gen = ctx->desc.gen_rep != 0;
     if (ctx->desc.model == GFT_MODEL_GLOBAL) {   // no links, no Gauss law
         if (gauss) return;
-        if (f64)
-            twoslot_step_global<double><<<nblocks, kBlock, 0, ctx->stream>>>(ctx->coef, (double*)ctx->S, now, me,
-                                                                             damping_step, ctx->partial);
-        else
-            twoslot_step_global<float><<<nblocks, kBlock, 0, ctx->stream>>>(ctx->coef, (float*)ctx->S, now, me,
-                                                                            damping_step, ctx->partial);
+        (f64 ? global_step<double> : global_step<float>)(ctx, nblocks, now, me, damping_step);
         return;
     }
     if (f64) (gen ? twoslot_dispatch<double, true> : twoslot_dispatch<double, false>)(ctx, nblocks, now, gauss, me,

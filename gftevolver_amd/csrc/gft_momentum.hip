@@ -1231,8 +1231,8 @@ __global__ void __launch_bounds__(256, MINW > 0 ? MINW : 1) mom_step(StepArgs<R>
         // Stage 2: the wave issues at raised priority until its own-state loads
         // are out (mom_block_shared drops it), so a wave starting a tile gets
         // its memory requests in ahead of the waves that are computing (1.5%
-        // faster at 256^3; the ensemble kernel measured 0.5% slower with it and
-        // keeps the default).  Priority changes issue order only, not results.
+        // faster at 256^3; the ensemble kernel, with 16 bands per member, 3.5%
+        // faster).  Priority changes issue order only, not results.
         if constexpr (PL == 2) __builtin_amdgcn_s_setprio(3);
         plane_fill<ND>(a, pl, t.y, t.z * a.by, t.w * a.bz, a.bz, a.by);
         pl.r0 = (ty + 1) * pl.w + tz + 1;
@@ -1292,6 +1292,9 @@ __global__ void __launch_bounds__(256, MINW) ens_step(const EnsMember* __restric
             extern __shared__ float4 gft_plane_lds[];
             char* lds = reinterpret_cast<char*>(gft_plane_lds);
             PlaneLds pl = plane_layout<1>(lds, a.bz, a.by);
+            // Raised issue priority until the own-state loads are out, as in
+            // mom_step (mom_block_shared drops it in every wave).
+            if constexpr (PL == 2) __builtin_amdgcn_s_setprio(3);
             plane_fill<1>(a, pl, t.y, t.z * a.by, t.w * a.bz, a.bz, a.by);
             pl.r0 = (ty + 1) * pl.w + tz + 1;
             if constexpr (PL == 2)
@@ -1612,14 +1615,15 @@ inline std::vector<int4> build_tiles(int member, int nx, int ny, int nz, int bz,
     return t;
 }
 
-// Tiles of one member in band-major order: 8 bands of y-tiles, and inside a
+// Tiles of one member in band-major order: 16 bands of y-tiles, and inside a
 // band x-planes in order with the band's (y, z) tiles inside a plane.  Split
 // into 8 contiguous ranges (the ensemble kernel's XCD assignment), a large
-// member gives each XCD one y-band to sweep along x, as block order 2 does
-// for a single lattice, so the x +- 1 planes a tile reads stay in that XCD's L2.
+// member gives each XCD two y-bands to sweep along x one after the other, as
+// block order 3 does for a single lattice, so the x +- 1 planes a tile reads
+// stay in that XCD's L2.
 inline std::vector<int4> build_tiles_banded(int member, int nx, int ny, int nz, int bz, int by) {
     const int nbz = (nz + bz - 1) / bz, nby = (ny + by - 1) / by;
-    const int br = (nby + 7) / 8;
+    const int br = (nby + 15) / 16;
     std::vector<int4> t;
     t.reserve((size_t)nbz * nby * nx);
     for (int b0 = 0; b0 < nby; b0 += br)

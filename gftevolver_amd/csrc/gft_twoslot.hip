@@ -436,61 +436,146 @@ __global__ void __launch_bounds__(256) twoslot_step(Coef c, T* __restrict__ S, T
     if (measure) block_reduce_store<5, 0>(dens, nullptr, lds, partial, blockIdx.x);
 }
 
+// The NS stored components of one site as doubles, in as few loads as the
+// site's alignment allows (a site starts at a multiple of NS elements of an
+// allocation aligned to 256 bytes): 16-byte loads when NS * sizeof(T) is a
+// multiple of 16, 8-byte loads when it is a multiple of 8, else one per value.
+template <class T, int NS>
+__device__ __forceinline__ void load_site(const T* p, double* o) {
+    constexpr int B = NS * static_cast<int>(sizeof(T));
+    if constexpr (sizeof(T) == 4 && B % 16 == 0) {
+#pragma unroll
+        for (int v = 0; v < NS / 4; v++) {
+            const float4 w = reinterpret_cast<const float4*>(p)[v];
+            o[4 * v] = w.x; o[4 * v + 1] = w.y; o[4 * v + 2] = w.z; o[4 * v + 3] = w.w;
+        }
+    } else if constexpr (sizeof(T) == 4 && B % 8 == 0) {
+#pragma unroll
+        for (int v = 0; v < NS / 2; v++) {
+            const float2 w = reinterpret_cast<const float2*>(p)[v];
+            o[2 * v] = w.x; o[2 * v + 1] = w.y;
+        }
+    } else if constexpr (sizeof(T) == 8 && B % 16 == 0) {
+#pragma unroll
+        for (int v = 0; v < NS / 2; v++) {
+            const double2 w = reinterpret_cast<const double2*>(p)[v];
+            o[2 * v] = w.x; o[2 * v + 1] = w.y;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < NS; k++) o[k] = static_cast<double>(p[k]);
+    }
+}
+template <class T, int NS>
+__device__ __forceinline__ void store_site(const double* v, T* p) {
+    constexpr int B = NS * static_cast<int>(sizeof(T));
+    if constexpr (sizeof(T) == 4 && B % 16 == 0) {
+#pragma unroll
+        for (int w = 0; w < NS / 4; w++)
+            reinterpret_cast<float4*>(p)[w] = make_float4(static_cast<float>(v[4 * w]), static_cast<float>(v[4 * w + 1]),
+                                                          static_cast<float>(v[4 * w + 2]),
+                                                          static_cast<float>(v[4 * w + 3]));
+    } else if constexpr (sizeof(T) == 4 && B % 8 == 0) {
+#pragma unroll
+        for (int w = 0; w < NS / 2; w++)
+            reinterpret_cast<float2*>(p)[w] = make_float2(static_cast<float>(v[2 * w]), static_cast<float>(v[2 * w + 1]));
+    } else if constexpr (sizeof(T) == 8 && B % 16 == 0) {
+#pragma unroll
+        for (int w = 0; w < NS / 2; w++) reinterpret_cast<double2*>(p)[w] = make_double2(v[2 * w], v[2 * w + 1]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < NS; k++) p[k] = static_cast<T>(v[k]);
+    }
+}
+
 // One leapfrog step of the global O(N) model (Gradients/Source/Global.cpp with
 // the null Wilson loop, WilsonLoops/Source/WilsonLoop.cpp:22-85): the plain
 // Laplacian in the reference's order (component, axis, stencil point;
 // Global.cpp:118-135), the potential and Model::evolve's scalar leapfrog
 // (Model.cpp:345-356).  measure != 0 reduces the FP64 energy densities
 // (potential, gradient, kinetic; the magnetic and electric parts are zero).
-template <class T>
+// NS, the component count, is a template parameter: every loop unrolls, a
+// site's components travel in vector loads and stores, and each thread decodes
+// its coordinates once (two divisions) and steps to its six neighbours by
+// wrapped increments.  The FP64 arithmetic is the same sequence of operations
+// whatever NS and the loads, so the fields are bitwise those of the reference.
+// Logical block b (sites 256 b .. 256 b + 255, energy partial b) runs on XCD k
+// for b in [k nb / 8, (k + 1) nb / 8): each XCD sweeps a contiguous range of
+// rows, so a row's y +- 1 and x +- 1 neighbours are mostly in its own L2 (the
+// hardware deals consecutive blocks to the 8 XCDs in turn).  The grid is 8
+// times the longest range; a hardware block past its XCD's range exits.
+template <class T, int NS>
 __global__ void __launch_bounds__(256) twoslot_step_global(Coef c, T* __restrict__ S, unsigned now, int measure,
-                                                           double damping_step, double* __restrict__ partial) {
+                                                           double damping_step, double* __restrict__ partial,
+                                                           unsigned nb) {
 #pragma clang fp contract(off)
     __shared__ double lds[4 * 5];
-    const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned xcd = blockIdx.x % 8, idx = blockIdx.x / 8;
+    const unsigned b = static_cast<unsigned>(static_cast<unsigned long long>(xcd) * nb / 8) + idx;
+    if (b >= static_cast<unsigned>(static_cast<unsigned long long>(xcd + 1) * nb / 8)) return;
+    const unsigned s = b * blockDim.x + threadIdx.x;
     double dens[5] = {0, 0, 0, 0, 0};
-    if (s < c.nsites && !site_frozen(c, s)) {
-        const int Ns = c.ns;
-        const long long N = c.nsites;
-        const T* Sn = S + (long long)now * N * Ns;
-        T* sp = S + (long long)(!now) * N * Ns + s * Ns;
-        const T* sn = Sn + s * Ns;
-        long long nb[3][3];
-        for (int a = 0; a < 3; a++) {
-            nb[a][0] = site_shift(c, s, a, -1);
-            nb[a][1] = s;
-            nb[a][2] = site_shift(c, s, a, +1);
-        }
-        double phi[8], pot[8], der[8];
-        for (int k = 0; k < Ns; k++) phi[k] = static_cast<double>(sn[k]);
-        potential_derivative_ref(c, phi, pot);
-        for (int k = 0; k < Ns; k++) {
-            der[k] = 0.0;
-            for (int a = 0; a < 3; a++)
-                for (int st = 0; st < 3; st++) der[k] += c.lap[a][st] * static_cast<double>(Sn[nb[a][st] * Ns + k]);
-        }
-        if (measure) {
-            dens[0] = potential_energy(c, phi);
-            double ge = 0.0, ke = 0.0;
-            for (int k = 0; k < Ns; k++)
-                for (int a = 0; a < 3; a++) {
-                    const double dd = -c.est[a] * phi[k] + c.est[a] * static_cast<double>(Sn[nb[a][2] * Ns + k]);
-                    ge += dd * dd;
-                }
-            for (int k = 0; k < Ns; k++) {
-                const double dd = (static_cast<double>(sn[k]) - static_cast<double>(sp[k])) / c.dt;
-                ke += dd * dd;
+    const unsigned N = static_cast<unsigned>(c.nsites);
+    if (s < N) {
+        const unsigned nz = c.nz, ny = c.ny, nx = c.nx;
+        const unsigned t = s / nz, z = s - t * nz, x = t / ny, y = t - x * ny;
+        if (!site_frozen(c, static_cast<int>(x), static_cast<int>(y), static_cast<int>(z))) {
+            const unsigned sx = ny * nz;
+            // Neighbour offsets from s, in elements of one slot's site array.
+            const unsigned xm = x == 0 ? s + (nx - 1) * sx : s - sx, xp = x + 1 == nx ? s - (nx - 1) * sx : s + sx;
+            const unsigned ym = y == 0 ? s + (ny - 1) * nz : s - nz, yp = y + 1 == ny ? s - (ny - 1) * nz : s + nz;
+            const unsigned zm = z == 0 ? s + (nz - 1) : s - 1, zp = z + 1 == nz ? s - (nz - 1) : s + 1;
+            const unsigned nb[3][3] = {{xm, s, xp}, {ym, s, yp}, {zm, s, zp}};
+            const T* Sn = S + static_cast<size_t>(now) * N * NS;
+            T* sp = S + static_cast<size_t>(!now) * N * NS + static_cast<size_t>(s) * NS;
+            // phi and pot are 8 wide: the potential's two-doublet form reads 8.
+            double phi[8] = {0, 0, 0, 0, 0, 0, 0, 0}, old[NS], f[3][3][NS], pot[8], der[NS];
+            load_site<T, NS>(Sn + static_cast<size_t>(s) * NS, phi);
+            load_site<T, NS>(sp, old);
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                load_site<T, NS>(Sn + static_cast<size_t>(nb[a][0]) * NS, f[a][0]);
+                load_site<T, NS>(Sn + static_cast<size_t>(nb[a][2]) * NS, f[a][2]);
+#pragma unroll
+                for (int k = 0; k < NS; k++) f[a][1][k] = phi[k];
             }
-            dens[1] = 0.5 * ge;
-            dens[2] = 0.5 * ke;
-        }
-        for (int k = 0; k < Ns; k++) {
-            const double rhs = c.dt * c.dt * (der[k] - pot[k])
-                             - damping_step * (static_cast<double>(sn[k]) - static_cast<double>(sp[k]));
-            sp[k] = static_cast<T>(2 * static_cast<double>(sn[k]) - static_cast<double>(sp[k]) + rhs);
+            potential_derivative_ref(c, phi, pot);
+#pragma unroll
+            for (int k = 0; k < NS; k++) {
+                der[k] = 0.0;
+#pragma unroll
+                for (int a = 0; a < 3; a++)
+#pragma unroll
+                    for (int st = 0; st < 3; st++) der[k] += c.lap[a][st] * f[a][st][k];
+            }
+            if (measure) {
+                dens[0] = potential_energy(c, phi);
+                double ge = 0.0, ke = 0.0;
+#pragma unroll
+                for (int k = 0; k < NS; k++)
+#pragma unroll
+                    for (int a = 0; a < 3; a++) {
+                        const double dd = -c.est[a] * phi[k] + c.est[a] * f[a][2][k];
+                        ge += dd * dd;
+                    }
+#pragma unroll
+                for (int k = 0; k < NS; k++) {
+                    const double dd = (phi[k] - old[k]) / c.dt;
+                    ke += dd * dd;
+                }
+                dens[1] = 0.5 * ge;
+                dens[2] = 0.5 * ke;
+            }
+            double nw[NS];
+#pragma unroll
+            for (int k = 0; k < NS; k++) {
+                const double rhs = c.dt * c.dt * (der[k] - pot[k]) - damping_step * (phi[k] - old[k]);
+                nw[k] = 2 * phi[k] - old[k] + rhs;
+            }
+            store_site<T, NS>(nw, sp);
         }
     }
-    if (measure) block_reduce_store<5, 0>(dens, nullptr, lds, partial, blockIdx.x);
+    if (measure) block_reduce_store<5, 0>(dens, nullptr, lds, partial, b);
 }
 
 // Gauss law after a step (slot `now` = t, slot !now = t+dt), FP64
