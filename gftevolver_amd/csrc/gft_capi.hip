@@ -161,6 +161,7 @@ static Coef make_coef(const gft_desc& d) {
     c.dvol = d.dx * d.dy * d.dz;
     c.fixed = d.fixed_faces;
     c.tol = d.div_tol;
+    c.G = gauge_num(d);
     return c;
 }
 
@@ -227,9 +228,6 @@ int gft_create(const gft_desc* desc, gft_ctx** out) {
     const bool family = gauge_num(d) == 5 || gauge_num(d) == 7;
     if (family && d.model != GFT_MODEL_2HDM)
         return fail(nullptr, GFT_EINVAL, "gft_create: a Higgs-family gauge field needs two doublets (GFT_MODEL_2HDM)");
-    if (family && !is_two_slot(d.form))
-        return fail(nullptr, GFT_EINVAL, "gft_create: gaugeNum 5 and 7 run in the two-slot forms (the momentum "
-                                         "forms carry the SU(2) x U(1) links only)");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
         return fail(nullptr, GFT_ENODEV, "gft_create: no HIP device");
@@ -351,8 +349,18 @@ static int import_two_slot(gft_ctx* ctx, const Src* scalar, const Src* vec, unsi
     const size_t ns = 2 * N * ctx->coef.ns, nv = 2 * N * link_reals(ctx->desc);
     if (!is_two_slot(ctx->form) && ctx->desc.gen_rep) {
         // Momentum forms: the links enter as FP64 quaternions, the scalars exactly.
-        std::vector<double> s(scalar, scalar + ns), v(2 * N * 15);
-        for (long long l = 0; l < 2 * N * 3; l++) gen_to_quat(vec + 4 * l, ctx->coef.tol, v.data() + 5 * l);
+        const int G = gauge_num(ctx->desc), vg = G == 5 ? 5 : G == 7 ? 7 : 4, vq = G == 5 ? 6 : G == 7 ? 9 : 5;
+        std::vector<double> s(scalar, scalar + ns), v(2 * N * 3 * vq);
+        for (long long l = 0; l < 2 * N * 3; l++) {
+            gen_to_quat(vec + vg * l, ctx->coef.tol, v.data() + vq * l);
+            if (G == 5) v[vq * l + 5] = (double)vec[vg * l + 4];
+            if (G == 7) {   // the family SU(2) from its generators w' (positions 4..6)
+                const Src w[4] = {Src(0), vec[vg * l + 4], vec[vg * l + 5], vec[vg * l + 6]};
+                double q[5];
+                gen_to_quat(w, ctx->coef.tol, q);
+                for (int k = 0; k < 4; k++) v[vq * l + 5 + k] = q[1 + k];
+            }
+        }
         int rc = momentum_import(ctx->mom, ctx->coef, s.data(), v.data(), time_iter, ctx->stream, ctx->err);
         if (rc) return rc;
         CK(hipStreamSynchronize(ctx->stream));
@@ -392,11 +400,21 @@ static int export_two_slot(gft_ctx* ctx, Dst* scalar, Dst* vec, unsigned* time_i
     const long long N = ctx->N;
     const size_t ns = 2 * N * ctx->coef.ns, nv = 2 * N * link_reals(ctx->desc);
     if (!is_two_slot(ctx->form) && ctx->desc.gen_rep) {
-        std::vector<double> s(ns), v(2 * N * 15);
+        const int G = gauge_num(ctx->desc), vg = G == 5 ? 5 : G == 7 ? 7 : 4, vq = G == 5 ? 6 : G == 7 ? 9 : 5;
+        std::vector<double> s(ns), v(2 * N * 3 * vq);
         int rc = momentum_export(ctx->mom, ctx->coef, s.data(), v.data(), ctx->time_iter, ctx->stream, ctx->err);
         if (rc) return rc;
         for (size_t i = 0; i < ns; i++) scalar[i] = (Dst)s[i];
-        for (long long l = 0; l < 2 * N * 3; l++) quat_to_gen(v.data() + 5 * l, ctx->coef.tol, vec + 4 * l);
+        for (long long l = 0; l < 2 * N * 3; l++) {
+            quat_to_gen(v.data() + vq * l, ctx->coef.tol, vec + vg * l);
+            if (G == 5) vec[vg * l + 4] = (Dst)v[vq * l + 5];
+            if (G == 7) {
+                const double q[5] = {0.0, v[vq * l + 5], v[vq * l + 6], v[vq * l + 7], v[vq * l + 8]};
+                Dst w[4];
+                quat_to_gen(q, ctx->coef.tol, w);
+                for (int k = 0; k < 3; k++) vec[vg * l + 4 + k] = w[1 + k];
+            }
+        }
         if (time_iter) *time_iter = ctx->time_iter;
         return GFT_OK;
     }
@@ -1311,6 +1329,22 @@ __global__ void __launch_bounds__(256) grf_assemble_mom(MomFields<R> f, const do
     }
     if (f.lo) f.lo[s] = make_uint2(0u, 0u);   // P = 0, E = 0: no residuals
     if (f.lo_P2) f.lo_P2[s] = 0;
+    // Higgs-family links at the identity with no electric field.
+    for (int i = 0; i < 3; i++) {
+        if (f.fy[0][i]) {
+            f.fy[0][i][s] = R(0);
+            f.fe[i][s] = R(0);
+        }
+        if (f.fq[0][i]) {
+            V4 e, q;
+            e.x = e.y = e.z = e.w = R(0);
+            q.x = R(1); q.y = q.z = q.w = R(0);
+            f.fq[0][i][s] = q;
+            f.fE[i][s] = e;
+        }
+    }
+    if (f.flo5) f.flo5[s] = 0u;
+    if (f.flo7) f.flo7[s] = make_uint2(0u, 0u);
 }
 
 template <class T>

@@ -41,6 +41,8 @@ struct Coef {
     double pot_offset;
     double dvol;
     int fixed;                   // fixed faces: bit 2a low, 2a+1 high face of axis a (0: periodic)
+    int G;                       // Gauss generators: 4 (SU(2) x U(1)), 5 or 7 (2HDM with the
+                                 // Higgs-family U(1) or SU(2)), 0 without links
     double tol;                  // division-by-zero tolerance of the generator-form conversions
 };
 

@@ -60,6 +60,16 @@ struct MomFields {
     // and those of the second doublet's P (2HDM) in lo_P2.
     uint2* lo;
     unsigned short* lo_P2;
+    // Higgs-family gauge field of the 2HDM (null unless gaugeNum is 5 or 7):
+    R* fy[2][3];        // gaugeNum 5: family U(1) phases y', wrapped   (ping-pong)
+    R* fe[3];           // gaugeNum 5: E' of the last update            (in place)
+    V4* fq[2][3];       // gaugeNum 7: family SU(2) quaternions         (ping-pong)
+    V4* fE[3];          // gaugeNum 7: (0, E'^1, E'^2, E'^3) of the last update (in place)
+    // Compensated storage: the residuals of the family E.  gaugeNum 5: nibble a
+    // of flo5 = E'[a]; gaugeNum 7: 16-bit group a of flo7 (x low, x high, y
+    // low) holds the residuals of fE[a] in nibbles 1..3 (nibble m = value m).
+    unsigned* flo5;
+    uint2* flo7;
 };
 
 struct MomentumState {
@@ -210,6 +220,8 @@ __device__ __forceinline__ void st_nt(const uint2& v, uint2* p) {
 }
 __device__ __forceinline__ void st_nt(float v, float* p) { __builtin_nontemporal_store(v, p); }
 __device__ __forceinline__ unsigned ld_nt(const unsigned* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ float ld_nt(const float* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ double ld_nt(const double* p) { return __builtin_nontemporal_load(p); }
 __device__ __forceinline__ void st_nt(unsigned v, unsigned* p) { __builtin_nontemporal_store(v, p); }
 __device__ __forceinline__ void st_nt(double v, double* p) { __builtin_nontemporal_store(v, p); }
 __device__ __forceinline__ unsigned short ld_nt(const unsigned short* p) { return __builtin_nontemporal_load(p); }
@@ -379,6 +391,71 @@ __device__ __forceinline__ void transport(const Qt<R>& u, R c, R s, bool conj, c
     act(conj ? qconj(u) : u, t, o);
 }
 
+// The Higgs-family part of one link of the 2HDM: gaugeNum 5 a phase y' given
+// by (c, s) = (cos y', sin y'), gaugeNum 7 an SU(2) quaternion v.
+template <class R>
+struct FamLink {
+    R c, s;
+    Qt<R> v;
+};
+
+// T phi for both doublets (8 reals): the U(1) phase and SU(2) of transport on
+// each doublet, then the Higgs-family part (Gradients/Source/TwoHDM.cpp:
+// 106-141): gaugeNum 5 the phase y' with charge +1 on the first doublet and -1
+// on the second, gaugeNum 7 the SU(2) v acting on the doublet index of
+// (f_i, f_i+1, f_i+4, f_i+5), i = 0, 2 (the real 4x4 action of act).  conj
+// gives T^dag.  The three factors commute.
+template <class R, int G>
+__device__ __forceinline__ void transport2(const Qt<R>& u, R c, R s, const FamLink<R>& f, bool conj, const R* h,
+                                           R* o) {
+#pragma clang fp contract(off)
+    R t[8];
+    transport(u, c, s, conj, h, t);
+    transport(u, c, s, conj, h + 4, t + 4);
+    if constexpr (G == 5) {
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            const R ss = (conj != (k == 1)) ? -f.s : f.s;
+            const R* p = t + 4 * k;
+            R* q = o + 4 * k;
+            q[0] = fma_r(f.c, p[0], -ss * p[1]);
+            q[1] = fma_r(f.c, p[1], ss * p[0]);
+            q[2] = fma_r(f.c, p[2], -ss * p[3]);
+            q[3] = fma_r(f.c, p[3], ss * p[2]);
+        }
+    } else {
+        const Qt<R> v = conj ? qconj(f.v) : f.v;
+#pragma unroll
+        for (int i = 0; i < 4; i += 2) {
+            const R hi[4] = {t[i], t[i + 1], t[i + 4], t[i + 5]};
+            R ho[4];
+            act(v, hi, ho);
+            o[i] = ho[0];
+            o[i + 1] = ho[1];
+            o[i + 4] = ho[2];
+            o[i + 5] = ho[3];
+        }
+    }
+}
+
+// Higgs-family bilinears of both doublets' t and n (8 reals each): the family
+// currents and charges (Gradients/Source/TwoHDM.cpp:401-424, 299-319),
+// accumulated into j[0] (gaugeNum 5) or j[0..2] (gaugeNum 7).  Antisymmetric:
+// they vanish for n = t, like the bilinears of bilinears_add.
+template <class R, int G>
+__device__ __forceinline__ void family_bilinears(const R* t, const R* n, R* j) {
+#pragma unroll
+    for (int i = 0; i < 4; i += 2) {
+        if constexpr (G == 5) {
+            j[0] += R(0.25) * (t[i] * n[i + 1] - t[i + 1] * n[i] - t[i + 4] * n[i + 5] + t[i + 5] * n[i + 4]);
+        } else {
+            j[0] += R(0.25) * (t[i] * n[i + 5] - t[i + 1] * n[i + 4] + t[i + 4] * n[i + 1] - t[i + 5] * n[i]);
+            j[1] += R(0.25) * (-t[i] * n[i + 4] - t[i + 1] * n[i + 5] + t[i + 4] * n[i] + t[i + 5] * n[i + 1]);
+            j[2] += R(0.25) * (t[i] * n[i + 1] - t[i + 1] * n[i] - t[i + 4] * n[i + 5] + t[i + 5] * n[i + 4]);
+        }
+    }
+}
+
 template <class V4, class R>
 __device__ __forceinline__ void unpack(const V4& v, R* o) { o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w; }
 template <class V4, class R>
@@ -404,6 +481,16 @@ struct StepArgs {
     V4* E[3];
     uint2* lo;               // compensated storage: residuals of P and E (see MomFields)
     unsigned short* lo_P2;
+    // Higgs-family gauge field (gaugeNum 5 or 7; see MomFields).
+    const R* fy[3];
+    R* fy_out[3];
+    R* fe[3];
+    const V4* fq[3];
+    V4* fq_out[3];
+    V4* fE[3];
+    unsigned* flo5;
+    uint2* flo7;
+    R gsq_f;                 // g''^2 (the family generators' coupling, rounded through float)
     V4 ref[2];               // reference doublets: phi = stored + ref
     int nx, ny, nz, nbz, nby;
     long long nblocks;
@@ -724,13 +811,111 @@ __device__ __forceinline__ void kick_drift(const StepArgs<R>& a, unsigned s, con
     }
 }
 
+// Kick and drift of the Higgs-family links of site s (gaugeNum 5: phases y'
+// with E' = sin(y'(t+dt) - y'(t)); gaugeNum 7: quaternions with
+// E' = vec(V(t+dt) V(t)^dag)), from their forces MF (staples + g''^2 J'), as the
+// SU(2) x U(1) links in kick_drift_t (WilsonLoops/Source/TwoHDM.cpp:645-702).
+template <class R, int G, bool CMP, bool U1, bool FZ>
+__device__ __forceinline__ void kick_drift_family_t(const StepArgs<R>& a, unsigned s, const R* fyv, const Qt<R>* FV,
+                                                    const R (*MF)[4], bool frozen) {
+    using V4 = typename V4T<R>::type;
+    if constexpr (!FZ) frozen = false;
+    unsigned bad = 0;
+    if constexpr (G == 5) {
+        unsigned lw = 0u, nw = 0u;
+        if constexpr (CMP) lw = ld_nt(ptr_at(a.flo5, s));
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const R e = ld_nt(ptr_at(a.fe[i], s));
+            R en;
+            if constexpr (CMP) {
+                float r;
+                if constexpr (U1) en = cmp_update1(e, res_val(lw, i, e), a.dt2 * MF[i][0], &r);
+                else en = cmp_update(e, res_val(lw, i, e), a.keep, a.dt2 * MF[i][0], &r);
+                nw |= (static_cast<unsigned>(res_q(r, en)) & 0xfu) << (4 * i);
+            } else {
+                en = fma(a.dt2, MF[i][0], U1 ? e : a.keep * e);
+            }
+            if (!frozen && !(fabs(en) < R(1))) bad = 1;
+            const R yn = wrap_pi(fyv[i] + asin_small(en));
+            st_nt(frozen ? e : en, ptr_at(a.fe[i], s));
+            st_nt(frozen ? fyv[i] : yn, ptr_at(a.fy_out[i], s));
+        }
+        if constexpr (CMP) st_nt(frozen ? lw : nw, ptr_at(a.flo5, s));
+    } else {
+        uint2 lw = make_uint2(0u, 0u);
+        unsigned ng[3] = {0u, 0u, 0u};
+        if constexpr (CMP) lw = ld_nt(ptr_at(a.flo7, s));
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            R e[4];
+            unpack(ld_nt(ptr_at(a.fE[i], s)), e);
+            V4 en;
+            en.x = R(0);
+            if constexpr (CMP) {
+                const unsigned L = res_group(lw, i);
+                float eh[4], er[4];
+                eh[0] = er[0] = 0.f;
+#pragma unroll
+                for (int m = 1; m < 4; m++) {
+                    if constexpr (U1) eh[m] = cmp_update1(e[m], res_val(L, m, e[m]), a.dt2 * MF[i][m], &er[m]);
+                    else eh[m] = cmp_update(e[m], res_val(L, m, e[m]), a.keep, a.dt2 * MF[i][m], &er[m]);
+                }
+                en.y = eh[1];
+                en.z = eh[2];
+                en.w = eh[3];
+                ng[i] = frozen ? L : res_pack4(0, res_q(er[1], eh[1]), res_q(er[2], eh[2]), res_q(er[3], eh[3]));
+            } else {
+                en.y = fma(a.dt2, MF[i][1], U1 ? e[1] : a.keep * e[1]);
+                en.z = fma(a.dt2, MF[i][2], U1 ? e[2] : a.keep * e[2]);
+                en.w = fma(a.dt2, MF[i][3], U1 ? e[3] : a.keep * e[3]);
+            }
+            const R e2 = fma(en.w, en.w, fma(en.z, en.z, en.y * en.y));
+            if (!frozen && !(e2 < R(1))) bad = 1;
+            const Qt<R> u{sqrt_unit(R(1) - e2), en.y, en.z, en.w};
+            const Qt<R> qn = qmul(u, FV[i]);
+            const R inv = rsqrt_r(fma(qn.z, qn.z, fma(qn.y, qn.y, fma(qn.x, qn.x, qn.w * qn.w))));
+            V4 qo;
+            qo.x = qn.w * inv;
+            qo.y = qn.x * inv;
+            qo.z = qn.y * inv;
+            qo.w = qn.z * inv;
+            if (frozen) {
+                en = pack<V4, R>(e);
+                qo.x = FV[i].w;
+                qo.y = FV[i].x;
+                qo.z = FV[i].y;
+                qo.w = FV[i].z;
+            }
+            st_nt(en, ptr_at(a.fE[i], s));
+            st_nt(qo, ptr_at(a.fq_out[i], s));
+        }
+        if constexpr (CMP) st_nt(make_uint2(ng[0] | (ng[1] << 16), ng[2]), ptr_at(a.flo7, s));
+    }
+    if (bad) atomicOr(a.err, 1u);
+}
+template <class R, int G, bool CMP>
+__device__ __forceinline__ void kick_drift_family(const StepArgs<R>& a, unsigned s, const R* fyv, const Qt<R>* FV,
+                                                  const R (*MF)[4], bool frozen, bool any_fixed) {
+    if (!any_fixed) {
+        if (a.keep == R(1)) kick_drift_family_t<R, G, CMP, true, false>(a, s, fyv, FV, MF, false);
+        else kick_drift_family_t<R, G, CMP, false, false>(a, s, fyv, FV, MF, false);
+    } else {
+        if (a.keep == R(1)) kick_drift_family_t<R, G, CMP, true, true>(a, s, fyv, FV, MF, frozen);
+        else kick_drift_family_t<R, G, CMP, false, true>(a, s, fyv, FV, MF, frozen);
+    }
+}
+
 // One site-update at (x, y, z): forces, kick and drift (shared by the
 // single-lattice kernel and the ensemble kernel).  PL: in-plane neighbours and
 // own q, y, phi come from the block's LDS tile `pl` (FP32 only).
-template <class R, int ND, bool CMP, bool PL = false>
+// G: the gauge group (4: SU(2) x U(1); 5, 7: the 2HDM with the Higgs-family
+// U(1) or SU(2), whose links are read from global memory in every staging mode).
+template <class R, int ND, bool CMP, bool PL = false, int G = 4>
 __device__ __forceinline__ void mom_site(const StepArgs<R>& a, const Coef& c, int x, int y, int z,
                                          const PlaneLds& pl = PlaneLds{}) {
     using V4 = typename V4T<R>::type;
+    static_assert(G == 4 || ND == 2, "a Higgs-family field acts on two doublets");
 
     // 32-bit site offsets (the host limits a lattice to 2^28 sites): addresses
     // become a scalar base plus one 32-bit vector offset, with no 64-bit math.
@@ -803,12 +988,70 @@ __device__ __forceinline__ void mom_site(const StepArgs<R>& a, const Coef& c, in
         add_ref(a.ref[k], praw + 4 * k, phi + 4 * k);
     }
 
+    // Higgs-family links (global memory): phase y' or quaternion of direction f
+    // at x + si e_i + sj e_j, and the site's own.
+    auto FY = [&](int f, int i, int si, int j, int sj) -> R { return at(a.fy[f], gsite(i, si, j, sj)); };
+    auto FQ = [&](int f, int i, int si, int j, int sj) -> Qt<R> {
+        return toq<V4, R>(at(a.fq[f], gsite(i, si, j, sj)));
+    };
+    auto FAM = [&](int f, int i, int si) -> FamLink<R> {
+        FamLink<R> l{};
+        if constexpr (G == 5) sincos_fast(FY(f, i, si, 0, 0), &l.s, &l.c);
+        else if constexpr (G == 7) l.v = FQ(f, i, si, 0, 0);
+        return l;
+    };
+    R fyv[3] = {R(0), R(0), R(0)};
+    Qt<R> FV[3];
+    FamLink<R> FL[3];
+    if constexpr (G != 4) {
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            FL[i] = FAM(i, 0, 0);
+            if constexpr (G == 5) fyv[i] = FY(i, 0, 0, 0, 0);
+            else FV[i] = FL[i].v;
+        }
+    }
+
     // Scalar force and currents.
     R F[4 * ND];
     potential_derivative<R>(c, phi, F);
 #pragma unroll
     for (int k = 0; k < 4 * ND; k++) F[k] = -F[k];
     R J[3][4];
+    R JF[3][3];   // Higgs-family currents (G 5: [i][0]; 7: [i][0..2])
+    if constexpr (G != 4) {
+        // Both doublets at once: the family part of T mixes them for G = 7.
+        // The same covariant differences as below carry the Laplacian and all
+        // currents (every bilinear vanishes on equal arguments).
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const Qt<R> Um = Q(i, i, -1, 0, 0);
+            R cm, smn;
+            sincos_fast(Y(i, i, -1, 0, 0), &smn, &cm);
+            const FamLink<R> Fm = FAM(i, i, -1);
+            J[i][0] = J[i][1] = J[i][2] = J[i][3] = R(0);
+            JF[i][0] = JF[i][1] = JF[i][2] = R(0);
+            R fp[8], fm[8], tx[8], tm[8], d[8], dmv[8], dpv[8];
+#pragma unroll
+            for (int k = 0; k < 2; k++) {
+                PHI(k, i, 1, fp + 4 * k);
+                PHI(k, i, -1, fm + 4 * k);
+            }
+            transport2<R, G>(U[i], cy[i], sy_[i], FL[i], false, phi, tx);   // T_i(x) phi(x)
+            transport2<R, G>(Um, cm, smn, Fm, false, fm, tm);                // T_i(x-i) phi(x-i)
+#pragma unroll
+            for (int m = 0; m < 8; m++) {
+                d[m] = fp[m] - tx[m];
+                dmv[m] = phi[m] - tm[m];
+            }
+            transport2<R, G>(U[i], cy[i], sy_[i], FL[i], true, d, dpv);      // T_i(x)^dag d
+#pragma unroll
+            for (int m = 0; m < 8; m++) F[m] += a.inv_sp[i] * (dpv[m] - dmv[m]);
+            bilinears_add(tx, d, J[i]);
+            bilinears_add(tx + 4, d + 4, J[i]);
+            family_bilinears<R, G>(tx, d, JF[i]);
+        }
+    } else {
 #pragma unroll
     for (int i = 0; i < 3; i++) {
         const Qt<R> Um = Q(i, i, -1, 0, 0);
@@ -839,6 +1082,7 @@ __device__ __forceinline__ void mom_site(const StepArgs<R>& a, const Coef& c, in
             for (int m = 0; m < 4; m++) F[4 * k + m] += a.inv_sp[i] * (dpv[m] - dmv[m]);
             bilinears_add(tx, d, J[i]);
         }
+    }
     }
 
     // Magnetic force (staples, WilsonLoops/Source/StandardModel.cpp:324-409).
@@ -916,6 +1160,72 @@ __device__ __forceinline__ void mom_site(const StepArgs<R>& a, const Coef& c, in
     }
 
     kick_drift<R, ND, CMP>(a, s, U, yv, praw, F, M, nullptr, frozen, c.fixed != 0);
+
+    if constexpr (G != 4) {
+        // Higgs-family staples (WilsonLoops/Source/TwoHDM.cpp:496-552), in the
+        // pattern of the U(1) (G = 5) or SU(2) (G = 7) staples above, then the
+        // family source g''^2 J' and the family kick and drift.
+        R MF[3][4];
+#pragma unroll
+        for (int i = 0; i < 3; i++) MF[i][0] = MF[i][1] = MF[i][2] = MF[i][3] = R(0);
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+#pragma unroll
+            for (int j = i + 1; j < 3; j++) {
+                if constexpr (G == 5) {
+                    const R th = (FY(i, j, 1, 0, 0) - fyv[i]) + (fyv[j] - FY(j, i, 1, 0, 0));
+                    const R st = sin_fast(th);
+                    MF[i][0] += a.inv_sp[j] * st;
+                    MF[j][0] -= a.inv_sp[i] * st;
+                } else {
+                    const Qt<R> A = qmul(qconj(FQ(j, i, 1, 0, 0)), FQ(i, j, 1, 0, 0));
+                    const Qt<R> C = qmul(FV[j], qconj(FV[i]));
+                    R v[3], w[3];
+                    qvec(A, C, v);
+                    qrot(C, v, w);
+#pragma unroll
+                    for (int m = 0; m < 3; m++) {
+                        MF[i][1 + m] += a.inv_sp[j] * v[m];
+                        MF[j][1 + m] -= a.inv_sp[i] * w[m];
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                if (i == j) continue;
+                if constexpr (G == 5) {
+                    const R yim = FY(i, j, -1, 0, 0), yjm = FY(j, j, -1, 0, 0), yjd = FY(j, i, 1, j, -1);
+                    if (i < j) MF[i][0] -= a.inv_sp[j] * sin_fast((fyv[i] - yim) + (yjm - yjd));
+                    else MF[i][0] += a.inv_sp[j] * sin_fast((yjd - yjm) + (yim - fyv[i]));
+                } else {
+                    const Qt<R> Vim = FQ(i, j, -1, 0, 0), Vjm = FQ(j, j, -1, 0, 0), Vjd = FQ(j, i, 1, j, -1);
+                    R v[3], w[3];
+                    if (i < j) {
+                        qvec(qmul(qconj(Vjd), FV[i]), qmul(Vjm, qconj(Vim)), v);
+                        qrot(Vjd, v, w);
+#pragma unroll
+                        for (int m = 0; m < 3; m++) MF[i][1 + m] -= a.inv_sp[j] * w[m];
+                    } else {
+                        qvec(qmul(qconj(FV[i]), Vjd), qmul(Vim, qconj(Vjm)), v);
+                        qrot(FV[i], v, w);
+#pragma unroll
+                        for (int m = 0; m < 3; m++) MF[i][1 + m] += a.inv_sp[j] * w[m];
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            if constexpr (G == 5) MF[i][0] = fma(a.gsq_f, JF[i][0], MF[i][0]);
+            else
+#pragma unroll
+                for (int m = 0; m < 3; m++) MF[i][1 + m] = fma(a.gsq_f, JF[i][m], MF[i][1 + m]);
+        }
+        kick_drift_family<R, G, CMP>(a, s, fyv, FV, MF, frozen, c.fixed != 0);
+    }
 }
 
 // ---- shared plaquettes and covariant differences (FP32 SM, staged) ----------
@@ -1215,7 +1525,8 @@ __device__ __forceinline__ void mom_block_shared(const StepArgs<float>& a, const
 // PL: 0 every neighbour from global memory; 1 in-plane neighbours from the
 // block's LDS plane tile; 2 as 1, with the in-plane plaquettes and covariant
 // differences shared (FP32 SM, bz + by <= 256).  All give bitwise the same fields.
-template <class R, int ND, int MINW, bool CMP, int PL = 0>
+// G: gauge group (4, or 5 / 7 for the 2HDM with a Higgs-family field; PL <= 1).
+template <class R, int ND, int MINW, bool CMP, int PL = 0, int G = 4>
 __global__ void __launch_bounds__(256, MINW > 0 ? MINW : 1) mom_step(StepArgs<R> a, Coef c) {
     const int4 t = a.tiles[blockIdx.x];
     const int tz = static_cast<int>(threadIdx.x) & (a.bz - 1), ty = static_cast<int>(threadIdx.x) >> a.bz_shift;
@@ -1224,7 +1535,7 @@ __global__ void __launch_bounds__(256, MINW > 0 ? MINW : 1) mom_step(StepArgs<R>
     const bool valid = z < a.nz && y < a.ny;
     if constexpr (PL > 0) {
         static_assert(sizeof(R) == 4, "plane staging is FP32 only");
-        static_assert(PL == 1 || ND == 1, "shared plaquettes are for one doublet");
+        static_assert(PL == 1 || (ND == 1 && G == 4), "shared plaquettes are for one doublet");
         extern __shared__ float4 gft_plane_lds[];
         char* lds = reinterpret_cast<char*>(gft_plane_lds);
         PlaneLds pl = plane_layout<ND>(lds, a.bz, a.by);
@@ -1242,11 +1553,11 @@ __global__ void __launch_bounds__(256, MINW > 0 ? MINW : 1) mom_step(StepArgs<R>
             mom_block_shared<CMP, MINW != 4>(a, c, t.y, t.z * a.by, t.w * a.bz, ty, tz, valid, pl, lds);
         } else {
             if (!valid) return;
-            mom_site<R, ND, CMP, true>(a, c, t.y, y, z, pl);
+            mom_site<R, ND, CMP, true, G>(a, c, t.y, y, z, pl);
         }
     } else {
         if (!valid) return;
-        mom_site<R, ND, CMP>(a, c, t.y, y, z);
+        mom_site<R, ND, CMP, false, G>(a, c, t.y, y, z);
     }
 }
 
@@ -1320,6 +1631,15 @@ struct ObsArgs {
     const V4* E[3];
     const uint2* lo;                 // residuals of compensated storage (null otherwise)
     const unsigned short* lo_P2;
+    // Higgs-family field (gaugeNum 5 or 7; see MomFields): current, and the
+    // family SU(2) one step earlier (Gauss only).
+    const R* fy[3];
+    const R* fe[3];
+    const V4* fq[3];
+    const V4* fq_old[3];
+    const V4* fE[3];
+    const unsigned* flo5;
+    const uint2* flo7;
     double ref[2][4];                // reference doublets: phi = stored + ref (exact in FP64)
     int nx, ny, nz;
 };
@@ -1368,13 +1688,55 @@ __device__ __forceinline__ void obs_E(const ObsArgs<R>& a, int i, long long s, d
     if (a.lo) obs_add_res(hi, res_group(a.lo[s], 1 + i), o);
 }
 
+// The Higgs-family electric field at site s: E' (gaugeNum 5, o[0]) or E'^1..3
+// (gaugeNum 7, o[1..3]), with its residuals.
+template <class R, int G>
+__device__ __forceinline__ void obs_fE(const ObsArgs<R>& a, int i, long long s, double* o) {
+    if constexpr (G == 5) {
+        const R hi = a.fe[i][s];
+        o[0] = (double)hi;
+        if (a.flo5) o[0] += ldexp((double)res_nib(a.flo5[s], i), res_exp((float)hi) - 153);
+    } else {
+        const auto hi = a.fE[i][s];
+        load_d(hi, o);
+        if (a.flo7) obs_add_res(hi, res_group(a.flo7[s], i), o);
+        o[0] = 0.0;
+    }
+}
+// The family part of the link of direction i at site s, in FP64.
+template <class R, int G>
+__device__ __forceinline__ FamLink<double> obs_fam(const ObsArgs<R>& a, const R* const* fy, const typename V4T<R>::type* const* fq,
+                                                   int i, long long s) {
+    FamLink<double> l{};
+    if constexpr (G == 5) {
+        const double y = (double)fy[i][s];
+        l.c = cos(y);
+        l.s = sin(y);
+    } else if constexpr (G == 7) {
+        double t[4];
+        load_d(fq[i][s], t);
+        l.v = Qt<double>{t[0], t[1], t[2], t[3]};
+    }
+    return l;
+}
+
 // 1 - sqrt(1 - x) without cancellation.
 __device__ __forceinline__ double one_minus_sqrt1m(double x) { return x / (1.0 + sqrt(1.0 - x)); }
+
+// 1 - P0 / |P| of a plaquette quaternion P without cancellation for P0 > 0.
+__device__ __forceinline__ double one_minus_p0(const Qt<double>& P1) {
+    const double n = sqrt(P1.w * P1.w + P1.x * P1.x + P1.y * P1.y + P1.z * P1.z);
+    const double v2 = P1.x * P1.x + P1.y * P1.y + P1.z * P1.z;
+    return P1.w > 0 ? v2 / (n * (n + P1.w)) : 1.0 - P1.w / n;
+}
 
 // Energy densities of the current state (potential, gradient, kinetic, magnetic,
 // electric) with the reference's definitions (Analysers/Source/Energy.cpp:126-132;
 // kinetic and electric from the last update, as the reference's two slots give).
-template <class R, int ND>
+// G 5 / 7: with the Higgs-family terms (WilsonLoops/Source/TwoHDM.cpp:496-552,
+// 600-634): the family part of the covariant gradient, (1 - cos theta') / g''^2
+// or (1 - P'0) / g''^2 per plaquette, (1 - cos) / g''^2 of the family electric field.
+template <class R, int ND, int G = 4>
 __global__ void __launch_bounds__(256) mom_energy(ObsArgs<R> a, Coef c, double* __restrict__ partial) {
     __shared__ double lds[4 * 5];
     const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1394,6 +1756,16 @@ __global__ void __launch_bounds__(256) mom_energy(ObsArgs<R> a, Coef c, double* 
         }
         for (int i = 0; i < 3; i++) {
             const long long sp = nb_index(a.nx, a.ny, a.nz, s, i, 1);
+            if constexpr (G != 4) {
+                double fp[8], tx[8];
+                for (int k = 0; k < 2; k++) obs_phi(a, k, sp, fp + 4 * k);
+                transport2<double, G>(U[i], cos(yv[i]), sin(yv[i]), obs_fam<R, G>(a, a.fy, a.fq, i, s), false, phi, tx);
+                for (int m = 0; m < 8; m++) {
+                    const double dd = (fp[m] - tx[m]) * c.est[i];
+                    ge += dd * dd;
+                }
+                continue;
+            }
             for (int k = 0; k < ND; k++) {
                 double fp[4], tx[4];
                 obs_phi(a, k, sp, fp);
@@ -1431,6 +1803,23 @@ __global__ void __launch_bounds__(256) mom_energy(ObsArgs<R> a, Coef c, double* 
                 const double sh = sin(0.5 * th);
                 // Both orientations (i,j) and (j,i) of the plane contribute equally.
                 me += 2.0 * 2.0 * c.inv_sp[i] * c.inv_sp[j] * (c.inv_g_sqr * one_m_p0 + c.inv_gp_sqr * 2.0 * sh * sh);
+                if constexpr (G == 5) {
+                    const double tf = -(double)a.fy[j][spi] + (double)a.fy[i][spj] + (double)a.fy[j][s] -
+                                      (double)a.fy[i][s];
+                    const double shf = sin(0.5 * tf);
+                    me += 2.0 * 2.0 * c.inv_sp[i] * c.inv_sp[j] * c.inv_gpp_sqr * 2.0 * shf * shf;
+                } else if constexpr (G == 7) {
+                    double v1[4], v2[4], vi[4], vj[4];
+                    load_d(a.fq[j][spi], v1);
+                    load_d(a.fq[i][spj], v2);
+                    load_d(a.fq[i][s], vi);
+                    load_d(a.fq[j][s], vj);
+                    const Qt<double> Af = qmul(qconj(Qt<double>{v1[0], v1[1], v1[2], v1[3]}),
+                                               Qt<double>{v2[0], v2[1], v2[2], v2[3]});
+                    const Qt<double> Pf = qmul(Af, qmul(Qt<double>{vj[0], vj[1], vj[2], vj[3]},
+                                                        qconj(Qt<double>{vi[0], vi[1], vi[2], vi[3]})));
+                    me += 2.0 * 2.0 * c.inv_sp[i] * c.inv_sp[j] * c.inv_gpp_sqr * one_minus_p0(Pf);
+                }
             }
         }
         d[3] = me;
@@ -1440,6 +1829,12 @@ __global__ void __launch_bounds__(256) mom_energy(ObsArgs<R> a, Coef c, double* 
             obs_E(a, i, s, e);
             const double ea2 = e[1] * e[1] + e[2] * e[2] + e[3] * e[3];
             ee += 4.0 * c.inv_sp[i] * (c.inv_g_sqr * one_minus_sqrt1m(ea2) + c.inv_gp_sqr * one_minus_sqrt1m(e[0] * e[0]));
+            if constexpr (G != 4) {
+                double f[4] = {0, 0, 0, 0};
+                obs_fE<R, G>(a, i, s, f);
+                const double f2 = G == 5 ? f[0] * f[0] : f[1] * f[1] + f[2] * f[2] + f[3] * f[3];
+                ee += 4.0 * c.inv_sp[i] * c.inv_gpp_sqr * one_minus_sqrt1m(f2);
+            }
         }
         d[4] = c.inv_sqr_dt * ee;
     }
@@ -1452,23 +1847,52 @@ __global__ void __launch_bounds__(256) mom_energy(ObsArgs<R> a, Coef c, double* 
 // rho = 1/4 bilinears(phi(t), phi(t+dt)) evaluated as 1/4 bilinears(phi(t), P).  This equals the reference's
 // link-difference form (WilsonLoops/Source/StandardModel.cpp:274-322) in exact
 // arithmetic, since U(t+dt) U(t)^dag = (sqrt(1-|E|^2), E).
-template <class R, int ND>
+// G 5 / 7: G generators, the Higgs-family ones from the family electric field
+// and charges (WilsonLoops/Source/TwoHDM.cpp:352-386, Gradients/Source/
+// TwoHDM.cpp:299-319) in the same form as the U(1) (G = 5) or SU(2) (G = 7) ones.
+template <class R, int ND, int G = 4>
 __global__ void __launch_bounds__(256) mom_gauss(ObsArgs<R> a, Coef c, double* __restrict__ partial) {
-    __shared__ double lds[4 * 12];
+    __shared__ double lds[4 * 3 * G];
     const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    double sums[8] = {0, 0, 0, 0, 0, 0, 0, 0}, maxs[4] = {0, 0, 0, 0};
+    double sums[2 * G], maxs[G];
+    for (int k = 0; k < G; k++) sums[k] = sums[G + k] = maxs[k] = 0.0;
     if (s < c.nsites && !site_frozen(c, s)) {
         // rho = 1/4 B(phi(t), phi(t+dt)) = 1/4 B(phi(t), P): the bilinears are
         // antisymmetric, so B(phi, phi) = 0.  Using the stored P avoids the
         // rounding of phi(t+dt) = phi(t) + P, which is large next to the charge.
-        double rho[4] = {0, 0, 0, 0};
+        double rho[G];
+        for (int k = 0; k < G; k++) rho[k] = 0.0;
+        double t8[8], n8[8];
         for (int k = 0; k < ND; k++) {
-            double t[4], n[4];
-            obs_P(a, k, s, n);
-            obs_phi_of(a, a.phi_old[k], k, s, t);   // phi(t) as stored in the previous buffer
-            bilinears_add(t, n, rho);
+            obs_P(a, k, s, n8 + 4 * k);
+            obs_phi_of(a, a.phi_old[k], k, s, t8 + 4 * k);   // phi(t) as stored in the previous buffer
+            bilinears_add(t8 + 4 * k, n8 + 4 * k, rho);
         }
-        double div[4] = {0, 0, 0, 0}, mag[4] = {0, 0, 0, 0};
+        if constexpr (G != 4) family_bilinears<double, G>(t8, n8, rho + 4);
+        double div[G], mag[G];
+        for (int k = 0; k < G; k++) div[k] = mag[k] = 0.0;
+        for (int i = 0; i < 3; i++) {
+            if constexpr (G != 4) {
+                const long long smf = nb_index(a.nx, a.ny, a.nz, s, i, -1);
+                double f[4] = {0, 0, 0, 0}, fm[4] = {0, 0, 0, 0};
+                obs_fE<R, G>(a, i, s, f);
+                obs_fE<R, G>(a, i, smf, fm);
+                if constexpr (G == 5) {
+                    div[4] += (f[0] - fm[0]) * c.inv_sp[i];
+                    mag[4] += (fabs(f[0]) + fabs(fm[0])) * c.inv_sp[i];
+                } else {
+                    double v[4];
+                    load_d(a.fq_old[i][s], v);
+                    const Qt<double> Vt{v[0], v[1], v[2], v[3]};
+                    const Qt<double> rot = qmul(qconj(Vt), qmul(Qt<double>{0.0, f[1], f[2], f[3]}, Vt));
+                    const double r3[3] = {rot.x, rot.y, rot.z};
+                    for (int m = 0; m < 3; m++) {
+                        div[4 + m] += (r3[m] - fm[1 + m]) * c.inv_sp[i];
+                        mag[4 + m] += (fabs(r3[m]) + fabs(fm[1 + m])) * c.inv_sp[i];
+                    }
+                }
+            }
+        }
         for (int i = 0; i < 3; i++) {
             const long long sm = nb_index(a.nx, a.ny, a.nz, s, i, -1);
             double e[4], em[4], u[4];
@@ -1487,14 +1911,14 @@ __global__ void __launch_bounds__(256) mom_gauss(ObsArgs<R> a, Coef c, double* _
             mag[2] += (fabs(rot.y) + fabs(em[2])) * c.inv_sp[i];
             mag[3] += (fabs(rot.z) + fabs(em[3])) * c.inv_sp[i];
         }
-        for (int k = 0; k < 4; k++) {
+        for (int k = 0; k < G; k++) {
             const double g = div[k] - c.gsq[k] * rho[k];
             sums[k] = fabs(g);
-            sums[4 + k] = mag[k] + fabs(c.gsq[k] * rho[k]);
+            sums[G + k] = mag[k] + fabs(c.gsq[k] * rho[k]);
             maxs[k] = fabs(g);
         }
     }
-    block_reduce_store<8, 4>(sums, maxs, lds, partial, blockIdx.x);
+    block_reduce_store<2 * G, G>(sums, maxs, lds, partial, blockIdx.x);
 }
 
 // Wilson loops: mean 1/2 tr and mean cos of the 1x1 and 2x2 loops over the three
@@ -1560,7 +1984,7 @@ __global__ void __launch_bounds__(256) mom_loops(ObsArgs<R> a, Coef c, double* _
 // ---- host side ----------------------------------------------------------------
 
 template <class R>
-inline size_t carve(MomFields<R>* f, char* base, long long N, bool cmp) {
+inline size_t carve(MomFields<R>* f, char* base, long long N, bool cmp, int G) {
     using V4 = typename V4T<R>::type;
     size_t off = 0;
     auto take = [&](size_t elems, size_t esz) {
@@ -1578,6 +2002,22 @@ inline size_t carve(MomFields<R>* f, char* base, long long N, bool cmp) {
     for (int i = 0; i < 3; i++) { char* p = take(N, sizeof(V4)); if (f) f->E[i] = reinterpret_cast<V4*>(p); }
     { char* p = cmp ? take(N, sizeof(uint2)) : nullptr; if (f) f->lo = reinterpret_cast<uint2*>(p); }
     { char* p = cmp ? take(N, sizeof(unsigned short)) : nullptr; if (f) f->lo_P2 = reinterpret_cast<unsigned short*>(p); }
+    // The Higgs-family field, only for gaugeNum 5 or 7.
+    for (int b = 0; b < 2; b++)
+        for (int i = 0; i < 3; i++) {
+            char* p = G == 5 ? take(N, sizeof(R)) : nullptr;
+            if (f) f->fy[b][i] = reinterpret_cast<R*>(p);
+            char* pq = G == 7 ? take(N, sizeof(V4)) : nullptr;
+            if (f) f->fq[b][i] = reinterpret_cast<V4*>(pq);
+        }
+    for (int i = 0; i < 3; i++) {
+        char* p = G == 5 ? take(N, sizeof(R)) : nullptr;
+        if (f) f->fe[i] = reinterpret_cast<R*>(p);
+        char* pe = G == 7 ? take(N, sizeof(V4)) : nullptr;
+        if (f) f->fE[i] = reinterpret_cast<V4*>(pe);
+    }
+    { char* p = cmp && G == 5 ? take(N, sizeof(unsigned)) : nullptr; if (f) f->flo5 = reinterpret_cast<unsigned*>(p); }
+    { char* p = cmp && G == 7 ? take(N, sizeof(uint2)) : nullptr; if (f) f->flo7 = reinterpret_cast<uint2*>(p); }
     return off;
 }
 
@@ -1586,10 +2026,11 @@ inline int momentum_alloc(MomentumState& m, const Coef& c, bool f64, bool cmp) {
     m.cmp = cmp && !f64;
     const long long N = c.nsites;
     // The second-doublet arrays are carved for the SM too (unused there).
-    m.bytes = f64 ? carve<double>(nullptr, nullptr, N, false) : carve<float>(nullptr, nullptr, N, m.cmp);
+    const int G = c.G == 5 || c.G == 7 ? c.G : 4;
+    m.bytes = f64 ? carve<double>(nullptr, nullptr, N, false, G) : carve<float>(nullptr, nullptr, N, m.cmp, G);
     if (hipMalloc(&m.base, m.bytes) != hipSuccess) { m.base = nullptr; return GFT_ENOMEM; }
-    if (f64) carve<double>(&m.fd, static_cast<char*>(m.base), N, false);
-    else carve<float>(&m.ff, static_cast<char*>(m.base), N, m.cmp);
+    if (f64) carve<double>(&m.fd, static_cast<char*>(m.base), N, false, G);
+    else carve<float>(&m.ff, static_cast<char*>(m.base), N, m.cmp, G);
     m.cur = 0;
     m.prev_valid = 0;
     return GFT_OK;
@@ -1694,6 +2135,11 @@ int momentum_import_t(MomentumState& m, MomFields<R>& f, const Coef& c, const Sr
     const bool cmp = f.lo != nullptr;
     std::vector<uint2> lo(cmp ? N : 0, make_uint2(0u, 0u));
     std::vector<unsigned short> lo_P2(cmp ? N : 0, 0);
+    // Link reals per direction: [y, c0..c3, Higgs-family part (y' or c0'..c3')].
+    const int G = c.G == 5 || c.G == 7 ? c.G : 4, VD = 5 + (G == 5 ? 1 : G == 7 ? 4 : 0);
+    std::vector<R> fy(G == 5 ? 3 * N : 0), fe(G == 5 ? 3 * N : 0), fq(G == 7 ? 12 * N : 0), fE(G == 7 ? 12 * N : 0);
+    std::vector<unsigned> flo5(cmp && G == 5 ? N : 0, 0u);
+    std::vector<uint2> flo7(cmp && G == 7 ? N : 0, make_uint2(0u, 0u));
     for (int k = 0; k < 2; k++)
         for (int m4 = 0; m4 < 4; m4++) m.ref[k][m4] = 0.0;
     if (sizeof(R) == 4) {
@@ -1720,8 +2166,31 @@ int momentum_import_t(MomentumState& m, MomFields<R>& f, const Coef& c, const Sr
             }
         }
         for (int i = 0; i < 3; i++) {
-            const Src* ln = V + (now * N + s) * 15 + 5 * i;
-            const Src* lp = V + (past * N + s) * 15 + 5 * i;
+            const Src* ln = V + (now * N + s) * (3 * VD) + VD * i;
+            const Src* lp = V + (past * N + s) * (3 * VD) + VD * i;
+            if (G == 5) {   // E' = sin(y'(t) - y'(t - dt))
+                const double ev = std::sin((double)ln[5] - (double)lp[5]);
+                fy[i * N + s] = (R)wrap_host((double)ln[5]);
+                fe[i * N + s] = (R)ev;
+                if (cmp) flo5[s] |= res_q_host(ev - (double)(float)ev, (float)ev) << (4 * i);
+            } else if (G == 7) {   // E' = vec(V(t) V(t - dt)^dag)
+                const double vn[4] = {(double)ln[5], (double)ln[6], (double)ln[7], (double)ln[8]};
+                const double vp[4] = {(double)lp[5], -(double)lp[6], -(double)lp[7], -(double)lp[8]};
+                double r[4];
+                qmul_full(vn, vp, r);
+                r[0] = 0.0;
+                unsigned g = 0;
+                for (int k = 0; k < 4; k++) {
+                    fq[(i * N + s) * 4 + k] = (R)vn[k];
+                    fE[(i * N + s) * 4 + k] = (R)r[k];
+                    if (k) g |= res_q_host(r[k] - (double)(float)r[k], (float)r[k]) << (4 * k);
+                }
+                if (cmp) {
+                    if (i == 0) flo7[s].x = g;
+                    else if (i == 1) flo7[s].x |= g << 16;
+                    else flo7[s].y = g;
+                }
+            }
             y[i * N + s] = (R)wrap_host((double)ln[0]);
             const double un[4] = {(double)ln[1], (double)ln[2], (double)ln[3], (double)ln[4]};
             const double up[4] = {(double)lp[1], -(double)lp[2], -(double)lp[3], -(double)lp[4]};
@@ -1761,6 +2230,17 @@ int momentum_import_t(MomentumState& m, MomFields<R>& f, const Coef& c, const Sr
         ok &= cp(f.lo, lo.data(), N * sizeof(uint2));
         ok &= cp(f.lo_P2, lo_P2.data(), N * sizeof(unsigned short));
     }
+    for (int i = 0; i < 3; i++) {
+        if (G == 5) {
+            ok &= cp(f.fy[0][i], &fy[i * N], N * sizeof(R));
+            ok &= cp(f.fe[i], &fe[i * N], N * sizeof(R));
+        } else if (G == 7) {
+            ok &= cp(f.fq[0][i], &fq[i * N * 4], N * 4 * sizeof(R));
+            ok &= cp(f.fE[i], &fE[i * N * 4], N * 4 * sizeof(R));
+        }
+    }
+    if (cmp && G == 5) ok &= cp(f.flo5, flo5.data(), N * sizeof(unsigned));
+    if (cmp && G == 7) ok &= cp(f.flo7, flo7.data(), N * sizeof(uint2));
     ok &= hipStreamSynchronize(st) == hipSuccess;
     if (!ok) { err = "momentum import: copy failed"; return GFT_EHIP; }
     return GFT_OK;
@@ -1778,10 +2258,25 @@ int momentum_export_t(MomentumState& m, MomFields<R>& f, const Coef& c, Dst* S, 
     const bool cmp = f.lo != nullptr;
     std::vector<uint2> lo(cmp ? N : 0);
     std::vector<unsigned short> lo_P2(cmp ? N : 0);
+    const int G = c.G == 5 || c.G == 7 ? c.G : 4, VD = 5 + (G == 5 ? 1 : G == 7 ? 4 : 0);
+    std::vector<R> fy(G == 5 ? 3 * N : 0), fe(G == 5 ? 3 * N : 0), fq(G == 7 ? 12 * N : 0), fE(G == 7 ? 12 * N : 0);
+    std::vector<unsigned> flo5(cmp && G == 5 ? N : 0);
+    std::vector<uint2> flo7(cmp && G == 7 ? N : 0);
     auto cp = [&](void* dst, const void* src, size_t bytes) {
         return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st) == hipSuccess;
     };
     bool ok = true;
+    for (int i = 0; i < 3; i++) {
+        if (G == 5) {
+            ok &= cp(&fy[i * N], f.fy[m.cur][i], N * sizeof(R));
+            ok &= cp(&fe[i * N], f.fe[i], N * sizeof(R));
+        } else if (G == 7) {
+            ok &= cp(&fq[i * N * 4], f.fq[m.cur][i], N * 4 * sizeof(R));
+            ok &= cp(&fE[i * N * 4], f.fE[i], N * 4 * sizeof(R));
+        }
+    }
+    if (cmp && G == 5) ok &= cp(flo5.data(), f.flo5, N * sizeof(unsigned));
+    if (cmp && G == 7) ok &= cp(flo7.data(), f.flo7, N * sizeof(uint2));
     for (int k = 0; k < nd; k++) {
         ok &= cp(&phi[k * N * 4], f.phi[m.cur][k], N * 4 * sizeof(R));
         ok &= cp(&P[k * N * 4], f.P[k], N * 4 * sizeof(R));
@@ -1811,8 +2306,33 @@ int momentum_export_t(MomentumState& m, MomFields<R>& f, const Coef& c, Dst* S, 
                 S[(past * N + s) * Ns + 4 * k + m4] = (Dst)(a - pp);
             }
         for (int i = 0; i < 3; i++) {
-            Dst* ln = V + (now * N + s) * 15 + 5 * i;
-            Dst* lp = V + (past * N + s) * 15 + 5 * i;
+            Dst* ln = V + (now * N + s) * (3 * VD) + VD * i;
+            Dst* lp = V + (past * N + s) * (3 * VD) + VD * i;
+            if (G == 5) {   // y'(t - dt) = y'(t) - asin(E')
+                double e = (double)fe[i * N + s];
+                if (cmp) e += res_val_host(flo5[s], i, (float)fe[i * N + s]);
+                const double yf = (double)fy[i * N + s];
+                ln[5] = (Dst)yf;
+                lp[5] = (Dst)(yf - std::asin(e));
+            } else if (G == 7) {   // V(t - dt) = (sqrt(1 - |E'|^2), E')^dag V(t)
+                double e[4];
+                const unsigned g = !cmp ? 0u : i == 0 ? flo7[s].x & 0xffffu : i == 1 ? flo7[s].x >> 16
+                                                                                   : flo7[s].y & 0xffffu;
+                for (int k = 0; k < 4; k++) {
+                    e[k] = (double)fE[(i * N + s) * 4 + k];
+                    if (cmp && k) e[k] += res_val_host(g, k, (float)fE[(i * N + s) * 4 + k]);
+                }
+                const double vn[4] = {(double)fq[(i * N + s) * 4], (double)fq[(i * N + s) * 4 + 1],
+                                      (double)fq[(i * N + s) * 4 + 2], (double)fq[(i * N + s) * 4 + 3]};
+                const double ud[4] = {std::sqrt(std::max(0.0, 1.0 - e[1] * e[1] - e[2] * e[2] - e[3] * e[3])), -e[1],
+                                      -e[2], -e[3]};
+                double vp[4];
+                qmul_full(ud, vn, vp);
+                for (int k = 0; k < 4; k++) {
+                    ln[5 + k] = (Dst)vn[k];
+                    lp[5 + k] = (Dst)vp[k];
+                }
+            }
             const double yy = (double)y[i * N + s];
             double e[4] = {(double)E[(i * N + s) * 4], (double)E[(i * N + s) * 4 + 1],
                            (double)E[(i * N + s) * 4 + 2], (double)E[(i * N + s) * 4 + 3]};
@@ -1868,6 +2388,17 @@ StepArgs<R> make_step_args(MomentumState& m, MomFields<R>& f, const Coef& c, dou
     }
     a.lo = f.lo;
     a.lo_P2 = f.lo_P2;
+    for (int i = 0; i < 3; i++) {
+        a.fy[i] = f.fy[cur][i];
+        a.fy_out[i] = f.fy[nxt][i];
+        a.fe[i] = f.fe[i];
+        a.fq[i] = f.fq[cur][i];
+        a.fq_out[i] = f.fq[nxt][i];
+        a.fE[i] = f.fE[i];
+    }
+    a.flo5 = f.flo5;
+    a.flo7 = f.flo7;
+    a.gsq_f = (R)c.gsq[4];
     for (int k = 0; k < 2; k++) {
         a.ref[k].x = (R)m.ref[k][0];
         a.ref[k].y = (R)m.ref[k][1];
@@ -1915,7 +2446,9 @@ void launch_step_f32(const StepArgs<float>& a, const Coef& c, int min_waves, lon
         else if (min_waves == 4) mom_step<float, 1, 4, CMP, 2><<<nb, 256, lds, st>>>(a, c);
         else mom_step<float, 1, 0, CMP, 2><<<nb, 256, lds, st>>>(a, c);
     } else {
-        if (c.nd == 2) mom_step<float, 2, 0, CMP, PL><<<nb, 256, lds, st>>>(a, c);
+        if (c.nd == 2 && c.G == 5) mom_step<float, 2, 0, CMP, PL, 5><<<nb, 256, lds, st>>>(a, c);
+        else if (c.nd == 2 && c.G == 7) mom_step<float, 2, 0, CMP, PL, 7><<<nb, 256, lds, st>>>(a, c);
+        else if (c.nd == 2) mom_step<float, 2, 0, CMP, PL><<<nb, 256, lds, st>>>(a, c);
         else if (min_waves == 3) mom_step<float, 1, 3, CMP, PL><<<nb, 256, lds, st>>>(a, c);
         else if (min_waves == 4) mom_step<float, 1, 4, CMP, PL><<<nb, 256, lds, st>>>(a, c);
         else mom_step<float, 1, 0, CMP, PL><<<nb, 256, lds, st>>>(a, c);
@@ -1930,6 +2463,8 @@ inline int momentum_step(MomentumState& m, const Coef& c, double damping_step, h
     if (m.f64) {
         auto a = make_step_args<double>(m, m.fd, c, damping_step, err);
         if (c.nd == 1) mom_step<double, 1, 0, false><<<nb, 256, 0, st>>>(a, c);
+        else if (c.G == 5) mom_step<double, 2, 0, false, 0, 5><<<nb, 256, 0, st>>>(a, c);
+        else if (c.G == 7) mom_step<double, 2, 0, false, 0, 7><<<nb, 256, 0, st>>>(a, c);
         else mom_step<double, 2, 0, false><<<nb, 256, 0, st>>>(a, c);
     } else {
         auto a = make_step_args<float>(m, m.ff, c, damping_step, err);
@@ -1969,6 +2504,15 @@ ObsArgs<R> make_obs_args(const MomentumState& m, const MomFields<R>& f, const Co
     }
     a.lo = f.lo;
     a.lo_P2 = f.lo_P2;
+    for (int i = 0; i < 3; i++) {
+        a.fy[i] = f.fy[cur][i];
+        a.fe[i] = f.fe[i];
+        a.fq[i] = f.fq[cur][i];
+        a.fq_old[i] = f.fq[old][i];
+        a.fE[i] = f.fE[i];
+    }
+    a.flo5 = f.flo5;
+    a.flo7 = f.flo7;
     for (int k = 0; k < 2; k++)
         for (int m4 = 0; m4 < 4; m4++) a.ref[k][m4] = m.ref[k][m4];
     a.nx = c.nx; a.ny = c.ny; a.nz = c.nz;
@@ -1983,20 +2527,32 @@ inline int momentum_observe(const MomentumState& m, const Coef& c, MomObs what, 
         msg = "Gauss law needs the state of the previous step: run at least one step after import";
         return GFT_EINVAL;
     }
-#define GFT_MOM_LAUNCH(KERNEL)                                                                           \
-    if (m.f64) {                                                                                         \
-        auto a = make_obs_args<double>(m, m.fd, c);                                                      \
-        if (c.nd == 1) KERNEL<double, 1><<<nblocks, 256, 0, st>>>(a, c, partial);                        \
-        else KERNEL<double, 2><<<nblocks, 256, 0, st>>>(a, c, partial);                                  \
-    } else {                                                                                             \
-        auto a = make_obs_args<float>(m, m.ff, c);                                                       \
-        if (c.nd == 1) KERNEL<float, 1><<<nblocks, 256, 0, st>>>(a, c, partial);                         \
-        else KERNEL<float, 2><<<nblocks, 256, 0, st>>>(a, c, partial);                                   \
+// Energy and Gauss law by gauge group (the Higgs-family terms for G 5, 7); the
+// Wilson loops are those of the SU(2) x U(1) links in every group.
+#define GFT_MOM_LAUNCH_R(KERNEL, R, FLD)                                                                 \
+    {                                                                                                    \
+        auto a = make_obs_args<R>(m, FLD, c);                                                            \
+        if (c.nd == 1) KERNEL<R, 1><<<nblocks, 256, 0, st>>>(a, c, partial);                             \
+        else if (c.G == 5) KERNEL<R, 2, 5><<<nblocks, 256, 0, st>>>(a, c, partial);                      \
+        else if (c.G == 7) KERNEL<R, 2, 7><<<nblocks, 256, 0, st>>>(a, c, partial);                      \
+        else KERNEL<R, 2><<<nblocks, 256, 0, st>>>(a, c, partial);                                       \
     }
+#define GFT_MOM_LAUNCH(KERNEL)                                                                           \
+    if (m.f64) GFT_MOM_LAUNCH_R(KERNEL, double, m.fd)                                                    \
+    else GFT_MOM_LAUNCH_R(KERNEL, float, m.ff)
     if (what == MomObs::Energy) { GFT_MOM_LAUNCH(mom_energy) }
     else if (what == MomObs::Gauss) { GFT_MOM_LAUNCH(mom_gauss) }
-    else { GFT_MOM_LAUNCH(mom_loops) }
+    else if (m.f64) {
+        auto a = make_obs_args<double>(m, m.fd, c);
+        if (c.nd == 1) mom_loops<double, 1><<<nblocks, 256, 0, st>>>(a, c, partial);
+        else mom_loops<double, 2><<<nblocks, 256, 0, st>>>(a, c, partial);
+    } else {
+        auto a = make_obs_args<float>(m, m.ff, c);
+        if (c.nd == 1) mom_loops<float, 1><<<nblocks, 256, 0, st>>>(a, c, partial);
+        else mom_loops<float, 2><<<nblocks, 256, 0, st>>>(a, c, partial);
+    }
 #undef GFT_MOM_LAUNCH
+#undef GFT_MOM_LAUNCH_R
     if (hipGetLastError() != hipSuccess) { msg = "momentum observable launch failed"; return GFT_EHIP; }
     return GFT_OK;
 }

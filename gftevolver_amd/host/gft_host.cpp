@@ -610,8 +610,7 @@ int Lattice::resolvedForm() const {
     if (form != GFT_FORM_AUTO) return form;
     gft_desc d{};
     model.fill(d);
-    const bool family = d.gauge_num == 5 || d.gauge_num == 7;
-    return (d.model == GFT_MODEL_GLOBAL || family) ? GFT_FORM_TWO_SLOT_F32 : GFT_FORM_MOMENTUM_F32;
+    return d.model == GFT_MODEL_GLOBAL ? GFT_FORM_TWO_SLOT_F32 : GFT_FORM_MOMENTUM_F32;
 }
 
 void Lattice::evolve() {

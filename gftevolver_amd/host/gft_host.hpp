@@ -25,9 +25,10 @@
 
 namespace gft {
 
-// Lattice's default form: the fast path (GFT_FORM_MOMENTUM_F32) for the models it
-// carries (SM, and the 2HDM with SU(2) x U(1) links), the reference-order FP32
-// two-slot form for the others (global O(N) model, 2HDM with gaugeNum 5 or 7).
+// Lattice's default form: the fast path (GFT_FORM_MOMENTUM_F32) for the gauged
+// models (SM; 2HDM with SU(2) x U(1) links or a Higgs-family U(1) or SU(2)), the
+// FP32 two-slot form for the global O(N) model (whose two-slot kernel streams
+// the scalars in the reference's float storage and operation order).
 constexpr int GFT_FORM_AUTO = -1;
 
 // Positional reader of the reference's .cfg format: descriptions end with ':'

@@ -5,8 +5,8 @@
 // CONFIG_ROOT holds the reference's config tree (<Lib>/Config/*.cfg, default
 // "."); DATA_DIR receives global_quantities*.dat and field dumps (default
 // CONFIG_ROOT/Data); FORM is auto (default: momentum_f32, the fast path, for
-// the SM and the 2HDM with SU(2) x U(1) links; two_slot_f32 for the global
-// O(N) model and gaugeNum 5 / 7), two_slot_f32 (the reference's float storage
+// the SM and the 2HDM with any of its gauge groups; two_slot_f32 for the global
+// O(N) model), two_slot_f32 (the reference's float storage
 // and operation order), two_slot_f64, momentum_f32, momentum_f32_plain or
 // momentum_f64;
 // FORM = dump_config parses the configuration, prints it and exits (no GPU).

@@ -68,7 +68,7 @@ def test_create_rejects_invalid_face_masks_before_touching_a_device(faces, msg):
 
 @pytest.mark.skipif(not os.path.exists(LIB), reason="libgft.so not built")
 @pytest.mark.parametrize("model,form,gauge_num,msg", [
-    (_abi.GFT_MODEL_2HDM, _abi.GFT_FORM_MOMENTUM_F32, 5, "two-slot forms"),
+    (_abi.GFT_MODEL_SM, _abi.GFT_FORM_MOMENTUM_F32, 5, "two doublets"),
     (_abi.GFT_MODEL_SM, _abi.GFT_FORM_TWO_SLOT_F64, 7, "two doublets"),
     (_abi.GFT_MODEL_2HDM, _abi.GFT_FORM_TWO_SLOT_F64, 6, "gauge_num is 4")])
 def test_create_rejects_unsupported_gauge_groups_before_touching_a_device(model, form, gauge_num, msg):

@@ -1,16 +1,15 @@
 """GPU parity of the 2HDM with a Higgs-family gauge field (gaugeNum 5: an extra
 U(1) under which the two doublets carry opposite charge; gaugeNum 7: an extra
 SU(2) acting on the doublet index; Gradients/Source/TwoHDM.cpp,
-WilsonLoops/Source/TwoHDM.cpp).  These run in the two-slot forms, which follow
-the reference's algorithm:
+WilsonLoops/Source/TwoHDM.cpp) in the two-slot forms, which follow the
+reference's algorithm (the momentum forms: test_gpu_family_momentum.py):
 
   * two-slot FP32 against the fields the reference's own code wrote from the
     same state (harness_2hdm_g<5|7>[_gen].npz): a few float ulps;
   * two-slot FP64 against the FP64 oracle: <= 1e-12 relative (fields),
     energies <= 1e-10 relative, all gaugeNum Gauss integrals to 1e-11 of the
     cancelling terms;
-  * the momentum forms refuse gaugeNum 5 and 7, and a single doublet refuses a
-    Higgs-family field, with GFT_EINVAL.
+  * a single doublet refuses a Higgs-family field with GFT_EINVAL.
 """
 from __future__ import annotations
 
@@ -114,10 +113,10 @@ def test_twoslot_f64_family_gauge_matches_oracle(rt, gauge_num, gen):
         np.testing.assert_allclose(np.array(a.gauss_scale[:gauge_num]), sc, rtol=1e-10)
 
 
-def test_family_gauge_needs_two_doublets_and_a_two_slot_form(rt):
-    for model, form, msg in ((GFT_MODEL_2HDM, GFT_FORM_MOMENTUM_F32, "two-slot forms"),
-                             (GFT_MODEL_SM, GFT_FORM_TWO_SLOT_F64, "two doublets")):
-        d = make_desc(model=model, form=form, n=(8, 8, 8), pot2=TWO_HDM_POTENTIAL if model else None, gauge_num=5)
+@pytest.mark.parametrize("form", [GFT_FORM_TWO_SLOT_F64, GFT_FORM_MOMENTUM_F32], ids=["two_slot", "momentum"])
+def test_family_gauge_needs_two_doublets(rt, form):
+    for g in (5, 7):
+        d = make_desc(model=GFT_MODEL_SM, form=form, n=(8, 8, 8), gauge_num=g)
         with pytest.raises(rt.GftError) as e:
             rt.Evolver(d)
-        assert e.value.code == -1 and msg in str(e.value)
+        assert e.value.code == -1 and "two doublets" in str(e.value)

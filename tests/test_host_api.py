@@ -238,23 +238,27 @@ def test_config_reader_family_gauge(tmp_path, nvd, gen, gauge_num):
                          check=True).stdout
     kv = {l.split()[0]: l.split()[1:] for l in out.strip().splitlines()}
     assert kv["gauge_num"] == [str(gauge_num)] and kv["nv"] == [str(3 * nvd)] and kv["gen_rep"] == [str(int(gen))]
-    # Family gauge fields run in the FP32 two-slot form (GFT_FORM_TWO_SLOT_F32 = 1) by default.
-    assert kv["form"] == ["1"]
+    # Family gauge fields run in the fast path (GFT_FORM_MOMENTUM_F32 = 2) by default.
+    assert kv["form"] == ["2"]
 
 
 @needs_cfg
 @pytest.mark.gpu
+@pytest.mark.parametrize("form", ["two_slot_f32", None], ids=["two_slot_f32", "default"])
 @pytest.mark.parametrize("gauge_num,gen", [(5, False), (5, True), (7, False), (7, True)],
                          ids=["g5", "g5-gen", "g7", "g7-gen"])
-def test_gft_run_reproduces_reference_program_family_gauge(tmp_path, gauge_num, gen):
+def test_gft_run_reproduces_reference_program_family_gauge(tmp_path, gauge_num, gen, form):
     """The reference program's 2HDM with a Higgs-family gauge field: energies and
-    all 2 x gaugeNum Gauss columns."""
+    all 2 x gaugeNum Gauss columns, in the reference-order FP32 two-slot form and
+    in the default form (the FP32 momentum form, the fast path), whose reordered
+    arithmetic agrees to the float reference's own accumulation error."""
     tag = f"2hdm_g{gauge_num}" + ("_gen" if gen else "")
     g = np.load(os.path.join(ROOT, "tests", "golden", f"program_{tag}.npz"))
     nvd = (4 if gen else 5) + (1 if gauge_num == 5 else (3 if gen else 4))
     root = _prepare(str(tmp_path), 1, (6, 5, 7), 12, 8, gen=gen, nvd=nvd)
     data = tmp_path / "data"
-    subprocess.run([GFT_RUN, root, str(data), "two_slot_f32"], check=True, capture_output=True, timeout=120)
+    subprocess.run([GFT_RUN, root, str(data)] + ([form] if form else []), check=True, capture_output=True,
+                   timeout=120)
     E = np.loadtxt(data / "global_quantities.dat", ndmin=2)
     G = np.loadtxt(data / "global_quantities_gauge.dat", ndmin=2)
     assert E.shape == g["energy"].shape and G.shape == g["gauss"].shape == (G.shape[0], 2 * gauge_num)
