@@ -203,8 +203,9 @@ def options_leg(rt, with_cpu):
     """The reference's other model options on the GPU, each beside the
     reference program on the host (one core, 32^3 sample): fixed faces in the
     fast path, the generator form (boundary conversion and the two-slot
-    kernel), the 2HDM with Higgs-family U(1) / SU(2) gauge fields and the
-    global O(4) model in the two-slot forms."""
+    kernel), the 2HDM with SU(2) x U(1) links and with Higgs-family U(1) /
+    SU(2) gauge fields in the reference-order two-slot form and in the fast
+    path, and the global O(4) model (two-slot form)."""
     import numpy as np
     from gftevolver_amd._abi import (GFT_FORM_MOMENTUM_F32, GFT_FORM_TWO_SLOT_F32, GFT_MODEL_2HDM,
                                      GFT_MODEL_GLOBAL, TWO_HDM_POTENTIAL, make_desc)
@@ -267,6 +268,9 @@ def options_leg(rt, with_cpu):
             ref = reference_program_rate(1, 32, 10, 8, nvd, TWO_HDM_POTENTIAL)
             r["cpu_reference"] = ref[0] if ref else None
         out[f"2hdm_gauge{g}_two_slot_f32"] = r
+        dm = make_desc(model=GFT_MODEL_2HDM, form=GFT_FORM_MOMENTUM_F32, n=n128, pot2=TWO_HDM_POTENTIAL,
+                       gauge_num=g, gpp=0.5)
+        out[f"2hdm_gauge{g}_momentum_f32"] = timed(dm, 200, state=(s8, v.reshape(2, N, 3 * nvd)))
     dgl = make_desc(model=GFT_MODEL_GLOBAL, form=GFT_FORM_TWO_SLOT_F32, n=n256, num_scalar=4, potential=1)
     Ng = 256 ** 3
     sg = (np.random.default_rng(2).normal(size=(2, Ng, 4)) * 0.1 + np.array([1.0, 0, 0, 0])).astype(np.float32)

@@ -100,8 +100,10 @@ typedef struct gft_desc {
      * SU(2) x U(1); 5 adds a Higgs-family U(1) (one more phase y' per link, opposite charge
      * on the two doublets); 7 adds a Higgs-family SU(2) acting on the doublet index.  Link
      * reals per direction: 4 -> 5 | 4 (quaternion | generator form), 5 -> 6 | 5
-     * ([.., y']), 7 -> 9 | 7 ([.., c0'..c3'] | [.., w1'..w3']).  5 and 7 need GFT_MODEL_2HDM
-     * and a two-slot form.  gpp is the Higgs-family coupling g'' (TwoHDM.cfg).          */
+     * ([.., y']), 7 -> 9 | 7 ([.., c0'..c3'] | [.., w1'..w3']).  5 and 7 need GFT_MODEL_2HDM;
+     * every form carries them (the momentum forms store the family phases or quaternions
+     * and their electric fields beside the SU(2) x U(1) ones).  gpp is the Higgs-family
+     * coupling g'' (TwoHDM.cfg).                                                          */
     int      gauge_num;
     double   gpp;
     /* GFT_MODEL_GLOBAL only (Gradients/Source/Global.cpp; Potentials/Source/SO_N.cpp takes any
