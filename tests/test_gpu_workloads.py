@@ -12,8 +12,13 @@ properties (the FP64 oracle on the CPU would take hours at these sizes):
     magnetic energy every 50 steps: FP32-bulk against FP64 within 1e-5
     relative, the magnetic energy within 1e-5 of the total energy (it starts at
     zero from identity links, so it is compared on the scale of the total).
+  * the 2HDM with a Higgs-family U(1) or SU(2) gauge field (gaugeNum 5, 7) at
+    128^3 over 1000 steps under a family-symmetric potential: FP32 against
+    FP64 every 200 steps, energies within 1e-5 of the total, loops within
+    1e-5, the FP32 Gauss violation of every generator <= 1e-6.
 The FP64 momentum form is itself held to the FP64 CPU oracle at 1e-11
-(tests/test_gpu_momentum.py), so these tie the full-size FP32 runs to it.
+(tests/test_gpu_momentum.py, tests/test_gpu_family_momentum.py), so these tie
+the full-size FP32 runs to it.
 """
 from __future__ import annotations
 
@@ -80,3 +85,23 @@ def test_2hdm_192_fp32_matches_fp64_wilson_loops(rt):
             print(f"step {50 * (k + 1)}: loops {l32} rel err {dl.max():.1e}, magnetic energy rel err {dm:.1e}")
         assert dl.max() <= 1e-5 and dm <= 1e-5, (50 * (k + 1), dl, dm)
     print(f"worst over 100 measurements: loops {worst[0]:.1e}, magnetic energy {worst[1]:.1e}")
+
+
+@pytest.mark.parametrize("G", [5, 7], ids=["g5", "g7"])
+def test_2hdm_family_128_fp32_matches_fp64(rt, G):
+    from tests.test_gpu_family_momentum import FAMILY_SYMMETRIC_POTENTIAL
+    n = (128, 128, 128)
+    kw = dict(model=GFT_MODEL_2HDM, n=n, pot2=FAMILY_SYMMETRIC_POTENTIAL, gauge_num=G, gpp=0.5)
+    _, r32 = _run(rt, make_desc(form=GFT_FORM_MOMENTUM_F32, **kw), 1000, 200, vevs=(0.8, 0.6))
+    _, r64 = _run(rt, make_desc(form=GFT_FORM_MOMENTUM_F64, **kw), 1000, 200, vevs=(0.8, 0.6))
+    assert len(r32) == len(r64) == 5
+    for k, ((e32, l32, g32), (e64, l64, g64)) in enumerate(zip(r32, r64)):
+        de = np.abs(e32 - e64).max() / abs(e64[0])
+        dl = np.abs(l32 - l64).max() / np.abs(l64).max()
+        assert g32.has_gauss and g32.num_eq == G
+        gs = max(g32.gauss_int[m] / g32.gauss_scale[m] for m in range(G))
+        g64s = max(g64.gauss_int[m] / g64.gauss_scale[m] for m in range(G))
+        print(f"gaugeNum {G} step {200 * (k + 1)}: energy {e32[0]:.9e} vs {e64[0]:.9e} (col err {de:.1e}), "
+              f"loops err {dl:.1e}, Gauss FP32 {gs:.1e} FP64 {g64s:.1e}")
+        assert de <= 1e-5 and dl <= 1e-5
+        assert gs <= 1e-6 and g64s <= 1e-12
