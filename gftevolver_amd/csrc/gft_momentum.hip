@@ -1273,13 +1273,17 @@ struct ShareLds {
     float4* p01;
     float4* p02;
     float4* p12;
+    float4* d1b;   // the second doublet's d_y, d_z (two doublets only)
+    float4* d2b;
     int w;   // bz + 1
 };
-__host__ __device__ inline size_t share_lds_bytes(int bz, int by) { return (size_t)(bz + 1) * (by + 1) * 5 * 16; }
+__host__ __device__ inline size_t share_lds_bytes(int bz, int by, int nd) {
+    return (size_t)(bz + 1) * (by + 1) * (nd == 2 ? 7 : 5) * 16;
+}
 __device__ __forceinline__ ShareLds share_layout(char* base, int bz, int by) {
     const int n = (bz + 1) * (by + 1);
     float4* f = reinterpret_cast<float4*>(base);
-    return ShareLds{f, f + n, f + 2 * n, f + 3 * n, f + 4 * n, bz + 1};
+    return ShareLds{f, f + n, f + 2 * n, f + 3 * n, f + 4 * n, f + 5 * n, f + 6 * n, bz + 1};
 }
 __device__ __forceinline__ float4 pack_wu(const Plaq<float>& p) { return make_float4(p.wu[0], p.wu[1], p.wu[2], p.st); }
 
@@ -1289,7 +1293,9 @@ __device__ __forceinline__ float4 pack_wu(const Plaq<float>& p) { return make_fl
 // EARLY: the site's own P, E and residuals are loaded at the start of phase A,
 // where their latency overlaps the phase (at the cost of registers held across
 // the exchange barriers); otherwise after the exchange, next to their use.
-template <bool CMP, bool EARLY = true>
+// ND: doublets (the 2HDM with SU(2) x U(1) links shares both doublets'
+// differences through LDS as well).
+template <int ND, bool CMP, bool EARLY = true>
 __device__ __forceinline__ void mom_block_shared(const StepArgs<float>& a, const Coef& c, int x, int y0, int z0, int ty,
                                                  int tz, bool valid, const PlaneLds& pl, char* lds) {
     using R = float;
@@ -1303,10 +1309,10 @@ __device__ __forceinline__ void mom_block_shared(const StepArgs<float>& a, const
     const int r0 = pl.r0;
     const unsigned sm0 = static_cast<unsigned>(xm * sx + y * a.nz + z);   // x - e_0
     Qt<R> U[3];
-    R cy[3], sy_[3], yv[3], phi[4], praw[4], F[4], J[3][4], M[3][4], dpv[3][4], dsh[3][4];
+    R cy[3], sy_[3], yv[3], phi[4 * ND], praw[4 * ND], F[4 * ND], J[3][4], M[3][4], dpv[3][4 * ND], dsh[3][4 * ND];
     Plaq<R> P[3];   // at x: (0,1), (0,2), (1,2)
-    OwnState<R, 1> own;
-    if (EARLY && valid) load_own<R, 1, CMP>(a, s, own);   // issued early: its latency overlaps phase A
+    OwnState<R, ND> own;
+    if (EARLY && valid) load_own<R, ND, CMP>(a, s, own);   // issued early: its latency overlaps phase A
     // Back to the default priority (mom_step raised it for the loads), in every
     // wave: a wave with no valid lane must not keep the raised priority.
     __builtin_amdgcn_s_setprio(0);
@@ -1317,39 +1323,49 @@ __device__ __forceinline__ void mom_block_shared(const StepArgs<float>& a, const
             yv[i] = pl.y[i][r0];
             sincos_fast(yv[i], &sy_[i], &cy[i]);
         }
-        unpack(pl.phi[0][r0], praw);
-        add_ref(a.ref[0], praw, phi);
+#pragma unroll
+        for (int k = 0; k < ND; k++) {
+            unpack(pl.phi[k][r0], praw + 4 * k);
+            add_ref(a.ref[k], praw + 4 * k, phi + 4 * k);
+        }
         potential_derivative<R>(c, phi, F);
 #pragma unroll
-        for (int k = 0; k < 4; k++) F[k] = -F[k];
+        for (int k = 0; k < 4 * ND; k++) F[k] = -F[k];
         // Neighbours x + e_i: plane x + 1 (global) for i = 0, LDS for i = 1, 2.
         const unsigned sp0 = static_cast<unsigned>(xp * sx + y * a.nz + z);
         const int step[3] = {0, W, 1};
 #pragma unroll
         for (int i = 0; i < 3; i++) {
-            R fr[4], fp[4], tx[4];
-            if (i == 0) unpack(at(a.phi[0], sp0), fr);
-            else unpack(pl.phi[0][r0 + step[i]], fr);
-            add_ref(a.ref[0], fr, fp);
-            transport(U[i], cy[i], sy_[i], false, phi, tx);
             J[i][0] = J[i][1] = J[i][2] = J[i][3] = R(0);
 #pragma unroll
-            for (int m = 0; m < 4; m++) dsh[i][m] = fp[m] - tx[m];
-            transport(U[i], cy[i], sy_[i], true, dsh[i], dpv[i]);
-            bilinears_add(tx, dsh[i], J[i]);
+            for (int k = 0; k < ND; k++) {
+                R fr[4], fp[4], tx[4];
+                if (i == 0) unpack(at(a.phi[k], sp0), fr);
+                else unpack(pl.phi[k][r0 + step[i]], fr);
+                add_ref(a.ref[k], fr, fp);
+                transport(U[i], cy[i], sy_[i], false, phi + 4 * k, tx);
+#pragma unroll
+                for (int m = 0; m < 4; m++) dsh[i][4 * k + m] = fp[m] - tx[m];
+                transport(U[i], cy[i], sy_[i], true, dsh[i] + 4 * k, dpv[i] + 4 * k);
+                bilinears_add(tx, dsh[i] + 4 * k, J[i]);
+            }
         }
         // d_x(x - e_x) from the plane x - 1; the scalar force's i = 0 term.
         {
             const Qt<R> Um = toq<float4, R>(at(a.q[0], sm0));
-            R cm, smn, fr[4], fm[4], tm[4], dmv0[4];
+            R cm, smn;
             sincos_fast(at(a.y[0], sm0), &smn, &cm);
-            unpack(at(a.phi[0], sm0), fr);
-            add_ref(a.ref[0], fr, fm);
-            transport(Um, cm, smn, false, fm, tm);
 #pragma unroll
-            for (int m = 0; m < 4; m++) dmv0[m] = phi[m] - tm[m];
+            for (int k = 0; k < ND; k++) {
+                R fr[4], fm[4], tm[4], dmv0[4];
+                unpack(at(a.phi[k], sm0), fr);
+                add_ref(a.ref[k], fr, fm);
+                transport(Um, cm, smn, false, fm, tm);
 #pragma unroll
-            for (int m = 0; m < 4; m++) F[m] += a.inv_sp[0] * (dpv[0][m] - dmv0[m]);
+                for (int m = 0; m < 4; m++) dmv0[m] = phi[4 * k + m] - tm[m];
+#pragma unroll
+                for (int m = 0; m < 4; m++) F[4 * k + m] += a.inv_sp[0] * (dpv[0][4 * k + m] - dmv0[m]);
+            }
         }
         // Plaquettes based at x and their contributions to the links at x.
 #pragma unroll
@@ -1407,7 +1423,7 @@ __device__ __forceinline__ void mom_block_shared(const StepArgs<float>& a, const
     const int rdir = k < bz ? 1 : 2;                      // the ring's direction
     const int rt = k - bz;                                // z-ring: tile row
     const bool ring_valid = ring && (rdir == 1 ? z0 + k < a.nz : y0 + rt < a.ny);
-    float4 ring_d, ring_p0, ring_p12;
+    float4 ring_d[ND], ring_p0, ring_p12;
     if (ring_valid) {
         const int rb = rdir == 1 ? k + 1 : (rt + 1) * W;      // plane slot of the ring site b
         const int rbr = rdir == 1 ? rb + W : rb + 1;          // b + e_r (a tile site)
@@ -1425,16 +1441,20 @@ __device__ __forceinline__ void mom_block_shared(const StepArgs<float>& a, const
         {
             const Qt<R> Ur = rdir == 1 ? Ub[1] : Ub[2];
             const R yr = rdir == 1 ? yb[1] : yb[2];
-            R cr, sr, rawb[4], rawn[4], fb[4], fn[4], tx[4], d[4];
+            R cr, sr;
             sincos_fast(yr, &sr, &cr);
-            unpack(pl.phi[0][rb], rawb);
-            unpack(pl.phi[0][rbr], rawn);
-            add_ref(a.ref[0], rawb, fb);
-            add_ref(a.ref[0], rawn, fn);
-            transport(Ur, cr, sr, false, fb, tx);
 #pragma unroll
-            for (int m = 0; m < 4; m++) d[m] = fn[m] - tx[m];
-            ring_d = make_float4(d[0], d[1], d[2], d[3]);
+            for (int kd = 0; kd < ND; kd++) {
+                R rawb[4], rawn[4], fb[4], fn[4], tx[4], d[4];
+                unpack(pl.phi[kd][rb], rawb);
+                unpack(pl.phi[kd][rbr], rawn);
+                add_ref(a.ref[kd], rawb, fb);
+                add_ref(a.ref[kd], rawn, fn);
+                transport(Ur, cr, sr, false, fb, tx);
+#pragma unroll
+                for (int m = 0; m < 4; m++) d[m] = fn[m] - tx[m];
+                ring_d[kd] = make_float4(d[0], d[1], d[2], d[3]);
+            }
         }
         // Plaquette (0, r) based at b: U_r(b + e_0) from the plane x + 1.
         {
@@ -1462,6 +1482,10 @@ __device__ __forceinline__ void mom_block_shared(const StepArgs<float>& a, const
     if (valid) {
         sh.d1[si] = make_float4(dsh[1][0], dsh[1][1], dsh[1][2], dsh[1][3]);
         sh.d2[si] = make_float4(dsh[2][0], dsh[2][1], dsh[2][2], dsh[2][3]);
+        if constexpr (ND == 2) {
+            sh.d1b[si] = make_float4(dsh[1][4], dsh[1][5], dsh[1][6], dsh[1][7]);
+            sh.d2b[si] = make_float4(dsh[2][4], dsh[2][5], dsh[2][6], dsh[2][7]);
+        }
         sh.p01[si] = pack_wu(P[0]);
         sh.p02[si] = pack_wu(P[1]);
         sh.p12[si] = pack_wu(P[2]);
@@ -1469,30 +1493,33 @@ __device__ __forceinline__ void mom_block_shared(const StepArgs<float>& a, const
     if (ring_valid) {
         const int ri = rdir == 1 ? k + 1 : (rt + 1) * sh.w;
         if (rdir == 1) {
-            sh.d1[ri] = ring_d;
+            sh.d1[ri] = ring_d[0];
+            if constexpr (ND == 2) sh.d1b[ri] = ring_d[1];
             sh.p01[ri] = ring_p0;
         } else {
-            sh.d2[ri] = ring_d;
+            sh.d2[ri] = ring_d[0];
+            if constexpr (ND == 2) sh.d2b[ri] = ring_d[1];
             sh.p02[ri] = ring_p0;
         }
         sh.p12[ri] = ring_p12;
     }
     __syncthreads();
     if (!valid) return;
-    if constexpr (!EARLY) load_own<R, 1, CMP>(a, s, own);
+    if constexpr (!EARLY) load_own<R, ND, CMP>(a, s, own);
     // Shell of a fixed face: its plaquettes and differences served the
     // neighbours above; the site is computed like any other, then carried over
     // unchanged by the kick's stores.
     const bool frozen = site_frozen(c, x, y, z);
 
     // ---- phase B: the scalar force's i = 1, 2 terms ----
-    {
-        const float4 d1m = sh.d1[si - sh.w], d2m = sh.d2[si - 1];
+#pragma unroll
+    for (int kd = 0; kd < ND; kd++) {
+        const float4 d1m = kd ? sh.d1b[si - sh.w] : sh.d1[si - sh.w], d2m = kd ? sh.d2b[si - 1] : sh.d2[si - 1];
         const R dm1[4] = {d1m.x, d1m.y, d1m.z, d1m.w}, dm2[4] = {d2m.x, d2m.y, d2m.z, d2m.w};
 #pragma unroll
-        for (int m = 0; m < 4; m++) F[m] += a.inv_sp[1] * (dpv[1][m] - dm1[m]);
+        for (int m = 0; m < 4; m++) F[4 * kd + m] += a.inv_sp[1] * (dpv[1][4 * kd + m] - dm1[m]);
 #pragma unroll
-        for (int m = 0; m < 4; m++) F[m] += a.inv_sp[2] * (dpv[2][m] - dm2[m]);
+        for (int m = 0; m < 4; m++) F[4 * kd + m] += a.inv_sp[2] * (dpv[2][4 * kd + m] - dm2[m]);
     }
     // Plaquettes based at x - e_1 and x - e_2 from LDS, in the order j = 1, 2.
     {
@@ -1517,7 +1544,7 @@ __device__ __forceinline__ void mom_block_shared(const StepArgs<float>& a, const
         M[1][2] -= a.inv_sp[2] * p12b.y;
         M[1][3] -= a.inv_sp[2] * p12b.z;
     }
-    kick_drift<R, 1, CMP, true>(a, s, U, yv, praw, F, M, &own, frozen, c.fixed != 0);
+    kick_drift<R, ND, CMP, true>(a, s, U, yv, praw, F, M, &own, frozen, c.fixed != 0);
 }
 
 // MINW: minimum waves per SIMD the register allocation must allow (0: the
@@ -1535,7 +1562,7 @@ __global__ void __launch_bounds__(256, MINW > 0 ? MINW : 1) mom_step(StepArgs<R>
     const bool valid = z < a.nz && y < a.ny;
     if constexpr (PL > 0) {
         static_assert(sizeof(R) == 4, "plane staging is FP32 only");
-        static_assert(PL == 1 || (ND == 1 && G == 4), "shared plaquettes are for one doublet");
+        static_assert(PL == 1 || G == 4, "shared plaquettes are for the SU(2) x U(1) links");
         extern __shared__ float4 gft_plane_lds[];
         char* lds = reinterpret_cast<char*>(gft_plane_lds);
         PlaneLds pl = plane_layout<ND>(lds, a.bz, a.by);
@@ -1548,9 +1575,10 @@ __global__ void __launch_bounds__(256, MINW > 0 ? MINW : 1) mom_step(StepArgs<R>
         plane_fill<ND>(a, pl, t.y, t.z * a.by, t.w * a.bz, a.bz, a.by);
         pl.r0 = (ty + 1) * pl.w + tz + 1;
         if constexpr (PL == 2) {
-            // At 4 waves per SIMD the registers do not hold the own state across the
-            // exchange barriers: it is loaded after them.
-            mom_block_shared<CMP, MINW != 4>(a, c, t.y, t.z * a.by, t.w * a.bz, ty, tz, valid, pl, lds);
+            // At 4 waves per SIMD (and with two doublets) the registers do not hold
+            // the own state across the exchange barriers: it is loaded after them.
+            mom_block_shared<ND, CMP, ND == 1 && MINW != 4>(a, c, t.y, t.z * a.by, t.w * a.bz, ty, tz, valid, pl,
+                                                            lds);
         } else {
             if (!valid) return;
             mom_site<R, ND, CMP, true, G>(a, c, t.y, y, z, pl);
@@ -1609,7 +1637,7 @@ __global__ void __launch_bounds__(256, MINW) ens_step(const EnsMember* __restric
             plane_fill<1>(a, pl, t.y, t.z * a.by, t.w * a.bz, a.bz, a.by);
             pl.r0 = (ty + 1) * pl.w + tz + 1;
             if constexpr (PL == 2)
-                mom_block_shared<true, MINW != 4>(a, c, t.y, t.z * a.by, t.w * a.bz, ty, tz, valid, pl, lds);
+                mom_block_shared<1, true, MINW != 4>(a, c, t.y, t.z * a.by, t.w * a.bz, ty, tz, valid, pl, lds);
             else if (valid) mom_site<float, 1, true, true>(a, c, t.y, y, z, pl);
         } else {
             if (valid) mom_site<float, 1, true>(a, c, t.y, y, z);
@@ -2428,12 +2456,14 @@ constexpr size_t kPlaneLdsMax = 65536;
 inline size_t step_lds_bytes(int bz, int by, int nd, int pl) {
     if (pl == 0) return 0;
     const size_t b = plane_lds_bytes(bz, by, nd);
-    return pl == 2 ? std::max(b, share_lds_bytes(bz, by)) : b;
+    return pl == 2 ? std::max(b, share_lds_bytes(bz, by, nd)) : b;
 }
 
 // The staging mode a launch runs with: the requested one where it applies.
-inline int step_stage_mode(int stage, int bz, int by, int nd) {
-    if (stage >= 2 && (nd != 1 || bz + by > 256 || step_lds_bytes(bz, by, nd, 2) > kPlaneLdsMax)) stage = 1;
+// Stage 2 needs bz + by <= 256 (one ring item per thread) and the SU(2) x U(1)
+// links alone (G = 4; a Higgs-family field runs stage <= 1).
+inline int step_stage_mode(int stage, int bz, int by, int nd, int G = 4) {
+    if (stage >= 2 && (G > 4 || bz + by > 256 || step_lds_bytes(bz, by, nd, 2) > kPlaneLdsMax)) stage = 1;
     if (stage >= 1 && step_lds_bytes(bz, by, nd, 1) > kPlaneLdsMax) stage = 0;
     return stage;
 }
@@ -2442,7 +2472,8 @@ template <bool CMP, int PL>
 void launch_step_f32(const StepArgs<float>& a, const Coef& c, int min_waves, long long nb, hipStream_t st) {
     const size_t lds = step_lds_bytes(a.bz, a.by, c.nd, PL);
     if constexpr (PL == 2) {
-        if (min_waves == 3) mom_step<float, 1, 3, CMP, 2><<<nb, 256, lds, st>>>(a, c);
+        if (c.nd == 2) mom_step<float, 2, 0, CMP, 2><<<nb, 256, lds, st>>>(a, c);
+        else if (min_waves == 3) mom_step<float, 1, 3, CMP, 2><<<nb, 256, lds, st>>>(a, c);
         else if (min_waves == 4) mom_step<float, 1, 4, CMP, 2><<<nb, 256, lds, st>>>(a, c);
         else mom_step<float, 1, 0, CMP, 2><<<nb, 256, lds, st>>>(a, c);
     } else {
@@ -2470,7 +2501,7 @@ inline int momentum_step(MomentumState& m, const Coef& c, double damping_step, h
         auto a = make_step_args<float>(m, m.ff, c, damping_step, err);
         // Every staging mode gives bitwise the same fields; a launch runs the
         // requested mode where it applies (see step_stage_mode), else a lower one.
-        const int pl = step_stage_mode(m.stage, m.bz, m.by, c.nd);
+        const int pl = step_stage_mode(m.stage, m.bz, m.by, c.nd, c.G);
         if (m.cmp) {
             if (pl == 2) launch_step_f32<true, 2>(a, c, m.min_waves, nb, st);
             else if (pl == 1) launch_step_f32<true, 1>(a, c, m.min_waves, nb, st);
