@@ -906,6 +906,75 @@ __device__ __forceinline__ void kick_drift_family(const StepArgs<R>& a, unsigned
     }
 }
 
+// Higgs-family link forces (WilsonLoops/Source/TwoHDM.cpp:496-552): the
+// staples in the pattern of the U(1) (G = 5) or SU(2) (G = 7) staples of
+// mom_site, each plaquette's contributions from one rounded angle or
+// quaternion, then the family source g''^2 J'.  FY(f, i, si, j, sj) / FQ(...)
+// read the family phase / quaternion of direction f at x + si e_i + sj e_j;
+// fyv, FV are the site's own.  MF[i][0] (G = 5) or MF[i][1..3] (G = 7).
+template <class R, int G, class FYF, class FQF>
+__device__ __forceinline__ void family_force(const StepArgs<R>& a, FYF FY, FQF FQ, const R* fyv, const Qt<R>* FV,
+                                             const R (*JF)[3], R (*MF)[4]) {
+#pragma unroll
+    for (int i = 0; i < 3; i++) MF[i][0] = MF[i][1] = MF[i][2] = MF[i][3] = R(0);
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+#pragma unroll
+        for (int j = i + 1; j < 3; j++) {
+            if constexpr (G == 5) {
+                const R th = (FY(i, j, 1, 0, 0) - fyv[i]) + (fyv[j] - FY(j, i, 1, 0, 0));
+                const R st = sin_fast(th);
+                MF[i][0] += a.inv_sp[j] * st;
+                MF[j][0] -= a.inv_sp[i] * st;
+            } else {
+                const Qt<R> A = qmul(qconj(FQ(j, i, 1, 0, 0)), FQ(i, j, 1, 0, 0));
+                const Qt<R> C = qmul(FV[j], qconj(FV[i]));
+                R v[3], w[3];
+                qvec(A, C, v);
+                qrot(C, v, w);
+#pragma unroll
+                for (int m = 0; m < 3; m++) {
+                    MF[i][1 + m] += a.inv_sp[j] * v[m];
+                    MF[j][1 + m] -= a.inv_sp[i] * w[m];
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            if (i == j) continue;
+            if constexpr (G == 5) {
+                const R yim = FY(i, j, -1, 0, 0), yjm = FY(j, j, -1, 0, 0), yjd = FY(j, i, 1, j, -1);
+                if (i < j) MF[i][0] -= a.inv_sp[j] * sin_fast((fyv[i] - yim) + (yjm - yjd));
+                else MF[i][0] += a.inv_sp[j] * sin_fast((yjd - yjm) + (yim - fyv[i]));
+            } else {
+                const Qt<R> Vim = FQ(i, j, -1, 0, 0), Vjm = FQ(j, j, -1, 0, 0), Vjd = FQ(j, i, 1, j, -1);
+                R v[3], w[3];
+                if (i < j) {
+                    qvec(qmul(qconj(Vjd), FV[i]), qmul(Vjm, qconj(Vim)), v);
+                    qrot(Vjd, v, w);
+#pragma unroll
+                    for (int m = 0; m < 3; m++) MF[i][1 + m] -= a.inv_sp[j] * w[m];
+                } else {
+                    qvec(qmul(qconj(FV[i]), Vjd), qmul(Vim, qconj(Vjm)), v);
+                    qrot(FV[i], v, w);
+#pragma unroll
+                    for (int m = 0; m < 3; m++) MF[i][1 + m] += a.inv_sp[j] * w[m];
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        if constexpr (G == 5) MF[i][0] = fma(a.gsq_f, JF[i][0], MF[i][0]);
+        else
+#pragma unroll
+            for (int m = 0; m < 3; m++) MF[i][1 + m] = fma(a.gsq_f, JF[i][m], MF[i][1 + m]);
+    }
+}
+
 // One site-update at (x, y, z): forces, kick and drift (shared by the
 // single-lattice kernel and the ensemble kernel).  PL: in-plane neighbours and
 // own q, y, phi come from the block's LDS tile `pl` (FP32 only).
@@ -1162,68 +1231,9 @@ __device__ __forceinline__ void mom_site(const StepArgs<R>& a, const Coef& c, in
     kick_drift<R, ND, CMP>(a, s, U, yv, praw, F, M, nullptr, frozen, c.fixed != 0);
 
     if constexpr (G != 4) {
-        // Higgs-family staples (WilsonLoops/Source/TwoHDM.cpp:496-552), in the
-        // pattern of the U(1) (G = 5) or SU(2) (G = 7) staples above, then the
-        // family source g''^2 J' and the family kick and drift.
+        // Higgs-family staples, source and kick.
         R MF[3][4];
-#pragma unroll
-        for (int i = 0; i < 3; i++) MF[i][0] = MF[i][1] = MF[i][2] = MF[i][3] = R(0);
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-#pragma unroll
-            for (int j = i + 1; j < 3; j++) {
-                if constexpr (G == 5) {
-                    const R th = (FY(i, j, 1, 0, 0) - fyv[i]) + (fyv[j] - FY(j, i, 1, 0, 0));
-                    const R st = sin_fast(th);
-                    MF[i][0] += a.inv_sp[j] * st;
-                    MF[j][0] -= a.inv_sp[i] * st;
-                } else {
-                    const Qt<R> A = qmul(qconj(FQ(j, i, 1, 0, 0)), FQ(i, j, 1, 0, 0));
-                    const Qt<R> C = qmul(FV[j], qconj(FV[i]));
-                    R v[3], w[3];
-                    qvec(A, C, v);
-                    qrot(C, v, w);
-#pragma unroll
-                    for (int m = 0; m < 3; m++) {
-                        MF[i][1 + m] += a.inv_sp[j] * v[m];
-                        MF[j][1 + m] -= a.inv_sp[i] * w[m];
-                    }
-                }
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-#pragma unroll
-            for (int i = 0; i < 3; i++) {
-                if (i == j) continue;
-                if constexpr (G == 5) {
-                    const R yim = FY(i, j, -1, 0, 0), yjm = FY(j, j, -1, 0, 0), yjd = FY(j, i, 1, j, -1);
-                    if (i < j) MF[i][0] -= a.inv_sp[j] * sin_fast((fyv[i] - yim) + (yjm - yjd));
-                    else MF[i][0] += a.inv_sp[j] * sin_fast((yjd - yjm) + (yim - fyv[i]));
-                } else {
-                    const Qt<R> Vim = FQ(i, j, -1, 0, 0), Vjm = FQ(j, j, -1, 0, 0), Vjd = FQ(j, i, 1, j, -1);
-                    R v[3], w[3];
-                    if (i < j) {
-                        qvec(qmul(qconj(Vjd), FV[i]), qmul(Vjm, qconj(Vim)), v);
-                        qrot(Vjd, v, w);
-#pragma unroll
-                        for (int m = 0; m < 3; m++) MF[i][1 + m] -= a.inv_sp[j] * w[m];
-                    } else {
-                        qvec(qmul(qconj(FV[i]), Vjd), qmul(Vim, qconj(Vjm)), v);
-                        qrot(FV[i], v, w);
-#pragma unroll
-                        for (int m = 0; m < 3; m++) MF[i][1 + m] += a.inv_sp[j] * w[m];
-                    }
-                }
-            }
-        }
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-            if constexpr (G == 5) MF[i][0] = fma(a.gsq_f, JF[i][0], MF[i][0]);
-            else
-#pragma unroll
-                for (int m = 0; m < 3; m++) MF[i][1 + m] = fma(a.gsq_f, JF[i][m], MF[i][1 + m]);
-        }
+        family_force<R, G>(a, FY, FQ, fyv, FV, JF, MF);
         kick_drift_family<R, G, CMP>(a, s, fyv, FV, MF, frozen, c.fixed != 0);
     }
 }
@@ -1293,9 +1303,12 @@ __device__ __forceinline__ float4 pack_wu(const Plaq<float>& p) { return make_fl
 // EARLY: the site's own P, E and residuals are loaded at the start of phase A,
 // where their latency overlaps the phase (at the cost of registers held across
 // the exchange barriers); otherwise after the exchange, next to their use.
-// ND: doublets (the 2HDM with SU(2) x U(1) links shares both doublets'
-// differences through LDS as well).
-template <int ND, bool CMP, bool EARLY = true>
+// ND: doublets (the 2HDM shares both doublets' differences through LDS as
+// well).  G 5 / 7 (two doublets): the differences carry the Higgs-family part
+// of the transport (the family links read from global memory), the family
+// staples are evaluated per site from global memory, and the family kick
+// follows the SU(2) x U(1) one.
+template <int ND, bool CMP, bool EARLY = true, int G = 4>
 __device__ __forceinline__ void mom_block_shared(const StepArgs<float>& a, const Coef& c, int x, int y0, int z0, int ty,
                                                  int tz, bool valid, const PlaneLds& pl, char* lds) {
     using R = float;
@@ -1311,6 +1324,35 @@ __device__ __forceinline__ void mom_block_shared(const StepArgs<float>& a, const
     Qt<R> U[3];
     R cy[3], sy_[3], yv[3], phi[4 * ND], praw[4 * ND], F[4 * ND], J[3][4], M[3][4], dpv[3][4 * ND], dsh[3][4 * ND];
     Plaq<R> P[3];   // at x: (0,1), (0,2), (1,2)
+    static_assert(G == 4 || ND == 2, "a Higgs-family field acts on two doublets");
+    // Higgs-family links: direction f at x + si e_i + sj e_j (global memory).
+    const int nz = a.nz;
+    int dp[3], dm[3];
+    dp[0] = (x == a.nx - 1) ? -(a.nx - 1) * sx : sx;
+    dm[0] = (x == 0) ? (a.nx - 1) * sx : -sx;
+    dp[1] = (y == a.ny - 1) ? -(a.ny - 1) * nz : nz;
+    dm[1] = (y == 0) ? (a.ny - 1) * nz : -nz;
+    dp[2] = (z == a.nz - 1) ? -(a.nz - 1) : 1;
+    dm[2] = (z == 0) ? (a.nz - 1) : -1;
+    auto fsite = [&](int i, int si, int j, int sj) -> unsigned {
+        int o = 0;
+        if (si > 0) o += dp[i];
+        else if (si < 0) o += dm[i];
+        if (sj > 0) o += dp[j];
+        else if (sj < 0) o += dm[j];
+        return s + o;
+    };
+    auto FY = [&](int f, int i, int si, int j, int sj) -> R { return at(a.fy[f], fsite(i, si, j, sj)); };
+    auto FQ = [&](int f, int i, int si, int j, int sj) -> Qt<R> { return toq<float4, R>(at(a.fq[f], fsite(i, si, j, sj))); };
+    auto FAMAT = [&](int f, unsigned site) -> FamLink<R> {
+        FamLink<R> l{};
+        if constexpr (G == 5) sincos_fast(at(a.fy[f], site), &l.s, &l.c);
+        else if constexpr (G == 7) l.v = toq<float4, R>(at(a.fq[f], site));
+        return l;
+    };
+    R fyv[3] = {R(0), R(0), R(0)}, JF[3][3], MF[3][4];
+    Qt<R> FV[3];
+    FamLink<R> FL[3];
     OwnState<R, ND> own;
     if (EARLY && valid) load_own<R, ND, CMP>(a, s, own);   // issued early: its latency overlaps phase A
     // Back to the default priority (mom_step raised it for the loads), in every
@@ -1334,9 +1376,37 @@ __device__ __forceinline__ void mom_block_shared(const StepArgs<float>& a, const
         // Neighbours x + e_i: plane x + 1 (global) for i = 0, LDS for i = 1, 2.
         const unsigned sp0 = static_cast<unsigned>(xp * sx + y * a.nz + z);
         const int step[3] = {0, W, 1};
+        if constexpr (G != 4) {
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                FL[i] = FAMAT(i, s);
+                if constexpr (G == 5) fyv[i] = at(a.fy[i], s);
+                else FV[i] = FL[i].v;
+            }
+        }
 #pragma unroll
         for (int i = 0; i < 3; i++) {
             J[i][0] = J[i][1] = J[i][2] = J[i][3] = R(0);
+            if constexpr (G != 4) {
+                // Both doublets at once: the family part of T mixes them (G = 7).
+                JF[i][0] = JF[i][1] = JF[i][2] = R(0);
+                R fp[8], tx[8];
+#pragma unroll
+                for (int k = 0; k < 2; k++) {
+                    R fr[4];
+                    if (i == 0) unpack(at(a.phi[k], sp0), fr);
+                    else unpack(pl.phi[k][r0 + step[i]], fr);
+                    add_ref(a.ref[k], fr, fp + 4 * k);
+                }
+                transport2<R, G>(U[i], cy[i], sy_[i], FL[i], false, phi, tx);
+#pragma unroll
+                for (int m = 0; m < 8; m++) dsh[i][m] = fp[m] - tx[m];
+                transport2<R, G>(U[i], cy[i], sy_[i], FL[i], true, dsh[i], dpv[i]);
+                bilinears_add(tx, dsh[i], J[i]);
+                bilinears_add(tx + 4, dsh[i] + 4, J[i]);
+                family_bilinears<R, G>(tx, dsh[i], JF[i]);
+                continue;
+            }
 #pragma unroll
             for (int k = 0; k < ND; k++) {
                 R fr[4], fp[4], tx[4];
@@ -1355,16 +1425,29 @@ __device__ __forceinline__ void mom_block_shared(const StepArgs<float>& a, const
             const Qt<R> Um = toq<float4, R>(at(a.q[0], sm0));
             R cm, smn;
             sincos_fast(at(a.y[0], sm0), &smn, &cm);
+            if constexpr (G != 4) {
+                R fm[8], tm[8];
 #pragma unroll
-            for (int k = 0; k < ND; k++) {
-                R fr[4], fm[4], tm[4], dmv0[4];
-                unpack(at(a.phi[k], sm0), fr);
-                add_ref(a.ref[k], fr, fm);
-                transport(Um, cm, smn, false, fm, tm);
+                for (int k = 0; k < 2; k++) {
+                    R fr[4];
+                    unpack(at(a.phi[k], sm0), fr);
+                    add_ref(a.ref[k], fr, fm + 4 * k);
+                }
+                transport2<R, G>(Um, cm, smn, FAMAT(0, sm0), false, fm, tm);
 #pragma unroll
-                for (int m = 0; m < 4; m++) dmv0[m] = phi[4 * k + m] - tm[m];
+                for (int m = 0; m < 8; m++) F[m] += a.inv_sp[0] * (dpv[0][m] - (phi[m] - tm[m]));
+            } else {
 #pragma unroll
-                for (int m = 0; m < 4; m++) F[4 * k + m] += a.inv_sp[0] * (dpv[0][4 * k + m] - dmv0[m]);
+                for (int k = 0; k < ND; k++) {
+                    R fr[4], fm[4], tm[4], dmv0[4];
+                    unpack(at(a.phi[k], sm0), fr);
+                    add_ref(a.ref[k], fr, fm);
+                    transport(Um, cm, smn, false, fm, tm);
+#pragma unroll
+                    for (int m = 0; m < 4; m++) dmv0[m] = phi[4 * k + m] - tm[m];
+#pragma unroll
+                    for (int m = 0; m < 4; m++) F[4 * k + m] += a.inv_sp[0] * (dpv[0][4 * k + m] - dmv0[m]);
+                }
             }
         }
         // Plaquettes based at x and their contributions to the links at x.
@@ -1415,6 +1498,7 @@ __device__ __forceinline__ void mom_block_shared(const StepArgs<float>& a, const
             }
         }
         fold_current(a, J, M);
+        if constexpr (G != 4) family_force<R, G>(a, FY, FQ, fyv, FV, JF, MF);
     }
 
     // ---- phase A: ring item (y-ring site for k < bz, z-ring site after) ----
@@ -1443,17 +1527,35 @@ __device__ __forceinline__ void mom_block_shared(const StepArgs<float>& a, const
             const R yr = rdir == 1 ? yb[1] : yb[2];
             R cr, sr;
             sincos_fast(yr, &sr, &cr);
+            if constexpr (G != 4) {
+                const unsigned sb = static_cast<unsigned>(x * sx + yy * a.nz + zz);   // the ring site b
+                R fb[8], fn[8], tx[8];
 #pragma unroll
-            for (int kd = 0; kd < ND; kd++) {
-                R rawb[4], rawn[4], fb[4], fn[4], tx[4], d[4];
-                unpack(pl.phi[kd][rb], rawb);
-                unpack(pl.phi[kd][rbr], rawn);
-                add_ref(a.ref[kd], rawb, fb);
-                add_ref(a.ref[kd], rawn, fn);
-                transport(Ur, cr, sr, false, fb, tx);
+                for (int kd = 0; kd < 2; kd++) {
+                    R rawb[4], rawn[4];
+                    unpack(pl.phi[kd][rb], rawb);
+                    unpack(pl.phi[kd][rbr], rawn);
+                    add_ref(a.ref[kd], rawb, fb + 4 * kd);
+                    add_ref(a.ref[kd], rawn, fn + 4 * kd);
+                }
+                transport2<R, G>(Ur, cr, sr, FAMAT(rdir, sb), false, fb, tx);
 #pragma unroll
-                for (int m = 0; m < 4; m++) d[m] = fn[m] - tx[m];
-                ring_d[kd] = make_float4(d[0], d[1], d[2], d[3]);
+                for (int kd = 0; kd < 2; kd++)
+                    ring_d[kd] = make_float4(fn[4 * kd] - tx[4 * kd], fn[4 * kd + 1] - tx[4 * kd + 1],
+                                             fn[4 * kd + 2] - tx[4 * kd + 2], fn[4 * kd + 3] - tx[4 * kd + 3]);
+            } else {
+#pragma unroll
+                for (int kd = 0; kd < ND; kd++) {
+                    R rawb[4], rawn[4], fb[4], fn[4], tx[4], d[4];
+                    unpack(pl.phi[kd][rb], rawb);
+                    unpack(pl.phi[kd][rbr], rawn);
+                    add_ref(a.ref[kd], rawb, fb);
+                    add_ref(a.ref[kd], rawn, fn);
+                    transport(Ur, cr, sr, false, fb, tx);
+#pragma unroll
+                    for (int m = 0; m < 4; m++) d[m] = fn[m] - tx[m];
+                    ring_d[kd] = make_float4(d[0], d[1], d[2], d[3]);
+                }
             }
         }
         // Plaquette (0, r) based at b: U_r(b + e_0) from the plane x + 1.
@@ -1545,6 +1647,7 @@ __device__ __forceinline__ void mom_block_shared(const StepArgs<float>& a, const
         M[1][3] -= a.inv_sp[2] * p12b.z;
     }
     kick_drift<R, ND, CMP, true>(a, s, U, yv, praw, F, M, &own, frozen, c.fixed != 0);
+    if constexpr (G != 4) kick_drift_family<R, G, CMP>(a, s, fyv, FV, MF, frozen, c.fixed != 0);
 }
 
 // MINW: minimum waves per SIMD the register allocation must allow (0: the
@@ -1562,7 +1665,7 @@ __global__ void __launch_bounds__(256, MINW > 0 ? MINW : 1) mom_step(StepArgs<R>
     const bool valid = z < a.nz && y < a.ny;
     if constexpr (PL > 0) {
         static_assert(sizeof(R) == 4, "plane staging is FP32 only");
-        static_assert(PL == 1 || G == 4, "shared plaquettes are for the SU(2) x U(1) links");
+        static_assert(PL == 1 || G == 4 || ND == 2, "a Higgs-family field acts on two doublets");
         extern __shared__ float4 gft_plane_lds[];
         char* lds = reinterpret_cast<char*>(gft_plane_lds);
         PlaneLds pl = plane_layout<ND>(lds, a.bz, a.by);
@@ -1577,8 +1680,8 @@ __global__ void __launch_bounds__(256, MINW > 0 ? MINW : 1) mom_step(StepArgs<R>
         if constexpr (PL == 2) {
             // At 4 waves per SIMD (and with two doublets) the registers do not hold
             // the own state across the exchange barriers: it is loaded after them.
-            mom_block_shared<ND, CMP, ND == 1 && MINW != 4>(a, c, t.y, t.z * a.by, t.w * a.bz, ty, tz, valid, pl,
-                                                            lds);
+            mom_block_shared<ND, CMP, ND == 1 && MINW != 4, G>(a, c, t.y, t.z * a.by, t.w * a.bz, ty, tz, valid,
+                                                               pl, lds);
         } else {
             if (!valid) return;
             mom_site<R, ND, CMP, true, G>(a, c, t.y, y, z, pl);
@@ -2460,10 +2563,10 @@ inline size_t step_lds_bytes(int bz, int by, int nd, int pl) {
 }
 
 // The staging mode a launch runs with: the requested one where it applies.
-// Stage 2 needs bz + by <= 256 (one ring item per thread) and the SU(2) x U(1)
-// links alone (G = 4; a Higgs-family field runs stage <= 1).
+// Stage 2 needs bz + by <= 256 (one ring item per thread).
 inline int step_stage_mode(int stage, int bz, int by, int nd, int G = 4) {
-    if (stage >= 2 && (G > 4 || bz + by > 256 || step_lds_bytes(bz, by, nd, 2) > kPlaneLdsMax)) stage = 1;
+    (void)G;
+    if (stage >= 2 && (bz + by > 256 || step_lds_bytes(bz, by, nd, 2) > kPlaneLdsMax)) stage = 1;
     if (stage >= 1 && step_lds_bytes(bz, by, nd, 1) > kPlaneLdsMax) stage = 0;
     return stage;
 }
@@ -2472,7 +2575,9 @@ template <bool CMP, int PL>
 void launch_step_f32(const StepArgs<float>& a, const Coef& c, int min_waves, long long nb, hipStream_t st) {
     const size_t lds = step_lds_bytes(a.bz, a.by, c.nd, PL);
     if constexpr (PL == 2) {
-        if (c.nd == 2) mom_step<float, 2, 0, CMP, 2><<<nb, 256, lds, st>>>(a, c);
+        if (c.nd == 2 && c.G == 5) mom_step<float, 2, 0, CMP, 2, 5><<<nb, 256, lds, st>>>(a, c);
+        else if (c.nd == 2 && c.G == 7) mom_step<float, 2, 0, CMP, 2, 7><<<nb, 256, lds, st>>>(a, c);
+        else if (c.nd == 2) mom_step<float, 2, 0, CMP, 2><<<nb, 256, lds, st>>>(a, c);
         else if (min_waves == 3) mom_step<float, 1, 3, CMP, 2><<<nb, 256, lds, st>>>(a, c);
         else if (min_waves == 4) mom_step<float, 1, 4, CMP, 2><<<nb, 256, lds, st>>>(a, c);
         else mom_step<float, 1, 0, CMP, 2><<<nb, 256, lds, st>>>(a, c);
