@@ -219,8 +219,8 @@ int gft_wilson_loops(gft_ctx* ctx, gft_loops* out);
  * unconstrained); and `stage` for the FP32 forms (0: every neighbour read from
  * global memory; 1: each block gathers q, y and phi of its tile and a one-site
  * ring in y and z into LDS, so that only the neighbours in the planes x +- 1
- * are global reads; 2, the default: as 1, and with SU(2) x U(1) links (one or
- * two doublets) the plaquettes and covariant differences inside the x-plane are
+ * are global reads; 2, the default: as 1, and the SU(2) x U(1) plaquettes and
+ * the covariant differences (of one or two doublets) inside the x-plane are
  * evaluated once per block and exchanged through LDS instead of by each of the
  * sites that use them, for tiles with bz + by <= 256, otherwise as 1; FP64
  * forms always run unstaged).
