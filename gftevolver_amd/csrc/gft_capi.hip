@@ -558,8 +558,22 @@ static void finish_gauss(gft_ctx* ctx, const double* partial, long long nblocks)
 // gauge group (gaugeNum 4, 5, 7).
 template <class T, bool GEN, int G>
 static void twoslot_step_as(gft_ctx* ctx, long long nblocks, unsigned now, bool me, double damping_step) {
-    twoslot_step<T, GEN, G><<<nblocks, kBlock, 0, ctx->stream>>>(ctx->coef, (T*)ctx->S, (T*)ctx->V, now, me,
-                                                                  damping_step, ctx->partial, ctx->err_dev);
+    // The doublet count as a template parameter where that measured faster
+    // (quaternion links with gaugeNum 4 or 5), at run time otherwise.
+    if constexpr (GEN || G == 7) {
+        twoslot_step<T, GEN, G, 0><<<nblocks, kBlock, 0, ctx->stream>>>(ctx->coef, (T*)ctx->S, (T*)ctx->V, now, me,
+                                                                         damping_step, ctx->partial, ctx->err_dev);
+    } else if constexpr (G == 4) {
+        if (ctx->coef.nd == 1)
+            twoslot_step<T, GEN, G, 1><<<nblocks, kBlock, 0, ctx->stream>>>(
+                ctx->coef, (T*)ctx->S, (T*)ctx->V, now, me, damping_step, ctx->partial, ctx->err_dev);
+        else
+            twoslot_step<T, GEN, G, 2><<<nblocks, kBlock, 0, ctx->stream>>>(
+                ctx->coef, (T*)ctx->S, (T*)ctx->V, now, me, damping_step, ctx->partial, ctx->err_dev);
+    } else {
+        twoslot_step<T, GEN, G, 2><<<nblocks, kBlock, 0, ctx->stream>>>(ctx->coef, (T*)ctx->S, (T*)ctx->V, now, me,
+                                                                         damping_step, ctx->partial, ctx->err_dev);
+    }
 }
 template <class T, bool GEN, int G>
 static void twoslot_gauss_as(gft_ctx* ctx, long long nblocks, unsigned now) {
