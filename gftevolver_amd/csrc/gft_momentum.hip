@@ -1692,6 +1692,16 @@ __global__ void __launch_bounds__(256, MINW > 0 ? MINW : 1) mom_step(StepArgs<R>
     }
 }
 
+// The default single-doublet FP32 step (4 waves per SIMD, shared staging,
+// compensated and plain storage) is compiled in its own translation unit,
+// gft_step_sm.hip, under the instruction-level-parallelism scheduler: the
+// same instructions in another order, 0.6% faster at 256^3 (1.0015 ->
+// 0.9958 ms, 127 VGPRs, still 4 waves), where that scheduler makes the
+// two-doublet kernels slower (they lose occupancy).  These declarations keep
+// every other translation unit from instantiating them.
+extern template __global__ void mom_step<float, 1, 4, true, 2, 4>(StepArgs<float>, Coef);
+extern template __global__ void mom_step<float, 1, 4, false, 2, 4>(StepArgs<float>, Coef);
+
 // One member of a batched ensemble: its step arguments and coefficients.
 struct EnsMember {
     StepArgs<float> a;
@@ -2113,6 +2123,9 @@ __global__ void __launch_bounds__(256) mom_loops(ObsArgs<R> a, Coef c, double* _
 }
 
 // ---- host side ----------------------------------------------------------------
+// (Left out of a translation unit that defines GFT_KERNELS_ONLY to compile
+// selected kernels alone: the launches below would make it emit every kernel.)
+#ifndef GFT_KERNELS_ONLY
 
 template <class R>
 inline size_t carve(MomFields<R>* f, char* base, long long N, bool cmp, int G) {
@@ -2692,5 +2705,7 @@ inline int momentum_observe(const MomentumState& m, const Coef& c, MomObs what, 
     if (hipGetLastError() != hipSuccess) { msg = "momentum observable launch failed"; return GFT_EHIP; }
     return GFT_OK;
 }
+
+#endif  // GFT_KERNELS_ONLY
 
 }  // namespace gft

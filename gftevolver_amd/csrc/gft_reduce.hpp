@@ -60,40 +60,10 @@ __device__ __forceinline__ void block_reduce_store(const double* sums, const dou
 // deterministic.  nsum + nmax <= kFinWidth.
 constexpr int kFinThreads = 256;
 constexpr int kFinWidth = 12;
+// Defined once, in gft_capi.hip (the library's other translation unit includes
+// this header too).
 __global__ void __launch_bounds__(kFinThreads) finalize_partials(const double* __restrict__ partial, long long nblocks,
                                                                  int stride, int nsum, int nmax,
-                                                                 double* __restrict__ out) {
-    __shared__ double s_hi[kFinWidth * kFinThreads], s_lo[kFinWidth * kFinThreads];
-    const int width = nsum + nmax;
-    Kahan k[kFinWidth];
-    double mx[kFinWidth];
-#pragma unroll
-    for (int q = 0; q < kFinWidth; q++) { k[q].init(); mx[q] = 0.0; }
-    for (long long b = threadIdx.x; b < nblocks; b += kFinThreads) {
-        const double* row = partial + b * stride;
-#pragma unroll
-        for (int q = 0; q < kFinWidth; q++) {
-            if (q < nsum) k[q].add(row[q]);
-            else if (q < width) mx[q] = fmax(mx[q], row[q]);
-        }
-    }
-#pragma unroll
-    for (int q = 0; q < kFinWidth; q++) {
-        s_hi[q * kFinThreads + threadIdx.x] = (q < nsum) ? k[q].s : mx[q];
-        s_lo[q * kFinThreads + threadIdx.x] = (q < nsum) ? k[q].c : 0.0;
-    }
-    __syncthreads();
-    if (threadIdx.x < width) {
-        const int q = threadIdx.x;
-        Kahan t;
-        t.init();
-        double m = 0.0;
-        for (int i = 0; i < kFinThreads; i++) {
-            if (q < nsum) { t.add(s_hi[q * kFinThreads + i]); t.add(s_lo[q * kFinThreads + i]); }
-            else m = fmax(m, s_hi[q * kFinThreads + i]);
-        }
-        out[q] = (q < nsum) ? t.value() : m;
-    }
-}
+                                                                 double* __restrict__ out);
 
 }  // namespace gft
