@@ -150,6 +150,33 @@ __device__ __forceinline__ float sin_fast(float x) {
     return (k & 2) ? -ss : ss;
 }
 
+// The same functions without the branch: the reduction always runs (for
+// |x| < 0.78 it returns x itself and quadrant 0, so the results are bitwise
+// those of sincos_fast / sin_fast), both polynomials are evaluated and the
+// quadrant selects.  About a dozen more VALU instructions per call and no
+// exec-mask branch, for code that waits on memory rather than on the VALU
+// (the Higgs-family phases of the gaugeNum-5 kernels).
+__device__ __forceinline__ void sincos_nb(float x, float* s, float* c) {
+    int k;
+    const float r = quadrant_reduce(x, &k);
+    const float r2 = r * r;
+    const float sr = sin_poly(r, r2), cr = cos_poly(r2);
+    const float ss = (k & 1) ? cr : sr;
+    const float cc = (k & 1) ? sr : cr;
+    *s = (k & 2) ? -ss : ss;
+    *c = ((k + 1) & 2) ? -cc : cc;
+}
+__device__ __forceinline__ float sin_nb(float x) {
+    int k;
+    const float r = quadrant_reduce(x, &k);
+    const float r2 = r * r;
+    const float sr = sin_poly(r, r2), cr = cos_poly(r2);
+    const float ss = (k & 1) ? cr : sr;
+    return (k & 2) ? -ss : ss;
+}
+__device__ __forceinline__ void sincos_nb(double x, double* s, double* c) { sincos(x, s, c); }
+__device__ __forceinline__ double sin_nb(double x) { return sin(x); }
+
 // asin by its Maclaurin series for |x| <= 1/8 (terms to x^11, error < 1e-13
 // relative), the library asinf beyond.
 __device__ __forceinline__ float asin_small(float x) {
@@ -165,6 +192,23 @@ __device__ __forceinline__ float asin_small(float x) {
 __device__ __forceinline__ void sincos_fast(double x, double* s, double* c) { sincos(x, s, c); }
 __device__ __forceinline__ double sin_fast(double x) { return sin(x); }
 __device__ __forceinline__ double asin_small(double x) { return asin(x); }
+
+// The step's trigonometric functions: NB selects the branch-free forms.  For
+// the U(1) phases they pay only in the family-SU(2) kernels (gaugeNum 7:
+// 192^3 step 1.071 -> 1.023 ms); the SM kernels, which are bound by VALU
+// issue, lose 4% with them, the SU(2) x U(1) and family-U(1) 2HDM kernels
+// 1-4%.  The family-U(1) phases always take them (gaugeNum 5: 1.136 ->
+// 0.913 ms at 192^3).  Either form gives bitwise the same fields.
+template <bool NB, class R>
+__device__ __forceinline__ void sincos_k(R x, R* s, R* c) {
+    if constexpr (NB) sincos_nb(x, s, c);
+    else sincos_fast(x, s, c);
+}
+template <bool NB, class R>
+__device__ __forceinline__ R sin_k(R x) {
+    if constexpr (NB) return sin_nb(x);
+    else return sin_fast(x);
+}
 __device__ __forceinline__ float rsqrt_r(float x) { return __frsqrt_rn(x); }
 __device__ __forceinline__ double rsqrt_r(double x) { return 1.0 / sqrt(x); }
 // Correctly rounded sqrt of x in [2^-24, 1] (1 - |E|^2 for a float |E|^2 < 1):
@@ -923,7 +967,7 @@ __device__ __forceinline__ void family_force(const StepArgs<R>& a, FYF FY, FQF F
         for (int j = i + 1; j < 3; j++) {
             if constexpr (G == 5) {
                 const R th = (FY(i, j, 1, 0, 0) - fyv[i]) + (fyv[j] - FY(j, i, 1, 0, 0));
-                const R st = sin_fast(th);
+                const R st = sin_nb(th);
                 MF[i][0] += a.inv_sp[j] * st;
                 MF[j][0] -= a.inv_sp[i] * st;
             } else {
@@ -947,8 +991,8 @@ __device__ __forceinline__ void family_force(const StepArgs<R>& a, FYF FY, FQF F
             if (i == j) continue;
             if constexpr (G == 5) {
                 const R yim = FY(i, j, -1, 0, 0), yjm = FY(j, j, -1, 0, 0), yjd = FY(j, i, 1, j, -1);
-                if (i < j) MF[i][0] -= a.inv_sp[j] * sin_fast((fyv[i] - yim) + (yjm - yjd));
-                else MF[i][0] += a.inv_sp[j] * sin_fast((yjd - yjm) + (yim - fyv[i]));
+                if (i < j) MF[i][0] -= a.inv_sp[j] * sin_nb((fyv[i] - yim) + (yjm - yjd));
+                else MF[i][0] += a.inv_sp[j] * sin_nb((yjd - yjm) + (yim - fyv[i]));
             } else {
                 const Qt<R> Vim = FQ(i, j, -1, 0, 0), Vjm = FQ(j, j, -1, 0, 0), Vjd = FQ(j, i, 1, j, -1);
                 R v[3], w[3];
@@ -985,6 +1029,7 @@ __device__ __forceinline__ void mom_site(const StepArgs<R>& a, const Coef& c, in
                                          const PlaneLds& pl = PlaneLds{}) {
     using V4 = typename V4T<R>::type;
     static_assert(G == 4 || ND == 2, "a Higgs-family field acts on two doublets");
+    constexpr bool NB = G == 7;
 
     // 32-bit site offsets (the host limits a lattice to 2^28 sites): addresses
     // become a scalar base plus one 32-bit vector offset, with no 64-bit math.
@@ -1047,7 +1092,7 @@ __device__ __forceinline__ void mom_site(const StepArgs<R>& a, const Coef& c, in
     for (int i = 0; i < 3; i++) {
         U[i] = Q(i, 0, 0, 0, 0);
         yv[i] = Y(i, 0, 0, 0, 0);
-        sincos_fast(yv[i], &sy_[i], &cy[i]);
+        sincos_k<NB>(yv[i], &sy_[i], &cy[i]);
     }
 
     R phi[4 * ND], praw[4 * ND];
@@ -1065,7 +1110,7 @@ __device__ __forceinline__ void mom_site(const StepArgs<R>& a, const Coef& c, in
     };
     auto FAM = [&](int f, int i, int si) -> FamLink<R> {
         FamLink<R> l{};
-        if constexpr (G == 5) sincos_fast(FY(f, i, si, 0, 0), &l.s, &l.c);
+        if constexpr (G == 5) sincos_nb(FY(f, i, si, 0, 0), &l.s, &l.c);
         else if constexpr (G == 7) l.v = FQ(f, i, si, 0, 0);
         return l;
     };
@@ -1096,7 +1141,7 @@ __device__ __forceinline__ void mom_site(const StepArgs<R>& a, const Coef& c, in
         for (int i = 0; i < 3; i++) {
             const Qt<R> Um = Q(i, i, -1, 0, 0);
             R cm, smn;
-            sincos_fast(Y(i, i, -1, 0, 0), &smn, &cm);
+            sincos_k<NB>(Y(i, i, -1, 0, 0), &smn, &cm);
             const FamLink<R> Fm = FAM(i, i, -1);
             J[i][0] = J[i][1] = J[i][2] = J[i][3] = R(0);
             JF[i][0] = JF[i][1] = JF[i][2] = R(0);
@@ -1125,7 +1170,7 @@ __device__ __forceinline__ void mom_site(const StepArgs<R>& a, const Coef& c, in
     for (int i = 0; i < 3; i++) {
         const Qt<R> Um = Q(i, i, -1, 0, 0);
         R cm, smn;
-        sincos_fast(Y(i, i, -1, 0, 0), &smn, &cm);
+        sincos_k<NB>(Y(i, i, -1, 0, 0), &smn, &cm);
         J[i][0] = J[i][1] = J[i][2] = J[i][3] = R(0);
 #pragma unroll
         for (int k = 0; k < ND; k++) {
@@ -1184,7 +1229,7 @@ __device__ __forceinline__ void mom_site(const StepArgs<R>& a, const Coef& c, in
             R v[3], w[3];
             qvec(A, C, v);
             qrot(C, v, w);
-            const R st = sin_fast(th);
+            const R st = sin_k<NB>(th);
             M[i][0] += a.inv_sp[j] * st;
             M[j][0] -= a.inv_sp[i] * st;
 #pragma unroll
@@ -1212,7 +1257,7 @@ __device__ __forceinline__ void mom_site(const StepArgs<R>& a, const Coef& c, in
                 const R th = (yv[i] - yim) + (yjm - yjd);
                 qvec(qmul(qconj(Ujd), U[i]), qmul(Ujm, qconj(Uim)), v);
                 qrot(Ujd, v, w);
-                M[i][0] -= a.inv_sp[j] * sin_fast(th);
+                M[i][0] -= a.inv_sp[j] * sin_k<NB>(th);
 #pragma unroll
                 for (int m = 0; m < 3; m++) M[i][1 + m] -= a.inv_sp[j] * w[m];
             } else {
@@ -1220,7 +1265,7 @@ __device__ __forceinline__ void mom_site(const StepArgs<R>& a, const Coef& c, in
                 const R th = (yjd - yjm) + (yim - yv[i]);
                 qvec(qmul(qconj(U[i]), Ujd), qmul(Uim, qconj(Ujm)), v);
                 qrot(U[i], v, w);
-                M[i][0] += a.inv_sp[j] * sin_fast(th);
+                M[i][0] += a.inv_sp[j] * sin_k<NB>(th);
 #pragma unroll
                 for (int m = 0; m < 3; m++) M[i][1 + m] += a.inv_sp[j] * w[m];
             }
@@ -1261,7 +1306,7 @@ template <class R>
 struct Plaq {
     R v[3], wc[3], wu[3], st;
 };
-template <class R>
+template <bool NB, class R>
 __device__ __forceinline__ void plaquette(const Qt<R>& Ulo, const Qt<R>& Uhi, const Qt<R>& Uhi_lo, const Qt<R>& Ulo_hi,
                                           R ylo, R yhi, R yhi_lo, R ylo_hi, Plaq<R>& p) {
     const R th = (ylo_hi - ylo) + (yhi - yhi_lo);
@@ -1270,7 +1315,7 @@ __device__ __forceinline__ void plaquette(const Qt<R>& Ulo, const Qt<R>& Uhi, co
     qvec(A, C, p.v);
     qrot(C, p.v, p.wc);
     qrot(Uhi_lo, p.v, p.wu);
-    p.st = sin_fast(th);
+    p.st = sin_k<NB>(th);
 }
 
 // Shared values in LDS, one float4 per site of the tile plus ring row and ring
@@ -1312,6 +1357,7 @@ template <int ND, bool CMP, bool EARLY = true, int G = 4>
 __device__ __forceinline__ void mom_block_shared(const StepArgs<float>& a, const Coef& c, int x, int y0, int z0, int ty,
                                                  int tz, bool valid, const PlaneLds& pl, char* lds) {
     using R = float;
+    constexpr bool NB = G == 7;
     const int bz = a.bz, W = pl.w;
     const int sx = a.ny * a.nz;
     const int xp = (x == a.nx - 1) ? 0 : x + 1, xm = (x == 0) ? a.nx - 1 : x - 1;
@@ -1346,7 +1392,7 @@ __device__ __forceinline__ void mom_block_shared(const StepArgs<float>& a, const
     auto FQ = [&](int f, int i, int si, int j, int sj) -> Qt<R> { return toq<float4, R>(at(a.fq[f], fsite(i, si, j, sj))); };
     auto FAMAT = [&](int f, unsigned site) -> FamLink<R> {
         FamLink<R> l{};
-        if constexpr (G == 5) sincos_fast(at(a.fy[f], site), &l.s, &l.c);
+        if constexpr (G == 5) sincos_nb(at(a.fy[f], site), &l.s, &l.c);
         else if constexpr (G == 7) l.v = toq<float4, R>(at(a.fq[f], site));
         return l;
     };
@@ -1363,7 +1409,7 @@ __device__ __forceinline__ void mom_block_shared(const StepArgs<float>& a, const
         for (int i = 0; i < 3; i++) {
             U[i] = toq<float4, R>(pl.q[i][r0]);
             yv[i] = pl.y[i][r0];
-            sincos_fast(yv[i], &sy_[i], &cy[i]);
+            sincos_k<NB>(yv[i], &sy_[i], &cy[i]);
         }
 #pragma unroll
         for (int k = 0; k < ND; k++) {
@@ -1424,7 +1470,7 @@ __device__ __forceinline__ void mom_block_shared(const StepArgs<float>& a, const
         {
             const Qt<R> Um = toq<float4, R>(at(a.q[0], sm0));
             R cm, smn;
-            sincos_fast(at(a.y[0], sm0), &smn, &cm);
+            sincos_k<NB>(at(a.y[0], sm0), &smn, &cm);
             if constexpr (G != 4) {
                 R fm[8], tm[8];
 #pragma unroll
@@ -1469,7 +1515,7 @@ __device__ __forceinline__ void mom_block_shared(const StepArgs<float>& a, const
                 }
                 Ulo_hi = toq<float4, R>(pl.q[i][r0 + step[j]]);
                 ylo_hi = pl.y[i][r0 + step[j]];
-                plaquette(U[i], U[j], Uhi_lo, Ulo_hi, yv[i], yv[j], yhi_lo, ylo_hi, pq);
+                plaquette<NB>(U[i], U[j], Uhi_lo, Ulo_hi, yv[i], yv[j], yhi_lo, ylo_hi, pq);
                 M[i][0] += a.inv_sp[j] * pq.st;
                 M[j][0] -= a.inv_sp[i] * pq.st;
 #pragma unroll
@@ -1490,7 +1536,7 @@ __device__ __forceinline__ void mom_block_shared(const StepArgs<float>& a, const
             for (int i = 1; i < 3; i++) {
                 const unsigned sd = i == 1 ? sm0p1 : sm0p2;       // x + e_i - e_0
                 Plaq<R> pq;
-                plaquette(U0m, toq<float4, R>(at(a.q[i], sm0)), U[i], toq<float4, R>(at(a.q[0], sd)), y0m,
+                plaquette<NB>(U0m, toq<float4, R>(at(a.q[i], sm0)), U[i], toq<float4, R>(at(a.q[0], sd)), y0m,
                           at(a.y[i], sm0), yv[i], at(a.y[0], sd), pq);
                 M[i][0] += a.inv_sp[0] * pq.st;
 #pragma unroll
@@ -1526,7 +1572,7 @@ __device__ __forceinline__ void mom_block_shared(const StepArgs<float>& a, const
             const Qt<R> Ur = rdir == 1 ? Ub[1] : Ub[2];
             const R yr = rdir == 1 ? yb[1] : yb[2];
             R cr, sr;
-            sincos_fast(yr, &sr, &cr);
+            sincos_k<NB>(yr, &sr, &cr);
             if constexpr (G != 4) {
                 const unsigned sb = static_cast<unsigned>(x * sx + yy * a.nz + zz);   // the ring site b
                 R fb[8], fn[8], tx[8];
@@ -1565,13 +1611,13 @@ __device__ __forceinline__ void mom_block_shared(const StepArgs<float>& a, const
             const R yr = rdir == 1 ? yb[1] : yb[2];
             const float4 qrn = rdir == 1 ? at(a.q[1], sbp0) : at(a.q[2], sbp0);
             const R yrn = rdir == 1 ? at(a.y[1], sbp0) : at(a.y[2], sbp0);
-            plaquette(Ub[0], Ur, toq<float4, R>(qrn), toq<float4, R>(pl.q[0][rbr]), yb[0], yr, yrn, pl.y[0][rbr], pq);
+            plaquette<NB>(Ub[0], Ur, toq<float4, R>(qrn), toq<float4, R>(pl.q[0][rbr]), yb[0], yr, yrn, pl.y[0][rbr], pq);
             ring_p0 = pack_wu(pq);
         }
         // Plaquette (1, 2) based at b: U_2(b + e_1), U_1(b + e_2) in the plane.
         {
             Plaq<R> pq;
-            plaquette(Ub[1], Ub[2], toq<float4, R>(pl.q[2][rb + W]), toq<float4, R>(pl.q[1][rb + 1]), yb[1], yb[2],
+            plaquette<NB>(Ub[1], Ub[2], toq<float4, R>(pl.q[2][rb + W]), toq<float4, R>(pl.q[1][rb + 1]), yb[1], yb[2],
                       pl.y[2][rb + W], pl.y[1][rb + 1], pq);
             ring_p12 = pack_wu(pq);
         }

@@ -154,14 +154,15 @@ def test_family_momentum_gauss_law_conserved(rt, G, form, tol):
 
 @pytest.mark.parametrize("G", [5, 7], ids=["g5", "g7"])
 def test_family_momentum_launch_shapes_bitwise_identical(rt, G):
-    """Tiles, block orders and the LDS staging of the SU(2) x U(1) plane change
-    where values come from, never the arithmetic (the family links are read
-    from global memory in every mode)."""
+    """Tiles, block orders and the staging modes (LDS plane tile; shared
+    SU(2) x U(1) plaquettes and covariant differences) change where values
+    come from, never the arithmetic: ragged tiles in every mode."""
     n = (16, 12, 20)
     d = _desc(G, GFT_FORM_MOMENTUM_F32, n)
     s, v = family_state(n, G, False, seed=5)
     cfgs = ((64, 4, 2, 4, 1), (64, 4, 2, 4, 0), (256, 1, 2, 4, 1), (16, 16, 3, 4, 1), (32, 8, 1, 4, 0),
-            (128, 2, 4, 4, 1), (16, 16, 0, 4, 0), (64, 4, 3, 4, 2))
+            (128, 2, 4, 4, 1), (16, 16, 0, 4, 0), (64, 4, 3, 4, 2), (16, 16, 0, 4, 2), (32, 8, 4, 4, 2),
+            (128, 2, 1, 4, 2))
     out = {}
     for c in cfgs:
         with rt.Evolver(d) as ev:
@@ -174,6 +175,38 @@ def test_family_momentum_launch_shapes_bitwise_identical(rt, G):
         nd_ = int((out[c][0] != ref[0]).sum() + (out[c][1] != ref[1]).sum())
         print(f"launch {c}: {nd_} values differ from {cfgs[0]}")
         assert nd_ == 0, c
+
+
+@pytest.mark.parametrize("G", [5, 7], ids=["g5", "g7"])
+@pytest.mark.parametrize("form,tol", [(GFT_FORM_MOMENTUM_F64, 1e-11), (GFT_FORM_MOMENTUM_F32, 2e-5)],
+                         ids=["f64", "f32"])
+def test_family_momentum_fixed_faces_and_damping_match_oracle(rt, G, form, tol):
+    """Fixed faces (both y faces and both x faces) and the damped first steps
+    (Model::update, Core/Source/Model.cpp:241-254) with the family fields: the
+    family links and electric fields take the frozen-shell and damping paths
+    of the kick and drift.  Fields vs the FP64 oracle after 30 steps (5 damped)
+    within the periodic tolerances; the shell bitwise unchanged."""
+    n = (8, 6, 10)
+    faces = 4 | 8 | 1 | 2
+    d = make_desc(model=GFT_MODEL_2HDM, form=form, n=n, d=(0.5, 0.45, 0.55), pot2=TWO_HDM_POTENTIAL,
+                  gauge_num=G, gpp=0.5, fixed_faces=faces, damping=1.0, nt_damped=5)
+    s, v = family_state(n, G, False, seed=23)
+    m = O.frozen_mask(n, faces)
+    s[0][m], v[0][m] = s[1][m], v[1][m]
+    with rt.Evolver(d) as ev:
+        ev.import_two_slot(s, v, 0)
+        base_s, base_v, _ = ev.export_two_slot(np.float64)
+    with rt.Evolver(d) as ev:
+        ev.import_two_slot(s, v, 0)
+        ev.evolve(30)
+        gs, gv, it = ev.export_two_slot(np.float64)
+    assert it == 30
+    O.evolve(d, s, v, 30, nthreads=8)
+    cur = (30 + 1) % 2
+    rs, rv = _rel(gs[cur], s[cur]), np.abs(_link_diff(gv[cur], v[cur], G)).max()
+    print(f"gaugeNum {G} form {form}: fixed faces + damping vs the oracle: scalar {rs:.2e}, links {rv:.2e}")
+    assert rs <= tol and rv <= tol
+    assert np.array_equal(gs[:, m], base_s[:, m]) and np.array_equal(gv[:, m], base_v[:, m])
 
 
 @pytest.mark.parametrize("G,form", [(5, GFT_FORM_MOMENTUM_F32), (7, GFT_FORM_MOMENTUM_F32),
