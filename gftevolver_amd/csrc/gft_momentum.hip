@@ -192,6 +192,25 @@ __device__ __forceinline__ float asin_small(float x) {
 __device__ __forceinline__ void sincos_fast(double x, double* s, double* c) { sincos(x, s, c); }
 __device__ __forceinline__ double sin_fast(double x) { return sin(x); }
 __device__ __forceinline__ double asin_small(double x) { return asin(x); }
+// asin_small without the branch: the library asinf and the series are both
+// evaluated and |x| selects (bitwise asin_small's result).  It pays only in
+// the family-U(1) kernels (gaugeNum 5: 192^3 step 0.913 -> 0.730 ms); the
+// SM kernels lose 1.4% with it, the other 2HDM kernels 0-0.6%.
+__device__ __forceinline__ float asin_nb(float x) {
+    const float big = asinf(x);
+    const float x2 = x * x;
+    float p = fmaf(x2, 2.23721590909090909e-2f, 3.03819444444444444e-2f);
+    p = fmaf(x2, p, 4.46428571428571429e-2f);
+    p = fmaf(x2, p, 7.5e-2f);
+    p = fmaf(x2, p, 1.66666666666666667e-1f);
+    return fabsf(x) > 0.125f ? big : fmaf(x * x2, p, x);
+}
+__device__ __forceinline__ double asin_nb(double x) { return asin(x); }
+template <bool NB, class R>
+__device__ __forceinline__ R asin_k(R x) {
+    if constexpr (NB) return asin_nb(x);
+    else return asin_small(x);
+}
 
 // The step's trigonometric functions: NB selects the branch-free forms.  For
 // the U(1) phases they pay only in the family-SU(2) kernels (gaugeNum 7:
@@ -719,7 +738,8 @@ __device__ __forceinline__ void fold_current(const StepArgs<R>& a, const R (*J)[
 // U1: no damping (keep = 1), with the kick's keep products dropped (exact).
 // praw: the site's stored doublet(s) (phi - ref), which the drift advances.
 // M: the link forces with the gauge source already folded in (fold_current).
-template <class R, int ND, bool CMP, bool PRE = false, bool U1 = false, bool FZ = true>
+template <class R, int ND, bool CMP, bool PRE = false, bool U1 = false, bool FZ = true, bool NBA = false>
+// NBA: the branch-free asin (asin_nb).
 // frozen: a site of a fixed face's shell -- every lane still stores, but a
 // frozen lane stores the values it read, so every store instruction writes
 // whole cache lines (a divergent copy branch would write partial lines twice).
@@ -813,7 +833,7 @@ __device__ __forceinline__ void kick_drift_t(const StepArgs<R>& a, unsigned s, c
         }
         const R e2 = fma(en.w, en.w, fma(en.z, en.z, en.y * en.y));
         if (!frozen && (!(e2 < R(1)) || !(fabs(en.x) < R(1)))) bad = 1;
-        const R yn = wrap_pi(yv[i] + asin_small(en.x));
+        const R yn = wrap_pi(yv[i] + asin_k<NBA>(en.x));
         const Qt<R> u{sqrt_unit(R(1) - e2), en.y, en.z, en.w};
         Qt<R> qn = qmul(u, U[i]);
         const R inv = rsqrt_r(fma(qn.z, qn.z, fma(qn.y, qn.y, fma(qn.x, qn.x, qn.w * qn.w))));
@@ -842,16 +862,16 @@ __device__ __forceinline__ void kick_drift_t(const StepArgs<R>& a, unsigned s, c
 
 // The kick with uniform branches on damping (keep = 1 takes the shorter path)
 // and on fixed faces (a periodic lattice has no frozen lane).
-template <class R, int ND, bool CMP, bool PRE = false>
+template <class R, int ND, bool CMP, bool PRE = false, bool NBA = false>
 __device__ __forceinline__ void kick_drift(const StepArgs<R>& a, unsigned s, const Qt<R>* U, const R* yv,
                                            const R* praw, const R* F, const R (*M)[4], const OwnState<R, ND>* o,
                                            bool frozen, bool any_fixed) {
     if (!any_fixed) {
-        if (a.keep == R(1)) kick_drift_t<R, ND, CMP, PRE, true, false>(a, s, U, yv, praw, F, M, o, false);
-        else kick_drift_t<R, ND, CMP, PRE, false, false>(a, s, U, yv, praw, F, M, o, false);
+        if (a.keep == R(1)) kick_drift_t<R, ND, CMP, PRE, true, false, NBA>(a, s, U, yv, praw, F, M, o, false);
+        else kick_drift_t<R, ND, CMP, PRE, false, false, NBA>(a, s, U, yv, praw, F, M, o, false);
     } else {
-        if (a.keep == R(1)) kick_drift_t<R, ND, CMP, PRE, true, true>(a, s, U, yv, praw, F, M, o, frozen);
-        else kick_drift_t<R, ND, CMP, PRE, false, true>(a, s, U, yv, praw, F, M, o, frozen);
+        if (a.keep == R(1)) kick_drift_t<R, ND, CMP, PRE, true, true, NBA>(a, s, U, yv, praw, F, M, o, frozen);
+        else kick_drift_t<R, ND, CMP, PRE, false, true, NBA>(a, s, U, yv, praw, F, M, o, frozen);
     }
 }
 
@@ -881,7 +901,7 @@ __device__ __forceinline__ void kick_drift_family_t(const StepArgs<R>& a, unsign
                 en = fma(a.dt2, MF[i][0], U1 ? e : a.keep * e);
             }
             if (!frozen && !(fabs(en) < R(1))) bad = 1;
-            const R yn = wrap_pi(fyv[i] + asin_small(en));
+            const R yn = wrap_pi(fyv[i] + asin_k<true>(en));
             st_nt(frozen ? e : en, ptr_at(a.fe[i], s));
             st_nt(frozen ? fyv[i] : yn, ptr_at(a.fy_out[i], s));
         }
@@ -1273,7 +1293,7 @@ __device__ __forceinline__ void mom_site(const StepArgs<R>& a, const Coef& c, in
         if (j == 0) fold_current(a, J, M);
     }
 
-    kick_drift<R, ND, CMP>(a, s, U, yv, praw, F, M, nullptr, frozen, c.fixed != 0);
+    kick_drift<R, ND, CMP, false, G == 5>(a, s, U, yv, praw, F, M, nullptr, frozen, c.fixed != 0);
 
     if constexpr (G != 4) {
         // Higgs-family staples, source and kick.
@@ -1692,7 +1712,7 @@ __device__ __forceinline__ void mom_block_shared(const StepArgs<float>& a, const
         M[1][2] -= a.inv_sp[2] * p12b.y;
         M[1][3] -= a.inv_sp[2] * p12b.z;
     }
-    kick_drift<R, ND, CMP, true>(a, s, U, yv, praw, F, M, &own, frozen, c.fixed != 0);
+    kick_drift<R, ND, CMP, true, G == 5>(a, s, U, yv, praw, F, M, &own, frozen, c.fixed != 0);
     if constexpr (G != 4) kick_drift_family<R, G, CMP>(a, s, fyv, FV, MF, frozen, c.fixed != 0);
 }
 
