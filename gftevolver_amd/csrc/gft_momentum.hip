@@ -2271,10 +2271,18 @@ inline std::vector<int4> build_tiles(int member, int nx, int ny, int nz, int bz,
 // into 8 contiguous ranges (the ensemble kernel's XCD assignment), a large
 // member gives each XCD two y-bands to sweep along x one after the other, as
 // block order 3 does for a single lattice, so the x +- 1 planes a tile reads
-// stay in that XCD's L2.
+// stay in that XCD's L2.  A member whose tiles span its z-rows in one or two
+// tiles and whose x-plane (148 stored bytes read per site) fits in 2.6 MB is
+// one band: its XCD sweeps whole planes, so the y-neighbours of a tile row are
+// read from the plane in flight rather than from a band swept long before
+// (ns per site-update, homogeneous ensembles: 32^3 0.0604 -> 0.0586, 64^3
+// 0.0603 -> 0.0588, 128^3 0.0654 -> 0.0640; the 48^3 and 96^3 members, three
+// z-tiles per row, measured no gain and keep their bands; the mixed
+// 64-member workload 0.906 -> 0.896 ms).
 inline std::vector<int4> build_tiles_banded(int member, int nx, int ny, int nz, int bz, int by) {
     const int nbz = (nz + bz - 1) / bz, nby = (ny + by - 1) / by;
-    const int br = (nby + 15) / 16;
+    const bool one_band = nbz <= 2 && static_cast<double>(ny) * nz * 148.0 <= 2.6e6;
+    const int br = one_band ? nby : (nby + 15) / 16;
     std::vector<int4> t;
     t.reserve((size_t)nbz * nby * nx);
     for (int b0 = 0; b0 < nby; b0 += br)
