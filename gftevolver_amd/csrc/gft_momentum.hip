@@ -738,8 +738,8 @@ __device__ __forceinline__ void fold_current(const StepArgs<R>& a, const R (*J)[
 // U1: no damping (keep = 1), with the kick's keep products dropped (exact).
 // praw: the site's stored doublet(s) (phi - ref), which the drift advances.
 // M: the link forces with the gauge source already folded in (fold_current).
+// NBA: the link update's asin in its branch-free form (asin_nb).
 template <class R, int ND, bool CMP, bool PRE = false, bool U1 = false, bool FZ = true, bool NBA = false>
-// NBA: the branch-free asin (asin_nb).
 // frozen: a site of a fixed face's shell -- every lane still stores, but a
 // frozen lane stores the values it read, so every store instruction writes
 // whole cache lines (a divergent copy branch would write partial lines twice).
