@@ -598,13 +598,16 @@ static void finish_gauss(gft_ctx* ctx, const double* partial, long long nblocks)
 template <class T, bool GEN, int G>
 static void twoslot_step_as(gft_ctx* ctx, long long nblocks, unsigned now, bool me, double damping_step) {
     // The doublet count as a template parameter where that measured faster
-    // (quaternion links with gaugeNum 4 or 5), at run time otherwise.
+    // (quaternion links with gaugeNum 4 or 5; FP32 17-21% at 128^3-256^3), at
+    // run time otherwise -- also for the FP64 SM kernel, whose compile-time
+    // form is 3% faster at 128^3 but 10% slower at 256^3 (11.03 against
+    // 9.95 ms).
     if constexpr (GEN || G == 7) {
         twoslot_step<T, GEN, G, 0><<<nblocks, kBlock, 0, ctx->stream>>>(ctx->coef, (T*)ctx->S, (T*)ctx->V, now, me,
                                                                          damping_step, ctx->partial, ctx->err_dev);
     } else if constexpr (G == 4) {
         if (ctx->coef.nd == 1)
-            twoslot_step<T, GEN, G, 1><<<nblocks, kBlock, 0, ctx->stream>>>(
+            twoslot_step<T, GEN, G, sizeof(T) == 8 ? 0 : 1><<<nblocks, kBlock, 0, ctx->stream>>>(
                 ctx->coef, (T*)ctx->S, (T*)ctx->V, now, me, damping_step, ctx->partial, ctx->err_dev);
         else
             twoslot_step<T, GEN, G, 2><<<nblocks, kBlock, 0, ctx->stream>>>(
