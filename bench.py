@@ -205,10 +205,11 @@ def options_leg(rt, with_cpu):
     fast path, the generator form (boundary conversion and the two-slot
     kernel), the 2HDM with SU(2) x U(1) links and with Higgs-family U(1) /
     SU(2) gauge fields in the reference-order two-slot form and in the fast
-    path, and the global O(4) model (two-slot form)."""
+    path, the global O(4) model (two-slot form), and the 2HDM workload
+    lattice (192^3) in the FP32 and FP64 momentum forms."""
     import numpy as np
-    from gftevolver_amd._abi import (GFT_FORM_MOMENTUM_F32, GFT_FORM_TWO_SLOT_F32, GFT_MODEL_2HDM,
-                                     GFT_MODEL_GLOBAL, TWO_HDM_POTENTIAL, make_desc)
+    from gftevolver_amd._abi import (GFT_FORM_MOMENTUM_F32, GFT_FORM_MOMENTUM_F64, GFT_FORM_TWO_SLOT_F32,
+                                     GFT_MODEL_2HDM, GFT_MODEL_GLOBAL, TWO_HDM_POTENTIAL, make_desc)
     out = {}
 
     def timed(desc, steps, init=None, state=None):
@@ -280,6 +281,24 @@ def options_leg(rt, with_cpu):
         ref = reference_program_rate(2, 32, 60, 4, 1)
         r["cpu_reference"] = ref[0] if ref else None
     out["global_o4_two_slot_f32"] = r
+    # The 2HDM workload lattice: 192^3, the two-Higgs-doublet potential, both
+    # doublets GRFs of rms 0.1 around vevs 0.8 and 0.6; FP32-bulk against FP64
+    # (accuracy over its 5000 steps: tests/test_gpu_workloads.py).
+    n192 = (192,) * 3
+
+    def grf2(ev):
+        ev.init_grf(seed=20261204, rms=0.1, slope=-1.0, vevs=(0.8, 0.6))
+
+    r32 = timed(make_desc(model=GFT_MODEL_2HDM, form=GFT_FORM_MOMENTUM_F32, n=n192, pot2=TWO_HDM_POTENTIAL), 200,
+                grf2)
+    r64 = timed(make_desc(model=GFT_MODEL_2HDM, form=GFT_FORM_MOMENTUM_F64, n=n192, pot2=TWO_HDM_POTENTIAL), 20,
+                grf2)
+    # 43 FP32 reals per site read and written once (two doublets' phi and P,
+    # the links and their electric fields).
+    r32["achieved_GBps"] = 344 * 192 ** 3 / (r32["kernel_ms"] * 1e-3) / 1e9
+    out["2hdm_192_momentum_f32"] = r32
+    out["2hdm_192_momentum_f64"] = r64
+    out["2hdm_192_speedup_fp32_vs_fp64"] = r64["kernel_ms"] / r32["kernel_ms"]
     for k, v in out.items():
         log(f"option {k}: {v}")
     return out
